@@ -1,0 +1,178 @@
+/*
+ * raft_slantstack.h -- exported C ABI of pyraft_amd/lib/libraft<EXT_SUFFIX>.
+ *
+ * Two groups of symbols:
+ *
+ *  1. Drop-in symbols with the reference's names and signatures (void, by-value
+ *     structs / host float pointers).  They run on the current HIP device and
+ *     print the failure reason to stderr (the reference is silent, see
+ *     raft/raft_radon_gpu.cu:16-35).
+ *
+ *  2. New stream-ordered device-pointer entry points (prefix raft_ss_) that return
+ *     0 or a negative RAFT_SS_E* code; raft_ss_last_error() gives the message.
+ *     They have no reference counterpart; they are what the drop-in symbols and the
+ *     Python package are built on.
+ *
+ * Layouts (row-major, FP32 on the device):
+ *   image    [nslices][N][N], row r <-> y = ymin + r*dy (row 0 at the BOTTOM, as in
+ *            raft/raft_radon_slantstack.c:146,184), column c <-> x = xmin + c*dx
+ *   sinogram [nslices][R][A] (angle contiguous, as pyraft's (rays, angles) arrays);
+ *            the *_range variants accept/produce the angle-major [nslices][A][R] layout
+ *            when angle_major != 0.
+ * Corners are {tl_x, tl_y, br_x, br_y} exactly as in raft_image (raft/raft_image.h:11-19);
+ * the sinogram's x axis is theta (theta_j = tl_x + j*(br_x-tl_x)/A) and its y axis is t
+ * (t_k = br_y + k*(tl_y-br_y)/(R-1)).
+ *
+ * Streams are hipStream_t passed as void* (NULL = the default stream).  A plan is
+ * immutable after creation and may be shared by concurrent calls on one device.
+ */
+#ifndef RAFT_SLANTSTACK_H
+#define RAFT_SLANTSTACK_H
+
+#include "raft_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define RAFT_SS_OK 0
+#define RAFT_SS_EINVAL (-1)    /* bad argument / unsupported geometry                */
+#define RAFT_SS_EHIP (-2)      /* a HIP runtime call failed                           */
+#define RAFT_SS_EINTERNAL (-3) /* internal consistency check failed                   */
+
+const char *raft_ss_last_error(void);
+int raft_ss_version(void);
+
+/* ================================================================ drop-in symbols */
+
+/* Replaces raft_radon_slantstack (raft/raft_radon_slantstack.h:10-12), bound at
+ * pyraft/raftypes.py:159-160 and called from pyraft.radon(..., method=SLANTSTACK)
+ * (pyraft/sinogram/sino.py:201-202).  FP64 strided input/output, FP32 on the GPU;
+ * nthreads is accepted and ignored. */
+void raft_radon_slantstack(raft_image phantom, raft_image radon, int nthreads);
+
+/* Replaces raft_backprojection_slantstack (raft/raft_backprojection_slantstack.h:10-13),
+ * bound at pyraft/raftypes.py:161-162, called from pyraft.backprojection(sino, plan)
+ * (pyraft/backprojection/backprojection.py:206-207). */
+void raft_backprojection_slantstack(raft_image sinogram, raft_image backprojection, int nthreads);
+
+/* Exact transpose of raft_radon_slantstack in the same by-value style.  No reference
+ * counterpart: pyraft.radon_transpose supports only BRESENHAM
+ * (pyraft/backprojection/backprojection.py:229-232); this is its SLANTSTACK case. */
+void raft_radon_transpose_slantstack(raft_image sinogram, raft_image image, int nthreads);
+
+/* Status (0 or RAFT_SS_E*) of the last drop-in call above on this thread: the drop-in
+ * symbols return void like the reference's, so callers that want to raise use this. */
+int raft_ss_legacy_status(void);
+
+/* Replaces raft_radon_slantstack_gpu (raft/raft_radon_gpu.h:8; pyraft/raftypes.py:208),
+ * called by pyraft.radon_gpu (pyraft/sinogram/sino.py:377).  Host float buffers,
+ * default corners (image [-1,1]^2, theta in [0,pi), t in [-1,1]).  Computes the
+ * CPU slant-stack semantics, not the reference CUDA kernel's theta = pi - j*pi/A
+ * texture variant. */
+void raft_radon_slantstack_gpu(float *h_output, float *h_input, int sizeImage, int nrays, int nangles);
+
+/* Replaces raft_backprojection_slantstack_gpu (raft/raft_backprojection_gpu.h:8;
+ * pyraft/raftypes.py:206), called by backprojection_gpu and fbp_gpu
+ * (pyraft/backprojection/backprojection.py:254, pyraft/inversion/inversion.py:56).
+ * Linear interpolation, no column mirroring (see raft_radon_slantstack_gpu). */
+void raft_backprojection_slantstack_gpu(float *image, float *sino, int sizeImage, int nrays, int nangles);
+
+/* Replaces raft_em_gpu (raft/raft_em_gpu.cu:20; pyraft/raftypes.py:211), called by
+ * pyraft.em_gpu (pyraft/iterative/em.py:202).  x0 = 1, niter plain EM iterations
+ * with the slant-stack pair; zero projections give a zero ratio instead of Inf/NaN. */
+void raft_em_gpu(float *output, float *sino, int sizeImage, int nrays, int nangles, int niter);
+
+/* Replaces raft_image_phantom_fromdesc (raft/raft_image_phantom_fromdesc.h:26;
+ * pyraft/raftypes.py:138) used by pyraft.shepplogan (pyraft/phantom/phantom.py:49).
+ * Adds into the image.  A description with < 6 columns is an error (the reference
+ * frees the caller's buffer there, raft/raft_image_phantom_fromdesc.c:11-15). */
+void raft_image_phantom_fromdesc(raft_image image, raft_matrix desc);
+
+/* Replaces raft_radon_fromdesc (raft/raft_radon_fromdesc.h:23-24; pyraft/raftypes.py:140),
+ * the exact Radon transform of an ellipse description used by pyraft.radon(ndarray)
+ * (pyraft/sinogram/sino.py:209-212).  Adds into the sinogram. */
+void raft_radon_fromdesc(raft_image sino, raft_matrix desc);
+
+/* ================================================================ plans */
+
+typedef struct raft_ss_plan raft_ss_plan;
+
+/* Build the geometry plan for an N x N image and an R x A sinogram: the FP64 angle
+ * table (glibc cos/sin/tan, as the reference evaluates them), the exact per-ray
+ * sample ranges of the slant-stack predicate and the backprojection staging
+ * parameters.  Synchronises `stream`. */
+int raft_ss_plan_create(raft_ss_plan **plan, int n, int nrays, int nangles, const double *img_corners,
+                        const double *sino_corners, void *stream);
+int raft_ss_plan_destroy(raft_ss_plan *plan);
+/* info[0..6] = n, nrays, nangles, #irregular rays, #degenerate angles, bp window, bp angles/stage */
+int raft_ss_plan_info(const raft_ss_plan *plan, int *info, int ninfo);
+/* copy the per-ray sample ranges [A][R] x (lo, hi) to host memory */
+int raft_ss_plan_bounds(const raft_ss_plan *plan, int *host_bounds);
+/* copy the per-(angle, 64-ray group) union/intersection of the ranges, [A][ceil(R/64)] x
+ * (union lo, union hi, intersection lo, intersection hi); (0,-1,0,-1) for an empty group */
+int raft_ss_plan_wave_bounds(const raft_ss_plan *plan, int *host_wbounds);
+/* the same ranges (and per-angle Q32.32 alpha0/gamma/beta, branch flags) computed on
+ * the host, for checking index ranges without a GPU */
+int raft_ss_plan_host_bounds(int n, int nrays, int nangles, const double *img_corners, const double *sino_corners,
+                             int *host_bounds, long long *host_fix, int *host_branch);
+
+/* ================================================================ device-pointer operators */
+
+/* Slant-stack Radon of nslices images (angle columns [j0, j1) of the sinogram are
+ * written; the rest is untouched). */
+int raft_ss_radon(const raft_ss_plan *plan, const float *d_image, float *d_sino, int nslices, void *stream);
+int raft_ss_radon_range(const raft_ss_plan *plan, const float *d_image, float *d_out, int angle_major, int j0, int j1,
+                        int nslices, void *stream);
+
+/* Slant-stack backprojection (sum over angles [j0, j1) times dtheta). */
+int raft_ss_backprojection(const raft_ss_plan *plan, const float *d_sino, float *d_image, int nslices, void *stream);
+int raft_ss_backprojection_range(const raft_ss_plan *plan, const float *d_sino, float *d_image, int angle_major,
+                                 int j0, int j1, int nslices, void *stream);
+
+/* Exact matched transpose of raft_ss_radon (no reference counterpart): the adjoint
+ * that the autograd backward needs, <R x, y> == <x, R^T y> to rounding. */
+int raft_ss_radon_transpose(const raft_ss_plan *plan, const float *d_sino, float *d_image, int nslices, void *stream);
+int raft_ss_radon_transpose_range(const raft_ss_plan *plan, const float *d_sino, float *d_image, int angle_major,
+                                  int j0, int j1, int nslices, void *stream);
+
+/* FBP row filter of pyraft.lowpassino_fbp (pyraft/filters/filters.py:36-68): each angle
+ * column is circularly convolved along t with g = Re(ifft(fftshift(|w| cos(pi w dt)))). */
+int raft_ss_fbp_filter(raft_ss_plan *plan, const float *d_sino, float *d_out, int angle_major, int nslices,
+                       void *stream);
+/* The FP32 convolution kernel g (length R) of lowpassino_fbp for a detector grid
+ * t = linspace(tmin, tmax, R), computed in FP64 on the host. */
+int raft_ss_ramp_kernel(int R, double tmin, double tmax, float *host_g);
+/* Circular convolution of nrows contiguous length-R rows with the device kernel d_g. */
+int raft_ss_filter_rows(const float *d_in, float *d_out, const float *d_g, int R, int nrows, void *stream);
+/* filter + backprojection (pyraft/inversion/inversion.py:48-56) */
+int raft_ss_fbp(raft_ss_plan *plan, const float *d_sino, float *d_image, int nslices, void *stream);
+
+/* Ordered-subsets EM (nsubsets = 1: the EM of raft/raft_em_gpu.cu:41-69).  d_x is the
+ * starting image on entry and the estimate on exit.  Per subset: forward projection
+ * with the guarded ratio s/p and FP64 log-likelihood partials fused in its epilogue,
+ * then backprojection with x <- x * BP_b(ratio) / BP_b(1) fused in its epilogue.
+ * ll_host (optional) receives niter values of sum(s log p - p) over rays with p > 0. */
+int raft_ss_em(raft_ss_plan *plan, const float *d_sino, float *d_x, int niter, int nsubsets, int nslices,
+               double *ll_host, void *stream);
+
+/* Rasterise ellipses (rows of 6 doubles: value, a, b, x0, y0, rotation) into a
+ * device FP32 image of lines x cols with corners {tl_x, tl_y, br_x, br_y}
+ * (row 0 at tl_y, as raft/raft_image_phantom_fromdesc.c:18-41). */
+int raft_ss_phantom(const double *desc_rows, int nell, int lines, int cols, const double *corners, float *d_out,
+                    void *stream);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAFT_SLANTSTACK_H */

@@ -1,0 +1,305 @@
+"""FP64 CPU oracle for the PyRAFT slant-stack path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker or the timed CPU
+baseline.  The shipped package ``pyraft_amd`` never imports it.
+
+Contents (each restates the cited reference routine):
+
+* ``radon`` / ``backprojection`` / ``radon_transpose`` -- ctypes calls into
+  ``liboracle.so`` (oracle.c), restating raft/raft_radon_slantstack.c:122-214 and
+  raft/raft_backprojection_slantstack.c:121-181, plus the exact adjoint of the
+  forward (no reference counterpart).
+* ``lowpassino_fbp`` -- numpy restatement of pyraft/filters/filters.py:36-68.
+* ``fbp`` -- filter then backprojection, the composition pyraft/inversion/inversion.py:48-56
+  performs (with the CPU slant-stack backprojection in place of the CUDA one).
+* ``em`` / ``osem`` -- the multiplicative EM recurrence of raft/raft_em_gpu.cu:41-69
+  (x0 = 1, W = BP(1), x <- x * BP(s / R x) / W) composed from the slant-stack
+  Radon and backprojection; OSEM walks contiguous angle blocks each with its own
+  weight BP_b(1) (pyraft/iterative/em.py:48-57, 282-318).  Zero denominators use
+  the convention documented below (the reference divides unguarded,
+  raft/raft_cuda_aux.cu:21).
+* ``loglik`` -- emission log-likelihood sum s*log(p) - sum p (xfct/likelihood.c:83-94).
+* ``phantom_fromdesc`` / ``radon_fromdesc`` -- raft/raft_image_phantom_fromdesc.c:4-59,
+  raft/raft_radon_fromdesc.c:4-67.
+* ``ref_*`` -- the same calls into ``_ref/libraft_ref.so``, the reference's own C
+  compiled from its sources by ``oracle/Makefile`` (present only where it was built).
+
+Conventions shared with the product (stated, not taken from the product):
+  EM ratio   rho = s / p  where p > 0, else 0.
+  EM update  x <- x * BP_b(rho) / W_b  where W_b > 0; pixels with W_b == 0 keep x.
+  LL         sum over rays with p > 0 of s*log(p) - p, accumulated in FP64.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class RaftMatrix(ctypes.Structure):
+    _fields_ = [("p_data", ctypes.POINTER(ctypes.c_double)), ("lines", ctypes.c_int),
+                ("line_stride", ctypes.c_int), ("columns", ctypes.c_int),
+                ("column_stride", ctypes.c_int)]
+
+
+class RaftImage(ctypes.Structure):
+    _fields_ = [("data", RaftMatrix), ("tl_x", ctypes.c_double), ("tl_y", ctypes.c_double),
+                ("br_x", ctypes.c_double), ("br_y", ctypes.c_double)]
+
+
+def _matrix(a):
+    assert a.dtype == np.float64 and a.ndim == 2
+    return RaftMatrix(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0],
+                      a.strides[1] // 8, a.shape[1], a.strides[0] // 8)
+
+
+def _image(a, tl, br):
+    return RaftImage(_matrix(a), float(tl[0]), float(tl[1]), float(br[0]), float(br[1]))
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        _lib = ctypes.CDLL(path)
+        for name in ("oracle_radon_slantstack", "oracle_backprojection_slantstack",
+                     "oracle_radon_transpose_slantstack"):
+            f = getattr(_lib, name)
+            f.argtypes = [RaftImage, RaftImage, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            f.restype = None
+        for name in ("oracle_phantom_fromdesc", "oracle_radon_fromdesc"):
+            f = getattr(_lib, name)
+            f.argtypes = [RaftImage, RaftMatrix]
+            f.restype = None
+        _lib.oracle_radon_sample_map.argtypes = [RaftImage, RaftImage, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_radon_sample_map.restype = None
+        _lib.oracle_bp_sample_map.argtypes = [RaftImage, RaftImage, ctypes.c_void_p]
+        _lib.oracle_bp_sample_map.restype = None
+    return _lib
+
+
+def ref_path():
+    return os.path.join(_HERE, "_ref", "libraft_ref.so")
+
+
+def ref_available():
+    return os.path.exists(ref_path())
+
+
+def ref():
+    """The reference's own C (compiled by oracle/Makefile), or None."""
+    global _ref
+    if _ref is None and ref_available():
+        _ref = ctypes.CDLL(ref_path())
+        for name in ("raft_radon_slantstack", "raft_backprojection_slantstack"):
+            f = getattr(_ref, name)
+            f.argtypes = [RaftImage, RaftImage, ctypes.c_int]
+            f.restype = None
+        for name in ("raft_image_phantom_fromdesc", "raft_radon_fromdesc"):
+            f = getattr(_ref, name)
+            f.argtypes = [RaftImage, RaftMatrix]
+            f.restype = None
+    return _ref
+
+
+IMG_TL, IMG_BR = (-1.0, 1.0), (1.0, -1.0)
+SINO_TL, SINO_BR = (0.0, 1.0), (np.pi, -1.0)
+
+
+def _threads(nthreads):
+    return int(nthreads) if nthreads else (os.cpu_count() or 1)
+
+
+# ---------------------------------------------------------------- projectors
+
+def radon(img, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
+          j0=0, j1=None, nthreads=None, out=None):
+    """Slant-stack Radon (raft/raft_radon_slantstack.c:44-214); returns (nrays, nangles) FP64."""
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    sino = np.zeros((nrays, nangles)) if out is None else out
+    j1 = nangles if j1 is None else j1
+    lib().oracle_radon_slantstack(_image(img, img_tl, img_br), _image(sino, sino_tl, sino_br),
+                                  j0, j1, _threads(nthreads))
+    return sino
+
+
+def backprojection(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
+                   j0=0, j1=None, nthreads=None):
+    """Slant-stack backprojection (raft/raft_backprojection_slantstack.c:43-181)."""
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    img = np.zeros((n, n))
+    j1 = sino.shape[1] if j1 is None else j1
+    lib().oracle_backprojection_slantstack(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br),
+                                           j0, j1, _threads(nthreads))
+    return img
+
+
+def radon_transpose(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
+                    j0=0, j1=None, nthreads=None):
+    """Exact adjoint of ``radon`` (same index pattern, scatter form)."""
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    img = np.zeros((n, n))
+    j1 = sino.shape[1] if j1 is None else j1
+    lib().oracle_radon_transpose_slantstack(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br),
+                                            j0, j1, _threads(nthreads))
+    return img
+
+
+def radon_sample_map(n, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR):
+    """(mask, coord) of shape (A, R, N): which samples the reference accumulates and
+    its FP64 interpolation coordinate for each."""
+    img = np.zeros((n, n))
+    sino = np.zeros((nrays, nangles))
+    mask = np.zeros((nangles, nrays, n), dtype=np.uint8)
+    coord = np.zeros((nangles, nrays, n))
+    lib().oracle_radon_sample_map(_image(img, img_tl, img_br), _image(sino, sino_tl, sino_br),
+                                  mask.ctypes.data, coord.ctypes.data)
+    return mask.astype(bool), coord
+
+
+def bp_sample_map(n, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR):
+    """mask of shape (A, N, N): which (angle, pixel) pairs the reference backprojection uses."""
+    img = np.zeros((n, n))
+    sino = np.zeros((nrays, nangles))
+    mask = np.zeros((nangles, n, n), dtype=np.uint8)
+    lib().oracle_bp_sample_map(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br), mask.ctypes.data)
+    return mask.astype(bool)
+
+
+# ---------------------------------------------------------------- phantoms
+
+def phantom_fromdesc(desc, n, tl=IMG_TL, br=IMG_BR):
+    desc = np.ascontiguousarray(desc, dtype=np.float64)
+    img = np.zeros((n, n))
+    lib().oracle_phantom_fromdesc(_image(img, tl, br), _matrix(desc))
+    return img
+
+
+def radon_fromdesc(desc, nrays, nangles, tl=SINO_TL, br=SINO_BR):
+    desc = np.ascontiguousarray(desc, dtype=np.float64)
+    sino = np.zeros((nrays, nangles))
+    lib().oracle_radon_fromdesc(_image(sino, tl, br), _matrix(desc))
+    return sino
+
+
+# ---------------------------------------------------------------- reference C
+
+def ref_radon(img, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
+              nthreads=1):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    sino = np.zeros((nrays, nangles))
+    ref().raft_radon_slantstack(_image(img, img_tl, img_br), _image(sino, sino_tl, sino_br), nthreads)
+    return sino
+
+
+def ref_backprojection(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
+                       nthreads=1):
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    img = np.zeros((n, n))
+    ref().raft_backprojection_slantstack(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br),
+                                         nthreads)
+    return img
+
+
+def ref_phantom_fromdesc(desc, n, tl=IMG_TL, br=IMG_BR):
+    desc = np.ascontiguousarray(desc, dtype=np.float64)
+    img = np.zeros((n, n))
+    ref().raft_image_phantom_fromdesc(_image(img, tl, br), _matrix(desc))
+    return img
+
+
+def ref_radon_fromdesc(desc, nrays, nangles, tl=SINO_TL, br=SINO_BR):
+    desc = np.ascontiguousarray(desc, dtype=np.float64)
+    sino = np.zeros((nrays, nangles))
+    ref().raft_radon_fromdesc(_image(sino, tl, br), _matrix(desc))
+    return sino
+
+
+# ---------------------------------------------------------------- FBP
+
+def lowpassino_fbp(sino, tl=SINO_TL, br=SINO_BR, t=None):
+    """pyraft/filters/filters.py:36-68: per column Re(ifft(fft(s) * fftshift(h)))."""
+    sino = np.asarray(sino, dtype=np.float64)
+    tmin, tmax = br[1], tl[1]
+    if t is None:
+        t = np.linspace(tmin, tmax, sino.shape[0])
+    R, V = int(sino.shape[0]), int(sino.shape[1])
+    dt = (t[R - 1] - t[0]) / (R - 1)
+    wc = 1 / (2 * dt)
+    w = np.linspace(-wc, wc, R)
+    h = np.abs(w) * np.cos(np.pi * w * dt)
+    G = np.fft.fftshift(np.transpose(np.kron(np.ones((V, 1)), h)))
+    return np.fft.ifft(np.fft.fft(sino, axis=0) * G, axis=0).real
+
+
+def ramp_kernel(R, tl=SINO_TL, br=SINO_BR):
+    """The real circular-convolution kernel g = Re(ifft(fftshift(h))) equivalent to
+    ``lowpassino_fbp`` (s is real, so only the even part of the spectrum acts)."""
+    tmin, tmax = br[1], tl[1]
+    t = np.linspace(tmin, tmax, R)
+    dt = (t[R - 1] - t[0]) / (R - 1)
+    wc = 1 / (2 * dt)
+    w = np.linspace(-wc, wc, R)
+    h = np.abs(w) * np.cos(np.pi * w * dt)
+    return np.fft.ifft(np.fft.fftshift(h)).real
+
+
+def fbp(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR, nthreads=None):
+    filt = lowpassino_fbp(sino, sino_tl, sino_br)
+    return backprojection(filt, n, img_tl, img_br, sino_tl, sino_br, nthreads=nthreads)
+
+
+# ---------------------------------------------------------------- EM / OSEM
+
+def subset_bounds(nangles, nsubsets):
+    """Contiguous angle blocks [b*A//S, (b+1)*A//S) (pyraft/iterative/em.py:284-312)."""
+    return [((b * nangles) // nsubsets, ((b + 1) * nangles) // nsubsets) for b in range(nsubsets)]
+
+
+def loglik(s, p):
+    """xfct/likelihood.c:83-94, over rays with p > 0."""
+    s = np.asarray(s, dtype=np.float64)
+    p = np.asarray(p, dtype=np.float64)
+    m = p > 0
+    return float(np.sum(s[m] * np.log(p[m])) - np.sum(p[m]))
+
+
+def osem(sino, n, niter, nsubsets=1, x0=None, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL,
+         sino_br=SINO_BR, nthreads=None, track_ll=True):
+    """Ordered-subsets EM with the raft_em_gpu recurrence (nsubsets=1 is plain EM).
+
+    Returns (x, ll) where ll[k] sums the subset log-likelihood terms seen during
+    iteration k (for nsubsets=1 this is LL(x_k) before the update)."""
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    R, A = sino.shape
+    x = np.ones((n, n)) if x0 is None else np.array(x0, dtype=np.float64)
+    geo = dict(img_tl=img_tl, img_br=img_br, sino_tl=sino_tl, sino_br=sino_br, nthreads=nthreads)
+    blocks = subset_bounds(A, nsubsets)
+    ones = np.ones_like(sino)
+    weights = [backprojection(ones, n, j0=b0, j1=b1, **geo) for (b0, b1) in blocks]
+    ll = []
+    for _ in range(niter):
+        acc = 0.0
+        for (b0, b1), W in zip(blocks, weights):
+            p = radon(x, R, A, j0=b0, j1=b1, **geo)[:, b0:b1]
+            s = sino[:, b0:b1]
+            if track_ll:
+                acc += loglik(s, p)
+            rho = np.zeros((R, A))
+            rho[:, b0:b1] = np.where(p > 0, s / np.where(p > 0, p, 1.0), 0.0)
+            bp = backprojection(rho, n, j0=b0, j1=b1, **geo)
+            x = np.where(W > 0, x * bp / np.where(W > 0, W, 1.0), x)
+        ll.append(acc)
+    return x, ll
+
+
+def em(sino, n, niter, **kw):
+    return osem(sino, n, niter, nsubsets=1, **kw)

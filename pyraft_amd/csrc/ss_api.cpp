@@ -1,0 +1,894 @@
+// ss_api.cpp -- the libraft C ABI: plans, device-pointer entry points and the
+// reference-compatible host-pointer entry points.
+//
+// Every function returns 0 on success or a negative RAFT_SS_E* code; the message
+// of the last failure on the calling thread is available from raft_ss_last_error().
+// The legacy `void` symbols print that message to stderr instead.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/raft_slantstack.h"
+#include "ss_kernels.h"
+#include "ss_plan_logic.h"
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_last_error;
+
+static int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return set_error(RAFT_SS_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                                      \
+  } while (0)
+
+extern "C" const char* raft_ss_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ plan
+
+struct raft_ss_plan {
+  int device = 0;
+  SSGeom g{};
+  double img_corners[4] = {0, 0, 0, 0};   // tl_x, tl_y, br_x, br_y
+  double sino_corners[4] = {0, 0, 0, 0};
+  std::vector<SSAngle> h_angles;
+  SSAngle* d_angles = nullptr;
+  int2* d_bounds = nullptr;
+  int4* d_wbounds = nullptr;
+  int* d_irregular = nullptr;
+  int nirregular = 0;
+  int ndegenerate = 0;
+  int nwaves = 0;
+  int bp_window = 0, bp_angles = 0;
+  float* d_ramp = nullptr;  // FBP convolution kernel, built on first use
+  std::mutex ramp_mutex;
+  SSDevPlan dev() const {
+    SSDevPlan p;
+    p.g = g;
+    p.angles = d_angles;
+    p.bounds = d_bounds;
+    p.wbounds = d_wbounds;
+    p.irregular = d_irregular;
+    p.nirregular = nirregular;
+    p.nwaves = nwaves;
+    p.bp_window = bp_window;
+    p.bp_angles = bp_angles;
+    return p;
+  }
+  ~raft_ss_plan() {
+    if (d_angles) (void)hipFree(d_angles);
+    if (d_bounds) (void)hipFree(d_bounds);
+    if (d_wbounds) (void)hipFree(d_wbounds);
+    if (d_irregular) (void)hipFree(d_irregular);
+    if (d_ramp) (void)hipFree(d_ramp);
+  }
+};
+
+static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Geometry in FP64 exactly as the reference derives it from the corners.
+static SSGeom make_geom(int n, int nrays, int nangles, const double* ic, const double* sc) {
+  SSGeom g;
+  g.n = n;
+  g.nrays = nrays;
+  g.nangles = nangles;
+  g.pitch = n + 2;
+  g.xmin = ic[0];
+  g.ymin = ic[3];
+  g.dx = (ic[2] - ic[0]) / (n - 1);   // raft/raft_radon_slantstack.c:150
+  g.dy = (ic[1] - ic[3]) / (n - 1);   // :151
+  g.tmin = sc[3];
+  g.dt = (sc[1] - sc[3]) / (nrays - 1);  // :170
+  g.th0 = sc[0];
+  g.dth = (sc[2] - sc[0]) / (nangles);   // :156
+  return g;
+}
+
+static int64_t to_fix(double v) { return (int64_t)llrint(v * SS_FIX_ONE); }
+
+// Host-side angle table (glibc cos/sin/tan, as the reference calls them).
+static void build_angles(const SSGeom& g, std::vector<SSAngle>& out, int* ndeg) {
+  const double tol = 1.0 / sqrt(2.0);  // raft/raft_radon_slantstack.c:139
+  out.resize(g.nangles);
+  *ndeg = 0;
+  for (int j = 0; j < g.nangles; ++j) {
+    SSAngle a;
+    memset(&a, 0, sizeof(a));
+    const double theta = g.th0 + j * g.dth;  // :158
+    a.cost = cos(theta);
+    a.sint = sin(theta);
+    a.tant = tan(theta);
+    const double fs = fabs(a.sint), fc = fabs(a.cost);
+    a.branch_a = fc < tol ? 1 : 0;  // :174
+    a.delta = (float)(a.branch_a ? (g.dy / fs) : (g.dx / fc));
+    const SSTraj tr = ss_trajectory(g, a);
+    a.alpha0 = to_fix(tr.alpha0);
+    a.gamma = to_fix(tr.gamma);
+    a.beta = to_fix(tr.beta);
+    a.degenerate = fabs(tr.beta) < SS_DEGENERATE_SLOPE ? 1 : 0;
+    *ndeg += a.degenerate;
+    a.bp_a = (g.xmin * a.cost + g.ymin * a.sint - g.tmin) / g.dt;
+    a.bp_b = (g.dx * a.cost) / g.dt;
+    a.bp_c = (g.dy * a.sint) / g.dt;
+    out[j] = a;
+  }
+}
+
+static int validate_geom(const SSGeom& g, const std::vector<SSAngle>& angles) {
+  if (g.n < 2) return set_error(RAFT_SS_EINVAL, "image side must be >= 2 (got %d)", g.n);
+  if (g.nrays < 2) return set_error(RAFT_SS_EINVAL, "nrays must be >= 2 (got %d)", g.nrays);
+  if (g.nangles < 1) return set_error(RAFT_SS_EINVAL, "nangles must be >= 1 (got %d)", g.nangles);
+  if (g.n > 32768 || g.nrays > (1 << 24) || g.nangles > (1 << 24))
+    return set_error(RAFT_SS_EINVAL, "geometry too large (n=%d R=%d A=%d)", g.n, g.nrays, g.nangles);
+  const double vals[] = {g.xmin, g.ymin, g.dx, g.dy, g.tmin, g.dt, g.th0, g.dth};
+  for (double v : vals)
+    if (!std::isfinite(v)) return set_error(RAFT_SS_EINVAL, "non-finite geometry (check the corners)");
+  if (g.dx == 0.0 || g.dy == 0.0 || g.dt == 0.0)
+    return set_error(RAFT_SS_EINVAL, "zero-extent field of view (dx=%g dy=%g dt=%g)", g.dx, g.dy, g.dt);
+  // the Q32.32 trajectory must stay far inside int64
+  for (const SSAngle& a : angles) {
+    const double amax = fabs((double)a.alpha0) + fabs((double)a.gamma) * g.nrays + fabs((double)a.beta) * g.n;
+    if (!(amax < 4.0e18))
+      return set_error(RAFT_SS_EINVAL, "ray coordinates out of fixed-point range; unsupported field of view");
+  }
+  return 0;
+}
+
+extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nangles, const double* img_corners,
+                                   const double* sino_corners, void* stream_) {
+  if (!out || !img_corners || !sino_corners) return set_error(RAFT_SS_EINVAL, "null argument");
+  *out = nullptr;
+  hipStream_t stream = as_stream(stream_);
+  std::unique_ptr<raft_ss_plan> p(new raft_ss_plan());
+  HIP_TRY(hipGetDevice(&p->device));
+  memcpy(p->img_corners, img_corners, sizeof(p->img_corners));
+  memcpy(p->sino_corners, sino_corners, sizeof(p->sino_corners));
+  p->g = make_geom(n, nrays, nangles, img_corners, sino_corners);
+  if (n < 2 || nrays < 2 || nangles < 1) return validate_geom(p->g, p->h_angles);
+  build_angles(p->g, p->h_angles, &p->ndegenerate);
+  int rc = validate_geom(p->g, p->h_angles);
+  if (rc) return rc;
+
+  const SSGeom& g = p->g;
+  p->nwaves = (nrays + 63) / 64;
+  // backprojection staging window: rays a 32x32 tile can touch at one angle
+  double span = 0.0;
+  for (const SSAngle& a : p->h_angles) span = std::max(span, fabs(a.bp_b) + fabs(a.bp_c));
+  int W = (int)ceil(31.0 * span) + 5;
+  W = (W + 3) & ~3;
+  if (W > 4096) return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)", W);
+  p->bp_window = W;
+  p->bp_angles = std::min(64, 4096 / W);
+
+  const int capacity = (p->ndegenerate + 2) * nrays + 4096;
+  HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSAngle) * nangles));
+  HIP_TRY(hipMalloc(&p->d_bounds, sizeof(int2) * (size_t)nangles * nrays));
+  HIP_TRY(hipMalloc(&p->d_wbounds, sizeof(int4) * (size_t)nangles * p->nwaves));
+  HIP_TRY(hipMalloc(&p->d_irregular, sizeof(int) * (capacity + 1)));
+  int* d_count = p->d_irregular + capacity;
+  HIP_TRY(hipMemcpyAsync(p->d_angles, p->h_angles.data(), sizeof(SSAngle) * nangles, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), stream));
+  HIP_TRY(ss_launch_plan_bounds(g, p->d_angles, p->d_bounds, p->d_irregular, d_count, capacity, stream));
+  HIP_TRY(ss_launch_plan_wave_bounds(g, p->d_bounds, p->d_wbounds, stream));
+  int count = 0;
+  HIP_TRY(hipMemcpyAsync(&count, d_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (count > capacity) return set_error(RAFT_SS_EINTERNAL, "irregular ray list overflow (%d > %d)", count, capacity);
+  p->nirregular = count;
+  *out = p.release();
+  return 0;
+}
+
+extern "C" int raft_ss_plan_destroy(raft_ss_plan* plan) {
+  delete plan;
+  return 0;
+}
+
+extern "C" int raft_ss_plan_info(const raft_ss_plan* p, int* info, int ninfo) {
+  if (!p || !info) return set_error(RAFT_SS_EINVAL, "null argument");
+  const int vals[] = {p->g.n, p->g.nrays, p->g.nangles, p->nirregular, p->ndegenerate, p->bp_window, p->bp_angles};
+  for (int i = 0; i < ninfo && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
+  return 0;
+}
+
+extern "C" int raft_ss_plan_bounds(const raft_ss_plan* p, int* host_bounds) {
+  if (!p || !host_bounds) return set_error(RAFT_SS_EINVAL, "null argument");
+  HIP_TRY(hipMemcpy(host_bounds, p->d_bounds, sizeof(int2) * (size_t)p->g.nangles * p->g.nrays,
+                    hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int raft_ss_plan_wave_bounds(const raft_ss_plan* p, int* host_wbounds) {
+  if (!p || !host_wbounds) return set_error(RAFT_SS_EINVAL, "null argument");
+  HIP_TRY(hipMemcpy(host_wbounds, p->d_wbounds, sizeof(int4) * (size_t)p->g.nangles * p->nwaves,
+                    hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Host evaluation of the same plan logic (no GPU needed): per-ray ranges in the
+// layout of raft_ss_plan_bounds plus the per-angle Q32.32 constants, so tests can
+// verify index ranges of every kernel before anything runs on a device.
+extern "C" int raft_ss_plan_host_bounds(int n, int nrays, int nangles, const double* img_corners,
+                                        const double* sino_corners, int* host_bounds, long long* host_fix,
+                                        int* host_branch) {
+  SSGeom g = make_geom(n, nrays, nangles, img_corners, sino_corners);
+  std::vector<SSAngle> angles;
+  int ndeg = 0;
+  if (n < 2 || nrays < 2 || nangles < 1) return validate_geom(g, angles);
+  build_angles(g, angles, &ndeg);
+  int rc = validate_geom(g, angles);
+  if (rc) return rc;
+  const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t)
+    th.emplace_back([&, t]() {
+      for (int j = t; j < nangles; j += nth) {
+        const SSAngle& a = angles[j];
+        const SSTraj tr = ss_trajectory(g, a);
+        for (int k = 0; k < nrays; ++k) {
+          int lo, hi;
+          if (ss_ray_range(g, a, tr, k, &lo, &hi)) {
+            lo = 0;
+            hi = -2;
+          }
+          host_bounds[2 * ((size_t)j * nrays + k)] = lo;
+          host_bounds[2 * ((size_t)j * nrays + k) + 1] = hi;
+        }
+      }
+    });
+  for (auto& t : th) t.join();
+  for (int j = 0; j < nangles; ++j) {
+    if (host_fix) {
+      host_fix[3 * j] = angles[j].alpha0;
+      host_fix[3 * j + 1] = angles[j].gamma;
+      host_fix[3 * j + 2] = angles[j].beta;
+    }
+    if (host_branch) host_branch[j] = angles[j].branch_a;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ workspace
+
+namespace {
+struct DevBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+  hipError_t alloc(size_t bytes, hipStream_t stream) {
+    s = stream;
+    return hipMallocAsync(&p, bytes ? bytes : 4, stream);
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+int check_plan(const raft_ss_plan* p, int nslices) {
+  if (!p) return set_error(RAFT_SS_EINVAL, "null plan");
+  if (nslices < 1) return set_error(RAFT_SS_EINVAL, "nslices must be >= 1 (got %d)", nslices);
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
+    return set_error(RAFT_SS_EINVAL, "plan was built on device %d but device %d is current", p->device, dev);
+  return 0;
+}
+
+int check_range(const raft_ss_plan* p, int j0, int j1) {
+  if (j0 < 0 || j1 > p->g.nangles || j0 > j1)
+    return set_error(RAFT_SS_EINVAL, "angle range [%d, %d) outside [0, %d)", j0, j1, p->g.nangles);
+  return 0;
+}
+
+// g = Re(ifft(fftshift(h))) with h = |w| cos(pi w dt), w = linspace(-wc, wc, R), wc = 1/(2 dt)
+// (pyraft/filters/filters.py:44-55), evaluated in FP64 by a direct real DFT.
+void compute_ramp(int R, double tmin, double tmax, float* gk) {
+  // numpy.linspace(tmin, tmax, R) stores the endpoint exactly, so t[R-1] - t[0] = tmax - tmin
+  const double dt = (tmax - tmin) / (R - 1);
+  const double wc = 1 / (2 * dt);
+  const double step = (wc - (-wc)) / (R - 1);
+  std::vector<double> h(R), G(R);
+  const double pi = 3.141592653589793;
+  for (int i = 0; i < R; ++i) {
+    const double w = (i == R - 1) ? wc : (i * step + (-wc));
+    h[i] = fabs(w) * cos(pi * w * dt);
+  }
+  const int shift = R / 2;  // numpy.fft.fftshift = roll by R//2
+  for (int f = 0; f < R; ++f) G[f] = h[((f - shift) % R + R) % R];
+  for (int m = 0; m < R; ++m) {
+    double acc = 0.0;
+    for (int f = 0; f < R; ++f) {
+      const long long ph = ((long long)f * m) % R;
+      acc += G[f] * cos(2.0 * pi * (double)ph / R);
+    }
+    gk[m] = (float)(acc / R);
+  }
+}
+
+int ensure_ramp(raft_ss_plan* p, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(p->ramp_mutex);
+  if (p->d_ramp) return 0;
+  const int R = p->g.nrays;
+  std::vector<float> gk(R);
+  compute_ramp(R, p->sino_corners[3], p->sino_corners[1], gk.data());
+  float* d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(float) * R));
+  HIP_TRY(hipMemcpyAsync(d, gk.data(), sizeof(float) * R, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  p->d_ramp = d;
+  return 0;
+}
+}  // namespace
+
+// ------------------------------------------------------------------ device-pointer entry points
+
+extern "C" int raft_ss_ramp_kernel(int R, double tmin, double tmax, float* host_g) {
+  if (R < 2 || !host_g) return set_error(RAFT_SS_EINVAL, "ramp kernel needs R >= 2 and an output buffer");
+  if (!std::isfinite(tmin) || !std::isfinite(tmax) || tmin == tmax)
+    return set_error(RAFT_SS_EINVAL, "degenerate detector extent");
+  compute_ramp(R, tmin, tmax, host_g);
+  return 0;
+}
+
+extern "C" int raft_ss_filter_rows(const float* d_in, float* d_out, const float* d_g, int R, int nrows,
+                                   void* stream) {
+  if (!d_in || !d_out || !d_g || R < 2 || nrows < 0) return set_error(RAFT_SS_EINVAL, "invalid filter arguments");
+  if (d_in == d_out) return set_error(RAFT_SS_EINVAL, "filter cannot run in place");
+  if (nrows == 0) return 0;
+  HIP_TRY(ss_launch_ramp_filter(d_in, d_out, d_g, R, nrows, as_stream(stream)));
+  return 0;
+}
+
+extern "C" int raft_ss_radon_range(const raft_ss_plan* p, const float* d_image, float* d_out, int angle_major, int j0,
+                                   int j1, int nslices, void* stream_) {
+  int rc = check_plan(p, nslices);
+  if (rc) return rc;
+  if ((rc = check_range(p, j0, j1))) return rc;
+  if (!d_image || !d_out) return set_error(RAFT_SS_EINVAL, "null device pointer");
+  hipStream_t stream = as_stream(stream_);
+  const SSDevPlan dp = p->dev();
+  const size_t layout = (size_t)nslices * p->g.n * p->g.pitch;
+  DevBuf ws;
+  HIP_TRY(ws.alloc(2 * layout * sizeof(float), stream));
+  float* gA = ws.as<float>();
+  float* gB = gA + layout;
+  HIP_TRY(ss_launch_gather_layouts(dp, d_image, gA, gB, nslices, stream));
+  SSEmRatioArgs none{nullptr, nullptr};
+  HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, angle_major ? SS_OUT_SINOT : SS_OUT_SINO, d_out, none, stream));
+  return 0;
+}
+
+extern "C" int raft_ss_radon(const raft_ss_plan* p, const float* d_image, float* d_sino, int nslices, void* stream) {
+  if (!p) return set_error(RAFT_SS_EINVAL, "null plan");
+  return raft_ss_radon_range(p, d_image, d_sino, 0, 0, p->g.nangles, nslices, stream);
+}
+
+extern "C" int raft_ss_backprojection_range(const raft_ss_plan* p, const float* d_sino, float* d_image,
+                                            int angle_major, int j0, int j1, int nslices, void* stream_) {
+  int rc = check_plan(p, nslices);
+  if (rc) return rc;
+  if ((rc = check_range(p, j0, j1))) return rc;
+  if (!d_image || !d_sino) return set_error(RAFT_SS_EINVAL, "null device pointer");
+  hipStream_t stream = as_stream(stream_);
+  const SSDevPlan dp = p->dev();
+  const float* sT = d_sino;
+  DevBuf ws;
+  if (!angle_major) {
+    HIP_TRY(ws.alloc((size_t)nslices * p->g.nrays * p->g.nangles * sizeof(float), stream));
+    HIP_TRY(ss_launch_sino_to_t(d_sino, ws.as<float>(), p->g.nrays, p->g.nangles, nslices, stream));
+    sT = ws.as<float>();
+  }
+  SSEmUpdateArgs none{nullptr, nullptr};
+  HIP_TRY(ss_launch_backprojection(dp, sT, j0, j1, nslices, SS_BP_PLAIN, d_image, none, stream));
+  return 0;
+}
+
+extern "C" int raft_ss_backprojection(const raft_ss_plan* p, const float* d_sino, float* d_image, int nslices,
+                                      void* stream) {
+  if (!p) return set_error(RAFT_SS_EINVAL, "null plan");
+  return raft_ss_backprojection_range(p, d_sino, d_image, 0, 0, p->g.nangles, nslices, stream);
+}
+
+extern "C" int raft_ss_radon_transpose_range(const raft_ss_plan* p, const float* d_sino, float* d_image,
+                                             int angle_major, int j0, int j1, int nslices, void* stream_) {
+  int rc = check_plan(p, nslices);
+  if (rc) return rc;
+  if ((rc = check_range(p, j0, j1))) return rc;
+  if (!d_image || !d_sino) return set_error(RAFT_SS_EINVAL, "null device pointer");
+  hipStream_t stream = as_stream(stream_);
+  const SSDevPlan dp = p->dev();
+  const float* sT = d_sino;
+  DevBuf ws;
+  if (!angle_major) {
+    HIP_TRY(ws.alloc((size_t)nslices * p->g.nrays * p->g.nangles * sizeof(float), stream));
+    HIP_TRY(ss_launch_sino_to_t(d_sino, ws.as<float>(), p->g.nrays, p->g.nangles, nslices, stream));
+    sT = ws.as<float>();
+  }
+  HIP_TRY(ss_launch_radon_transpose(dp, sT, j0, j1, nslices, d_image, stream));
+  return 0;
+}
+
+extern "C" int raft_ss_radon_transpose(const raft_ss_plan* p, const float* d_sino, float* d_image, int nslices,
+                                       void* stream) {
+  if (!p) return set_error(RAFT_SS_EINVAL, "null plan");
+  return raft_ss_radon_transpose_range(p, d_sino, d_image, 0, 0, p->g.nangles, nslices, stream);
+}
+
+extern "C" int raft_ss_fbp_filter(raft_ss_plan* p, const float* d_sino, float* d_out, int angle_major, int nslices,
+                                  void* stream_) {
+  int rc = check_plan(p, nslices);
+  if (rc) return rc;
+  if (!d_sino || !d_out) return set_error(RAFT_SS_EINVAL, "null device pointer");
+  hipStream_t stream = as_stream(stream_);
+  if ((rc = ensure_ramp(p, stream))) return rc;
+  const int R = p->g.nrays, A = p->g.nangles;
+  const size_t total = (size_t)nslices * R * A;
+  if (angle_major) {
+    HIP_TRY(ss_launch_ramp_filter(d_sino, d_out, p->d_ramp, R, nslices * A, stream));
+    return 0;
+  }
+  DevBuf ws;
+  HIP_TRY(ws.alloc(2 * total * sizeof(float), stream));
+  float* t0 = ws.as<float>();
+  float* t1 = t0 + total;
+  HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
+  HIP_TRY(ss_launch_ramp_filter(t0, t1, p->d_ramp, R, nslices * A, stream));
+  HIP_TRY(ss_launch_t_to_sino(t1, d_out, R, A, nslices, stream));
+  return 0;
+}
+
+extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image, int nslices, void* stream_) {
+  int rc = check_plan(p, nslices);
+  if (rc) return rc;
+  if (!d_sino || !d_image) return set_error(RAFT_SS_EINVAL, "null device pointer");
+  hipStream_t stream = as_stream(stream_);
+  if ((rc = ensure_ramp(p, stream))) return rc;
+  const int R = p->g.nrays, A = p->g.nangles;
+  const size_t total = (size_t)nslices * R * A;
+  DevBuf ws;
+  HIP_TRY(ws.alloc(2 * total * sizeof(float), stream));
+  float* t0 = ws.as<float>();
+  float* t1 = t0 + total;
+  HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
+  HIP_TRY(ss_launch_ramp_filter(t0, t1, p->d_ramp, R, nslices * A, stream));
+  SSEmUpdateArgs none{nullptr, nullptr};
+  HIP_TRY(ss_launch_backprojection(p->dev(), t1, 0, A, nslices, SS_BP_PLAIN, d_image, none, stream));
+  return 0;
+}
+
+// Ordered-subsets EM.  d_x holds the starting image on entry (the reference starts
+// from ones, raft/raft_em_gpu.cu:41-46) and the result on exit.  Subset b covers
+// angle columns [b*A/S, (b+1)*A/S); S = 1 is the plain EM of raft_em_gpu.
+// ll_host (optional, niter doubles) receives the per-iteration log-likelihood.
+extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int niter, int nsubsets, int nslices,
+                          double* ll_host, void* stream_) {
+  int rc = check_plan(p, nslices);
+  if (rc) return rc;
+  if (!d_sino || !d_x) return set_error(RAFT_SS_EINVAL, "null device pointer");
+  if (niter < 0) return set_error(RAFT_SS_EINVAL, "niter must be >= 0");
+  const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
+  if (nsubsets < 1 || nsubsets > A) return set_error(RAFT_SS_EINVAL, "nsubsets must be in [1, %d]", A);
+  hipStream_t stream = as_stream(stream_);
+  const SSDevPlan dp = p->dev();
+  const size_t sino_sz = (size_t)nslices * R * A;
+  const size_t img_sz = (size_t)nslices * n * n;
+  const size_t layout = (size_t)nslices * n * p->g.pitch;
+  // partial buffer large enough for the biggest subset
+  int max_partials = 0;
+  for (int b = 0; b < nsubsets; ++b) {
+    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
+    max_partials = std::max(max_partials, ss_radon_partials(dp, j0, j1, nslices));
+  }
+  const size_t bytes = sizeof(float) * (2 * sino_sz + (size_t)nsubsets * img_sz + 2 * layout) +
+                       sizeof(double) * ((size_t)max_partials + (size_t)std::max(niter, 1) + 8);
+  DevBuf ws;
+  HIP_TRY(ws.alloc(bytes, stream));
+  float* measT = ws.as<float>();
+  float* rhoT = measT + sino_sz;
+  float* weights = rhoT + sino_sz;
+  float* gA = weights + (size_t)nsubsets * img_sz;
+  float* gB = gA + layout;
+  double* partials = reinterpret_cast<double*>(gB + layout);
+  partials = reinterpret_cast<double*>(((uintptr_t)partials + 7) & ~(uintptr_t)7);
+  double* d_ll = partials + max_partials;
+
+  HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
+  // W_b = BP_b(1) with the same backprojection kernel
+  HIP_TRY(ss_launch_fill(rhoT, 1.0f, sino_sz, stream));
+  SSEmUpdateArgs none{nullptr, nullptr};
+  for (int b = 0; b < nsubsets; ++b) {
+    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
+    HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
+                                     stream));
+  }
+  if (niter > 0) HIP_TRY(hipMemsetAsync(d_ll, 0, sizeof(double) * niter, stream));
+  for (int it = 0; it < niter; ++it) {
+    for (int b = 0; b < nsubsets; ++b) {
+      const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
+      HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
+      SSEmRatioArgs ea{measT, partials};
+      HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, stream));
+      HIP_TRY(ss_launch_sum_partials(partials, ss_radon_partials(dp, j0, j1, nslices), d_ll + it, stream));
+      SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
+      HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, stream));
+    }
+  }
+  if (ll_host && niter > 0) {
+    HIP_TRY(hipMemcpyAsync(ll_host, d_ll, sizeof(double) * niter, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ phantoms
+
+static int upload_ellipses(const raft_matrix& desc, std::vector<SSEllipse>& ell) {
+  if (desc.columns < 6) return set_error(RAFT_SS_EINVAL, "phantom description needs 6 columns (got %d)", desc.columns);
+  ell.resize(desc.lines);
+  for (int e = 0; e < desc.lines; ++e) {
+    SSEllipse E;
+    E.val = RAFT_MATRIX_AT(desc, e, 0);
+    E.a = RAFT_MATRIX_AT(desc, e, 1);
+    E.b = RAFT_MATRIX_AT(desc, e, 2);
+    E.x0 = RAFT_MATRIX_AT(desc, e, 3);
+    E.y0 = RAFT_MATRIX_AT(desc, e, 4);
+    const double rot = RAFT_MATRIX_AT(desc, e, 5);
+    E.c = cos(rot);
+    E.s = sin(rot);
+    ell[e] = E;
+  }
+  return 0;
+}
+
+// Rasterise an ellipse description into a device FP32 image (row 0 = tl_y, the
+// rasteriser's own orientation, raft/raft_image_phantom_fromdesc.c:18-41).
+extern "C" int raft_ss_phantom(const double* desc_rows, int nell, int lines, int cols, const double* corners,
+                               float* d_out, void* stream_) {
+  if (!desc_rows || !corners || !d_out || nell < 0 || lines < 2 || cols < 2)
+    return set_error(RAFT_SS_EINVAL, "invalid phantom arguments");
+  hipStream_t stream = as_stream(stream_);
+  raft_matrix desc{const_cast<double*>(desc_rows), nell, 1, 6, 6};
+  std::vector<SSEllipse> ell;
+  int rc = upload_ellipses(desc, ell);
+  if (rc) return rc;
+  DevBuf d_ell;
+  HIP_TRY(d_ell.alloc(sizeof(SSEllipse) * std::max(1, nell), stream));
+  if (nell) HIP_TRY(hipMemcpyAsync(d_ell.p, ell.data(), sizeof(SSEllipse) * nell, hipMemcpyHostToDevice, stream));
+  const double hs = (corners[2] - corners[0]) / (cols - 1), vs = (corners[3] - corners[1]) / (lines - 1);
+  HIP_TRY(ss_launch_phantom(d_ell.as<SSEllipse>(), nell, lines, cols, corners[0], corners[1], hs, vs, nullptr, d_out,
+                            stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return 0;
+}
+
+// ------------------------------------------------------------------ host <-> raft_image staging
+
+namespace {
+struct PinnedF32 {
+  float* p = nullptr;
+  ~PinnedF32() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipHostMalloc(reinterpret_cast<void**>(&p), n * sizeof(float) + 4, 0); }
+};
+
+void gather_f64(const raft_matrix& m, float* dst) {
+  for (int i = 0; i < m.lines; ++i)
+    for (int j = 0; j < m.columns; ++j) dst[(size_t)i * m.columns + j] = (float)RAFT_MATRIX_AT(m, i, j);
+}
+
+void scatter_f64(const float* src, const raft_matrix& m) {
+  for (int i = 0; i < m.lines; ++i)
+    for (int j = 0; j < m.columns; ++j) RAFT_MATRIX_AT(m, i, j) = (double)src[(size_t)i * m.columns + j];
+}
+
+// plan cache for the host-pointer entry points
+struct PlanKey {
+  int device, n, R, A;
+  double c[8];
+  bool operator==(const PlanKey& o) const {
+    return device == o.device && n == o.n && R == o.R && A == o.A && memcmp(c, o.c, sizeof(c)) == 0;
+  }
+};
+std::mutex g_cache_mutex;
+std::list<std::pair<PlanKey, std::shared_ptr<raft_ss_plan>>> g_cache;
+
+int cached_plan(int n, int R, int A, const double* ic, const double* sc, std::shared_ptr<raft_ss_plan>* out) {
+  PlanKey key;
+  memset(&key, 0, sizeof(key));
+  HIP_TRY(hipGetDevice(&key.device));
+  key.n = n;
+  key.R = R;
+  key.A = A;
+  memcpy(key.c, ic, 4 * sizeof(double));
+  memcpy(key.c + 4, sc, 4 * sizeof(double));
+  {
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+      if (it->first == key) {
+        g_cache.splice(g_cache.begin(), g_cache, it);
+        *out = g_cache.front().second;
+        return 0;
+      }
+  }
+  raft_ss_plan* raw = nullptr;
+  int rc = raft_ss_plan_create(&raw, n, R, A, ic, sc, nullptr);
+  if (rc) return rc;
+  std::shared_ptr<raft_ss_plan> sp(raw);
+  std::lock_guard<std::mutex> lock(g_cache_mutex);
+  g_cache.emplace_front(key, sp);
+  while (g_cache.size() > 8) g_cache.pop_back();
+  *out = sp;
+  return 0;
+}
+
+void corners_of(const raft_image& im, double* c) {
+  c[0] = im.tl_x;
+  c[1] = im.tl_y;
+  c[2] = im.br_x;
+  c[3] = im.br_y;
+}
+
+int host_radon(const raft_image& phantom, const raft_image& radon) {
+  if (!phantom.data.p_data || !radon.data.p_data) return set_error(RAFT_SS_EINVAL, "empty image");
+  const int n = phantom.data.lines;
+  if (phantom.data.columns != n)
+    return set_error(RAFT_SS_EINVAL, "slant-stack needs a square image (got %dx%d)", n, phantom.data.columns);
+  const int R = radon.data.lines, A = radon.data.columns;
+  double ic[4], sc[4];
+  corners_of(phantom, ic);
+  corners_of(radon, sc);
+  std::shared_ptr<raft_ss_plan> plan;
+  int rc = cached_plan(n, R, A, ic, sc, &plan);
+  if (rc) return rc;
+  PinnedF32 hi, ho;
+  HIP_TRY(hi.alloc((size_t)n * n));
+  HIP_TRY(ho.alloc((size_t)R * A));
+  gather_f64(phantom.data, hi.p);
+  DevBuf di, dso;
+  HIP_TRY(di.alloc(sizeof(float) * n * n, nullptr));
+  HIP_TRY(dso.alloc(sizeof(float) * (size_t)R * A, nullptr));
+  HIP_TRY(hipMemcpyAsync(di.p, hi.p, sizeof(float) * n * n, hipMemcpyHostToDevice, nullptr));
+  rc = raft_ss_radon(plan.get(), di.as<float>(), dso.as<float>(), 1, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ho.p, dso.p, sizeof(float) * (size_t)R * A, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  scatter_f64(ho.p, radon.data);
+  return 0;
+}
+
+int host_backprojection(const raft_image& sino, const raft_image& img) {
+  if (!sino.data.p_data || !img.data.p_data) return set_error(RAFT_SS_EINVAL, "empty image");
+  const int n = img.data.lines;
+  if (img.data.columns != n)
+    return set_error(RAFT_SS_EINVAL, "slant-stack needs a square image (got %dx%d)", n, img.data.columns);
+  const int R = sino.data.lines, A = sino.data.columns;
+  double ic[4], sc[4];
+  corners_of(img, ic);
+  corners_of(sino, sc);
+  std::shared_ptr<raft_ss_plan> plan;
+  int rc = cached_plan(n, R, A, ic, sc, &plan);
+  if (rc) return rc;
+  PinnedF32 hs, ho;
+  HIP_TRY(hs.alloc((size_t)R * A));
+  HIP_TRY(ho.alloc((size_t)n * n));
+  gather_f64(sino.data, hs.p);
+  DevBuf ds, di;
+  HIP_TRY(ds.alloc(sizeof(float) * (size_t)R * A, nullptr));
+  HIP_TRY(di.alloc(sizeof(float) * n * n, nullptr));
+  HIP_TRY(hipMemcpyAsync(ds.p, hs.p, sizeof(float) * (size_t)R * A, hipMemcpyHostToDevice, nullptr));
+  rc = raft_ss_backprojection(plan.get(), ds.as<float>(), di.as<float>(), 1, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ho.p, di.p, sizeof(float) * n * n, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  scatter_f64(ho.p, img.data);
+  return 0;
+}
+
+int host_radon_transpose(const raft_image& sino, const raft_image& img) {
+  if (!sino.data.p_data || !img.data.p_data) return set_error(RAFT_SS_EINVAL, "empty image");
+  const int n = img.data.lines;
+  if (img.data.columns != n)
+    return set_error(RAFT_SS_EINVAL, "slant-stack needs a square image (got %dx%d)", n, img.data.columns);
+  const int R = sino.data.lines, A = sino.data.columns;
+  double ic[4], sc[4];
+  corners_of(img, ic);
+  corners_of(sino, sc);
+  std::shared_ptr<raft_ss_plan> plan;
+  int rc = cached_plan(n, R, A, ic, sc, &plan);
+  if (rc) return rc;
+  PinnedF32 hs, ho;
+  HIP_TRY(hs.alloc((size_t)R * A));
+  HIP_TRY(ho.alloc((size_t)n * n));
+  gather_f64(sino.data, hs.p);
+  DevBuf ds, di;
+  HIP_TRY(ds.alloc(sizeof(float) * (size_t)R * A, nullptr));
+  HIP_TRY(di.alloc(sizeof(float) * n * n, nullptr));
+  HIP_TRY(hipMemcpyAsync(ds.p, hs.p, sizeof(float) * (size_t)R * A, hipMemcpyHostToDevice, nullptr));
+  rc = raft_ss_radon_transpose(plan.get(), ds.as<float>(), di.as<float>(), 1, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ho.p, di.p, sizeof(float) * n * n, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  scatter_f64(ho.p, img.data);
+  return 0;
+}
+
+const double kDefaultImg[4] = {-1.0, 1.0, 1.0, -1.0};
+const double kDefaultSino[4] = {0.0, 1.0, 3.14159265358979323846, -1.0};
+
+int host_f32_op(int which, float* h_out, const float* h_in, int n, int R, int A, int niter) {
+  if (!h_out || !h_in) return set_error(RAFT_SS_EINVAL, "null host pointer");
+  std::shared_ptr<raft_ss_plan> plan;
+  int rc = cached_plan(n, R, A, kDefaultImg, kDefaultSino, &plan);
+  if (rc) return rc;
+  const size_t nimg = (size_t)n * n, nsino = (size_t)R * A;
+  DevBuf da, db;
+  HIP_TRY(da.alloc(sizeof(float) * nimg, nullptr));
+  HIP_TRY(db.alloc(sizeof(float) * nsino, nullptr));
+  if (which == 0) {  // radon: in = image, out = sino
+    HIP_TRY(hipMemcpyAsync(da.p, h_in, sizeof(float) * nimg, hipMemcpyHostToDevice, nullptr));
+    if ((rc = raft_ss_radon(plan.get(), da.as<float>(), db.as<float>(), 1, nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, db.p, sizeof(float) * nsino, hipMemcpyDeviceToHost, nullptr));
+  } else if (which == 1) {  // backprojection: in = sino, out = image
+    HIP_TRY(hipMemcpyAsync(db.p, h_in, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
+    if ((rc = raft_ss_backprojection(plan.get(), db.as<float>(), da.as<float>(), 1, nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, da.p, sizeof(float) * nimg, hipMemcpyDeviceToHost, nullptr));
+  } else {  // em from ones
+    HIP_TRY(hipMemcpyAsync(db.p, h_in, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
+    HIP_TRY(ss_launch_fill(da.as<float>(), 1.0f, nimg, nullptr));
+    if ((rc = raft_ss_em(plan.get(), db.as<float>(), da.as<float>(), niter, 1, 1, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, da.p, sizeof(float) * nimg, hipMemcpyDeviceToHost, nullptr));
+  }
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  return 0;
+}
+
+thread_local int g_legacy_status = 0;
+
+void report(const char* fn, int rc) {
+  g_legacy_status = rc;
+  if (rc) fprintf(stderr, "libraft %s: %s\n", fn, g_last_error.c_str());
+}
+}  // namespace
+
+extern "C" int raft_ss_legacy_status(void) { return g_legacy_status; }
+
+// ------------------------------------------------------------------ legacy symbols (reference ABI)
+
+extern "C" void raft_radon_slantstack(raft_image phantom, raft_image radon, int nthreads) {
+  (void)nthreads;
+  report("raft_radon_slantstack", host_radon(phantom, radon));
+}
+
+extern "C" void raft_backprojection_slantstack(raft_image sinogram, raft_image backprojection, int nthreads) {
+  (void)nthreads;
+  report("raft_backprojection_slantstack", host_backprojection(sinogram, backprojection));
+}
+
+extern "C" void raft_radon_transpose_slantstack(raft_image sinogram, raft_image image, int nthreads) {
+  (void)nthreads;
+  report("raft_radon_transpose_slantstack", host_radon_transpose(sinogram, image));
+}
+
+extern "C" void raft_radon_slantstack_gpu(float* h_output, float* h_input, int sizeImage, int nrays, int nangles) {
+  report("raft_radon_slantstack_gpu", host_f32_op(0, h_output, h_input, sizeImage, nrays, nangles, 0));
+}
+
+extern "C" void raft_backprojection_slantstack_gpu(float* image, float* sino, int sizeImage, int nrays,
+                                                   int nangles) {
+  report("raft_backprojection_slantstack_gpu", host_f32_op(1, image, sino, sizeImage, nrays, nangles, 0));
+}
+
+extern "C" void raft_em_gpu(float* output, float* sino, int sizeImage, int nrays, int nangles, int niter) {
+  report("raft_em_gpu", host_f32_op(2, output, sino, sizeImage, nrays, nangles, niter));
+}
+
+static int host_phantom(const raft_image& image, const raft_matrix& desc) {
+  if (!image.data.p_data) return 0;  // nothing to do (raft/raft_image_phantom_fromdesc.c:7-8)
+  std::vector<SSEllipse> ell;
+  int rc = upload_ellipses(desc, ell);
+  if (rc) return rc;
+  const int lines = image.data.lines, cols = image.data.columns;
+  if (lines < 2 || cols < 2) return set_error(RAFT_SS_EINVAL, "phantom image must be at least 2x2");
+  const size_t total = (size_t)lines * cols;
+  std::vector<double> h(total);
+  for (int i = 0; i < lines; ++i)
+    for (int j = 0; j < cols; ++j) h[(size_t)i * cols + j] = RAFT_MATRIX_AT(image.data, i, j);
+  DevBuf d_ell, d_img;
+  HIP_TRY(d_ell.alloc(sizeof(SSEllipse) * std::max<size_t>(1, ell.size()), nullptr));
+  HIP_TRY(d_img.alloc(sizeof(double) * total, nullptr));
+  if (!ell.empty())
+    HIP_TRY(hipMemcpyAsync(d_ell.p, ell.data(), sizeof(SSEllipse) * ell.size(), hipMemcpyHostToDevice, nullptr));
+  HIP_TRY(hipMemcpyAsync(d_img.p, h.data(), sizeof(double) * total, hipMemcpyHostToDevice, nullptr));
+  const double hs = (image.br_x - image.tl_x) / (cols - 1), vs = (image.br_y - image.tl_y) / (lines - 1);
+  HIP_TRY(ss_launch_phantom(d_ell.as<SSEllipse>(), (int)ell.size(), lines, cols, image.tl_x, image.tl_y, hs, vs,
+                            d_img.as<double>(), nullptr, nullptr));
+  HIP_TRY(hipMemcpyAsync(h.data(), d_img.p, sizeof(double) * total, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  for (int i = 0; i < lines; ++i)
+    for (int j = 0; j < cols; ++j) RAFT_MATRIX_AT(image.data, i, j) = h[(size_t)i * cols + j];
+  return 0;
+}
+
+extern "C" void raft_image_phantom_fromdesc(raft_image image, raft_matrix desc) {
+  report("raft_image_phantom_fromdesc", host_phantom(image, desc));
+}
+
+static int host_radon_fromdesc(const raft_image& sino, const raft_matrix& desc) {
+  if (!sino.data.p_data) return 0;
+  if (desc.columns < 6) return set_error(RAFT_SS_EINVAL, "phantom description needs 6 columns");
+  const int lines = sino.data.lines, cols = sino.data.columns, nell = desc.lines;
+  if (lines < 2 || cols < 2) return set_error(RAFT_SS_EINVAL, "sinogram must be at least 2x2");
+  // per (ellipse, column) constants, FP64 with glibc trig (raft/raft_radon_fromdesc.c:9-50)
+  const double ths = (sino.br_x - sino.tl_x) / (cols - 1);
+  const double tss = (sino.br_y - sino.tl_y) / (lines - 1);
+  std::vector<SSEllipseView> views((size_t)std::max(1, nell) * cols);
+  for (int e = 0; e < nell; ++e) {
+    const double val = RAFT_MATRIX_AT(desc, e, 0), a = RAFT_MATRIX_AT(desc, e, 1), b = RAFT_MATRIX_AT(desc, e, 2);
+    const double rot = RAFT_MATRIX_AT(desc, e, 5), ex = RAFT_MATRIX_AT(desc, e, 3), ey = RAFT_MATRIX_AT(desc, e, 4);
+    for (int j = 0; j < cols; ++j) {
+      double th = sino.tl_x + j * ths;
+      const double c = cos(th), s = sin(th);
+      th -= rot;
+      const double cp = cos(th), sp = sin(th);
+      double m = 1.0 / sqrt(((cp * cp) / (b * b)) + ((sp * sp) / (a * a)));
+      double mab = m / (a * b);
+      m *= 2.0;
+      m *= val;
+      const double delt = mab * ((ex * c) + (ey * s) - sino.tl_y);
+      mab *= tss;
+      views[(size_t)e * cols + j] = SSEllipseView{m, mab, delt};
+    }
+  }
+  const size_t total = (size_t)lines * cols;
+  std::vector<double> h(total);
+  for (int i = 0; i < lines; ++i)
+    for (int j = 0; j < cols; ++j) h[(size_t)i * cols + j] = RAFT_MATRIX_AT(sino.data, i, j);
+  DevBuf d_views, d_out;
+  HIP_TRY(d_views.alloc(sizeof(SSEllipseView) * views.size(), nullptr));
+  HIP_TRY(d_out.alloc(sizeof(double) * total, nullptr));
+  HIP_TRY(hipMemcpyAsync(d_views.p, views.data(), sizeof(SSEllipseView) * views.size(), hipMemcpyHostToDevice,
+                         nullptr));
+  HIP_TRY(hipMemcpyAsync(d_out.p, h.data(), sizeof(double) * total, hipMemcpyHostToDevice, nullptr));
+  HIP_TRY(ss_launch_radon_fromdesc(d_views.as<SSEllipseView>(), nell, lines, cols, d_out.as<double>(), nullptr));
+  HIP_TRY(hipMemcpyAsync(h.data(), d_out.p, sizeof(double) * total, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  for (int i = 0; i < lines; ++i)
+    for (int j = 0; j < cols; ++j) RAFT_MATRIX_AT(sino.data, i, j) = h[(size_t)i * cols + j];
+  return 0;
+}
+
+extern "C" void raft_radon_fromdesc(raft_image sino, raft_matrix desc) {
+  report("raft_radon_fromdesc", host_radon_fromdesc(sino, desc));
+}
+
+extern "C" int raft_ss_version(void) { return 1; }

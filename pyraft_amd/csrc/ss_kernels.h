@@ -1,0 +1,85 @@
+// ss_kernels.h -- launch wrappers for the slant-stack HIP kernels (ss_kernels.hip).
+// All device pointers; every wrapper enqueues on `stream` and returns hipSuccess or
+// the launch error.  Layout names:
+//   image      [slice][n][n]        row-major, row 0 = ymin (bottom)
+//   sino       [slice][R][A]        the reference layout (angle contiguous)
+//   sinoT      [slice][A][R]        angle-major (detector contiguous), internal
+//   gatherA/B  [slice][n][n+2]      zero-padded Radon gather layouts (A: transposed)
+#ifndef PYRAFT_AMD_SS_KERNELS_H
+#define PYRAFT_AMD_SS_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ss_geometry.h"
+
+// Radon output modes
+enum { SS_OUT_SINO = 0, SS_OUT_SINOT = 1, SS_OUT_EM_RATIO = 2 };
+// Backprojection output modes
+enum { SS_BP_PLAIN = 0, SS_BP_EM_UPDATE = 1 };
+
+struct SSDevPlan {
+  SSGeom g;
+  const SSAngle* angles;   // [A]
+  const int2* bounds;      // [A][R]   exact sample range per ray; (0,-1) empty, (0,-2) irregular
+  const int4* wbounds;     // [A][ceil(R/64)] (union lo, union hi, inter lo, inter hi)
+  const int* irregular;    // list of j*R + k
+  int nirregular;
+  int nwaves;              // ceil(R/64)
+  int bp_window;           // rays staged per angle per 32x32 pixel tile
+  int bp_angles;           // angles per LDS stage
+};
+
+// ---- plan construction
+hipError_t ss_launch_plan_bounds(const SSGeom& g, const SSAngle* angles, int2* bounds, int* irregular,
+                                 int* nirregular, int capacity, hipStream_t stream);
+hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4* wbounds, hipStream_t stream);
+
+// ---- layouts
+hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float* gatherA, float* gatherB,
+                                    int nslices, hipStream_t stream);
+hipError_t ss_launch_sino_to_t(const float* sino, float* sinoT, int R, int A, int nslices, hipStream_t stream);
+hipError_t ss_launch_t_to_sino(const float* sinoT, float* sino, int R, int A, int nslices, hipStream_t stream);
+
+// ---- forward
+struct SSEmRatioArgs {
+  const float* meas;   // measured sinogram, angle-major [slice][A][R]
+  double* ll_partials; // one FP64 partial per block (+ one per irregular ray)
+};
+hipError_t ss_launch_radon(const SSDevPlan& p, const float* gatherA, const float* gatherB, int j0, int j1,
+                           int nslices, int mode, float* out, SSEmRatioArgs em, hipStream_t stream);
+int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices);
+
+// ---- backprojection (reads angle-major sinogram)
+struct SSEmUpdateArgs {
+  float* x;            // image updated in place
+  const float* weight; // BP_b(1)
+};
+hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices, int mode,
+                                    float* out, SSEmUpdateArgs em, hipStream_t stream);
+
+// ---- exact transpose of the Radon (reads angle-major sinogram)
+hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
+                                     float* out, hipStream_t stream);
+
+// ---- FBP row filter (rows of length R, circular convolution with g)
+hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream);
+
+// ---- FP64 reductions / EM helpers
+hipError_t ss_launch_sum_partials(const double* partials, int n, double* dst, hipStream_t stream);
+hipError_t ss_launch_fill(float* p, float v, size_t n, hipStream_t stream);
+
+// ---- phantoms (FP64)
+struct SSEllipse {  // value, a, b, x0, y0, cos(rot), sin(rot)
+  double val, a, b, x0, y0, c, s;
+};
+hipError_t ss_launch_phantom(const SSEllipse* ell, int nell, int lines, int cols, double tl_x, double tl_y,
+                             double hs, double vs, double* out64, float* out32, hipStream_t stream);
+// per (ellipse, column): c, s, m, mab, delt  (host-built, see raft/raft_radon_fromdesc.c)
+struct SSEllipseView {
+  double m, mab, delt;
+};
+hipError_t ss_launch_radon_fromdesc(const SSEllipseView* views, int nell, int lines, int cols, double* out64,
+                                    hipStream_t stream);
+
+#endif

@@ -1,0 +1,761 @@
+// ss_kernels.hip -- gfx950 kernels for the slant-stack projector pair, its exact
+// transpose, the FBP row filter, the fused EM epilogues and the ellipse phantoms.
+//
+// Arithmetic is FP32 with FP32 accumulation.  FP64 appears only in the plan (angle
+// tables, exact sample ranges), in the knife-edge predicate fallback of the
+// backprojection and in the EM log-likelihood sums.
+#include <hip/hip_runtime.h>
+
+#include "ss_kernels.h"
+#include "ss_plan_logic.h"
+
+#define SS_WAVE 64
+#define SS_INV_FIX 2.3283064365386963e-10f  // 2^-32
+
+// =====================================================================================
+// plan: exact per-ray sample ranges
+// =====================================================================================
+
+__global__ __launch_bounds__(256) void k_plan_bounds(SSGeom g, const SSAngle* __restrict__ angles,
+                                                     int2* __restrict__ bounds, int* __restrict__ irregular,
+                                                     int* __restrict__ nirregular, int capacity) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (k >= g.nrays) return;
+  const SSAngle a = angles[j];
+  const SSTraj tr = ss_trajectory(g, a);
+  int lo, hi;
+  const int irr = ss_ray_range(g, a, tr, k, &lo, &hi);
+  if (irr) {
+    lo = 0;
+    hi = -2;  // empty for the main kernels; the fixup kernels evaluate it sample by sample
+    const int slot = atomicAdd(nirregular, 1);
+    if (slot < capacity) irregular[slot] = j * g.nrays + k;
+  }
+  bounds[(size_t)j * g.nrays + k] = make_int2(lo, hi);
+}
+
+// Per (angle, group of 64 rays): union and intersection of the non-empty ranges.
+__global__ __launch_bounds__(64) void k_plan_wave_bounds(SSGeom g, const int2* __restrict__ bounds,
+                                                         int4* __restrict__ wbounds) {
+  const int j = blockIdx.y;
+  const int grp = blockIdx.x;
+  const int k = grp * SS_WAVE + threadIdx.x;
+  int lo = 0x7fffffff, hi = -1, ilo = -0x7fffffff, ihi = 0x7fffffff;
+  bool any = false;
+  if (k < g.nrays) {
+    const int2 b = bounds[(size_t)j * g.nrays + k];
+    if (b.y >= b.x) {
+      lo = b.x;
+      hi = b.y;
+      ilo = b.x;
+      ihi = b.y;
+      any = true;
+    }
+  }
+  // every lane takes part in every shuffle (no short-circuit around them)
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off));
+    hi = max(hi, __shfl_xor(hi, off));
+    ilo = max(ilo, __shfl_xor(ilo, off));
+    ihi = min(ihi, __shfl_xor(ihi, off));
+  }
+  const bool any_lane = __ballot(any ? 1 : 0) != 0ull;
+  if (threadIdx.x == 0) {
+    int4 r;
+    if (!any_lane) {
+      r = make_int4(0, -1, 0, -1);
+    } else {
+      r = make_int4(lo, hi, ilo, ihi);
+    }
+    wbounds[(size_t)j * gridDim.x + grp] = r;
+  }
+}
+
+hipError_t ss_launch_plan_bounds(const SSGeom& g, const SSAngle* angles, int2* bounds, int* irregular,
+                                 int* nirregular, int capacity, hipStream_t stream) {
+  dim3 grid((g.nrays + 255) / 256, g.nangles);
+  hipLaunchKernelGGL(k_plan_bounds, grid, dim3(256), 0, stream, g, angles, bounds, irregular, nirregular, capacity);
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4* wbounds, hipStream_t stream) {
+  dim3 grid((g.nrays + SS_WAVE - 1) / SS_WAVE, g.nangles);
+  hipLaunchKernelGGL(k_plan_wave_bounds, grid, dim3(SS_WAVE), 0, stream, g, bounds, wbounds);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// layouts
+// =====================================================================================
+
+// image [s][n][n] -> gatherB [s][n][n+2] (rows, zero pad at both ends)
+//                 -> gatherA [s][n][n+2] (columns: gatherA[c][r+1] = img[r][c])
+__global__ __launch_bounds__(256) void k_gather_layouts(const float* __restrict__ img, float* __restrict__ gA,
+                                                        float* __restrict__ gB, int n, int pitch) {
+  __shared__ float tile[32][33];
+  const int slice = blockIdx.z;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const float* src = img + (size_t)slice * n * n;
+  float* dA = gA + (size_t)slice * n * pitch;
+  float* dB = gB + (size_t)slice * n * pitch;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int r = by + yy, c = bx + tx;
+    float v = 0.f;
+    if (r < n && c < n) {
+      v = src[(size_t)r * n + c];
+      dB[(size_t)r * pitch + c + 1] = v;
+    }
+    tile[yy][tx] = v;
+  }
+  __syncthreads();
+  for (int xx = ty; xx < 32; xx += 8) {
+    const int c = bx + xx, r = by + tx;
+    if (r < n && c < n) dA[(size_t)c * pitch + r + 1] = tile[tx][xx];
+  }
+  // zero pads: column 0 and n+1 of every row (done by the blocks of the first column)
+  if (blockIdx.x == 0) {
+    for (int yy = threadIdx.x; yy < 32; yy += 256) {
+      const int r = by + yy;
+      if (r < n) {
+        dB[(size_t)r * pitch] = 0.f;
+        dB[(size_t)r * pitch + n + 1] = 0.f;
+        dA[(size_t)r * pitch] = 0.f;
+        dA[(size_t)r * pitch + n + 1] = 0.f;
+      }
+    }
+  }
+}
+
+hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float* gatherA, float* gatherB,
+                                    int nslices, hipStream_t stream) {
+  const int n = p.g.n;
+  dim3 grid((n + 31) / 32, (n + 31) / 32, nslices);
+  hipLaunchKernelGGL(k_gather_layouts, grid, dim3(256), 0, stream, image, gatherA, gatherB, n, p.g.pitch);
+  return hipGetLastError();
+}
+
+// [s][rows][cols] -> [s][cols][rows]
+__global__ __launch_bounds__(256) void k_transpose2d(const float* __restrict__ in, float* __restrict__ out, int rows,
+                                                     int cols) {
+  __shared__ float tile[32][33];
+  const int slice = blockIdx.z;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const float* src = in + (size_t)slice * rows * cols;
+  float* dst = out + (size_t)slice * rows * cols;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int r = by + yy, c = bx + tx;
+    if (r < rows && c < cols) tile[yy][tx] = src[(size_t)r * cols + c];
+  }
+  __syncthreads();
+  for (int xx = ty; xx < 32; xx += 8) {
+    const int c = bx + xx, r = by + tx;
+    if (r < rows && c < cols) dst[(size_t)c * rows + r] = tile[tx][xx];
+  }
+}
+
+hipError_t ss_launch_sino_to_t(const float* sino, float* sinoT, int R, int A, int nslices, hipStream_t stream) {
+  dim3 grid((A + 31) / 32, (R + 31) / 32, nslices);
+  hipLaunchKernelGGL(k_transpose2d, grid, dim3(256), 0, stream, sino, sinoT, R, A);
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_t_to_sino(const float* sinoT, float* sino, int R, int A, int nslices, hipStream_t stream) {
+  dim3 grid((R + 31) / 32, (A + 31) / 32, nslices);
+  hipLaunchKernelGGL(k_transpose2d, grid, dim3(256), 0, stream, sinoT, sino, A, R);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// forward: ray-driven slant-stack Radon
+//
+// One wave = 64 consecutive rays of one angle; a block = 4 angles.  Sample s walks
+// gather rows (image rows for branch B, image columns for branch A, stored
+// transposed), so at each s the 64 lanes read a contiguous stretch of one padded
+// row.  Loop bounds come from the plan's exact FP64 ranges: predicated head and
+// tail, unpredicated middle.
+// =====================================================================================
+
+__device__ __forceinline__ double ss_block_sum_f64(double v, double* scratch) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) scratch[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += scratch[i];
+  return r;  // valid in thread 0
+}
+
+template <int MODE>
+__device__ __forceinline__ void ss_radon_store(const SSDevPlan& p, int slice, int j, int k, float val, float* out,
+                                               const SSEmRatioArgs& em, double* ll) {
+  const int R = p.g.nrays, A = p.g.nangles;
+  if (MODE == SS_OUT_SINO) {
+    out[((size_t)slice * R + k) * A + j] = val;
+  } else if (MODE == SS_OUT_SINOT) {
+    out[((size_t)slice * A + j) * R + k] = val;
+  } else {
+    const size_t idx = ((size_t)slice * A + j) * R + k;
+    const float s = em.meas[idx];
+    float rho = 0.f;
+    if (val > 0.f) {
+      rho = s / val;
+      *ll += (double)s * log((double)val) - (double)val;
+    }
+    out[idx] = rho;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_radon(SSDevPlan p, const float* __restrict__ gA,
+                                               const float* __restrict__ gB, int j0, int j1,
+                                               float* __restrict__ out, SSEmRatioArgs em) {
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = j0 + blockIdx.y * 4 + wv;
+  const int slice = blockIdx.z;
+  const int R = p.g.nrays, n = p.g.n, pitch = p.g.pitch;
+  const int grp = blockIdx.x;
+  const int k = grp * SS_WAVE + lane;
+  double ll = 0.0;
+  if (j < j1) {
+    const SSAngle& a = p.angles[j];
+    const int4 wb = p.wbounds[(size_t)j * p.nwaves + grp];
+    int lo = 0, hi = -1;
+    if (k < R) {
+      const int2 b = p.bounds[(size_t)j * R + k];
+      lo = b.x;
+      hi = b.y;
+    }
+    int64_t beta = a.beta;
+    int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)wb.x * beta;
+    if (hi < lo) {
+      // an empty ray reads the zero pad with weight 0: exactly 0 in the unpredicated loop
+      u = -((int64_t)1 << 32);
+      beta = 0;
+    }
+    const float* img = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch + 1;
+    const unsigned span = (unsigned)(hi - lo);
+    float acc_a = 0.f, acc_d = 0.f;
+    int s = wb.x;
+    const int e1 = min(wb.z, wb.y + 1);
+    for (; s < e1; ++s) {
+      if ((unsigned)(s - lo) <= span) {
+        const int m = (int)(u >> 32);
+        const float w = (float)(uint32_t)u;
+        const float* row = img + (size_t)s * pitch;
+        const float va = row[m], vb = row[m + 1];
+        acc_a += va;
+        acc_d = fmaf(w, vb - va, acc_d);
+      }
+      u += beta;
+    }
+    for (; s <= wb.w; ++s) {
+      const int m = (int)(u >> 32);
+      const float w = (float)(uint32_t)u;
+      const float* row = img + (size_t)s * pitch;
+      const float va = row[m], vb = row[m + 1];
+      acc_a += va;
+      acc_d = fmaf(w, vb - va, acc_d);
+      u += beta;
+    }
+    for (; s <= wb.y; ++s) {
+      if ((unsigned)(s - lo) <= span) {
+        const int m = (int)(u >> 32);
+        const float w = (float)(uint32_t)u;
+        const float* row = img + (size_t)s * pitch;
+        const float va = row[m], vb = row[m + 1];
+        acc_a += va;
+        acc_d = fmaf(w, vb - va, acc_d);
+      }
+      u += beta;
+    }
+    const float val = fmaf(acc_d, SS_INV_FIX, acc_a) * a.delta;
+    if (k < R) ss_radon_store<MODE>(p, slice, j, k, val, out, em, &ll);
+  }
+  if (MODE == SS_OUT_EM_RATIO) {
+    const double tot = ss_block_sum_f64(ll, red);
+    if (threadIdx.x == 0)
+      em.ll_partials[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = tot;
+  }
+}
+
+// Irregular rays (knife-edge angles): one block per ray, samples across threads,
+// validity evaluated per sample with the exact FP64 predicate.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float* __restrict__ gA,
+                                                     const float* __restrict__ gB, int j0, int j1,
+                                                     float* __restrict__ out, SSEmRatioArgs em, int partial_base) {
+  __shared__ float redf[4];
+  __shared__ double red[4];
+  const int id = p.irregular[blockIdx.x];
+  const int slice = blockIdx.y;
+  const int R = p.g.nrays, n = p.g.n, pitch = p.g.pitch;
+  const int j = id / R, k = id % R;
+  if (j < j0 || j >= j1) {
+    if (MODE == SS_OUT_EM_RATIO && threadIdx.x == 0)
+      em.ll_partials[partial_base + (size_t)slice * gridDim.x + blockIdx.x] = 0.0;
+    return;
+  }
+  const SSAngle& a = p.angles[j];
+  const float* img = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch + 1;
+  float acc = 0.f;
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    if (ss_radon_sample_valid(p.g, a, k, s)) {
+      const int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)s * a.beta;
+      const int m = (int)(u >> 32);
+      const float w = (float)(uint32_t)u * SS_INV_FIX;
+      const float* row = img + (size_t)s * pitch;
+      const float va = row[m], vb = row[m + 1];
+      acc += fmaf(w, vb - va, va);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  double ll = 0.0;
+  if (threadIdx.x == 0) {
+    float tot = 0.f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) tot += redf[i];
+    ss_radon_store<MODE>(p, slice, j, k, tot * a.delta, out, em, &ll);
+    if (MODE == SS_OUT_EM_RATIO) em.ll_partials[partial_base + (size_t)slice * gridDim.x + blockIdx.x] = ll;
+  }
+  (void)red;
+}
+
+int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices) {
+  const int gy = (j1 - j0 + 3) / 4;
+  return p.nwaves * gy * nslices + p.nirregular * nslices;
+}
+
+hipError_t ss_launch_radon(const SSDevPlan& p, const float* gatherA, const float* gatherB, int j0, int j1,
+                           int nslices, int mode, float* out, SSEmRatioArgs em, hipStream_t stream) {
+  if (j1 <= j0) return hipSuccess;
+  dim3 grid(p.nwaves, (j1 - j0 + 3) / 4, nslices);
+  const int partial_base = p.nwaves * grid.y * nslices;
+  switch (mode) {
+    case SS_OUT_SINO:
+      hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out, em);
+      break;
+    case SS_OUT_SINOT:
+      hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out, em);
+      break;
+    default:
+      hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out,
+                         em);
+      break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || p.nirregular == 0) return e;
+  dim3 fgrid(p.nirregular, nslices);
+  switch (mode) {
+    case SS_OUT_SINO:
+      hipLaunchKernelGGL(k_radon_fixup<SS_OUT_SINO>, fgrid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out,
+                         em, partial_base);
+      break;
+    case SS_OUT_SINOT:
+      hipLaunchKernelGGL(k_radon_fixup<SS_OUT_SINOT>, fgrid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out,
+                         em, partial_base);
+      break;
+    default:
+      hipLaunchKernelGGL(k_radon_fixup<SS_OUT_EM_RATIO>, fgrid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1,
+                         out, em, partial_base);
+      break;
+  }
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// backprojection: pixel-driven, 32x32 pixel tile per block, angle blocks staged in LDS
+//
+// For each angle of a stage the block computes (FP64) the ray coordinate U of the
+// tile origin and the tile's U range, then stages the W consecutive rays
+// [base, base+W) that the tile can touch.  Each thread owns 4 pixels (one column,
+// rows ty, ty+8, ty+16, ty+24) and interpolates from LDS with FP32 offsets
+// relative to `base` (|u| < ~W, so FP32 resolves ~1e-5 of a detector bin).
+//
+// Exactness of the 0 <= m <= R-2 predicate: a tile whose U range lies inside
+// [eps, R-1-eps] needs no test; one entirely outside contributes nothing; the rest
+// test each pixel, and pixels within 1e-3 bins of an edge re-evaluate the
+// reference's FP64 expression.
+// =====================================================================================
+
+#define BP_TILE 32
+#define BP_MAX_STAGE 4096  // floats of LDS for the sinogram windows
+
+struct BPAngle {
+  float f0, b, c;
+  int flag;  // 0 outside, 1 interior, 2 boundary
+  int base;
+  int pad0, pad1, pad2;
+};
+
+__device__ __noinline__ int ss_bp_exact_valid(const SSDevPlan& p, int j, int row, int col) {
+  const SSAngle& a = p.angles[j];
+  return ss_bp_sample_valid(p.g, a.cost, a.sint, row, col) ? 1 : 0;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float* __restrict__ sinoT, int j0, int j1,
+                                                        float* __restrict__ out, SSEmUpdateArgs em) {
+  __shared__ float win[BP_MAX_STAGE];
+  __shared__ BPAngle par[64];
+  const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
+  const int W = p.bp_window;
+  const int J = p.bp_angles;
+  const int c0 = blockIdx.x * BP_TILE, r0 = blockIdx.y * BP_TILE;
+  const int slice = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int tx = tid & 31, ty = tid >> 5;
+  const float* sl = sinoT + (size_t)slice * A * R;
+  const int col = c0 + tx;
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+  const float ftx = (float)tx;
+  const float lo_rel_eps = 1e-3f;
+
+  for (int jb = j0; jb < j1; jb += J) {
+    const int nj = min(J, j1 - jb);
+    if (tid < nj) {
+      const SSAngle& a = p.angles[jb + tid];
+      const double U = a.bp_a + (double)c0 * a.bp_b + (double)r0 * a.bp_c;
+      const double ec = (double)(BP_TILE - 1) * a.bp_b, er = (double)(BP_TILE - 1) * a.bp_c;
+      const double umin = U + fmin(0.0, ec) + fmin(0.0, er);
+      const double umax = U + fmax(0.0, ec) + fmax(0.0, er);
+      const double base = floor(umin) - 1.0;
+      BPAngle q;
+      q.f0 = (float)(U - base);
+      q.b = (float)a.bp_b;
+      q.c = (float)a.bp_c;
+      const double eps = 1e-6;
+      if (umax < -eps || umin >= (double)(R - 1) + eps)
+        q.flag = 0;
+      else if (umin >= eps && umax < (double)(R - 1) - eps)
+        q.flag = 1;
+      else
+        q.flag = 2;
+      q.base = (int)base;
+      q.pad0 = q.pad1 = q.pad2 = 0;
+      par[tid] = q;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nj * W; idx += 256) {
+      const int jj = idx / W, w = idx - jj * W;
+      const BPAngle& q = par[jj];
+      const int ray = q.base + w;
+      float v = 0.f;
+      if (q.flag != 0 && ray >= 0 && ray < R) v = sl[(size_t)(jb + jj) * R + ray];
+      win[jj * W + w] = v;
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nj; ++jj) {
+      const BPAngle q = par[jj];
+      const int flag = __builtin_amdgcn_readfirstlane(q.flag);
+      if (flag == 0) continue;
+      const float* wrow = win + jj * W;
+      const float t0 = fmaf(ftx, q.b, q.f0);
+      float us[4];
+      us[0] = fmaf((float)(ty), q.c, t0);
+      us[1] = fmaf((float)(ty + 8), q.c, t0);
+      us[2] = fmaf((float)(ty + 16), q.c, t0);
+      us[3] = fmaf((float)(ty + 24), q.c, t0);
+      float vals[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float u = us[i];
+        const float fl = floorf(u);
+        const int ii = (int)fl;
+        const float w = u - fl;
+        const float va = wrow[ii], vb = wrow[ii + 1];
+        vals[i] = fmaf(w, vb - va, va);
+      }
+      if (flag == 2) {
+        // boundary tile: per-pixel 0 <= m <= R-2 test; pixels within 1e-3 bins of an
+        // edge take the reference's FP64 expression.  The result is an integer mask
+        // applied by select, keeping the divergent region free of value phis.
+        const float lo_rel = (float)(-q.base), hi_rel = (float)(R - 1 - q.base);
+        int keep = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float u = us[i];
+          const bool inside = (u > lo_rel + lo_rel_eps) && (u < hi_rel - lo_rel_eps);
+          const bool outside = (u < lo_rel - lo_rel_eps) || (u > hi_rel + lo_rel_eps);
+          int ok = inside ? 1 : 0;
+          if (!inside && !outside) ok = ss_bp_exact_valid(p, jb + jj, r0 + ty + 8 * i, col);
+          keep |= ok << i;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) vals[i] = ((keep >> i) & 1) ? vals[i] : 0.f;
+      }
+      acc0 += vals[0];
+      acc1 += vals[1];
+      acc2 += vals[2];
+      acc3 += vals[3];
+    }
+    __syncthreads();
+  }
+  if (col >= n) return;
+  const float dth = (float)p.g.dth;
+  float accs[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = r0 + ty + 8 * i;
+    if (row >= n) continue;
+    const size_t idx = ((size_t)slice * n + row) * n + col;
+    const float bp = accs[i] * dth;
+    if (MODE == SS_BP_PLAIN) {
+      out[idx] = bp;
+    } else {
+      const float wgt = em.weight[idx];
+      const float x = em.x[idx];
+      em.x[idx] = wgt > 0.f ? x * bp / wgt : x;
+    }
+  }
+}
+
+hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices, int mode,
+                                    float* out, SSEmUpdateArgs em, hipStream_t stream) {
+  const int n = p.g.n;
+  dim3 grid((n + BP_TILE - 1) / BP_TILE, (n + BP_TILE - 1) / BP_TILE, nslices);
+  if (mode == SS_BP_PLAIN)
+    hipLaunchKernelGGL(k_backprojection<SS_BP_PLAIN>, grid, dim3(256), 0, stream, p, sinoT, j0, j1, out, em);
+  else
+    hipLaunchKernelGGL(k_backprojection<SS_BP_EM_UPDATE>, grid, dim3(256), 0, stream, p, sinoT, j0, j1, out, em);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// exact transpose of the Radon: pixel-driven gather over the forward's own index map
+//
+// Pixel (r,c) receives, from ray (j,k), (1-w)*delta*q when the forward's sample at
+// s (= c in branch A, r in branch B) has floor index m == p (= r in A, c in B) and
+// w*delta*q when m == p-1, provided s lies in the ray's exact range.  The candidate
+// rays are those with u in [p-1, p+1); u is recomputed with the forward's integer
+// expression, so indices and weights are bit-identical to the forward's.
+// =====================================================================================
+
+__global__ __launch_bounds__(256) void k_radon_transpose(SSDevPlan p, const float* __restrict__ qT, int j0, int j1,
+                                                         float* __restrict__ out) {
+  const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
+  const int c = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int r = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int slice = blockIdx.z;
+  if (r >= n || c >= n) return;
+  const float* q = qT + (size_t)slice * A * R;
+  float acc = 0.f;
+  for (int j = j0; j < j1; ++j) {
+    const SSAngle& a = p.angles[j];
+    const int s = a.branch_a ? c : r;
+    const int pp = a.branch_a ? r : c;
+    const int64_t base = a.alpha0 + (int64_t)s * a.beta;
+    const double g = (double)a.gamma;
+    // k with u = base + k*gamma in [(pp-1), (pp+1)) grid units
+    const double ka = ((double)(pp - 1) * SS_FIX_ONE - (double)base) / g;
+    const double kb = ((double)(pp + 1) * SS_FIX_ONE - (double)base) / g;
+    double klo = fmin(ka, kb), khi = fmax(ka, kb);
+    if (khi < -2.0 || klo > (double)R + 1.0) continue;
+    int k0 = (int)floor(fmax(klo, -2.0)) - 1;
+    int k1 = (int)ceil(fmin(khi, (double)R + 1.0)) + 1;
+    if (k0 < 0) k0 = 0;
+    if (k1 > R - 1) k1 = R - 1;
+    float sum = 0.f;
+    for (int k = k0; k <= k1; ++k) {
+      const int64_t u = base + (int64_t)k * a.gamma;
+      const int m = (int)(u >> 32);
+      if (m != pp && m != pp - 1) continue;
+      const int2 b = p.bounds[(size_t)j * R + k];
+      if (s < b.x || s > b.y) continue;
+      const float w = (float)(uint32_t)u * SS_INV_FIX;
+      const float wt = (m == pp) ? (1.f - w) : w;
+      sum = fmaf(wt, q[(size_t)j * R + k], sum);
+    }
+    acc = fmaf(sum, a.delta, acc);
+  }
+  out[((size_t)slice * n + r) * n + c] = acc;
+}
+
+// irregular rays: scatter their exact contributions
+__global__ __launch_bounds__(256) void k_radon_transpose_fixup(SSDevPlan p, const float* __restrict__ qT, int j0,
+                                                               int j1, float* __restrict__ out) {
+  const int id = p.irregular[blockIdx.x];
+  const int slice = blockIdx.y;
+  const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
+  const int j = id / R, k = id % R;
+  if (j < j0 || j >= j1) return;
+  const SSAngle& a = p.angles[j];
+  const float v = qT[((size_t)slice * A + j) * R + k] * a.delta;
+  float* img = out + (size_t)slice * n * n;
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    if (!ss_radon_sample_valid(p.g, a, k, s)) continue;
+    const int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)s * a.beta;
+    const int m = (int)(u >> 32);
+    const float w = (float)(uint32_t)u * SS_INV_FIX;
+    // pixel index along the interpolation axis m (weight 1-w) and m+1 (weight w)
+    if (m >= 0 && m < n) {
+      const size_t idx = a.branch_a ? (size_t)m * n + s : (size_t)s * n + m;
+      atomicAdd(img + idx, (1.f - w) * v);
+    }
+    if (m + 1 >= 0 && m + 1 < n) {
+      const size_t idx = a.branch_a ? (size_t)(m + 1) * n + s : (size_t)s * n + m + 1;
+      atomicAdd(img + idx, w * v);
+    }
+  }
+}
+
+hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
+                                     float* out, hipStream_t stream) {
+  const int n = p.g.n;
+  dim3 grid((n + 15) / 16, (n + 15) / 16, nslices);
+  hipLaunchKernelGGL(k_radon_transpose, grid, dim3(256), 0, stream, p, sinoT, j0, j1, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || p.nirregular == 0) return e;
+  hipLaunchKernelGGL(k_radon_transpose_fixup, dim3(p.nirregular, nslices), dim3(256), 0, stream, p, sinoT, j0, j1,
+                     out);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// FBP row filter: circular convolution of each length-R row with g
+// (g = Re(ifft(fftshift(h))), the real kernel equivalent to pyraft's lowpassino_fbp)
+// 4 rows per block; g (duplicated, length 2R) and the rows in LDS.
+// =====================================================================================
+
+#define RF_ROWS 4
+__global__ __launch_bounds__(256) void k_ramp_filter(const float* __restrict__ in, float* __restrict__ out,
+                                                     const float* __restrict__ g, int R, int nrows) {
+  extern __shared__ float lds[];
+  float* g2 = lds;                          // 2R
+  float* rows = lds + ((2 * R + 3) & ~3);   // R x RF_ROWS, interleaved [m][row], 16-byte aligned
+  const int row0 = blockIdx.x * RF_ROWS;
+  for (int i = threadIdx.x; i < R; i += 256) {
+    const float v = g[i];
+    g2[i] = v;
+    g2[i + R] = v;
+  }
+  for (int i = threadIdx.x; i < R * RF_ROWS; i += 256) {
+    const int rr = i / R, m = i - rr * R;
+    const int row = row0 + rr;
+    rows[m * RF_ROWS + rr] = row < nrows ? in[(size_t)row * R + m] : 0.f;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < R; k += 256) {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    const float* gk = g2 + k + R;
+    for (int m = 0; m < R; ++m) {
+      const float gv = gk[-m];
+      const float4 s = *reinterpret_cast<const float4*>(rows + m * RF_ROWS);
+      a0 = fmaf(gv, s.x, a0);
+      a1 = fmaf(gv, s.y, a1);
+      a2 = fmaf(gv, s.z, a2);
+      a3 = fmaf(gv, s.w, a3);
+    }
+    if (row0 + 0 < nrows) out[(size_t)(row0 + 0) * R + k] = a0;
+    if (row0 + 1 < nrows) out[(size_t)(row0 + 1) * R + k] = a1;
+    if (row0 + 2 < nrows) out[(size_t)(row0 + 2) * R + k] = a2;
+    if (row0 + 3 < nrows) out[(size_t)(row0 + 3) * R + k] = a3;
+  }
+}
+
+hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream) {
+  const size_t lds = (size_t)(((2 * R + 3) & ~3) + RF_ROWS * R) * sizeof(float);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ramp_filter, dim3((nrows + RF_ROWS - 1) / RF_ROWS), dim3(256), lds, stream, in, out, g, R,
+                     nrows);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// small helpers
+// =====================================================================================
+
+// dst += sum(partials[0..n)) in a fixed order (one block)
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int n,
+                                                      double* __restrict__ dst) {
+  __shared__ double red[4];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += partials[i];
+  const double tot = ss_block_sum_f64(v, red);
+  if (threadIdx.x == 0) *dst += tot;
+}
+
+hipError_t ss_launch_sum_partials(const double* partials, int n, double* dst, hipStream_t stream) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream, partials, n, dst);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, float v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
+hipError_t ss_launch_fill(float* p, float v, size_t n, hipStream_t stream) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, stream, p, v, n);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// ellipse phantoms, FP64 (raft/raft_image_phantom_fromdesc.c:4-59, raft/raft_radon_fromdesc.c:4-67)
+// =====================================================================================
+
+__global__ __launch_bounds__(256) void k_phantom(const SSEllipse* __restrict__ ell, int nell, int lines, int cols,
+                                                 double tl_x, double tl_y, double hs, double vs,
+                                                 double* __restrict__ out64, float* __restrict__ out32) {
+  SS_NOCONTRACT
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)lines * cols) return;
+  const int i = (int)(idx / cols), j = (int)(idx % cols);
+  double acc = out64 ? out64[idx] : 0.0;
+  for (int e = 0; e < nell; ++e) {
+    const SSEllipse E = ell[e];
+    const double x = tl_x + j * hs - E.x0;
+    const double y = tl_y + i * vs - E.y0;
+    double pv = (y * E.s) + (x * E.c);
+    pv /= E.a;
+    pv *= pv;
+    double qv = (y * E.c) - (x * E.s);
+    qv /= E.b;
+    qv *= qv;
+    pv += qv;
+    pv = sqrt(pv);
+    if (pv <= 1) acc += E.val;
+  }
+  if (out64) out64[idx] = acc;
+  if (out32) out32[idx] = (float)acc;
+}
+
+hipError_t ss_launch_phantom(const SSEllipse* ell, int nell, int lines, int cols, double tl_x, double tl_y,
+                             double hs, double vs, double* out64, float* out32, hipStream_t stream) {
+  const size_t total = (size_t)lines * cols;
+  hipLaunchKernelGGL(k_phantom, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, ell, nell, lines, cols,
+                     tl_x, tl_y, hs, vs, out64, out32);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_radon_fromdesc(const SSEllipseView* __restrict__ views, int nell,
+                                                        int lines, int cols, double* __restrict__ out64) {
+  SS_NOCONTRACT
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)lines * cols) return;
+  const int i = (int)(idx / cols), j = (int)(idx % cols);
+  double acc = out64[idx];
+  for (int e = 0; e < nell; ++e) {
+    const SSEllipseView v = views[(size_t)e * cols + j];
+    double tp = v.mab * i - v.delt;
+    tp *= tp;
+    if (tp <= 1.0) acc += v.m * sqrt(1.0 - tp);
+  }
+  out64[idx] = acc;
+}
+
+hipError_t ss_launch_radon_fromdesc(const SSEllipseView* views, int nell, int lines, int cols, double* out64,
+                                    hipStream_t stream) {
+  const size_t total = (size_t)lines * cols;
+  hipLaunchKernelGGL(k_radon_fromdesc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, views, nell,
+                     lines, cols, out64);
+  return hipGetLastError();
+}

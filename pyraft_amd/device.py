@@ -1,0 +1,224 @@
+"""Device-resident API over the libraft ``raft_ss_*`` entry points.
+
+PyTorch supplies device memory and the current stream; every operator runs a
+hand-written HIP kernel of libraft through ctypes (``data_ptr()`` in, status code
+out).  Tensors are float32 and contiguous:
+
+    image     (..., N, N)   row r <-> y = ymin + r*dy (row 0 at the bottom)
+    sinogram  (..., R, A)   the pyraft (rays, angles) layout
+    sinogramT (..., A, R)   angle-major (``angle_major=True`` variants)
+
+Leading dimensions are flattened into the slice count of one launch.
+"""
+import ctypes
+import math
+
+import torch
+
+from .raftypes import check, libraft
+
+DEFAULT_IMG_CORNERS = (-1.0, 1.0, 1.0, -1.0)
+DEFAULT_SINO_CORNERS = (0.0, 1.0, math.pi, -1.0)
+
+
+def current_stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyraft_amd.device needs a ROCm GPU (torch.cuda.is_available() is False)")
+
+
+def _check_tensor(t, shape_tail, name):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch.Tensor" % name)
+    if t.dtype != torch.float32:
+        raise TypeError("%s must be float32 (got %s)" % (name, t.dtype))
+    if t.device.type != "cuda":
+        raise TypeError("%s must be a device tensor" % name)
+    if tuple(t.shape[-len(shape_tail):]) != tuple(shape_tail):
+        raise ValueError("%s has shape %s, expected (..., %s)" % (name, tuple(t.shape),
+                                                                  ", ".join(map(str, shape_tail))))
+    return t.contiguous()
+
+
+def _nslices(t, tail):
+    n = 1
+    for d in t.shape[:-tail]:
+        n *= int(d)
+    return n
+
+
+class Plan:
+    """Geometry plan of an N x N image and an R x A sinogram on one device.
+
+    ``img_corners`` / ``sino_corners`` are (tl_x, tl_y, br_x, br_y) as in
+    raft_image: the image spans x in [tl_x, br_x], y in [br_y, tl_y]; the sinogram
+    spans theta = tl_x + j*(br_x - tl_x)/A and t = br_y + k*(tl_y - br_y)/(R - 1)."""
+
+    def __init__(self, n, nrays, nangles, img_corners=DEFAULT_IMG_CORNERS,
+                 sino_corners=DEFAULT_SINO_CORNERS, device=None):
+        _require_gpu()
+        self.n, self.nrays, self.nangles = int(n), int(nrays), int(nangles)
+        self.img_corners = tuple(float(v) for v in img_corners)
+        self.sino_corners = tuple(float(v) for v in sino_corners)
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else
+                                   torch.device(device).index or 0)
+        self._handle = ctypes.c_void_p()
+        ic = (ctypes.c_double * 4)(*self.img_corners)
+        sc = (ctypes.c_double * 4)(*self.sino_corners)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_plan_create(ctypes.byref(self._handle), self.n, self.nrays, self.nangles,
+                                              ic, sc, current_stream_handle()), "raft_ss_plan_create")
+
+    # -- bookkeeping
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                libraft.raft_ss_plan_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+            self._handle = None
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def info(self):
+        out = (ctypes.c_int * 7)()
+        check(libraft.raft_ss_plan_info(self._handle, out, 7), "raft_ss_plan_info")
+        keys = ("n", "nrays", "nangles", "irregular_rays", "degenerate_angles", "bp_window", "bp_angles")
+        return dict(zip(keys, list(out)))
+
+    def bounds(self):
+        import numpy
+        b = numpy.zeros((self.nangles, self.nrays, 2), numpy.int32)
+        check(libraft.raft_ss_plan_bounds(self._handle, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+              "raft_ss_plan_bounds")
+        return b
+
+    def wave_bounds(self):
+        import numpy
+        nw = (self.nrays + 63) // 64
+        b = numpy.zeros((self.nangles, nw, 4), numpy.int32)
+        check(libraft.raft_ss_plan_wave_bounds(self._handle, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+              "raft_ss_plan_wave_bounds")
+        return b
+
+    def _stream(self):
+        return current_stream_handle(self.device)
+
+    def _range(self, angles):
+        if angles is None:
+            return 0, self.nangles
+        j0, j1 = int(angles[0]), int(angles[1])
+        if not (0 <= j0 <= j1 <= self.nangles):
+            raise ValueError("angle range (%d, %d) outside [0, %d]" % (j0, j1, self.nangles))
+        return j0, j1
+
+    # -- operators
+    def radon(self, image, angle_major=False, angles=None, out=None):
+        """Slant-stack Radon of (..., N, N) -> (..., R, A) (or (..., A, R))."""
+        x = _check_tensor(image, (self.n, self.n), "image")
+        lead = x.shape[:-2]
+        shape = lead + ((self.nangles, self.nrays) if angle_major else (self.nrays, self.nangles))
+        if out is None:
+            out = torch.zeros(shape, dtype=torch.float32, device=x.device)
+        j0, j1 = self._range(angles)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_radon_range(self._handle, x.data_ptr(), out.data_ptr(), int(angle_major), j0, j1,
+                                              _nslices(x, 2), self._stream()), "raft_ss_radon_range")
+        return out
+
+    def backprojection(self, sino, angle_major=False, angles=None):
+        """Slant-stack backprojection (reference BP semantics) -> (..., N, N)."""
+        tail = (self.nangles, self.nrays) if angle_major else (self.nrays, self.nangles)
+        s = _check_tensor(sino, tail, "sinogram")
+        out = torch.empty(s.shape[:-2] + (self.n, self.n), dtype=torch.float32, device=s.device)
+        j0, j1 = self._range(angles)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_backprojection_range(self._handle, s.data_ptr(), out.data_ptr(), int(angle_major),
+                                                       j0, j1, _nslices(s, 2), self._stream()),
+                  "raft_ss_backprojection_range")
+        return out
+
+    def radon_transpose(self, sino, angle_major=False, angles=None):
+        """Exact adjoint of ``radon`` -> (..., N, N)."""
+        tail = (self.nangles, self.nrays) if angle_major else (self.nrays, self.nangles)
+        s = _check_tensor(sino, tail, "sinogram")
+        out = torch.empty(s.shape[:-2] + (self.n, self.n), dtype=torch.float32, device=s.device)
+        j0, j1 = self._range(angles)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_radon_transpose_range(self._handle, s.data_ptr(), out.data_ptr(), int(angle_major),
+                                                        j0, j1, _nslices(s, 2), self._stream()),
+                  "raft_ss_radon_transpose_range")
+        return out
+
+    def fbp_filter(self, sino, angle_major=False):
+        tail = (self.nangles, self.nrays) if angle_major else (self.nrays, self.nangles)
+        s = _check_tensor(sino, tail, "sinogram")
+        out = torch.empty_like(s)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_fbp_filter(self._handle, s.data_ptr(), out.data_ptr(), int(angle_major),
+                                             _nslices(s, 2), self._stream()), "raft_ss_fbp_filter")
+        return out
+
+    def fbp(self, sino):
+        s = _check_tensor(sino, (self.nrays, self.nangles), "sinogram")
+        out = torch.empty(s.shape[:-2] + (self.n, self.n), dtype=torch.float32, device=s.device)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_fbp(self._handle, s.data_ptr(), out.data_ptr(), _nslices(s, 2), self._stream()),
+                  "raft_ss_fbp")
+        return out
+
+    def em(self, sino, niter, nsubsets=1, x0=None, return_ll=False):
+        """OSEM (nsubsets=1: EM) from x0 (default ones); returns x (and the LL trace)."""
+        s = _check_tensor(sino, (self.nrays, self.nangles), "sinogram")
+        lead = s.shape[:-2]
+        if x0 is None:
+            x = torch.ones(lead + (self.n, self.n), dtype=torch.float32, device=s.device)
+        else:
+            x = _check_tensor(x0, (self.n, self.n), "x0").clone()
+        ll = (ctypes.c_double * max(1, int(niter)))()
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_em(self._handle, s.data_ptr(), x.data_ptr(), int(niter), int(nsubsets),
+                                     _nslices(s, 2), ll if return_ll else None, self._stream()), "raft_ss_em")
+        if return_ll:
+            return x, [ll[i] for i in range(int(niter))]
+        return x
+
+
+def ramp_kernel(nrays, tmin=-1.0, tmax=1.0):
+    """FP32 convolution kernel (length nrays) of lowpassino_fbp for t = linspace(tmin, tmax, nrays)."""
+    import numpy
+    g = numpy.zeros(int(nrays), numpy.float32)
+    check(libraft.raft_ss_ramp_kernel(int(nrays), float(tmin), float(tmax),
+                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "raft_ss_ramp_kernel")
+    return g
+
+
+def filter_rows(rows, g):
+    """Circular convolution of each contiguous row of ``rows`` (..., R) with ``g`` (R,)."""
+    r = rows.contiguous()
+    if r.dtype != torch.float32 or g.dtype != torch.float32:
+        raise TypeError("filter_rows needs float32 tensors")
+    R = r.shape[-1]
+    out = torch.empty_like(r)
+    check(libraft.raft_ss_filter_rows(r.data_ptr(), out.data_ptr(), g.contiguous().data_ptr(), int(R),
+                                      int(r.numel() // R), current_stream_handle(r.device)), "raft_ss_filter_rows")
+    return out
+
+
+def phantom(desc, n, corners=DEFAULT_IMG_CORNERS, device=None):
+    """Rasterise an ellipse description (K x 6) into a device float32 (n, n) image."""
+    import numpy
+    d = numpy.ascontiguousarray(desc, dtype=numpy.float64).reshape(-1, 6)
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    out = torch.empty((int(n), int(n)), dtype=torch.float32, device=dev)
+    c = (ctypes.c_double * 4)(*[float(v) for v in corners])
+    with torch.cuda.device(dev):
+        check(libraft.raft_ss_phantom(d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), d.shape[0], int(n), int(n),
+                                      c, out.data_ptr(), current_stream_handle(dev)), "raft_ss_phantom")
+    return out

@@ -1,0 +1,201 @@
+"""GPU parity of the HIP slant-stack path against the FP64 oracle.
+
+Inputs are FP32 arrays; the oracle receives the same values cast to FP64.  Bars:
+relative L2 <= 1e-5 for Radon, backprojection, FBP and EM,
+adjoint mismatch <= 1e-6 for the exact transpose, bit-exact plan ranges.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL_L2 = 1e-5
+ADJOINT = 1e-6
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _plan(n, R, A, ic=(-1.0, 1.0, 1.0, -1.0), sc=(0.0, 1.0, math.pi, -1.0)):
+    from pyraft_amd.device import Plan
+    return Plan(n, R, A, ic, sc)
+
+
+def _geo(ic, sc):
+    return dict(img_tl=(ic[0], ic[1]), img_br=(ic[2], ic[3]), sino_tl=(sc[0], sc[1]), sino_br=(sc[2], sc[3]))
+
+
+def _ellipse_image(n, seed, count=32):
+    from pyraft_amd.phantom import random_ellipses
+    desc = random_ellipses(count, seed)
+    return O.phantom_fromdesc(desc, n).astype(np.float32)
+
+
+GEOMS = [
+    # n, R, A, img corners, sino corners
+    (64, 64, 45, (-1.0, 1.0, 1.0, -1.0), (0.0, 1.0, math.pi, -1.0)),
+    (65, 70, 37, (-1.0, 1.0, 1.0, -1.0), (0.0, 1.0, math.pi, -1.0)),
+    (48, 40, 91, (-0.7, 1.3, 1.2, -0.9), (-math.pi / 2, 1.1, math.pi / 2, -0.8)),
+    (256, 256, 180, (-1.0, 1.0, 1.0, -1.0), (0.0, 1.0, math.pi, -1.0)),
+]
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: "n%d_R%d_A%d" % g[:3])
+def test_plan_bounds_match_host(torch_mod, geom):
+    import ctypes
+    from pyraft_amd.raftypes import libraft
+    n, R, A, ic, sc = geom
+    print("plan", geom, flush=True)
+    p = _plan(n, R, A, ic, sc)
+    dev = p.bounds()
+    host = np.zeros((A, R, 2), np.int32)
+    rc = libraft.raft_ss_plan_host_bounds(n, R, A, (ctypes.c_double * 4)(*ic), (ctypes.c_double * 4)(*sc),
+                                          host.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None)
+    assert rc == 0
+    assert np.array_equal(dev, host)
+    # per 64-ray group: union / intersection over the non-empty rays
+    wb = p.wave_bounds()
+    for j in range(A):
+        for g in range((R + 63) // 64):
+            rows = host[j, g * 64:(g + 1) * 64]
+            ne = rows[:, 1] >= rows[:, 0]
+            if not ne.any():
+                expect = (0, -1, 0, -1)
+            else:
+                expect = (rows[ne, 0].min(), rows[ne, 1].max(), rows[ne, 0].max(), rows[ne, 1].min())
+            assert tuple(wb[j, g]) == tuple(expect), (j, g)
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: "n%d_R%d_A%d" % g[:3])
+def test_radon_parity(torch_mod, geom):
+    torch = torch_mod
+    n, R, A, ic, sc = geom
+    print("radon", geom, flush=True)
+    rng = np.random.default_rng(11)
+    x = rng.random((n, n)).astype(np.float32)
+    p = _plan(n, R, A, ic, sc)
+    got = p.radon(torch.from_numpy(x).cuda()).cpu().numpy()
+    ref = O.radon(x.astype(np.float64), R, A, **_geo(ic, sc))
+    err = rel_l2(got, ref)
+    print("rel-L2", err)
+    assert err <= REL_L2
+    # angle-major output is the transpose
+    gotT = p.radon(torch.from_numpy(x).cuda(), angle_major=True).cpu().numpy()
+    assert np.array_equal(gotT, got.T)
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: "n%d_R%d_A%d" % g[:3])
+def test_backprojection_parity(torch_mod, geom):
+    torch = torch_mod
+    n, R, A, ic, sc = geom
+    print("bp", geom, flush=True)
+    rng = np.random.default_rng(12)
+    s = rng.random((R, A)).astype(np.float32)
+    p = _plan(n, R, A, ic, sc)
+    got = p.backprojection(torch.from_numpy(s).cuda()).cpu().numpy()
+    ref = O.backprojection(s.astype(np.float64), n, **_geo(ic, sc))
+    err = rel_l2(got, ref)
+    print("rel-L2", err)
+    assert err <= REL_L2
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: "n%d_R%d_A%d" % g[:3])
+def test_transpose_adjoint(torch_mod, geom):
+    torch = torch_mod
+    n, R, A, ic, sc = geom
+    print("transpose", geom, flush=True)
+    rng = np.random.default_rng(13)
+    x = rng.random((n, n)).astype(np.float32)
+    y = rng.random((R, A)).astype(np.float32)
+    p = _plan(n, R, A, ic, sc)
+    Rx = p.radon(torch.from_numpy(x).cuda()).cpu().numpy().astype(np.float64)
+    Rty = p.radon_transpose(torch.from_numpy(y).cuda()).cpu().numpy().astype(np.float64)
+    lhs = float(np.sum(Rx * y.astype(np.float64)))
+    rhs = float(np.sum(x.astype(np.float64) * Rty))
+    mism = abs(lhs - rhs) / abs(lhs)
+    print("adjoint mismatch", mism)
+    assert mism <= ADJOINT
+    ref = O.radon_transpose(y.astype(np.float64), n, **_geo(ic, sc))
+    assert rel_l2(Rty, ref) <= REL_L2
+
+
+def test_radon_random_ellipses_512(torch_mod):
+    torch = torch_mod
+    n = R = 512
+    A = 360
+    print("ellipses 512", flush=True)
+    x = _ellipse_image(n, 2)
+    p = _plan(n, R, A)
+    got = p.radon(torch.from_numpy(x).cuda()).cpu().numpy()
+    ref = O.radon(x.astype(np.float64), R, A)
+    err = rel_l2(got, ref)
+    print("radon rel-L2", err)
+    assert err <= REL_L2
+    bp = p.backprojection(torch.from_numpy(ref.astype(np.float32)).cuda()).cpu().numpy()
+    bref = O.backprojection(ref.astype(np.float32).astype(np.float64), n)
+    err = rel_l2(bp, bref)
+    print("bp rel-L2", err)
+    assert err <= REL_L2
+
+
+def test_fbp_parity(torch_mod):
+    torch = torch_mod
+    n = R = 256
+    A = 180
+    print("fbp", flush=True)
+    x = _ellipse_image(n, 7)
+    s = O.radon(x.astype(np.float64), R, A).astype(np.float32)
+    p = _plan(n, R, A)
+    filt = p.fbp_filter(torch.from_numpy(s).cuda()).cpu().numpy()
+    fref = O.lowpassino_fbp(s.astype(np.float64))
+    assert rel_l2(filt, fref) <= REL_L2
+    got = p.fbp(torch.from_numpy(s).cuda()).cpu().numpy()
+    ref = O.fbp(s.astype(np.float64), n)
+    err = rel_l2(got, ref)
+    print("fbp rel-L2", err)
+    assert err <= REL_L2
+
+
+@pytest.mark.parametrize("nsubsets", [1, 3])
+def test_em_parity(torch_mod, nsubsets):
+    torch = torch_mod
+    n = R = 96
+    A = 60
+    niter = 4
+    print("em", nsubsets, flush=True)
+    x = _ellipse_image(n, 3, count=12)
+    p_true = O.radon(x.astype(np.float64), R, A)
+    s = (p_true * 50.0).astype(np.float32)
+    p = _plan(n, R, A)
+    got, ll = p.em(torch.from_numpy(s).cuda(), niter, nsubsets, return_ll=True)
+    got = got.cpu().numpy()
+    ref, llref = O.osem(s.astype(np.float64), n, niter, nsubsets)
+    err = rel_l2(got, ref)
+    print("em rel-L2", err, "ll", ll, llref)
+    assert err <= REL_L2
+    for a, b in zip(ll, llref):
+        assert abs(a - b) <= 1e-5 * abs(b)
+
+
+def test_phantom_device(torch_mod):
+    from pyraft_amd.device import phantom
+    from pyraft_amd.phantom import random_ellipses
+    desc = random_ellipses(32, 5)
+    got = phantom(desc, 200).cpu().numpy()
+    ref = O.phantom_fromdesc(desc, 200)
+    assert np.array_equal(got, ref.astype(np.float32))
