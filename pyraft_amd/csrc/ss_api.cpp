@@ -407,7 +407,7 @@ extern "C" int raft_ss_backprojection_range(const raft_ss_plan* p, const float* 
     HIP_TRY(ss_launch_sino_to_t(d_sino, ws.as<float>(), p->g.nrays, p->g.nangles, nslices, stream));
     sT = ws.as<float>();
   }
-  SSEmUpdateArgs none{nullptr, nullptr};
+  SSEmUpdateArgs none{nullptr, nullptr, nullptr, nullptr};
   HIP_TRY(ss_launch_backprojection(dp, sT, j0, j1, nslices, SS_BP_PLAIN, d_image, none, stream));
   return 0;
 }
@@ -480,7 +480,7 @@ extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image,
   float* t1 = t0 + total;
   HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
   HIP_TRY(ss_launch_ramp_filter(t0, t1, p->d_ramp, R, nslices * A, stream));
-  SSEmUpdateArgs none{nullptr, nullptr};
+  SSEmUpdateArgs none{nullptr, nullptr, nullptr, nullptr};
   HIP_TRY(ss_launch_backprojection(p->dev(), t1, 0, A, nslices, SS_BP_PLAIN, d_image, none, stream));
   return 0;
 }
@@ -502,14 +502,15 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   const size_t sino_sz = (size_t)nslices * R * A;
   const size_t img_sz = (size_t)nslices * n * n;
   const size_t layout = (size_t)nslices * n * p->g.pitch;
-  // partial buffer large enough for the biggest subset
-  int max_partials = 0;
+  // LL partials of all subsets of one iteration, reduced once per iteration
+  std::vector<int> poff(nsubsets + 1, 0);
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-    max_partials = std::max(max_partials, ss_radon_partials(dp, j0, j1, nslices));
+    poff[b + 1] = poff[b] + ss_radon_partials(dp, j0, j1, nslices);
   }
+  const int npartials = poff[nsubsets];
   const size_t bytes = sizeof(float) * (2 * sino_sz + (size_t)nsubsets * img_sz + 2 * layout) +
-                       sizeof(double) * ((size_t)max_partials + (size_t)std::max(niter, 1) + 8);
+                       sizeof(double) * ((size_t)npartials + (size_t)std::max(niter, 1) + 8);
   DevBuf ws;
   HIP_TRY(ws.alloc(bytes, stream));
   float* measT = ws.as<float>();
@@ -519,28 +520,29 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   float* gB = gA + layout;
   double* partials = reinterpret_cast<double*>(gB + layout);
   partials = reinterpret_cast<double*>(((uintptr_t)partials + 7) & ~(uintptr_t)7);
-  double* d_ll = partials + max_partials;
+  double* d_ll = partials + npartials;
 
   HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
   // W_b = BP_b(1) with the same backprojection kernel
   HIP_TRY(ss_launch_fill(rhoT, 1.0f, sino_sz, stream));
-  SSEmUpdateArgs none{nullptr, nullptr};
+  SSEmUpdateArgs none{nullptr, nullptr, nullptr, nullptr};
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
     HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
                                      stream));
   }
   if (niter > 0) HIP_TRY(hipMemsetAsync(d_ll, 0, sizeof(double) * niter, stream));
+  // gather layouts of x0; afterwards every update epilogue refreshes them in place
+  HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
   for (int it = 0; it < niter; ++it) {
     for (int b = 0; b < nsubsets; ++b) {
       const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-      HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
-      SSEmRatioArgs ea{measT, partials};
+      SSEmRatioArgs ea{measT, partials + poff[b]};
       HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, stream));
-      HIP_TRY(ss_launch_sum_partials(partials, ss_radon_partials(dp, j0, j1, nslices), d_ll + it, stream));
-      SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
+      SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz, gA, gB};
       HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, stream));
     }
+    HIP_TRY(ss_launch_sum_partials(partials, npartials, d_ll + it, stream));
   }
   if (ll_host && niter > 0) {
     HIP_TRY(hipMemcpyAsync(ll_host, d_ll, sizeof(double) * niter, hipMemcpyDeviceToHost, stream));

@@ -54,6 +54,8 @@ int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices);
 struct SSEmUpdateArgs {
   float* x;            // image updated in place
   const float* weight; // BP_b(1)
+  float* gatherA;      // optional: the updated image is also written into the Radon
+  float* gatherB;      //   gather layouts (same layout as ss_launch_gather_layouts)
 };
 hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices, int mode,
                                     float* out, SSEmUpdateArgs em, hipStream_t stream);

@@ -171,7 +171,7 @@ hipError_t ss_launch_t_to_sino(const float* sinoT, float* sino, int R, int A, in
 // =====================================================================================
 // forward: ray-driven slant-stack Radon
 //
-// One wave = 64 consecutive rays of one angle; a block = 4 angles.  Sample s walks
+// One wave = 64 consecutive rays of one angle.  Sample s walks
 // gather rows (image rows for branch B, image columns for branch A, stored
 // transposed), so at each s the 64 lanes read a contiguous stretch of one padded
 // row.  Loop bounds come from the plan's exact FP64 ranges: predicated head and
@@ -209,21 +209,34 @@ __device__ __forceinline__ void ss_radon_store(const SSDevPlan& p, int slice, in
   }
 }
 
+// One (angle, 64-ray group) is handled by `split` waves, each taking a contiguous
+// share of the group's sample range; the shares are summed through LDS.  A block
+// is 8 waves = 8/split angles.  split grows when the launch is too small to fill
+// the chip (few angles, e.g. one OSEM subset).
+#define RADON_WAVES 8
+
 template <int MODE>
-__global__ __launch_bounds__(256) void k_radon(SSDevPlan p, const float* __restrict__ gA,
-                                               const float* __restrict__ gB, int j0, int j1,
+__global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float* __restrict__ gA,
+                                               const float* __restrict__ gB, int j0, int j1, int split,
                                                float* __restrict__ out, SSEmRatioArgs em) {
-  __shared__ double red[4];
+  __shared__ float comb_a[RADON_WAVES][SS_WAVE];
+  __shared__ float comb_d[RADON_WAVES][SS_WAVE];
+  __shared__ double red[RADON_WAVES];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int j = j0 + blockIdx.y * 4 + wv;
+  const int per_block = RADON_WAVES / split;
+  const int aw = wv / split, part = wv - aw * split;
+  const int j = j0 + blockIdx.y * per_block + aw;
   const int slice = blockIdx.z;
   const int R = p.g.nrays, n = p.g.n, pitch = p.g.pitch;
   const int grp = blockIdx.x;
   const int k = grp * SS_WAVE + lane;
-  double ll = 0.0;
-  if (j < j1) {
+  const bool active = j < j1;
+  float acc_a = 0.f, acc_d = 0.f;
+  float delta = 0.f;
+  if (active) {
     const SSAngle& a = p.angles[j];
+    delta = a.delta;
     const int4 wb = p.wbounds[(size_t)j * p.nwaves + grp];
     int lo = 0, hi = -1;
     if (k < R) {
@@ -231,8 +244,13 @@ __global__ __launch_bounds__(256) void k_radon(SSDevPlan p, const float* __restr
       lo = b.x;
       hi = b.y;
     }
+    // this wave's share [sa, sb) of the group's sample range [wb.x, wb.y]
+    const int len = wb.y - wb.x + 1;
+    const int chunk = (len + split - 1) / split;
+    const int sa = wb.x + part * chunk;
+    const int sb = min(wb.y + 1, sa + chunk);
     int64_t beta = a.beta;
-    int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)wb.x * beta;
+    int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)sa * beta;
     if (hi < lo) {
       // an empty ray reads the zero pad with weight 0: exactly 0 in the unpredicated loop
       u = -((int64_t)1 << 32);
@@ -240,42 +258,71 @@ __global__ __launch_bounds__(256) void k_radon(SSDevPlan p, const float* __restr
     }
     const float* img = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch + 1;
     const unsigned span = (unsigned)(hi - lo);
-    float acc_a = 0.f, acc_d = 0.f;
-    int s = wb.x;
-    const int e1 = min(wb.z, wb.y + 1);
-    for (; s < e1; ++s) {
+    int s = sa;
+    // head: predicated (before the intersection of the group's ranges)
+    const int h_end = min(sb, wb.z);
+    for (; s < h_end; ++s) {
       if ((unsigned)(s - lo) <= span) {
         const int m = (int)(u >> 32);
-        const float w = (float)(uint32_t)u;
         const float* row = img + (size_t)s * pitch;
         const float va = row[m], vb = row[m + 1];
         acc_a += va;
-        acc_d = fmaf(w, vb - va, acc_d);
+        acc_d = fmaf((float)(uint32_t)u, vb - va, acc_d);
       }
       u += beta;
     }
-    for (; s <= wb.w; ++s) {
+    // middle: every lane valid; 4 samples (8 loads) in flight per iteration
+    const int m_end = min(sb, wb.w + 1);
+    for (; s + 4 <= m_end; s += 4) {
+      const int64_t u0 = u, u1 = u0 + beta, u2 = u1 + beta, u3 = u2 + beta;
+      u = u3 + beta;
+      const float* r0 = img + (size_t)s * pitch;
+      const int m0 = (int)(u0 >> 32), m1 = (int)(u1 >> 32), m2 = (int)(u2 >> 32), m3 = (int)(u3 >> 32);
+      const float a0 = r0[m0], b0 = r0[m0 + 1];
+      const float a1 = r0[pitch + m1], b1 = r0[pitch + m1 + 1];
+      const float a2 = r0[2 * pitch + m2], b2 = r0[2 * pitch + m2 + 1];
+      const float a3 = r0[3 * pitch + m3], b3 = r0[3 * pitch + m3 + 1];
+      acc_a += (a0 + a1) + (a2 + a3);
+      acc_d = fmaf((float)(uint32_t)u0, b0 - a0, acc_d);
+      acc_d = fmaf((float)(uint32_t)u1, b1 - a1, acc_d);
+      acc_d = fmaf((float)(uint32_t)u2, b2 - a2, acc_d);
+      acc_d = fmaf((float)(uint32_t)u3, b3 - a3, acc_d);
+    }
+    for (; s < m_end; ++s) {
       const int m = (int)(u >> 32);
-      const float w = (float)(uint32_t)u;
       const float* row = img + (size_t)s * pitch;
       const float va = row[m], vb = row[m + 1];
       acc_a += va;
-      acc_d = fmaf(w, vb - va, acc_d);
+      acc_d = fmaf((float)(uint32_t)u, vb - va, acc_d);
       u += beta;
     }
-    for (; s <= wb.y; ++s) {
+    // tail: predicated
+    for (; s < sb; ++s) {
       if ((unsigned)(s - lo) <= span) {
         const int m = (int)(u >> 32);
-        const float w = (float)(uint32_t)u;
         const float* row = img + (size_t)s * pitch;
         const float va = row[m], vb = row[m + 1];
         acc_a += va;
-        acc_d = fmaf(w, vb - va, acc_d);
+        acc_d = fmaf((float)(uint32_t)u, vb - va, acc_d);
       }
       u += beta;
     }
-    const float val = fmaf(acc_d, SS_INV_FIX, acc_a) * a.delta;
-    if (k < R) ss_radon_store<MODE>(p, slice, j, k, val, out, em, &ll);
+  }
+  if (split > 1) {
+    comb_a[wv][lane] = acc_a;
+    comb_d[wv][lane] = acc_d;
+    __syncthreads();
+    if (part == 0) {
+      for (int q = 1; q < split; ++q) {
+        acc_a += comb_a[wv + q][lane];
+        acc_d += comb_d[wv + q][lane];
+      }
+    }
+  }
+  double ll = 0.0;
+  if (active && part == 0 && k < R) {
+    const float val = fmaf(acc_d, SS_INV_FIX, acc_a) * delta;
+    ss_radon_store<MODE>(p, slice, j, k, val, out, em, &ll);
   }
   if (MODE == SS_OUT_EM_RATIO) {
     const double tot = ss_block_sum_f64(ll, red);
@@ -327,25 +374,37 @@ __global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float* _
   (void)red;
 }
 
+// waves per (angle, ray group): enough to put ~32 waves on each of the 256 CUs
+static int ss_radon_split(const SSDevPlan& p, int j0, int j1, int nslices) {
+  const long long waves = (long long)p.nwaves * (j1 - j0) * nslices;
+  int split = 1;
+  while (split < RADON_WAVES && waves * split < 256LL * 32) split *= 2;
+  return split;
+}
+
 int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices) {
-  const int gy = (j1 - j0 + 3) / 4;
+  const int per_block = RADON_WAVES / ss_radon_split(p, j0, j1, nslices);
+  const int gy = (j1 - j0 + per_block - 1) / per_block;
   return p.nwaves * gy * nslices + p.nirregular * nslices;
 }
 
 hipError_t ss_launch_radon(const SSDevPlan& p, const float* gatherA, const float* gatherB, int j0, int j1,
                            int nslices, int mode, float* out, SSEmRatioArgs em, hipStream_t stream) {
   if (j1 <= j0) return hipSuccess;
-  dim3 grid(p.nwaves, (j1 - j0 + 3) / 4, nslices);
+  const int split = ss_radon_split(p, j0, j1, nslices);
+  const int per_block = RADON_WAVES / split;
+  dim3 grid(p.nwaves, (j1 - j0 + per_block - 1) / per_block, nslices);
+  dim3 block(RADON_WAVES * SS_WAVE);
   const int partial_base = p.nwaves * grid.y * nslices;
   switch (mode) {
     case SS_OUT_SINO:
-      hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out, em);
+      hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, split, out, em);
       break;
     case SS_OUT_SINOT:
-      hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out, em);
+      hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, split, out, em);
       break;
     default:
-      hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, dim3(256), 0, stream, p, gatherA, gatherB, j0, j1, out,
+      hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, split, out,
                          em);
       break;
   }
@@ -511,7 +570,13 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
     } else {
       const float wgt = em.weight[idx];
       const float x = em.x[idx];
-      em.x[idx] = wgt > 0.f ? x * bp / wgt : x;
+      const float xn = wgt > 0.f ? x * bp / wgt : x;
+      em.x[idx] = xn;
+      if (em.gatherB) {
+        const int pitch = p.g.pitch;
+        em.gatherB[((size_t)slice * n + row) * pitch + col + 1] = xn;
+        em.gatherA[((size_t)slice * n + col) * pitch + row + 1] = xn;
+      }
     }
   }
 }
