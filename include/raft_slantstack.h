@@ -122,6 +122,11 @@ int raft_ss_plan_wave_bounds(const raft_ss_plan *plan, int *host_wbounds);
 int raft_ss_plan_host_bounds(int n, int nrays, int nangles, const double *img_corners, const double *sino_corners,
                              int *host_bounds, long long *host_fix, int *host_branch);
 
+/* plan summary computed on the host (no device needed): info[0..4] = #degenerate
+ * angles, bp window, bp angles per stage, 64-ray groups, 0 */
+int raft_ss_plan_host_info(int n, int nrays, int nangles, const double *img_corners, const double *sino_corners,
+                           int *info);
+
 /* ================================================================ device-pointer operators */
 
 /* Slant-stack Radon of nslices images (angle columns [j0, j1) of the sinogram are

@@ -158,6 +158,19 @@ static int validate_geom(const SSGeom& g, const std::vector<SSAngle>& angles) {
   return 0;
 }
 
+// Backprojection staging: rays a 32x32 pixel tile can touch at one angle (+margins
+// for the floor of the tile minimum and the second tap), and angles per LDS stage.
+static int bp_staging(const std::vector<SSAngle>& angles, int* window, int* per_stage) {
+  double span = 0.0;
+  for (const SSAngle& a : angles) span = std::max(span, fabs(a.bp_b) + fabs(a.bp_c));
+  int W = (int)ceil(31.0 * span) + 5;
+  W = (W + 3) & ~3;
+  if (W > 4096) return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)", W);
+  *window = W;
+  *per_stage = std::min(64, 4096 / W);
+  return 0;
+}
+
 extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nangles, const double* img_corners,
                                    const double* sino_corners, void* stream_) {
   if (!out || !img_corners || !sino_corners) return set_error(RAFT_SS_EINVAL, "null argument");
@@ -175,14 +188,7 @@ extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nan
 
   const SSGeom& g = p->g;
   p->nwaves = (nrays + 63) / 64;
-  // backprojection staging window: rays a 32x32 tile can touch at one angle
-  double span = 0.0;
-  for (const SSAngle& a : p->h_angles) span = std::max(span, fabs(a.bp_b) + fabs(a.bp_c));
-  int W = (int)ceil(31.0 * span) + 5;
-  W = (W + 3) & ~3;
-  if (W > 4096) return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)", W);
-  p->bp_window = W;
-  p->bp_angles = std::min(64, 4096 / W);
+  if ((rc = bp_staging(p->h_angles, &p->bp_window, &p->bp_angles))) return rc;
 
   const int capacity = (p->ndegenerate + 2) * nrays + 4096;
   HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSAngle) * nangles));
@@ -269,6 +275,28 @@ extern "C" int raft_ss_plan_host_bounds(int n, int nrays, int nangles, const dou
     }
     if (host_branch) host_branch[j] = angles[j].branch_a;
   }
+  return 0;
+}
+
+// Host-side plan summary without a device: info[0..4] = #degenerate angles,
+// bp window, bp angles per stage, 64-ray groups, plan status.
+extern "C" int raft_ss_plan_host_info(int n, int nrays, int nangles, const double* img_corners,
+                                      const double* sino_corners, int* info) {
+  if (!img_corners || !sino_corners || !info) return set_error(RAFT_SS_EINVAL, "null argument");
+  SSGeom g = make_geom(n, nrays, nangles, img_corners, sino_corners);
+  std::vector<SSAngle> angles;
+  int ndeg = 0;
+  if (n < 2 || nrays < 2 || nangles < 1) return validate_geom(g, angles);
+  build_angles(g, angles, &ndeg);
+  int rc = validate_geom(g, angles);
+  if (rc) return rc;
+  int W = 0, J = 0;
+  if ((rc = bp_staging(angles, &W, &J))) return rc;
+  info[0] = ndeg;
+  info[1] = W;
+  info[2] = J;
+  info[3] = (nrays + 63) / 64;
+  info[4] = 0;
   return 0;
 }
 

@@ -1,0 +1,112 @@
+"""Emission EM and OSEM (pyraft/iterative/em.py).
+
+All variants run the multiplicative recurrence of raft/raft_em_gpu.cu:41-69,
+x <- x * BP_b(s / R_b x) / BP_b(1), with the slant-stack Radon R and the slant-stack
+backprojection BP, as fused HIP kernels: the forward projection writes the
+guarded ratio (0 where R x <= 0; the reference divides unguarded,
+raft/raft_cuda_aux.cu:21) and FP64 log-likelihood partials, the backprojection
+applies the update in its epilogue.  OSEM subset b covers the contiguous angle
+columns [b*A/S, (b+1)*A/S) (pyraft/iterative/em.py:284-312) on the full
+sinogram's angle grid.  Pixels with BP_b(1) = 0 keep their value.
+
+The reference's CPU ``em``/``make_em`` default to the Bresenham pair and its
+``osem`` to the NFFT slice pair; those projectors are not part of this build.
+"""
+import ctypes
+import math
+
+import numpy
+
+from .raftypes import check_legacy, image, libraft
+from .sinogram import radon_method
+
+__all__ = ["make_em", "em", "em_gpu", "osem"]
+
+
+def _corners(arr, default_tl, default_br):
+    tl = getattr(arr, 'top_left', None) or default_tl
+    br = getattr(arr, 'bottom_right', None) or default_br
+    return (float(tl[0]), float(tl[1]), float(br[0]), float(br[1]))
+
+
+def _sino_corners(sino):
+    return _corners(sino, (0.0, 1.0), (math.pi, -1.0))
+
+
+def _plan_for(x_shape, x_corners, sino):
+    from .device import Plan
+    if x_shape[0] != x_shape[1]:
+        raise ValueError("slant-stack EM needs a square image (got %s)" % (tuple(x_shape),))
+    return Plan(x_shape[0], sino.shape[0], sino.shape[1], x_corners, _sino_corners(sino))
+
+
+def _device(a):
+    import torch
+    return torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(a, dtype=numpy.float32))).cuda()
+
+
+def make_em(sino, shape, projection_method=radon_method.SLANTSTACK, backprojection_method=radon_method.SLANTSTACK,
+            weight=None, **kwargs):
+    """One-iteration EM closure (pyraft/iterative/em.py:109-144) on the slant-stack pair."""
+    for m in (projection_method, backprojection_method):
+        if m != radon_method.SLANTSTACK:
+            raise NotImplementedError("make_em: only radon_method.SLANTSTACK is part of this build")
+    if weight is not None:
+        raise NotImplementedError("make_em: the weight is BP(1) of the slant-stack backprojection")
+    if isinstance(shape, image):
+        xs, xc = shape.shape, _corners(shape, (-1.0, 1.0), (1.0, -1.0))
+    else:
+        tmp = image(shape, **kwargs)
+        xs, xc = tmp.shape, _corners(tmp, (-1.0, 1.0), (1.0, -1.0))
+    plan = _plan_for(xs, xc, sino)
+    s_dev = _device(sino)
+
+    def em_iter(x):
+        out = plan.em(s_dev, 1, 1, x0=_device(x)).cpu().numpy().astype(numpy.float64)
+        return image(out, top_left=(xc[0], xc[1]), bottom_right=(xc[2], xc[3]))
+
+    return em_iter
+
+
+def em(sino, shape=None, niter=5, **kwargs):
+    """EM from ones (or from ``shape`` when it is an image), pyraft/iterative/em.py:151-169."""
+    if shape is None:
+        shape = [sino.shape[1], sino.shape[1]]
+    if isinstance(shape, image):
+        if kwargs:
+            raise TypeError('Unhandled arguments!')
+        x = shape
+    else:
+        x = image(shape, **kwargs)
+        x[:] = 1.0
+    xc = _corners(x, (-1.0, 1.0), (1.0, -1.0))
+    plan = _plan_for(x.shape, xc, sino)
+    out = plan.em(_device(sino), int(niter), 1, x0=_device(x)).cpu().numpy().astype(numpy.float64)
+    return image(out, top_left=(xc[0], xc[1]), bottom_right=(xc[2], xc[3]))
+
+
+def em_gpu(*args):
+    """em_gpu(s, img_size=R, niter=20) -> float32 (pyraft/iterative/em.py:176-206)."""
+    if len(args) == 0:
+        raise TypeError('Not enough arguments!')
+    s = args[0]
+    rays, angles = s.shape[0], s.shape[1]
+    img_size = int(args[1]) if len(args) > 1 else rays
+    niter = int(args[2]) if len(args) > 2 else 20
+    sino_buff = numpy.ascontiguousarray(s, dtype=numpy.float32)
+    img = numpy.zeros(img_size * img_size, dtype=numpy.float32)
+    libraft.raft_em_gpu(img.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                        sino_buff.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), img_size, rays, angles, niter)
+    check_legacy("raft_em_gpu")
+    return img.reshape([img_size, img_size])
+
+
+def osem(sino, shape=None, niter=5, nblocks=4, **kwargs):
+    """OSEM with ``nblocks`` contiguous angle subsets from x = 1 on an R x R image
+    (pyraft/iterative/em.py:265-320; ``shape`` selects another square size)."""
+    n = sino.shape[0] if shape is None else int(shape[0])
+    x = image((n, n), **kwargs)
+    xc = _corners(x, (-1.0, 1.0), (1.0, -1.0))
+    plan = _plan_for(x.shape, xc, sino)
+    out = plan.em(_device(sino), int(niter), int(nblocks)).cpu().numpy().astype(numpy.float64)
+    return image(out, top_left=(xc[0], xc[1]), bottom_right=(xc[2], xc[3]))

@@ -12,7 +12,9 @@ import math
 
 import numpy
 
-from .raftypes import image, libraft, make_RAFT_IMAGE, make_RAFT_MATRIX
+from .raftypes import check_legacy, image, libraft, make_RAFT_IMAGE, make_RAFT_MATRIX
+
+__all__ = ["phantom", "shepplogan", "random_ellipses"]
 
 
 def _shepp_logan_desc():
@@ -38,6 +40,7 @@ def phantom(desc, shape=(256, 256), **kwargs):
     img = image(shape, **kwargs)
     libraft.raft_image_phantom_fromdesc(make_RAFT_IMAGE(img, img.top_left, img.bottom_right),
                                         make_RAFT_MATRIX(desc))
+    check_legacy("raft_image_phantom_fromdesc")
     return img
 
 

@@ -20,6 +20,10 @@ import sysconfig
 
 import numpy
 
+__all__ = ["nthreads", "libraft", "LIBRARY_PATH", "RAFT_MATRIX", "RAFT_IMAGE", "RAFT_VECTOR", "RaftError",
+           "check", "check_legacy", "image", "make_RAFT_MATRIX", "make_RAFT_IMAGE", "make_RAFT_VECTOR",
+           "EXPORTED_SYMBOLS"]
+
 nthreads = multiprocessing.cpu_count()
 
 _LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
@@ -93,6 +97,7 @@ _decl("raft_ss_plan_bounds", [_vp, ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_wave_bounds", [_vp, ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_host_bounds", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i),
                                    ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_i)], _i)
+_decl("raft_ss_plan_host_info", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i)], _i)
 _decl("raft_ss_radon", [_vp, _vp, _vp, _i, _vp], _i)
 _decl("raft_ss_radon_range", [_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i)
 _decl("raft_ss_backprojection", [_vp, _vp, _vp, _i, _vp], _i)
@@ -265,7 +270,7 @@ EXPORTED_SYMBOLS = (
     "raft_radon_slantstack", "raft_backprojection_slantstack", "raft_radon_slantstack_gpu",
     "raft_backprojection_slantstack_gpu", "raft_em_gpu", "raft_image_phantom_fromdesc",
     "raft_radon_fromdesc", "raft_radon_transpose_slantstack", "raft_ss_legacy_status", "raft_ss_last_error", "raft_ss_version", "raft_ss_plan_create",
-    "raft_ss_plan_destroy", "raft_ss_plan_info", "raft_ss_plan_bounds", "raft_ss_plan_wave_bounds", "raft_ss_plan_host_bounds",
+    "raft_ss_plan_destroy", "raft_ss_plan_info", "raft_ss_plan_bounds", "raft_ss_plan_wave_bounds", "raft_ss_plan_host_bounds", "raft_ss_plan_host_info",
     "raft_ss_radon", "raft_ss_radon_range", "raft_ss_backprojection", "raft_ss_backprojection_range",
     "raft_ss_radon_transpose", "raft_ss_radon_transpose_range", "raft_ss_fbp_filter", "raft_ss_fbp",
     "raft_ss_em", "raft_ss_phantom", "raft_ss_ramp_kernel", "raft_ss_filter_rows",
