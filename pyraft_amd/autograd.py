@@ -18,7 +18,7 @@ import torch.utils.checkpoint
 
 from .device import Plan
 
-__all__ = ["radon", "radon_transpose", "RadonOperator", "tv_smooth", "unrolled_ls_tv"]
+__all__ = ["radon", "radon_transpose", "RadonOperator", "tv_smooth", "operator_norm_sq", "unrolled_ls_tv"]
 
 
 class _Radon(torch.autograd.Function):
@@ -87,6 +87,19 @@ def _tv_grad(x, eps):
     return g
 
 
+def operator_norm_sq(plan, iters=12, angle_major=False, device=None):
+    """||R||_2^2 by power iteration on R^T R (a lower bound that converges from below)."""
+    with torch.no_grad():
+        v = torch.rand((plan.n, plan.n), device=device or plan.device, dtype=torch.float32)
+        lam = 0.0
+        for _ in range(iters):
+            v = v / v.norm()
+            w = plan.radon_transpose(plan.radon(v, angle_major=angle_major), angle_major=angle_major)
+            lam = float((w * v).sum())
+            v = w
+    return lam
+
+
 def unrolled_ls_tv(y, plan, steps=20, step_size=None, lam=1e-3, eps=1e-8, x0=None, checkpoint=True,
                    angle_major=False):
     """Unrolled gradient descent on 0.5||Rx - y||^2 + lam*TV_eps(x).
@@ -98,12 +111,7 @@ def unrolled_ls_tv(y, plan, steps=20, step_size=None, lam=1e-3, eps=1e-8, x0=Non
     if x0 is None:
         x0 = torch.zeros(lead + (plan.n, plan.n), dtype=y.dtype, device=y.device)
     if step_size is None:
-        # 1 / ||R||^2 estimate from one power iteration on a random image
-        with torch.no_grad():
-            v = torch.rand((plan.n, plan.n), device=y.device, dtype=y.dtype)
-            v = plan.radon_transpose(plan.radon(v, angle_major=angle_major), angle_major=angle_major)
-            v = plan.radon_transpose(plan.radon(v / v.norm(), angle_major=angle_major), angle_major=angle_major)
-            step_size = 1.0 / float(v.norm())
+        step_size = 0.95 / operator_norm_sq(plan, angle_major=angle_major, device=y.device)
     losses = []
 
     def step(x, y_, alpha, lam_):

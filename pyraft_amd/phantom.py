@@ -14,7 +14,7 @@ import numpy
 
 from .raftypes import check_legacy, image, libraft, make_RAFT_IMAGE, make_RAFT_MATRIX
 
-__all__ = ["phantom", "shepplogan", "random_ellipses"]
+__all__ = ["ellipses_phantom", "shepplogan", "random_ellipses"]
 
 
 def _shepp_logan_desc():
@@ -32,7 +32,7 @@ def _shepp_logan_desc():
                         [0.1, 0.0230, 0.046, 0.06, -0.605, 0.0]])
 
 
-def phantom(desc, shape=(256, 256), **kwargs):
+def ellipses_phantom(desc, shape=(256, 256), **kwargs):
     """Rasterise an ellipse description into a float64 ``image`` (additive)."""
     desc = numpy.ascontiguousarray(desc, dtype=numpy.float64)
     if desc.ndim != 2 or desc.shape[1] < 6:
@@ -46,7 +46,7 @@ def phantom(desc, shape=(256, 256), **kwargs):
 
 def shepplogan(shape=(256, 256), **kwargs):
     """Shepp-Logan head phantom (pyraft/phantom/phantom.py:33-52)."""
-    return phantom(_shepp_logan_desc(), shape, **kwargs)
+    return ellipses_phantom(_shepp_logan_desc(), shape, **kwargs)
 
 
 def random_ellipses(count=32, seed=0):

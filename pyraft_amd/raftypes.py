@@ -20,7 +20,7 @@ import sysconfig
 
 import numpy
 
-__all__ = ["nthreads", "libraft", "LIBRARY_PATH", "RAFT_MATRIX", "RAFT_IMAGE", "RAFT_VECTOR", "RaftError",
+__all__ = ["nthreads", "libraft", "LIBRARY_PATH", "HIP_RUNTIME", "RAFT_MATRIX", "RAFT_IMAGE", "RAFT_VECTOR", "RaftError",
            "check", "check_legacy", "image", "make_RAFT_MATRIX", "make_RAFT_IMAGE", "make_RAFT_VECTOR",
            "EXPORTED_SYMBOLS"]
 
@@ -33,8 +33,37 @@ LIBRARY_PATH = os.path.join(_LIBDIR, "libraft" + _EXT)
 if not os.path.exists(LIBRARY_PATH):
     raise ImportError(
         "pyraft_amd: native library %s is not built; run __graft_entry__.build() "
-        "(or python -m pyraft_amd.build)" % LIBRARY_PATH)
+        "(or python pyraft_amd/build.py)" % LIBRARY_PATH)
 
+
+def _preload_hip_runtime():
+    """One HIP runtime per process.
+
+    ROCm PyTorch wheels carry their own libamdhip64 / libhsa-runtime64 and link them
+    by the bare file name, which does not match the SONAME of the system copy that
+    libraft links.  If libraft were loaded first, importing torch afterwards would
+    bring up a second HIP + HSA runtime and one of them would find no device.  Loading
+    torch's copy first makes libraft's libamdhip64.so.7 dependency resolve to it (same
+    SONAME), whatever the import order.  PYRAFT_AMD_HIP_RUNTIME selects another file."""
+    cands = []
+    if os.environ.get("PYRAFT_AMD_HIP_RUNTIME"):
+        cands.append(os.environ["PYRAFT_AMD_HIP_RUNTIME"])
+    else:
+        try:
+            import importlib.util
+            spec = importlib.util.find_spec("torch")
+            for d in (spec.submodule_search_locations or []) if spec else []:
+                cands.append(os.path.join(d, "lib", "libamdhip64.so"))
+        except (ImportError, ValueError):
+            pass
+    for c in cands:
+        if os.path.exists(c):
+            ctypes.CDLL(c, mode=ctypes.RTLD_GLOBAL)
+            return c
+    return None
+
+
+HIP_RUNTIME = _preload_hip_runtime()
 libraft = ctypes.CDLL(LIBRARY_PATH)
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)

@@ -1,0 +1,123 @@
+"""The pyraft-compatible entry points on the GPU against the FP64 oracle.
+
+Covers the reference's CPU-runnable workload (256 x 256 Shepp-Logan, 180 angles,
+256 bins: Radon, backprojection, FBP) and every drop-in symbol: radon/backprojection
+through raft_image structs (including strided views), radon_gpu, backprojection_gpu,
+fbp_gpu, em_gpu, lowpassino_fbp, radon_transpose, em/make_em/osem.
+Tolerance: relative L2 <= 1e-5 against the oracle on the same (FP32-rounded) inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+REL_L2 = 1e-5
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def pr():
+    import torch
+    assert torch.cuda.is_available()
+    import pyraft_amd
+    return pyraft_amd
+
+
+def _shepp_logan_table():
+    import importlib
+    return importlib.import_module("pyraft_amd.phantom")._shepp_logan_desc()
+
+
+def _f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def test_shepp_logan_256_radon_bp_fbp(pr):
+    print("C1 shepp-logan", flush=True)
+    f = pr.shepplogan((256, 256))
+    assert np.array_equal(f, O.phantom_fromdesc(_shepp_logan_table(), 256))
+    s = pr.radon(f, (256, 180), method=pr.radon_method.SLANTSTACK)
+    assert s.top_left == (0.0, 1.0) and s.bottom_right == (math.pi, -1.0)
+    s_ref = O.radon(_f32(f), 256, 180)
+    e = rel_l2(s, s_ref)
+    print("radon", e)
+    assert e <= REL_L2
+    plan = pr.backprojection_plan(s.shape, (256, 256), method=pr.radon_method.SLANTSTACK)
+    b = pr.backprojection(s, plan)
+    e = rel_l2(b, O.backprojection(_f32(s), 256))
+    print("backprojection", e)
+    assert e <= REL_L2
+    rec = pr.fbp(s)
+    e = rel_l2(rec, O.fbp(_f32(s), 256))
+    print("fbp", e)
+    assert e <= REL_L2
+    filt = pr.lowpassino_fbp(s)
+    assert filt.top_left == s.top_left
+    assert rel_l2(filt, O.lowpassino_fbp(_f32(s))) <= REL_L2
+
+
+def test_strided_and_flipped_views(pr):
+    rng = np.random.default_rng(3)
+    base = rng.random((96, 96))
+    view = pr.image(base)[::-1, :]  # negative column stride through raft_matrix
+    view.top_left, view.bottom_right = (-1.0, 1.0), (1.0, -1.0)
+    s = pr.radon(view, (96, 40), method=pr.radon_method.SLANTSTACK)
+    assert rel_l2(s, O.radon(_f32(np.ascontiguousarray(view)), 96, 40)) <= REL_L2
+    sF = np.asfortranarray(np.asarray(s))  # column-major sinogram (line_stride = R)
+    assert not sF.flags.c_contiguous
+    b = pr.backprojection(sF, (pr.radon_method.SLANTSTACK, (96, 96), 2))
+    assert rel_l2(b, O.backprojection(_f32(s), 96)) <= REL_L2
+
+
+def test_legacy_float_entry_points(pr):
+    n, R, A = 128, 128, 90
+    f = pr.ellipses_phantom(pr.random_ellipses(20, 4), (n, n))
+    s = pr.radon_gpu(np.asarray(f), R, A)
+    assert s.top_left == (0, 1.0) and s.bottom_right == (math.pi, -1.0)
+    assert rel_l2(s, O.radon(_f32(f), R, A)) <= REL_L2
+    b = pr.backprojection_gpu(np.asarray(s), (n, n))
+    assert b.dtype == np.float64
+    assert rel_l2(b, O.backprojection(_f32(s), n)) <= REL_L2
+    fb = pr.fbp_gpu(s)
+    assert fb.dtype == np.float32 and fb.shape == (n, n)
+    assert rel_l2(fb, O.fbp(_f32(s), n)) <= REL_L2
+    em = pr.em_gpu(np.asarray(s) * 100.0, n, 4)
+    assert em.dtype == np.float32
+    ref, _ = O.em(_f32(np.asarray(s) * 100.0), n, 4)
+    assert rel_l2(em, ref) <= REL_L2
+
+
+def test_radon_transpose_and_fromdesc(pr):
+    n, R, A = 80, 90, 50
+    rng = np.random.default_rng(8)
+    y = pr.image(rng.random((R, A)), x_extent=(0.0, math.pi))
+    g = pr.radon_transpose(y, (n, n), method=pr.radon_method.SLANTSTACK)
+    assert rel_l2(g, O.radon_transpose(_f32(y), n)) <= REL_L2
+    d = pr.random_ellipses(8, 6)
+    exact = pr.radon(d, (R, A))
+    assert rel_l2(exact, O.radon_fromdesc(d, R, A)) <= 1e-12
+
+
+def test_em_osem_make_em(pr):
+    n = R = 64
+    A = 48
+    f = pr.ellipses_phantom(pr.random_ellipses(10, 9), (n, n))
+    s = pr.radon(f, (R, A), method=pr.radon_method.SLANTSTACK) * 40.0
+    x = pr.em(s, shape=(n, n), niter=3)
+    ref, _ = O.em(_f32(s), n, 3)
+    assert rel_l2(x, ref) <= REL_L2
+    step = pr.make_em(s, (n, n))
+    x1 = step(pr.image(np.ones((n, n))))
+    ref1, _ = O.em(_f32(s), n, 1)
+    assert rel_l2(x1, ref1) <= REL_L2
+    xo = pr.osem(s, niter=2, nblocks=4)
+    refo, _ = O.osem(_f32(s), n, 2, nsubsets=4)
+    assert rel_l2(xo, refo) <= REL_L2
