@@ -38,7 +38,25 @@ def parse():
     ap.add_argument("--angles", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-em", action="store_true", help="skip the EM leg")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="run one 64 MiB device-to-device copy first (reference for HBM byte counters)")
     return ap.parse_args()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_bytes_*2048x1500*.md"))):
+        m = re.search(r"```json\n(.*?)\n```", open(path).read(), flags=re.S)
+        if not m:
+            continue
+        table = json.loads(m.group(1)).get("bytes_per_launch", {})
+        for key, val in table.items():
+            if key.split("/")[0] == kernel:
+                return float(val), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_model():
@@ -117,6 +135,12 @@ def main():
     R = n
     upd = float(n) * n * A
 
+    if args.calibration_copy:
+        src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print("bench: plan n=%d R=%d A=%d" % (n, R, A), file=sys.stderr, flush=True)
     plan = Plan(n, R, A)
     img = phantom(random_ellipses(32, 2), n)
@@ -167,9 +191,10 @@ def main():
     torch.cuda.synchronize()
     fbp_ms = e0.elapsed_time(e1) / 3
 
-    dominant = "k_backprojection" if t_bp >= t_radon else "k_radon"
+    dominant = "k_backprojection<0>" if t_bp >= t_radon else "k_radon<1>"
     t_dom = max(t_bp, t_radon)
     achieved = BYTES_PER_UPDATE * upd / t_dom / 1e9
+    traffic, traffic_src = measured_traffic(dominant)
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -191,8 +216,10 @@ def main():
         "backprojection_ms": round(t_bp * 1e3, 4),
         "fbp_ms": round(fbp_ms, 4),
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "model": "8 B (two FP32 taps) per update x N^2*A updates per launch / mean launch time"},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "model": "8 B (two FP32 taps) per update x N^2*A updates per launch / mean launch time; "
+                              "traffic = measured HBM bytes per launch (copy-calibrated FETCH_SIZE + WRITE_SIZE)"},
     }
 
     if not args.no_em:
