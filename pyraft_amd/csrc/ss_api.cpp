@@ -142,7 +142,7 @@ static int validate_geom(const SSGeom& g, const std::vector<SSAngle>& angles) {
   if (g.n < 2) return set_error(RAFT_SS_EINVAL, "image side must be >= 2 (got %d)", g.n);
   if (g.nrays < 2) return set_error(RAFT_SS_EINVAL, "nrays must be >= 2 (got %d)", g.nrays);
   if (g.nangles < 1) return set_error(RAFT_SS_EINVAL, "nangles must be >= 1 (got %d)", g.nangles);
-  if (g.n > 32768 || g.nrays > (1 << 24) || g.nangles > (1 << 24))
+  if (g.n > 16384 || g.nrays > (1 << 24) || g.nangles > (1 << 24))
     return set_error(RAFT_SS_EINVAL, "geometry too large (n=%d R=%d A=%d)", g.n, g.nrays, g.nangles);
   const double vals[] = {g.xmin, g.ymin, g.dx, g.dy, g.tmin, g.dt, g.th0, g.dth};
   for (double v : vals)
@@ -165,9 +165,9 @@ static int bp_staging(const std::vector<SSAngle>& angles, int* window, int* per_
   for (const SSAngle& a : angles) span = std::max(span, fabs(a.bp_b) + fabs(a.bp_c));
   int W = (int)ceil(31.0 * span) + 5;
   W = (W + 3) & ~3;
-  if (W > 4096) return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)", W);
+  if (W > SS_BP_STAGE_PAIRS) return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)", W);
   *window = W;
-  *per_stage = std::min(64, 4096 / W);
+  *per_stage = std::min(64, SS_BP_STAGE_PAIRS / W);
   return 0;
 }
 
@@ -406,9 +406,9 @@ extern "C" int raft_ss_radon_range(const raft_ss_plan* p, const float* d_image, 
   const SSDevPlan dp = p->dev();
   const size_t layout = (size_t)nslices * p->g.n * p->g.pitch;
   DevBuf ws;
-  HIP_TRY(ws.alloc(2 * layout * sizeof(float), stream));
-  float* gA = ws.as<float>();
-  float* gB = gA + layout;
+  HIP_TRY(ws.alloc(2 * layout * sizeof(float2), stream));
+  float2* gA = ws.as<float2>();
+  float2* gB = gA + layout;
   HIP_TRY(ss_launch_gather_layouts(dp, d_image, gA, gB, nslices, stream));
   SSEmRatioArgs none{nullptr, nullptr};
   HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, angle_major ? SS_OUT_SINOT : SS_OUT_SINO, d_out, none, stream));
@@ -435,7 +435,7 @@ extern "C" int raft_ss_backprojection_range(const raft_ss_plan* p, const float* 
     HIP_TRY(ss_launch_sino_to_t(d_sino, ws.as<float>(), p->g.nrays, p->g.nangles, nslices, stream));
     sT = ws.as<float>();
   }
-  SSEmUpdateArgs none{nullptr, nullptr, nullptr, nullptr};
+  SSEmUpdateArgs none{nullptr, nullptr};
   HIP_TRY(ss_launch_backprojection(dp, sT, j0, j1, nslices, SS_BP_PLAIN, d_image, none, stream));
   return 0;
 }
@@ -508,7 +508,7 @@ extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image,
   float* t1 = t0 + total;
   HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
   HIP_TRY(ss_launch_ramp_filter(t0, t1, p->d_ramp, R, nslices * A, stream));
-  SSEmUpdateArgs none{nullptr, nullptr, nullptr, nullptr};
+  SSEmUpdateArgs none{nullptr, nullptr};
   HIP_TRY(ss_launch_backprojection(p->dev(), t1, 0, A, nslices, SS_BP_PLAIN, d_image, none, stream));
   return 0;
 }
@@ -537,37 +537,36 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
     poff[b + 1] = poff[b] + ss_radon_partials(dp, j0, j1, nslices);
   }
   const int npartials = poff[nsubsets];
-  const size_t bytes = sizeof(float) * (2 * sino_sz + (size_t)nsubsets * img_sz + 2 * layout) +
+  const size_t bytes = sizeof(float2) * 2 * layout + sizeof(float) * (2 * sino_sz + (size_t)nsubsets * img_sz) +
                        sizeof(double) * ((size_t)npartials + (size_t)std::max(niter, 1) + 8);
   DevBuf ws;
   HIP_TRY(ws.alloc(bytes, stream));
-  float* measT = ws.as<float>();
+  float2* gA = ws.as<float2>();
+  float2* gB = gA + layout;
+  float* measT = reinterpret_cast<float*>(gB + layout);
   float* rhoT = measT + sino_sz;
   float* weights = rhoT + sino_sz;
-  float* gA = weights + (size_t)nsubsets * img_sz;
-  float* gB = gA + layout;
-  double* partials = reinterpret_cast<double*>(gB + layout);
+  double* partials = reinterpret_cast<double*>(weights + (size_t)nsubsets * img_sz);
   partials = reinterpret_cast<double*>(((uintptr_t)partials + 7) & ~(uintptr_t)7);
   double* d_ll = partials + npartials;
 
   HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
   // W_b = BP_b(1) with the same backprojection kernel
   HIP_TRY(ss_launch_fill(rhoT, 1.0f, sino_sz, stream));
-  SSEmUpdateArgs none{nullptr, nullptr, nullptr, nullptr};
+  SSEmUpdateArgs none{nullptr, nullptr};
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
     HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
                                      stream));
   }
   if (niter > 0) HIP_TRY(hipMemsetAsync(d_ll, 0, sizeof(double) * niter, stream));
-  // gather layouts of x0; afterwards every update epilogue refreshes them in place
-  HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
   for (int it = 0; it < niter; ++it) {
     for (int b = 0; b < nsubsets; ++b) {
       const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
+      HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
       SSEmRatioArgs ea{measT, partials + poff[b]};
       HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, stream));
-      SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz, gA, gB};
+      SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
       HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, stream));
     }
     HIP_TRY(ss_launch_sum_partials(partials, npartials, d_ll + it, stream));

@@ -4,7 +4,8 @@
 //   image      [slice][n][n]        row-major, row 0 = ymin (bottom)
 //   sino       [slice][R][A]        the reference layout (angle contiguous)
 //   sinoT      [slice][A][R]        angle-major (detector contiguous), internal
-//   gatherA/B  [slice][n][n+2]      zero-padded Radon gather layouts (A: transposed)
+//   gatherA/B  [slice][n][n+2]      Radon gather layouts of float2 (lower tap, difference to the
+//                                   upper tap) per interpolation cell; A: image columns, B: rows
 #ifndef PYRAFT_AMD_SS_KERNELS_H
 #define PYRAFT_AMD_SS_KERNELS_H
 
@@ -17,6 +18,12 @@
 enum { SS_OUT_SINO = 0, SS_OUT_SINOT = 1, SS_OUT_EM_RATIO = 2 };
 // Backprojection output modes
 enum { SS_BP_PLAIN = 0, SS_BP_EM_UPDATE = 1 };
+
+// backprojection: LDS capacity for the staged sinogram windows, in (value, difference)
+// pairs of 8 bytes; a stage holds min(64, SS_BP_STAGE_PAIRS / window) angles
+#ifndef SS_BP_STAGE_PAIRS
+#define SS_BP_STAGE_PAIRS 4096
+#endif
 
 struct SSDevPlan {
   SSGeom g;
@@ -36,7 +43,7 @@ hipError_t ss_launch_plan_bounds(const SSGeom& g, const SSAngle* angles, int2* b
 hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4* wbounds, hipStream_t stream);
 
 // ---- layouts
-hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float* gatherA, float* gatherB,
+hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
                                     int nslices, hipStream_t stream);
 hipError_t ss_launch_sino_to_t(const float* sino, float* sinoT, int R, int A, int nslices, hipStream_t stream);
 hipError_t ss_launch_t_to_sino(const float* sinoT, float* sino, int R, int A, int nslices, hipStream_t stream);
@@ -46,7 +53,7 @@ struct SSEmRatioArgs {
   const float* meas;   // measured sinogram, angle-major [slice][A][R]
   double* ll_partials; // one FP64 partial per block (+ one per irregular ray)
 };
-hipError_t ss_launch_radon(const SSDevPlan& p, const float* gatherA, const float* gatherB, int j0, int j1,
+hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const float2* gatherB, int j0, int j1,
                            int nslices, int mode, float* out, SSEmRatioArgs em, hipStream_t stream);
 int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices);
 
@@ -54,8 +61,6 @@ int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices);
 struct SSEmUpdateArgs {
   float* x;            // image updated in place
   const float* weight; // BP_b(1)
-  float* gatherA;      // optional: the updated image is also written into the Radon
-  float* gatherB;      //   gather layouts (same layout as ss_launch_gather_layouts)
 };
 hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices, int mode,
                                     float* out, SSEmUpdateArgs em, hipStream_t stream);

@@ -89,50 +89,50 @@ hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4*
 // layouts
 // =====================================================================================
 
-// image [s][n][n] -> gatherB [s][n][n+2] (rows, zero pad at both ends)
-//                 -> gatherA [s][n][n+2] (columns: gatherA[c][r+1] = img[r][c])
-__global__ __launch_bounds__(256) void k_gather_layouts(const float* __restrict__ img, float* __restrict__ gA,
-                                                        float* __restrict__ gB, int n, int pitch) {
-  __shared__ float tile[32][33];
+// image [s][n][n] -> pair layouts [s][n][n+2] of float2, q in [0, n+1]:
+//   gatherB[r][q] = (F(r, q-1), F(r, q) - F(r, q-1))      image rows
+//   gatherA[c][q] = (F(q-1, c), F(q, c) - F(q-1, c))      image columns
+// with F = the image inside [0, n)^2 and 0 outside.  Element m+1 of a row holds the
+// lower tap of interpolation cell m and the difference to the upper tap, so a Radon
+// sample is one aligned 8-byte load and a + w*d.  One block = one 32x32 tile of q
+// in both directions, staged with a one-pixel halo.
+__global__ __launch_bounds__(256) void k_gather_pairs(const float* __restrict__ img, float2* __restrict__ gA,
+                                                      float2* __restrict__ gB, int n, int pitch) {
+  __shared__ float T[33][34];  // F(by-1+a, bx-1+b)
   const int slice = blockIdx.z;
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   const float* src = img + (size_t)slice * n * n;
-  float* dA = gA + (size_t)slice * n * pitch;
-  float* dB = gB + (size_t)slice * n * pitch;
-  for (int yy = ty; yy < 32; yy += 8) {
-    const int r = by + yy, c = bx + tx;
-    float v = 0.f;
-    if (r < n && c < n) {
-      v = src[(size_t)r * n + c];
-      dB[(size_t)r * pitch + c + 1] = v;
-    }
-    tile[yy][tx] = v;
+  for (int idx = threadIdx.x; idx < 33 * 33; idx += 256) {
+    const int a = idx / 33, b = idx - a * 33;
+    const int r = by - 1 + a, c = bx - 1 + b;
+    T[a][b] = (r >= 0 && r < n && c >= 0 && c < n) ? src[(size_t)r * n + c] : 0.f;
   }
   __syncthreads();
-  for (int xx = ty; xx < 32; xx += 8) {
-    const int c = bx + xx, r = by + tx;
-    if (r < n && c < n) dA[(size_t)c * pitch + r + 1] = tile[tx][xx];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  float2* dB = gB + (size_t)slice * n * pitch;
+  float2* dA = gA + (size_t)slice * n * pitch;
+  for (int i = ty; i < 32; i += 8) {
+    const int r = by + i, q = bx + tx;
+    if (r < n && q <= n + 1) {
+      const float v = T[i + 1][tx];
+      dB[(size_t)r * pitch + q] = make_float2(v, T[i + 1][tx + 1] - v);
+    }
   }
-  // zero pads: column 0 and n+1 of every row (done by the blocks of the first column)
-  if (blockIdx.x == 0) {
-    for (int yy = threadIdx.x; yy < 32; yy += 256) {
-      const int r = by + yy;
-      if (r < n) {
-        dB[(size_t)r * pitch] = 0.f;
-        dB[(size_t)r * pitch + n + 1] = 0.f;
-        dA[(size_t)r * pitch] = 0.f;
-        dA[(size_t)r * pitch + n + 1] = 0.f;
-      }
+  for (int j = ty; j < 32; j += 8) {
+    const int c = bx + j, q = by + tx;
+    if (c < n && q <= n + 1) {
+      const float v = T[tx][j + 1];
+      dA[(size_t)c * pitch + q] = make_float2(v, T[tx + 1][j + 1] - v);
     }
   }
 }
 
-hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float* gatherA, float* gatherB,
+hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
                                     int nslices, hipStream_t stream) {
   const int n = p.g.n;
-  dim3 grid((n + 31) / 32, (n + 31) / 32, nslices);
-  hipLaunchKernelGGL(k_gather_layouts, grid, dim3(256), 0, stream, image, gatherA, gatherB, n, p.g.pitch);
+  const int tiles = (n + 2 + 31) / 32;
+  dim3 grid(tiles, tiles, nslices);
+  hipLaunchKernelGGL(k_gather_pairs, grid, dim3(256), 0, stream, image, gatherA, gatherB, n, p.g.pitch);
   return hipGetLastError();
 }
 
@@ -215,9 +215,24 @@ __device__ __forceinline__ void ss_radon_store(const SSDevPlan& p, int slice, in
 // the chip (few angles, e.g. one OSEM subset).
 #define RADON_WAVES 8
 
+typedef float ss_f32x2 __attribute__((ext_vector_type(2)));
+#define SS_RSRC_DWORD3 0x00020000  // gfx9 buffer word 3: 32-bit data format, no swizzle
+
+// Structured buffer load of one 8-byte record: address = base + soffset + vindex * stride.
+// The record index comes straight from the high word of the Q32.32 trajectory, so a
+// sample costs no address arithmetic.
+__device__ ss_f32x2 ss_struct_load_f32x2(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                         int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v2f32");
+
+// tap pair (lower tap, difference) at record `rec` of the gather row at byte offset
+// `row_off` (per-slice offsets stay below 2^32 for n <= 16384; unsigned on the hardware)
+__device__ __forceinline__ ss_f32x2 ss_pair(__amdgpu_buffer_rsrc_t rs, uint32_t row_off, int rec) {
+  return ss_struct_load_f32x2(rs, rec, 0, (int)row_off, 0);
+}
+
 template <int MODE>
-__global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float* __restrict__ gA,
-                                               const float* __restrict__ gB, int j0, int j1, int split,
+__global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
+                                               const float2* __restrict__ gB, int j0, int j1, int split,
                                                float* __restrict__ out, SSEmRatioArgs em) {
   __shared__ float comb_a[RADON_WAVES][SS_WAVE];
   __shared__ float comb_d[RADON_WAVES][SS_WAVE];
@@ -249,61 +264,70 @@ __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float* __restr
     const int chunk = (len + split - 1) / split;
     const int sa = wb.x + part * chunk;
     const int sb = min(wb.y + 1, sa + chunk);
+    // u carries the trajectory plus one cell: its high word is the record index m + 1
+    // of the lower tap in the gather row, its low word the interpolation weight
     int64_t beta = a.beta;
-    int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)sa * beta;
+    int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)sa * beta + ((int64_t)1 << 32);
     if (hi < lo) {
-      // an empty ray reads the zero pad with weight 0: exactly 0 in the unpredicated loop
-      u = -((int64_t)1 << 32);
+      // an empty ray reads record 0 (lower tap 0) with weight 0: exactly 0 in the unpredicated loop
+      u = 0;
       beta = 0;
     }
-    const float* img = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch + 1;
+    // buffer of 8-byte records over this slice's gather layout
+    const float2* lay = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)lay, (short)sizeof(float2), (int)((uint32_t)n * (uint32_t)pitch * 8u), SS_RSRC_DWORD3);
+    const uint32_t row_bytes = (uint32_t)pitch * 8u;
     const unsigned span = (unsigned)(hi - lo);
     int s = sa;
     // head: predicated (before the intersection of the group's ranges)
     const int h_end = min(sb, wb.z);
     for (; s < h_end; ++s) {
       if ((unsigned)(s - lo) <= span) {
-        const int m = (int)(u >> 32);
-        const float* row = img + (size_t)s * pitch;
-        const float va = row[m], vb = row[m + 1];
-        acc_a += va;
-        acc_d = fmaf((float)(uint32_t)u, vb - va, acc_d);
+        const ss_f32x2 t = ss_pair(rs, (uint32_t)s * row_bytes, (int)(u >> 32));
+        acc_a += t.x;
+        acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
       }
       u += beta;
     }
-    // middle: every lane valid; 4 samples (8 loads) in flight per iteration
+    // middle: every lane valid; 4 samples in flight per iteration, each on its own
+    // trajectory register (one 64-bit add per sample; the empty asm keeps the compiler
+    // from re-deriving the four from one base, which costs extra adds)
     const int m_end = min(sb, wb.w + 1);
-    for (; s + 4 <= m_end; s += 4) {
-      const int64_t u0 = u, u1 = u0 + beta, u2 = u1 + beta, u3 = u2 + beta;
-      u = u3 + beta;
-      const float* r0 = img + (size_t)s * pitch;
-      const int m0 = (int)(u0 >> 32), m1 = (int)(u1 >> 32), m2 = (int)(u2 >> 32), m3 = (int)(u3 >> 32);
-      const float a0 = r0[m0], b0 = r0[m0 + 1];
-      const float a1 = r0[pitch + m1], b1 = r0[pitch + m1 + 1];
-      const float a2 = r0[2 * pitch + m2], b2 = r0[2 * pitch + m2 + 1];
-      const float a3 = r0[3 * pitch + m3], b3 = r0[3 * pitch + m3 + 1];
-      acc_a += (a0 + a1) + (a2 + a3);
-      acc_d = fmaf((float)(uint32_t)u0, b0 - a0, acc_d);
-      acc_d = fmaf((float)(uint32_t)u1, b1 - a1, acc_d);
-      acc_d = fmaf((float)(uint32_t)u2, b2 - a2, acc_d);
-      acc_d = fmaf((float)(uint32_t)u3, b3 - a3, acc_d);
+    if (s + 4 <= m_end) {
+      int64_t u0 = u, u1 = u + beta, u2 = u + 2 * beta, u3 = u + 3 * beta;
+      const int64_t beta4 = 4 * beta;
+      for (; s + 4 <= m_end; s += 4) {
+        const uint32_t off = (uint32_t)s * row_bytes;
+        const ss_f32x2 t0 = ss_pair(rs, off, (int)(u0 >> 32));
+        const ss_f32x2 t1 = ss_pair(rs, off + row_bytes, (int)(u1 >> 32));
+        const ss_f32x2 t2 = ss_pair(rs, off + 2 * row_bytes, (int)(u2 >> 32));
+        const ss_f32x2 t3 = ss_pair(rs, off + 3 * row_bytes, (int)(u3 >> 32));
+        acc_a += (t0.x + t1.x) + (t2.x + t3.x);
+        acc_d = fmaf((float)(uint32_t)u0, t0.y, acc_d);
+        acc_d = fmaf((float)(uint32_t)u1, t1.y, acc_d);
+        acc_d = fmaf((float)(uint32_t)u2, t2.y, acc_d);
+        acc_d = fmaf((float)(uint32_t)u3, t3.y, acc_d);
+        u0 += beta4;
+        u1 += beta4;
+        u2 += beta4;
+        u3 += beta4;
+        __asm__ volatile("" : "+v"(u0), "+v"(u1), "+v"(u2), "+v"(u3));
+      }
+      u = u0;
     }
     for (; s < m_end; ++s) {
-      const int m = (int)(u >> 32);
-      const float* row = img + (size_t)s * pitch;
-      const float va = row[m], vb = row[m + 1];
-      acc_a += va;
-      acc_d = fmaf((float)(uint32_t)u, vb - va, acc_d);
+      const ss_f32x2 t = ss_pair(rs, (uint32_t)s * row_bytes, (int)(u >> 32));
+      acc_a += t.x;
+      acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
       u += beta;
     }
     // tail: predicated
     for (; s < sb; ++s) {
       if ((unsigned)(s - lo) <= span) {
-        const int m = (int)(u >> 32);
-        const float* row = img + (size_t)s * pitch;
-        const float va = row[m], vb = row[m + 1];
-        acc_a += va;
-        acc_d = fmaf((float)(uint32_t)u, vb - va, acc_d);
+        const ss_f32x2 t = ss_pair(rs, (uint32_t)s * row_bytes, (int)(u >> 32));
+        acc_a += t.x;
+        acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
       }
       u += beta;
     }
@@ -334,8 +358,8 @@ __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float* __restr
 // Irregular rays (knife-edge angles): one block per ray, samples across threads,
 // validity evaluated per sample with the exact FP64 predicate.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float* __restrict__ gA,
-                                                     const float* __restrict__ gB, int j0, int j1,
+__global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* __restrict__ gA,
+                                                     const float2* __restrict__ gB, int j0, int j1,
                                                      float* __restrict__ out, SSEmRatioArgs em, int partial_base) {
   __shared__ float redf[4];
   __shared__ double red[4];
@@ -349,16 +373,15 @@ __global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float* _
     return;
   }
   const SSAngle& a = p.angles[j];
-  const float* img = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch + 1;
+  const float2* img = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch + 1;
   float acc = 0.f;
   for (int s = threadIdx.x; s < n; s += blockDim.x) {
     if (ss_radon_sample_valid(p.g, a, k, s)) {
       const int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)s * a.beta;
       const int m = (int)(u >> 32);
       const float w = (float)(uint32_t)u * SS_INV_FIX;
-      const float* row = img + (size_t)s * pitch;
-      const float va = row[m], vb = row[m + 1];
-      acc += fmaf(w, vb - va, va);
+      const float2 t = img[(size_t)s * pitch + m];
+      acc += fmaf(w, t.y, t.x);
     }
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
@@ -388,7 +411,7 @@ int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices) {
   return p.nwaves * gy * nslices + p.nirregular * nslices;
 }
 
-hipError_t ss_launch_radon(const SSDevPlan& p, const float* gatherA, const float* gatherB, int j0, int j1,
+hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const float2* gatherB, int j0, int j1,
                            int nslices, int mode, float* out, SSEmRatioArgs em, hipStream_t stream) {
   if (j1 <= j0) return hipSuccess;
   const int split = ss_radon_split(p, j0, j1, nslices);
@@ -444,7 +467,6 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float* gatherA, const float
 // =====================================================================================
 
 #define BP_TILE 32
-#define BP_MAX_STAGE 4096  // floats of LDS for the sinogram windows
 
 struct BPAngle {
   float f0, b, c;
@@ -453,7 +475,7 @@ struct BPAngle {
   int pad0, pad1, pad2;
 };
 
-__device__ __noinline__ int ss_bp_exact_valid(const SSDevPlan& p, int j, int row, int col) {
+__device__ __forceinline__ int ss_bp_exact_valid(const SSDevPlan& p, int j, int row, int col) {
   const SSAngle& a = p.angles[j];
   return ss_bp_sample_valid(p.g, a.cost, a.sint, row, col) ? 1 : 0;
 }
@@ -461,7 +483,7 @@ __device__ __noinline__ int ss_bp_exact_valid(const SSDevPlan& p, int j, int row
 template <int MODE>
 __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float* __restrict__ sinoT, int j0, int j1,
                                                         float* __restrict__ out, SSEmUpdateArgs em) {
-  __shared__ float win[BP_MAX_STAGE];
+  __shared__ float2 win[SS_BP_STAGE_PAIRS];  // (value, difference to the next ray)
   __shared__ BPAngle par[64];
   const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
   const int W = p.bp_window;
@@ -469,23 +491,27 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
   const int c0 = blockIdx.x * BP_TILE, r0 = blockIdx.y * BP_TILE;
   const int slice = blockIdx.z;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tx = tid & 31, ty = tid >> 5;
   const float* sl = sinoT + (size_t)slice * A * R;
   const int col = c0 + tx;
   float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
   const float ftx = (float)tx;
+  const float fty0 = (float)ty, fty1 = (float)(ty + 8), fty2 = (float)(ty + 16), fty3 = (float)(ty + 24);
   const float lo_rel_eps = 1e-3f;
 
   for (int jb = j0; jb < j1; jb += J) {
     const int nj = min(J, j1 - jb);
-    if (tid < nj) {
-      const SSAngle& a = p.angles[jb + tid];
+    // every wave: lane l holds the parameters of angle jb + l (J <= 64)
+    BPAngle q{0.f, 0.f, 0.f, 0, 0, 0, 0, 0};
+    if (lane < nj) {
+      const SSAngle& a = p.angles[jb + lane];
       const double U = a.bp_a + (double)c0 * a.bp_b + (double)r0 * a.bp_c;
       const double ec = (double)(BP_TILE - 1) * a.bp_b, er = (double)(BP_TILE - 1) * a.bp_c;
       const double umin = U + fmin(0.0, ec) + fmin(0.0, er);
       const double umax = U + fmax(0.0, ec) + fmax(0.0, er);
       const double base = floor(umin) - 1.0;
-      BPAngle q;
       q.f0 = (float)(U - base);
       q.b = (float)a.bp_b;
       q.c = (float)a.bp_c;
@@ -497,64 +523,86 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
       else
         q.flag = 2;
       q.base = (int)base;
-      q.pad0 = q.pad1 = q.pad2 = 0;
-      par[tid] = q;
     }
-    __syncthreads();
-    for (int idx = tid; idx < nj * W; idx += 256) {
-      const int jj = idx / W, w = idx - jj * W;
-      const BPAngle& q = par[jj];
-      const int ray = q.base + w;
-      float v = 0.f;
-      if (q.flag != 0 && ray >= 0 && ray < R) v = sl[(size_t)(jb + jj) * R + ray];
-      win[jj * W + w] = v;
-    }
-    __syncthreads();
-    for (int jj = 0; jj < nj; ++jj) {
-      const BPAngle q = par[jj];
-      const int flag = __builtin_amdgcn_readfirstlane(q.flag);
-      if (flag == 0) continue;
-      const float* wrow = win + jj * W;
-      const float t0 = fmaf(ftx, q.b, q.f0);
-      float us[4];
-      us[0] = fmaf((float)(ty), q.c, t0);
-      us[1] = fmaf((float)(ty + 8), q.c, t0);
-      us[2] = fmaf((float)(ty + 16), q.c, t0);
-      us[3] = fmaf((float)(ty + 24), q.c, t0);
-      float vals[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float u = us[i];
-        const float fl = floorf(u);
-        const int ii = (int)fl;
-        const float w = u - fl;
-        const float va = wrow[ii], vb = wrow[ii + 1];
-        vals[i] = fmaf(w, vb - va, va);
+    // a stage whose angles are all interior for this tile runs without predicates
+    const bool mixed = __ballot(lane < nj && q.flag != 1) != 0ull;
+    __syncthreads();  // the previous stage is done with win / par
+    if (wv == 0) par[lane] = q;
+    // staging: wave wv fills the windows of angles wv, wv+4, ...; base is at least one
+    // ray below the tile's lowest U, so every pixel has u >= 1 and (int)u < W - 1
+    for (int jj = wv; jj < nj; jj += 4) {
+      const int fl = __builtin_amdgcn_readlane(q.flag, jj);
+      const int bs = __builtin_amdgcn_readlane(q.base, jj);
+      const float* row = sl + (size_t)(jb + jj) * R;
+      for (int w = lane; w < W; w += 64) {
+        const int ray = bs + w;
+        float v0 = 0.f, v1 = 0.f;
+        if (fl != 0) {
+          if ((unsigned)ray < (unsigned)R) v0 = row[ray];
+          if ((unsigned)(ray + 1) < (unsigned)R) v1 = row[ray + 1];
+        }
+        win[jj * W + w] = make_float2(v0, v1 - v0);
       }
-      if (flag == 2) {
-        // boundary tile: per-pixel 0 <= m <= R-2 test; pixels within 1e-3 bins of an
-        // edge take the reference's FP64 expression.  The result is an integer mask
-        // applied by select, keeping the divergent region free of value phis.
-        const float lo_rel = (float)(-q.base), hi_rel = (float)(R - 1 - q.base);
-        int keep = 0;
+    }
+    __syncthreads();
+    if (!mixed) {
+#pragma unroll 2
+      for (int jj = 0; jj < nj; ++jj) {
+        const float f0 = par[jj].f0, pb = par[jj].b, pc = par[jj].c;
+        const float2* wrow = win + jj * W;
+        const float t0 = fmaf(ftx, pb, f0);
+        // u >= 1 here, so truncation is the floor
+        const float u0 = fmaf(fty0, pc, t0), u1 = fmaf(fty1, pc, t0);
+        const float u2 = fmaf(fty2, pc, t0), u3 = fmaf(fty3, pc, t0);
+        const float2 s0 = wrow[(int)u0], s1 = wrow[(int)u1], s2 = wrow[(int)u2], s3 = wrow[(int)u3];
+        acc0 += fmaf(__builtin_amdgcn_fractf(u0), s0.y, s0.x);
+        acc1 += fmaf(__builtin_amdgcn_fractf(u1), s1.y, s1.x);
+        acc2 += fmaf(__builtin_amdgcn_fractf(u2), s2.y, s2.x);
+        acc3 += fmaf(__builtin_amdgcn_fractf(u3), s3.y, s3.x);
+      }
+    } else {
+      for (int jj = 0; jj < nj; ++jj) {
+        const BPAngle qa = par[jj];
+        const int flag = __builtin_amdgcn_readfirstlane(qa.flag);
+        if (flag == 0) continue;
+        const float2* wrow = win + jj * W;
+        const float t0 = fmaf(ftx, qa.b, qa.f0);
+        float us[4];
+        us[0] = fmaf(fty0, qa.c, t0);
+        us[1] = fmaf(fty1, qa.c, t0);
+        us[2] = fmaf(fty2, qa.c, t0);
+        us[3] = fmaf(fty3, qa.c, t0);
+        float vals[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float u = us[i];
-          const bool inside = (u > lo_rel + lo_rel_eps) && (u < hi_rel - lo_rel_eps);
-          const bool outside = (u < lo_rel - lo_rel_eps) || (u > hi_rel + lo_rel_eps);
-          int ok = inside ? 1 : 0;
-          if (!inside && !outside) ok = ss_bp_exact_valid(p, jb + jj, r0 + ty + 8 * i, col);
-          keep |= ok << i;
+          const float2 t = wrow[(int)u];
+          vals[i] = fmaf(__builtin_amdgcn_fractf(u), t.y, t.x);
         }
+        if (flag == 2) {
+          // boundary tile: per-pixel 0 <= m <= R-2 test; pixels within 1e-3 bins of an
+          // edge take the reference's FP64 expression.  The result is an integer mask
+          // applied by select, keeping the divergent region free of value phis.
+          const float lo_rel = (float)(-qa.base), hi_rel = (float)(R - 1 - qa.base);
+          int keep = 0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) vals[i] = ((keep >> i) & 1) ? vals[i] : 0.f;
+          for (int i = 0; i < 4; ++i) {
+            const float u = us[i];
+            const bool inside = (u > lo_rel + lo_rel_eps) && (u < hi_rel - lo_rel_eps);
+            const bool outside = (u < lo_rel - lo_rel_eps) || (u > hi_rel + lo_rel_eps);
+            int ok = inside ? 1 : 0;
+            if (!inside && !outside) ok = ss_bp_exact_valid(p, jb + jj, r0 + ty + 8 * i, col);
+            keep |= ok << i;
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) vals[i] = ((keep >> i) & 1) ? vals[i] : 0.f;
+        }
+        acc0 += vals[0];
+        acc1 += vals[1];
+        acc2 += vals[2];
+        acc3 += vals[3];
       }
-      acc0 += vals[0];
-      acc1 += vals[1];
-      acc2 += vals[2];
-      acc3 += vals[3];
     }
-    __syncthreads();
   }
   if (col >= n) return;
   const float dth = (float)p.g.dth;
@@ -572,11 +620,6 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
       const float x = em.x[idx];
       const float xn = wgt > 0.f ? x * bp / wgt : x;
       em.x[idx] = xn;
-      if (em.gatherB) {
-        const int pitch = p.g.pitch;
-        em.gatherB[((size_t)slice * n + row) * pitch + col + 1] = xn;
-        em.gatherA[((size_t)slice * n + col) * pitch + row + 1] = xn;
-      }
     }
   }
 }
