@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--angles", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-em", action="store_true", help="skip the EM leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-slices leg")
+    ap.add_argument("--batch-slices", type=int, default=256, help="slices in the batched leg")
+    ap.add_argument("--batch-iters", type=int, default=20, help="OSEM iterations in the batched leg")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run one 64 MiB device-to-device copy first (reference for HBM byte counters)")
     return ap.parse_args()
@@ -110,6 +113,126 @@ def cpu_baseline(n, angles):
         "radon_gups": round(upd / tr / 1e9, 4),
         "backprojection_gups": round(upd / tb / 1e9, 4),
         "cpu_model": cpu_model(),
+    }
+
+
+def batched_leg(dev, nslices, niter, nsubsets=8, seed=4):
+    """Uneven slices: n from {256,...,2048}, A ~ U[180,1800], R = n, 32-ellipse phantoms.
+    Radon + backprojection and OSEM (nsubsets x niter, noiseless data s = R f), each as
+    (a) one batched persistent launch per kernel captured in a HIP graph, after the
+    autotuner picked the backprojection variant per image size, and (b) plain
+    per-slice launches of the same operators with the default variant."""
+    import numpy as np
+    import torch
+    from pyraft_amd.batch import Batch
+    from pyraft_amd.device import Plan, phantom
+    from pyraft_amd.phantom import random_ellipses
+
+    rng = np.random.default_rng(seed)
+    sizes = [int(v) for v in rng.choice([256, 512, 768, 1024, 1536, 2048], nslices)]
+    angles = [int(v) for v in rng.integers(180, 1801, nslices)]
+    upd = float(sum(n * n * a for n, a in zip(sizes, angles)))
+    print("bench: batch of %d slices, %.3g updates per pass" % (nslices, upd), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = Batch(sizes, sizes, angles)
+    plans = {}
+    for n, a in zip(sizes, angles):
+        if (n, a) not in plans:
+            plans[(n, a)] = Plan(n, n, a)
+    t_plan = time.perf_counter() - t0
+    images = b.empty_images()
+    for i, n in enumerate(sizes):
+        b.image(images, i).copy_(phantom(random_ellipses(32, 1000 + i), n))
+    t0 = time.perf_counter()
+    variants = b.tune(reps=1)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t0
+    tuned = {}
+    for n, v in zip(sizes, variants):
+        tuned[str(n)] = v
+
+    # ---- Radon + backprojection pass
+    sinos = b.empty_sinos()
+    bps = b.empty_images()
+    b.radon(images, out=sinos, graph=True)          # capture
+    b.backprojection(sinos, out=bps, graph=True)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.radon(images, out=sinos, graph=True)
+        b.backprojection(sinos, out=bps, graph=True)
+    torch.cuda.synchronize()
+    t_batched = (time.perf_counter() - t0) / reps
+    print("bench: batched Radon+BP %.1f ms" % (t_batched * 1e3), file=sys.stderr, flush=True)
+    views = [b.image(images, i) for i in range(nslices)]
+    plain_sinos = [None] * nslices
+    plain_bps = [None] * nslices
+
+    def plain_pass():
+        for i, (n, a) in enumerate(zip(sizes, angles)):
+            p = plans[(n, a)]
+            plain_sinos[i] = p.radon(views[i])
+            plain_bps[i] = p.backprojection(plain_sinos[i])
+
+    plain_pass()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_pass()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    print("bench: plain Radon+BP %.1f ms" % (t_plain * 1e3), file=sys.stderr, flush=True)
+
+    def worst(a_list, b_list):
+        w = 0.0
+        for x, y in zip(a_list, b_list):
+            d = float(torch.linalg.vector_norm((x - y).double()))
+            r = float(torch.linalg.vector_norm(y.double()))
+            w = max(w, d / (r if r > 0 else 1.0))
+        return w
+
+    diff_sino = worst([b.sino(sinos, i) for i in range(nslices)], plain_sinos)
+    diff_bp = worst([b.image(bps, i) for i in range(nslices)], plain_bps)
+
+    # ---- OSEM
+    x = torch.ones(b.img_total, dtype=torch.float32, device=dev)
+    b.em(sinos, 1, nsubsets, x=x)                   # weights, tables, kernels
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.em(sinos, niter, nsubsets, x=x, graph=True)    # capture + one graph launch
+    torch.cuda.synchronize()
+    t_em_b = time.perf_counter() - t0
+    print("bench: batched OSEM %.2f s" % t_em_b, file=sys.stderr, flush=True)
+    plain_x = [None] * nslices
+    plans[(sizes[0], angles[0])].em(plain_sinos[0], 1, nsubsets)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (n, a) in enumerate(zip(sizes, angles)):
+        plain_x[i] = plans[(n, a)].em(plain_sinos[i], niter, nsubsets)
+    torch.cuda.synchronize()
+    t_em_p = time.perf_counter() - t0
+    print("bench: plain OSEM %.2f s" % t_em_p, file=sys.stderr, flush=True)
+    diff_em = worst([b.image(x, i) for i in range(nslices)], plain_x)
+    return {
+        "workload": "%d slices, n in {256..2048} (seed %d), A ~ U[180,1800], R = n, 32-ellipse phantoms; "
+                    "Radon+backprojection pass and OSEM %d subsets x %d iterations on s = R f"
+                    % (nslices, seed, nsubsets, niter),
+        "updates_per_pass": upd,
+        "radon_bp_ms_batched": round(t_batched * 1e3, 3),
+        "radon_bp_ms_plain": round(t_plain * 1e3, 3),
+        "radon_bp_gups_batched": round(2 * upd / t_batched / 1e9, 2),
+        "radon_bp_gups_plain": round(2 * upd / t_plain / 1e9, 2),
+        "radon_bp_speedup": round(t_plain / t_batched, 3),
+        "em_iters_per_s_batched": round(niter / t_em_b, 3),
+        "em_iters_per_s_plain": round(niter / t_em_p, 3),
+        "em_speedup": round(t_em_p / t_em_b, 3),
+        "em_gups_batched": round(2 * upd * niter / t_em_b / 1e9, 2),
+        "tuned_bp_variant_by_n": tuned,
+        "tune_s": round(t_tune, 3),
+        "plan_s": round(t_plan, 3),
+        "max_rel_l2_batched_vs_plain": {"radon": diff_sino, "backprojection": diff_bp, "osem": diff_em},
     }
 
 
@@ -246,6 +369,9 @@ def main():
                         "iters_per_s_plain": round(it_s, 2),
                         "gups": round(2.0 * ne * ne * Ae * it_s / 1e9, 2),
                         "ll_first_last": [round(ll[0], 3), round(ll[-1], 3)]}
+
+    if not args.no_batch:
+        result["batched"] = batched_leg(dev, args.batch_slices, args.batch_iters)
 
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)

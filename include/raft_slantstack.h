@@ -23,8 +23,9 @@
  * the sinogram's x axis is theta (theta_j = tl_x + j*(br_x-tl_x)/A) and its y axis is t
  * (t_k = br_y + k*(tl_y-br_y)/(R-1)).
  *
- * Streams are hipStream_t passed as void* (NULL = the default stream).  A plan is
- * immutable after creation and may be shared by concurrent calls on one device.
+ * Streams are hipStream_t passed as void* (NULL = the default stream).  A plan may be
+ * shared by concurrent calls on one device; only raft_ss_plan_set_bp_variant and
+ * raft_ss_plan_tune change it.
  */
 #ifndef RAFT_SLANTSTACK_H
 #define RAFT_SLANTSTACK_H
@@ -110,7 +111,8 @@ typedef struct raft_ss_plan raft_ss_plan;
 int raft_ss_plan_create(raft_ss_plan **plan, int n, int nrays, int nangles, const double *img_corners,
                         const double *sino_corners, void *stream);
 int raft_ss_plan_destroy(raft_ss_plan *plan);
-/* info[0..6] = n, nrays, nangles, #irregular rays, #degenerate angles, bp window, bp angles/stage */
+/* info[0..7] = n, nrays, nangles, #irregular rays, #degenerate angles, bp window,
+ * bp angles/stage, bp kernel variant (window and stage are those of the variant) */
 int raft_ss_plan_info(const raft_ss_plan *plan, int *info, int ninfo);
 /* copy the per-ray sample ranges [A][R] x (lo, hi) to host memory */
 int raft_ss_plan_bounds(const raft_ss_plan *plan, int *host_bounds);
@@ -126,6 +128,12 @@ int raft_ss_plan_host_bounds(int n, int nrays, int nangles, const double *img_co
  * angles, bp window, bp angles per stage, 64-ray groups, 0 */
 int raft_ss_plan_host_info(int n, int nrays, int nangles, const double *img_corners, const double *sino_corners,
                            int *info);
+
+/* backprojection staging of every kernel variant computed on the host:
+ * windows[v] = rays staged per angle, per_stage[v] = angles per LDS stage (0 when the
+ * window does not fit variant v); arrays of RAFT_SS_BP_NVARIANTS ints */
+int raft_ss_plan_host_bp_staging(int n, int nrays, int nangles, const double *img_corners,
+                                 const double *sino_corners, int *windows, int *per_stage);
 
 /* ================================================================ device-pointer operators */
 
@@ -171,6 +179,69 @@ int raft_ss_em(raft_ss_plan *plan, const float *d_sino, float *d_x, int niter, i
  * (row 0 at tl_y, as raft/raft_image_phantom_fromdesc.c:18-41). */
 int raft_ss_phantom(const double *desc_rows, int nell, int lines, int cols, const double *corners, float *d_out,
                     void *stream);
+
+/* ================================================================ kernel variants and the autotuner
+ *
+ * No reference counterpart (the reference has one fixed CUDA launch shape,
+ * raft/raft_backprojection_gpu_function.cu:76-113).  The backprojection has
+ * RAFT_SS_BP_NVARIANTS tile shapes: variant v covers 32 columns x 8*rows[v] pixel rows
+ * per 256-thread block and stages up to stage[v] (value, difference) pairs of
+ * sinogram windows in LDS per block.  Every variant sums the same terms in the same
+ * per-pixel angle order; they differ only in the FP32 rounding of the detector
+ * coordinate, which is taken relative to the tile origin (~1e-7 relative). */
+#define RAFT_SS_BP_NVARIANTS 6
+int raft_ss_bp_variant_shape(int variant, int *rows, int *stage);
+/* select a variant for a plan (RAFT_SS_EINVAL if the window does not fit its stage) */
+int raft_ss_plan_set_bp_variant(raft_ss_plan *plan, int variant);
+/* time every runnable variant on this plan's full angle range (warm-up + best of
+ * `reps` launches, HIP events on `stream`), keep the fastest and return its index.
+ * times_us (optional, RAFT_SS_BP_NVARIANTS doubles) receives the times, -1 for
+ * variants that do not fit.  Results are cached per process by geometry; flag
+ * RAFT_SS_TUNE_FORCE measures again.  Not to be called while another thread uses
+ * the plan. */
+#define RAFT_SS_TUNE_FORCE 1
+int raft_ss_plan_tune(raft_ss_plan *plan, int reps, int flags, double *times_us, void *stream);
+
+/* ================================================================ batches of uneven slices
+ *
+ * No reference counterpart: the reference runs one slice per call
+ * (raft/raft_radon_gpu.cu:16-35, raft/raft_em_gpu.cu:20-88, and pyraft.osem loops
+ * over subsets in Python, pyraft/iterative/em.py:265-320).  A batch holds nslices
+ * slices with their own n_i, R_i, A_i and corners.  Images and sinograms are packed
+ * back to back: slice i's image starts at img_off[i] floats (n_i x n_i) and its
+ * sinogram at sino_off[i] floats (R_i x A_i, or A_i x R_i angle-major).  Each
+ * operator runs every slice in one persistent launch per kernel (a fixed grid of
+ * resident blocks walking the work items of all slices, heaviest first), and with
+ * RAFT_SS_BATCH_GRAPH the whole call is captured once into a HIP graph and
+ * replayed by a single graph launch on later calls with the same arguments. */
+typedef struct raft_ss_batch raft_ss_batch;
+#define RAFT_SS_BATCH_GRAPH 1
+/* img_corners / sino_corners: nslices x 4 doubles, or NULL for the defaults (image
+ * [-1,1]^2; theta in [0, pi), t in [-1, 1]).  Synchronises `stream`. */
+int raft_ss_batch_create(raft_ss_batch **batch, int nslices, const int *n, const int *nrays, const int *nangles,
+                         const double *img_corners, const double *sino_corners, void *stream);
+int raft_ss_batch_destroy(raft_ss_batch *batch);
+/* img_off / sino_off (nslices each, optional), totals[2] = total image / sinogram floats */
+int raft_ss_batch_layout(const raft_ss_batch *batch, long long *img_off, long long *sino_off, long long *totals);
+/* info[0..5] of one slice: n, nrays, nangles, #irregular rays, bp variant, bp angles/stage */
+int raft_ss_batch_info(const raft_ss_batch *batch, int slice, int *info, int ninfo);
+/* bp variant of one slice (slice = -1: all slices) */
+int raft_ss_batch_set_bp_variant(raft_ss_batch *batch, int slice, int variant);
+/* autotune: per distinct image side n, time the variants on that class's median-A
+ * slice with the batched kernel and give the class the fastest */
+int raft_ss_batch_tune(raft_ss_batch *batch, int reps, int flags, void *stream);
+/* Radon of all slices: packed images -> packed sinograms ([R][A], or [A][R] if
+ * angle_major) */
+int raft_ss_batch_radon(raft_ss_batch *batch, const float *d_images, float *d_sinos, int angle_major, int flags,
+                        void *stream);
+/* backprojection of all slices over their full angle ranges */
+int raft_ss_batch_backprojection(raft_ss_batch *batch, const float *d_sinos, float *d_images, int angle_major,
+                                 int flags, void *stream);
+/* OSEM of all slices (raft_ss_em per slice, subset b of slice i = angles
+ * [b*A_i/S, (b+1)*A_i/S)); d_sinos in [R][A] layout, d_x the packed starting images
+ * (estimates on exit).  ll_host (optional): niter x nslices log-likelihoods. */
+int raft_ss_batch_em(raft_ss_batch *batch, const float *d_sinos, float *d_x, int niter, int nsubsets,
+                     double *ll_host, int flags, void *stream);
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

@@ -20,15 +20,14 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/raft_slantstack.h"
-#include "ss_kernels.h"
+#include "ss_internal.h"
 #include "ss_plan_logic.h"
 
 // ------------------------------------------------------------------ errors
 
 static thread_local std::string g_last_error;
 
-static int set_error(int code, const char* fmt, ...) {
+int ss_set_error(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -38,57 +37,9 @@ static int set_error(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                                  \
-  do {                                                                                                 \
-    hipError_t e_ = (expr);                                                                            \
-    if (e_ != hipSuccess)                                                                              \
-      return set_error(RAFT_SS_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                       __LINE__);                                                                      \
-  } while (0)
+#define set_error ss_set_error
 
 extern "C" const char* raft_ss_last_error(void) { return g_last_error.c_str(); }
-
-// ------------------------------------------------------------------ plan
-
-struct raft_ss_plan {
-  int device = 0;
-  SSGeom g{};
-  double img_corners[4] = {0, 0, 0, 0};   // tl_x, tl_y, br_x, br_y
-  double sino_corners[4] = {0, 0, 0, 0};
-  std::vector<SSAngle> h_angles;
-  SSAngle* d_angles = nullptr;
-  int2* d_bounds = nullptr;
-  int4* d_wbounds = nullptr;
-  int* d_irregular = nullptr;
-  int nirregular = 0;
-  int ndegenerate = 0;
-  int nwaves = 0;
-  int bp_window = 0, bp_angles = 0;
-  float* d_ramp = nullptr;  // FBP convolution kernel, built on first use
-  std::mutex ramp_mutex;
-  SSDevPlan dev() const {
-    SSDevPlan p;
-    p.g = g;
-    p.angles = d_angles;
-    p.bounds = d_bounds;
-    p.wbounds = d_wbounds;
-    p.irregular = d_irregular;
-    p.nirregular = nirregular;
-    p.nwaves = nwaves;
-    p.bp_window = bp_window;
-    p.bp_angles = bp_angles;
-    return p;
-  }
-  ~raft_ss_plan() {
-    if (d_angles) (void)hipFree(d_angles);
-    if (d_bounds) (void)hipFree(d_bounds);
-    if (d_wbounds) (void)hipFree(d_wbounds);
-    if (d_irregular) (void)hipFree(d_irregular);
-    if (d_ramp) (void)hipFree(d_ramp);
-  }
-};
-
-static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Geometry in FP64 exactly as the reference derives it from the corners.
 static SSGeom make_geom(int n, int nrays, int nangles, const double* ic, const double* sc) {
@@ -158,16 +109,31 @@ static int validate_geom(const SSGeom& g, const std::vector<SSAngle>& angles) {
   return 0;
 }
 
-// Backprojection staging: rays a 32x32 pixel tile can touch at one angle (+margins
-// for the floor of the tile minimum and the second tap), and angles per LDS stage.
-static int bp_staging(const std::vector<SSAngle>& angles, int* window, int* per_stage) {
+// Backprojection staging for kernel variant v: rays a 32 x (8*rows) pixel tile can
+// touch at one angle (+margins for the floor of the tile minimum and the second tap),
+// and angles per LDS stage.  A variant whose window does not fit its LDS stage gets
+// per_stage = 0 (not runnable for this geometry).
+static void bp_staging(const std::vector<SSAngle>& angles, int v, int* window, int* per_stage) {
+  const double th = 8.0 * kSSBpRows[v] - 1.0;
   double span = 0.0;
-  for (const SSAngle& a : angles) span = std::max(span, fabs(a.bp_b) + fabs(a.bp_c));
-  int W = (int)ceil(31.0 * span) + 5;
+  for (const SSAngle& a : angles) span = std::max(span, 31.0 * fabs(a.bp_b) + th * fabs(a.bp_c));
+  int W = (int)ceil(span) + 5;
   W = (W + 3) & ~3;
-  if (W > SS_BP_STAGE_PAIRS) return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)", W);
   *window = W;
-  *per_stage = std::min(64, SS_BP_STAGE_PAIRS / W);
+  *per_stage = W <= kSSBpStage[v] ? std::min(64, kSSBpStage[v] / W) : 0;
+}
+
+// all variants' staging; the default variant, or the first runnable one
+static int bp_variants(raft_ss_plan* p) {
+  int first = -1;
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v) {
+    bp_staging(p->h_angles, v, &p->bp_window_v[v], &p->bp_angles_v[v]);
+    if (p->bp_angles_v[v] > 0 && first < 0) first = v;
+  }
+  if (first < 0)
+    return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)",
+                     p->bp_window_v[SS_BP_DEFAULT_VARIANT]);
+  p->bp_variant = p->bp_angles_v[SS_BP_DEFAULT_VARIANT] > 0 ? SS_BP_DEFAULT_VARIANT : first;
   return 0;
 }
 
@@ -188,7 +154,7 @@ extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nan
 
   const SSGeom& g = p->g;
   p->nwaves = (nrays + 63) / 64;
-  if ((rc = bp_staging(p->h_angles, &p->bp_window, &p->bp_angles))) return rc;
+  if ((rc = bp_variants(p.get()))) return rc;
 
   const int capacity = (p->ndegenerate + 2) * nrays + 4096;
   HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSAngle) * nangles));
@@ -216,7 +182,14 @@ extern "C" int raft_ss_plan_destroy(raft_ss_plan* plan) {
 
 extern "C" int raft_ss_plan_info(const raft_ss_plan* p, int* info, int ninfo) {
   if (!p || !info) return set_error(RAFT_SS_EINVAL, "null argument");
-  const int vals[] = {p->g.n, p->g.nrays, p->g.nangles, p->nirregular, p->ndegenerate, p->bp_window, p->bp_angles};
+  const int vals[] = {p->g.n,
+                      p->g.nrays,
+                      p->g.nangles,
+                      p->nirregular,
+                      p->ndegenerate,
+                      p->bp_window_v[p->bp_variant],
+                      p->bp_angles_v[p->bp_variant],
+                      p->bp_variant};
   for (int i = 0; i < ninfo && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
   return 0;
 }
@@ -291,7 +264,15 @@ extern "C" int raft_ss_plan_host_info(int n, int nrays, int nangles, const doubl
   int rc = validate_geom(g, angles);
   if (rc) return rc;
   int W = 0, J = 0;
-  if ((rc = bp_staging(angles, &W, &J))) return rc;
+  bp_staging(angles, SS_BP_DEFAULT_VARIANT, &W, &J);
+  if (J == 0) {
+    int v = 0;
+    for (; v < SS_BP_NVARIANTS; ++v) {
+      bp_staging(angles, v, &W, &J);
+      if (J > 0) break;
+    }
+    if (J == 0) return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)", W);
+  }
   info[0] = ndeg;
   info[1] = W;
   info[2] = J;
@@ -300,43 +281,24 @@ extern "C" int raft_ss_plan_host_info(int n, int nrays, int nangles, const doubl
   return 0;
 }
 
-// ------------------------------------------------------------------ workspace
+// Host-side backprojection staging of every kernel variant (no device needed):
+// windows[v] = staged rays per angle, per_stage[v] = angles per stage (0: the window
+// does not fit the variant's LDS stage).
+extern "C" int raft_ss_plan_host_bp_staging(int n, int nrays, int nangles, const double* img_corners,
+                                            const double* sino_corners, int* windows, int* per_stage) {
+  if (!img_corners || !sino_corners || !windows || !per_stage) return set_error(RAFT_SS_EINVAL, "null argument");
+  SSGeom g = make_geom(n, nrays, nangles, img_corners, sino_corners);
+  std::vector<SSAngle> angles;
+  int ndeg = 0;
+  if (n < 2 || nrays < 2 || nangles < 1) return validate_geom(g, angles);
+  build_angles(g, angles, &ndeg);
+  int rc = validate_geom(g, angles);
+  if (rc) return rc;
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v) bp_staging(angles, v, &windows[v], &per_stage[v]);
+  return 0;
+}
 
 namespace {
-struct DevBuf {
-  void* p = nullptr;
-  hipStream_t s = nullptr;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFreeAsync(p, s);
-  }
-  hipError_t alloc(size_t bytes, hipStream_t stream) {
-    s = stream;
-    return hipMallocAsync(&p, bytes ? bytes : 4, stream);
-  }
-  template <class T>
-  T* as() const {
-    return reinterpret_cast<T*>(p);
-  }
-};
-
-int check_plan(const raft_ss_plan* p, int nslices) {
-  if (!p) return set_error(RAFT_SS_EINVAL, "null plan");
-  if (nslices < 1) return set_error(RAFT_SS_EINVAL, "nslices must be >= 1 (got %d)", nslices);
-  int dev = -1;
-  if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
-    return set_error(RAFT_SS_EINVAL, "plan was built on device %d but device %d is current", p->device, dev);
-  return 0;
-}
-
-int check_range(const raft_ss_plan* p, int j0, int j1) {
-  if (j0 < 0 || j1 > p->g.nangles || j0 > j1)
-    return set_error(RAFT_SS_EINVAL, "angle range [%d, %d) outside [0, %d)", j0, j1, p->g.nangles);
-  return 0;
-}
-
 // g = Re(ifft(fftshift(h))) with h = |w| cos(pi w dt), w = linspace(-wc, wc, R), wc = 1/(2 dt)
 // (pyraft/filters/filters.py:44-55), evaluated in FP64 by a direct real DFT.
 void compute_ramp(int R, double tmin, double tmax, float* gk) {

@@ -1,0 +1,112 @@
+// ss_internal.h -- declarations shared by the host-side translation units of libraft
+// (plans and operators in ss_api.cpp, batches and the autotuner in ss_batch.cpp).
+#ifndef PYRAFT_AMD_SS_INTERNAL_H
+#define PYRAFT_AMD_SS_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/raft_slantstack.h"
+#include "ss_kernels.h"
+
+// ------------------------------------------------------------------ errors
+
+// records the message for raft_ss_last_error() on this thread and returns code
+int ss_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define HIP_TRY(expr)                                                                                     \
+  do {                                                                                                    \
+    hipError_t e_ = (expr);                                                                               \
+    if (e_ != hipSuccess)                                                                                 \
+      return ss_set_error(RAFT_SS_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                          __LINE__);                                                                      \
+  } while (0)
+
+// ------------------------------------------------------------------ plan
+
+struct raft_ss_plan {
+  int device = 0;
+  SSGeom g{};
+  double img_corners[4] = {0, 0, 0, 0};   // tl_x, tl_y, br_x, br_y
+  double sino_corners[4] = {0, 0, 0, 0};
+  std::vector<SSAngle> h_angles;
+  SSAngle* d_angles = nullptr;
+  int2* d_bounds = nullptr;
+  int4* d_wbounds = nullptr;
+  int* d_irregular = nullptr;
+  int nirregular = 0;
+  int ndegenerate = 0;
+  int nwaves = 0;
+  int bp_variant = SS_BP_DEFAULT_VARIANT;
+  int bp_window_v[SS_BP_NVARIANTS] = {0};  // per backprojection variant: staged window
+  int bp_angles_v[SS_BP_NVARIANTS] = {0};  //   and angles per stage (0: not runnable)
+  float* d_ramp = nullptr;  // FBP convolution kernel, built on first use
+  std::mutex ramp_mutex;
+  SSDevPlan dev() const { return dev(bp_variant); }
+  // the plan as the kernels see it, with backprojection variant v
+  SSDevPlan dev(int v) const {
+    SSDevPlan p;
+    p.g = g;
+    p.angles = d_angles;
+    p.bounds = d_bounds;
+    p.wbounds = d_wbounds;
+    p.irregular = d_irregular;
+    p.nirregular = nirregular;
+    p.nwaves = nwaves;
+    p.bp_variant = v;
+    p.bp_window = bp_window_v[v];
+    p.bp_angles = bp_angles_v[v];
+    return p;
+  }
+  ~raft_ss_plan() {
+    if (d_angles) (void)hipFree(d_angles);
+    if (d_bounds) (void)hipFree(d_bounds);
+    if (d_wbounds) (void)hipFree(d_wbounds);
+    if (d_irregular) (void)hipFree(d_irregular);
+    if (d_ramp) (void)hipFree(d_ramp);
+  }
+};
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------------ workspace
+
+// stream-ordered scratch allocation, freed on the same stream
+struct DevBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+  hipError_t alloc(size_t bytes, hipStream_t stream) {
+    s = stream;
+    return hipMallocAsync(&p, bytes ? bytes : 4, stream);
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+inline int check_plan(const raft_ss_plan* p, int nslices) {
+  if (!p) return ss_set_error(RAFT_SS_EINVAL, "null plan");
+  if (nslices < 1) return ss_set_error(RAFT_SS_EINVAL, "nslices must be >= 1 (got %d)", nslices);
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
+    return ss_set_error(RAFT_SS_EINVAL, "plan was built on device %d but device %d is current", p->device, dev);
+  return 0;
+}
+
+inline int check_range(const raft_ss_plan* p, int j0, int j1) {
+  if (j0 < 0 || j1 > p->g.nangles || j0 > j1)
+    return ss_set_error(RAFT_SS_EINVAL, "angle range [%d, %d) outside [0, %d)", j0, j1, p->g.nangles);
+  return 0;
+}
+
+#endif

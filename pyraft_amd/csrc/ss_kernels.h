@@ -19,11 +19,15 @@ enum { SS_OUT_SINO = 0, SS_OUT_SINOT = 1, SS_OUT_EM_RATIO = 2 };
 // Backprojection output modes
 enum { SS_BP_PLAIN = 0, SS_BP_EM_UPDATE = 1 };
 
-// backprojection: LDS capacity for the staged sinogram windows, in (value, difference)
-// pairs of 8 bytes; a stage holds min(64, SS_BP_STAGE_PAIRS / window) angles
-#ifndef SS_BP_STAGE_PAIRS
-#define SS_BP_STAGE_PAIRS 4096
-#endif
+// Backprojection kernel variants (chosen per plan, by the autotuner or by hand).
+// A block is 256 threads over a tile of 32 columns x 8*rows pixel rows; each thread
+// owns `rows` pixels of one column.  `stage` is the LDS capacity for the staged
+// sinogram windows in (value, difference) pairs of 8 bytes; a stage holds
+// min(64, stage / window) angles.
+#define SS_BP_NVARIANTS 6
+#define SS_BP_DEFAULT_VARIANT 0
+static const int kSSBpRows[SS_BP_NVARIANTS] = {4, 2, 8, 4, 2, 8};
+static const int kSSBpStage[SS_BP_NVARIANTS] = {4096, 2048, 4096, 2048, 4096, 2048};
 
 struct SSDevPlan {
   SSGeom g;
@@ -33,8 +37,31 @@ struct SSDevPlan {
   const int* irregular;    // list of j*R + k
   int nirregular;
   int nwaves;              // ceil(R/64)
-  int bp_window;           // rays staged per angle per 32x32 pixel tile
-  int bp_angles;           // angles per LDS stage
+  int bp_variant;          // index into kSSBpRows / kSSBpStage
+  int bp_window;           // rays staged per angle per pixel tile (for bp_variant)
+  int bp_angles;           // angles per LDS stage (for bp_variant)
+};
+
+// ---- batched execution over slices of different geometry
+// Packed buffers hold every slice at its own offset.  One launch covers a list of
+// entries (slice, angle range); its work items are numbered consecutively entry by
+// entry (item0 = first item of the entry) and a fixed grid of resident blocks walks
+// them with stride gridDim.x.
+struct SSBatchSlice {
+  SSDevPlan p;
+  int64_t img_off;   // floats: image [n][n]
+  int64_t sino_off;  // floats: sinogram of R*A values (either layout)
+  int64_t lay_off;   // float2: one gather layout [n][n+2]
+};
+struct SSBatchEntry {
+  int slice, j0, j1, item0;
+};
+struct SSBatchLaunch {
+  const SSBatchSlice* slices;  // device
+  const SSBatchEntry* entries; // device
+  int nentries;
+  int total;                   // work items
+  int grid;                    // blocks launched (<= total)
 };
 
 // ---- plan construction
@@ -68,6 +95,30 @@ hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int 
 // ---- exact transpose of the Radon (reads angle-major sinogram)
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, hipStream_t stream);
+
+// ---- batched forms (items: gather = 32x32 tiles; radon = 64-ray groups x angle blocks of
+//      RADON_WAVES/split angles; fixup = irregular rays; bp = pixel tiles of the variant;
+//      transpose = 32x32 tiles)
+int ss_radon_angles_per_block(int split);
+int ss_bp_tiles(int n, int variant);
+hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, const float* images, float2* gatherA, float2* gatherB,
+                                  hipStream_t stream);
+hipError_t ss_launch_batch_transpose(const SSBatchLaunch& L, const float* in, float* out, int to_angle_major,
+                                     hipStream_t stream);
+hipError_t ss_launch_batch_radon(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, int split,
+                                 int mode, float* out, const float* meas, double* ll_partials, hipStream_t stream);
+hipError_t ss_launch_batch_radon_fixup(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, int mode,
+                                       float* out, const float* meas, double* ll_partials, hipStream_t stream);
+hipError_t ss_launch_batch_backprojection(const SSBatchLaunch& L, int variant, const float* sinoT, int mode,
+                                          float* out, float* x, const float* weight, hipStream_t stream);
+// per-slice sums of LL partials: ll_out[i] = sum of the `nranges` (start, count) ranges of slice i
+hipError_t ss_launch_batch_ll(const int2* ranges, int nranges, int nslices, const double* partials, double* ll_out,
+                              hipStream_t stream);
+hipError_t ss_launch_fill_f64(double* p, double v, size_t n, hipStream_t stream);
+
+// occupancy-based resident grid of a batched kernel (blocks per CU x CUs)
+int ss_batch_grid(int kind, int variant, int mode, int total);
+enum { SS_KIND_GATHER = 0, SS_KIND_TRANSPOSE, SS_KIND_RADON, SS_KIND_FIXUP, SS_KIND_BP };
 
 // ---- FBP row filter (rows of length R, circular convolution with g)
 hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream);

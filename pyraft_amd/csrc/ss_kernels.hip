@@ -89,19 +89,16 @@ hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4*
 // layouts
 // =====================================================================================
 
-// image [s][n][n] -> pair layouts [s][n][n+2] of float2, q in [0, n+1]:
+// image [n][n] -> pair layouts [n][n+2] of float2, q in [0, n+1]:
 //   gatherB[r][q] = (F(r, q-1), F(r, q) - F(r, q-1))      image rows
 //   gatherA[c][q] = (F(q-1, c), F(q, c) - F(q-1, c))      image columns
 // with F = the image inside [0, n)^2 and 0 outside.  Element m+1 of a row holds the
 // lower tap of interpolation cell m and the difference to the upper tap, so a Radon
-// sample is one aligned 8-byte load and a + w*d.  One block = one 32x32 tile of q
-// in both directions, staged with a one-pixel halo.
-__global__ __launch_bounds__(256) void k_gather_pairs(const float* __restrict__ img, float2* __restrict__ gA,
-                                                      float2* __restrict__ gB, int n, int pitch) {
-  __shared__ float T[33][34];  // F(by-1+a, bx-1+b)
-  const int slice = blockIdx.z;
-  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-  const float* src = img + (size_t)slice * n * n;
+// sample is one aligned 8-byte load and a + w*d.  One tile = 32x32 values of q in
+// both directions, staged with a one-pixel halo.
+__device__ __forceinline__ void ss_gather_tile(const float* __restrict__ src, float2* __restrict__ dA,
+                                               float2* __restrict__ dB, int n, int pitch, int bx, int by,
+                                               float (*T)[34]) {
   for (int idx = threadIdx.x; idx < 33 * 33; idx += 256) {
     const int a = idx / 33, b = idx - a * 33;
     const int r = by - 1 + a, c = bx - 1 + b;
@@ -109,8 +106,6 @@ __global__ __launch_bounds__(256) void k_gather_pairs(const float* __restrict__ 
   }
   __syncthreads();
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  float2* dB = gB + (size_t)slice * n * pitch;
-  float2* dA = gA + (size_t)slice * n * pitch;
   for (int i = ty; i < 32; i += 8) {
     const int r = by + i, q = bx + tx;
     if (r < n && q <= n + 1) {
@@ -127,6 +122,14 @@ __global__ __launch_bounds__(256) void k_gather_pairs(const float* __restrict__ 
   }
 }
 
+__global__ __launch_bounds__(256) void k_gather_pairs(const float* __restrict__ img, float2* __restrict__ gA,
+                                                      float2* __restrict__ gB, int n, int pitch) {
+  __shared__ float T[33][34];  // F(by-1+a, bx-1+b)
+  const size_t slice = blockIdx.z;
+  ss_gather_tile(img + slice * n * n, gA + slice * n * pitch, gB + slice * n * pitch, n, pitch, blockIdx.x * 32,
+                 blockIdx.y * 32, T);
+}
+
 hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
                                     int nslices, hipStream_t stream) {
   const int n = p.g.n;
@@ -136,15 +139,10 @@ hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, floa
   return hipGetLastError();
 }
 
-// [s][rows][cols] -> [s][cols][rows]
-__global__ __launch_bounds__(256) void k_transpose2d(const float* __restrict__ in, float* __restrict__ out, int rows,
-                                                     int cols) {
-  __shared__ float tile[32][33];
-  const int slice = blockIdx.z;
-  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+// [rows][cols] -> [cols][rows], one 32x32 tile
+__device__ __forceinline__ void ss_transpose_tile(const float* __restrict__ src, float* __restrict__ dst, int rows,
+                                                  int cols, int bx, int by, float (*tile)[33]) {
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  const float* src = in + (size_t)slice * rows * cols;
-  float* dst = out + (size_t)slice * rows * cols;
   for (int yy = ty; yy < 32; yy += 8) {
     const int r = by + yy, c = bx + tx;
     if (r < rows && c < cols) tile[yy][tx] = src[(size_t)r * cols + c];
@@ -154,6 +152,14 @@ __global__ __launch_bounds__(256) void k_transpose2d(const float* __restrict__ i
     const int c = bx + xx, r = by + tx;
     if (r < rows && c < cols) dst[(size_t)c * rows + r] = tile[tx][xx];
   }
+}
+
+// [s][rows][cols] -> [s][cols][rows]
+__global__ __launch_bounds__(256) void k_transpose2d(const float* __restrict__ in, float* __restrict__ out, int rows,
+                                                     int cols) {
+  __shared__ float tile[32][33];
+  const size_t off = (size_t)blockIdx.z * rows * cols;
+  ss_transpose_tile(in + off, out + off, rows, cols, blockIdx.x * 32, blockIdx.y * 32, tile);
 }
 
 hipError_t ss_launch_sino_to_t(const float* sino, float* sinoT, int R, int A, int nslices, hipStream_t stream) {
@@ -189,17 +195,18 @@ __device__ __forceinline__ double ss_block_sum_f64(double v, double* scratch) {
   return r;  // valid in thread 0
 }
 
+// out / meas point at this slice's sinogram
 template <int MODE>
-__device__ __forceinline__ void ss_radon_store(const SSDevPlan& p, int slice, int j, int k, float val, float* out,
-                                               const SSEmRatioArgs& em, double* ll) {
+__device__ __forceinline__ void ss_radon_store(const SSDevPlan& p, int j, int k, float val, float* out,
+                                               const float* meas, double* ll) {
   const int R = p.g.nrays, A = p.g.nangles;
   if (MODE == SS_OUT_SINO) {
-    out[((size_t)slice * R + k) * A + j] = val;
+    out[(size_t)k * A + j] = val;
   } else if (MODE == SS_OUT_SINOT) {
-    out[((size_t)slice * A + j) * R + k] = val;
+    out[(size_t)j * R + k] = val;
   } else {
-    const size_t idx = ((size_t)slice * A + j) * R + k;
-    const float s = em.meas[idx];
+    const size_t idx = (size_t)j * R + k;
+    const float s = meas[idx];
     float rho = 0.f;
     if (val > 0.f) {
       rho = s / val;
@@ -230,21 +237,26 @@ __device__ __forceinline__ ss_f32x2 ss_pair(__amdgpu_buffer_rsrc_t rs, uint32_t 
   return ss_struct_load_f32x2(rs, rec, 0, (int)row_off, 0);
 }
 
+struct SSRadonLds {
+  float comb_a[RADON_WAVES][SS_WAVE];
+  float comb_d[RADON_WAVES][SS_WAVE];
+  double red[RADON_WAVES];
+};
+
+// One block's work: rays [grp*64, grp*64+64) of the RADON_WAVES/split angles starting
+// at j0 + ablk*(RADON_WAVES/split), clipped to [j0, j1).  gA/gB/out/meas point at this
+// slice's data; in EM mode the block's FP64 log-likelihood partial goes to *ll_slot.
 template <int MODE>
-__global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
-                                               const float2* __restrict__ gB, int j0, int j1, int split,
-                                               float* __restrict__ out, SSEmRatioArgs em) {
-  __shared__ float comb_a[RADON_WAVES][SS_WAVE];
-  __shared__ float comb_d[RADON_WAVES][SS_WAVE];
-  __shared__ double red[RADON_WAVES];
+__device__ __forceinline__ void ss_radon_item(const SSDevPlan& p, const float2* __restrict__ gA,
+                                              const float2* __restrict__ gB, int j0, int j1, int split, int grp,
+                                              int ablk, float* __restrict__ out, const float* __restrict__ meas,
+                                              double* ll_slot, SSRadonLds& lds) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int per_block = RADON_WAVES / split;
   const int aw = wv / split, part = wv - aw * split;
-  const int j = j0 + blockIdx.y * per_block + aw;
-  const int slice = blockIdx.z;
+  const int j = j0 + ablk * per_block + aw;
   const int R = p.g.nrays, n = p.g.n, pitch = p.g.pitch;
-  const int grp = blockIdx.x;
   const int k = grp * SS_WAVE + lane;
   const bool active = j < j1;
   float acc_a = 0.f, acc_d = 0.f;
@@ -274,7 +286,7 @@ __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __rest
       beta = 0;
     }
     // buffer of 8-byte records over this slice's gather layout
-    const float2* lay = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch;
+    const float2* lay = a.branch_a ? gA : gB;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void*)lay, (short)sizeof(float2), (int)((uint32_t)n * (uint32_t)pitch * 8u), SS_RSRC_DWORD3);
     const uint32_t row_bytes = (uint32_t)pitch * 8u;
@@ -333,47 +345,58 @@ __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __rest
     }
   }
   if (split > 1) {
-    comb_a[wv][lane] = acc_a;
-    comb_d[wv][lane] = acc_d;
+    lds.comb_a[wv][lane] = acc_a;
+    lds.comb_d[wv][lane] = acc_d;
     __syncthreads();
     if (part == 0) {
       for (int q = 1; q < split; ++q) {
-        acc_a += comb_a[wv + q][lane];
-        acc_d += comb_d[wv + q][lane];
+        acc_a += lds.comb_a[wv + q][lane];
+        acc_d += lds.comb_d[wv + q][lane];
       }
     }
   }
   double ll = 0.0;
   if (active && part == 0 && k < R) {
     const float val = fmaf(acc_d, SS_INV_FIX, acc_a) * delta;
-    ss_radon_store<MODE>(p, slice, j, k, val, out, em, &ll);
+    ss_radon_store<MODE>(p, j, k, val, out, meas, &ll);
   }
   if (MODE == SS_OUT_EM_RATIO) {
-    const double tot = ss_block_sum_f64(ll, red);
-    if (threadIdx.x == 0)
-      em.ll_partials[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = tot;
+    const double tot = ss_block_sum_f64(ll, lds.red);
+    if (threadIdx.x == 0) *ll_slot = tot;
   }
 }
 
-// Irregular rays (knife-edge angles): one block per ray, samples across threads,
-// validity evaluated per sample with the exact FP64 predicate.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* __restrict__ gA,
-                                                     const float2* __restrict__ gB, int j0, int j1,
-                                                     float* __restrict__ out, SSEmRatioArgs em, int partial_base) {
-  __shared__ float redf[4];
-  __shared__ double red[4];
-  const int id = p.irregular[blockIdx.x];
-  const int slice = blockIdx.y;
+__global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
+                                               const float2* __restrict__ gB, int j0, int j1, int split,
+                                               float* __restrict__ out, SSEmRatioArgs em) {
+  __shared__ SSRadonLds lds;
+  const size_t slice = blockIdx.z;
+  const size_t lay = slice * p.g.n * p.g.pitch;
+  const size_t so = slice * p.g.nangles * p.g.nrays;
+  double* slot = nullptr;
+  if (MODE == SS_OUT_EM_RATIO) slot = em.ll_partials + (slice * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  ss_radon_item<MODE>(p, gA + lay, gB + lay, j0, j1, split, blockIdx.x, blockIdx.y, out + so,
+                      MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr, slot, lds);
+}
+
+// Irregular rays (knife-edge angles): one block per ray, samples across threads,
+// validity evaluated per sample with the exact FP64 predicate.  `idx` indexes the
+// plan's irregular list; rays outside [j0, j1) write a zero partial only.
+template <int MODE>
+__device__ __forceinline__ void ss_radon_fixup_item(const SSDevPlan& p, const float2* __restrict__ gA,
+                                                    const float2* __restrict__ gB, int j0, int j1, int idx,
+                                                    float* __restrict__ out, const float* __restrict__ meas,
+                                                    double* ll_slot, float* redf) {
+  const int id = p.irregular[idx];
   const int R = p.g.nrays, n = p.g.n, pitch = p.g.pitch;
   const int j = id / R, k = id % R;
   if (j < j0 || j >= j1) {
-    if (MODE == SS_OUT_EM_RATIO && threadIdx.x == 0)
-      em.ll_partials[partial_base + (size_t)slice * gridDim.x + blockIdx.x] = 0.0;
+    if (MODE == SS_OUT_EM_RATIO && threadIdx.x == 0) *ll_slot = 0.0;
     return;
   }
   const SSAngle& a = p.angles[j];
-  const float2* img = (a.branch_a ? gA : gB) + (size_t)slice * n * pitch + 1;
+  const float2* img = (a.branch_a ? gA : gB) + 1;
   float acc = 0.f;
   for (int s = threadIdx.x; s < n; s += blockDim.x) {
     if (ss_radon_sample_valid(p.g, a, k, s)) {
@@ -387,14 +410,27 @@ __global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* 
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = acc;
   __syncthreads();
-  double ll = 0.0;
   if (threadIdx.x == 0) {
     float tot = 0.f;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) tot += redf[i];
-    ss_radon_store<MODE>(p, slice, j, k, tot * a.delta, out, em, &ll);
-    if (MODE == SS_OUT_EM_RATIO) em.ll_partials[partial_base + (size_t)slice * gridDim.x + blockIdx.x] = ll;
+    double ll = 0.0;
+    ss_radon_store<MODE>(p, j, k, tot * a.delta, out, meas, &ll);
+    if (MODE == SS_OUT_EM_RATIO) *ll_slot = ll;
   }
-  (void)red;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* __restrict__ gA,
+                                                     const float2* __restrict__ gB, int j0, int j1,
+                                                     float* __restrict__ out, SSEmRatioArgs em, int partial_base) {
+  __shared__ float redf[4];
+  const size_t slice = blockIdx.y;
+  const size_t lay = slice * p.g.n * p.g.pitch;
+  const size_t so = slice * p.g.nangles * p.g.nrays;
+  double* slot = nullptr;
+  if (MODE == SS_OUT_EM_RATIO) slot = em.ll_partials + partial_base + slice * gridDim.x + blockIdx.x;
+  ss_radon_fixup_item<MODE>(p, gA + lay, gB + lay, j0, j1, blockIdx.x, out + so,
+                            MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr, slot, redf);
 }
 
 // waves per (angle, ray group): enough to put ~32 waves on each of the 256 CUs
@@ -404,6 +440,8 @@ static int ss_radon_split(const SSDevPlan& p, int j0, int j1, int nslices) {
   while (split < RADON_WAVES && waves * split < 256LL * 32) split *= 2;
   return split;
 }
+
+int ss_radon_angles_per_block(int split) { return RADON_WAVES / split; }
 
 int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices) {
   const int per_block = RADON_WAVES / ss_radon_split(p, j0, j1, nslices);
@@ -452,13 +490,14 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const floa
 }
 
 // =====================================================================================
-// backprojection: pixel-driven, 32x32 pixel tile per block, angle blocks staged in LDS
+// backprojection: pixel-driven, 32 x (8*ROWS) pixel tile per block, angle blocks
+// staged in LDS
 //
 // For each angle of a stage the block computes (FP64) the ray coordinate U of the
 // tile origin and the tile's U range, then stages the W consecutive rays
-// [base, base+W) that the tile can touch.  Each thread owns 4 pixels (one column,
-// rows ty, ty+8, ty+16, ty+24) and interpolates from LDS with FP32 offsets
-// relative to `base` (|u| < ~W, so FP32 resolves ~1e-5 of a detector bin).
+// [base, base+W) that the tile can touch.  Each thread owns ROWS pixels (one column,
+// rows ty, ty+8, ...) and interpolates from LDS with FP32 offsets relative to `base`
+// (|u| < ~W, so FP32 resolves ~1e-5 of a detector bin).
 //
 // Exactness of the 0 <= m <= R-2 predicate: a tile whose U range lies inside
 // [eps, R-1-eps] needs no test; one entirely outside contributes nothing; the rest
@@ -466,7 +505,7 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const floa
 // reference's FP64 expression.
 // =====================================================================================
 
-#define BP_TILE 32
+#define BP_TILE_W 32
 
 struct BPAngle {
   float f0, b, c;
@@ -480,25 +519,35 @@ __device__ __forceinline__ int ss_bp_exact_valid(const SSDevPlan& p, int j, int 
   return ss_bp_sample_valid(p.g, a.cost, a.sint, row, col) ? 1 : 0;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float* __restrict__ sinoT, int j0, int j1,
-                                                        float* __restrict__ out, SSEmUpdateArgs em) {
-  __shared__ float2 win[SS_BP_STAGE_PAIRS];  // (value, difference to the next ray)
-  __shared__ BPAngle par[64];
-  const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
+template <int STAGE>
+struct SSBpLds {
+  float2 win[STAGE];  // (value, difference to the next ray)
+  BPAngle par[64];
+};
+
+// One pixel tile [c0, c0+32) x [r0, r0+8*ROWS) over angles [j0, j1).  sl points at this
+// slice's angle-major sinogram; out (plain) or x/weight (EM update) at its image.
+template <int MODE, int ROWS, int STAGE>
+__device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __restrict__ sl, int j0, int j1, int c0,
+                                           int r0, float* __restrict__ out, float* __restrict__ xs,
+                                           const float* __restrict__ ws, SSBpLds<STAGE>& lds) {
+  const int n = p.g.n, R = p.g.nrays;
   const int W = p.bp_window;
   const int J = p.bp_angles;
-  const int c0 = blockIdx.x * BP_TILE, r0 = blockIdx.y * BP_TILE;
-  const int slice = blockIdx.z;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tx = tid & 31, ty = tid >> 5;
-  const float* sl = sinoT + (size_t)slice * A * R;
   const int col = c0 + tx;
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+  constexpr int TH = 8 * ROWS;
+  float acc[ROWS];
+  float fty[ROWS];
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    acc[i] = 0.f;
+    fty[i] = (float)(ty + 8 * i);
+  }
   const float ftx = (float)tx;
-  const float fty0 = (float)ty, fty1 = (float)(ty + 8), fty2 = (float)(ty + 16), fty3 = (float)(ty + 24);
   const float lo_rel_eps = 1e-3f;
 
   for (int jb = j0; jb < j1; jb += J) {
@@ -508,7 +557,7 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
     if (lane < nj) {
       const SSAngle& a = p.angles[jb + lane];
       const double U = a.bp_a + (double)c0 * a.bp_b + (double)r0 * a.bp_c;
-      const double ec = (double)(BP_TILE - 1) * a.bp_b, er = (double)(BP_TILE - 1) * a.bp_c;
+      const double ec = (double)(BP_TILE_W - 1) * a.bp_b, er = (double)(TH - 1) * a.bp_c;
       const double umin = U + fmin(0.0, ec) + fmin(0.0, er);
       const double umax = U + fmax(0.0, ec) + fmax(0.0, er);
       const double base = floor(umin) - 1.0;
@@ -526,8 +575,8 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
     }
     // a stage whose angles are all interior for this tile runs without predicates
     const bool mixed = __ballot(lane < nj && q.flag != 1) != 0ull;
-    __syncthreads();  // the previous stage is done with win / par
-    if (wv == 0) par[lane] = q;
+    __syncthreads();  // the previous stage (or item) is done with win / par
+    if (wv == 0) lds.par[lane] = q;
     // staging: wave wv fills the windows of angles wv, wv+4, ...; base is at least one
     // ray below the tile's lowest U, so every pixel has u >= 1 and (int)u < W - 1
     for (int jj = wv; jj < nj; jj += 4) {
@@ -541,43 +590,37 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
           if ((unsigned)ray < (unsigned)R) v0 = row[ray];
           if ((unsigned)(ray + 1) < (unsigned)R) v1 = row[ray + 1];
         }
-        win[jj * W + w] = make_float2(v0, v1 - v0);
+        lds.win[jj * W + w] = make_float2(v0, v1 - v0);
       }
     }
     __syncthreads();
     if (!mixed) {
 #pragma unroll 2
       for (int jj = 0; jj < nj; ++jj) {
-        const float f0 = par[jj].f0, pb = par[jj].b, pc = par[jj].c;
-        const float2* wrow = win + jj * W;
+        const float f0 = lds.par[jj].f0, pb = lds.par[jj].b, pc = lds.par[jj].c;
+        const float2* wrow = lds.win + jj * W;
         const float t0 = fmaf(ftx, pb, f0);
         // u >= 1 here, so truncation is the floor
-        const float u0 = fmaf(fty0, pc, t0), u1 = fmaf(fty1, pc, t0);
-        const float u2 = fmaf(fty2, pc, t0), u3 = fmaf(fty3, pc, t0);
-        const float2 s0 = wrow[(int)u0], s1 = wrow[(int)u1], s2 = wrow[(int)u2], s3 = wrow[(int)u3];
-        acc0 += fmaf(__builtin_amdgcn_fractf(u0), s0.y, s0.x);
-        acc1 += fmaf(__builtin_amdgcn_fractf(u1), s1.y, s1.x);
-        acc2 += fmaf(__builtin_amdgcn_fractf(u2), s2.y, s2.x);
-        acc3 += fmaf(__builtin_amdgcn_fractf(u3), s3.y, s3.x);
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+          const float u = fmaf(fty[i], pc, t0);
+          const float2 t = wrow[(int)u];
+          acc[i] += fmaf(__builtin_amdgcn_fractf(u), t.y, t.x);
+        }
       }
     } else {
       for (int jj = 0; jj < nj; ++jj) {
-        const BPAngle qa = par[jj];
+        const BPAngle qa = lds.par[jj];
         const int flag = __builtin_amdgcn_readfirstlane(qa.flag);
         if (flag == 0) continue;
-        const float2* wrow = win + jj * W;
+        const float2* wrow = lds.win + jj * W;
         const float t0 = fmaf(ftx, qa.b, qa.f0);
-        float us[4];
-        us[0] = fmaf(fty0, qa.c, t0);
-        us[1] = fmaf(fty1, qa.c, t0);
-        us[2] = fmaf(fty2, qa.c, t0);
-        us[3] = fmaf(fty3, qa.c, t0);
-        float vals[4];
+        float us[ROWS], vals[ROWS];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float u = us[i];
-          const float2 t = wrow[(int)u];
-          vals[i] = fmaf(__builtin_amdgcn_fractf(u), t.y, t.x);
+        for (int i = 0; i < ROWS; ++i) {
+          us[i] = fmaf(fty[i], qa.c, t0);
+          const float2 t = wrow[(int)us[i]];
+          vals[i] = fmaf(__builtin_amdgcn_fractf(us[i]), t.y, t.x);
         }
         if (flag == 2) {
           // boundary tile: per-pixel 0 <= m <= R-2 test; pixels within 1e-3 bins of an
@@ -586,7 +629,7 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
           const float lo_rel = (float)(-qa.base), hi_rel = (float)(R - 1 - qa.base);
           int keep = 0;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < ROWS; ++i) {
             const float u = us[i];
             const bool inside = (u > lo_rel + lo_rel_eps) && (u < hi_rel - lo_rel_eps);
             const bool outside = (u < lo_rel - lo_rel_eps) || (u > hi_rel + lo_rel_eps);
@@ -595,43 +638,72 @@ __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float
             keep |= ok << i;
           }
 #pragma unroll
-          for (int i = 0; i < 4; ++i) vals[i] = ((keep >> i) & 1) ? vals[i] : 0.f;
+          for (int i = 0; i < ROWS; ++i) vals[i] = ((keep >> i) & 1) ? vals[i] : 0.f;
         }
-        acc0 += vals[0];
-        acc1 += vals[1];
-        acc2 += vals[2];
-        acc3 += vals[3];
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) acc[i] += vals[i];
       }
     }
   }
   if (col >= n) return;
   const float dth = (float)p.g.dth;
-  float accs[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < ROWS; ++i) {
     const int row = r0 + ty + 8 * i;
     if (row >= n) continue;
-    const size_t idx = ((size_t)slice * n + row) * n + col;
-    const float bp = accs[i] * dth;
+    const size_t idx = (size_t)row * n + col;
+    const float bp = acc[i] * dth;
     if (MODE == SS_BP_PLAIN) {
       out[idx] = bp;
     } else {
-      const float wgt = em.weight[idx];
-      const float x = em.x[idx];
-      const float xn = wgt > 0.f ? x * bp / wgt : x;
-      em.x[idx] = xn;
+      const float wgt = ws[idx];
+      const float x = xs[idx];
+      xs[idx] = wgt > 0.f ? x * bp / wgt : x;
     }
   }
 }
 
+template <int MODE, int ROWS, int STAGE>
+__global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float* __restrict__ sinoT, int j0, int j1,
+                                                        float* __restrict__ out, SSEmUpdateArgs em) {
+  __shared__ SSBpLds<STAGE> lds;
+  const size_t slice = blockIdx.z;
+  const size_t so = slice * p.g.nangles * p.g.nrays;
+  const size_t io = slice * p.g.n * p.g.n;
+  ss_bp_item<MODE, ROWS, STAGE>(p, sinoT + so, j0, j1, blockIdx.x * BP_TILE_W, blockIdx.y * 8 * ROWS,
+                                MODE == SS_BP_PLAIN ? out + io : nullptr, MODE == SS_BP_PLAIN ? nullptr : em.x + io,
+                                MODE == SS_BP_PLAIN ? nullptr : em.weight + io, lds);
+}
+
+int ss_bp_tiles(int n, int variant) {
+  const int th = 8 * kSSBpRows[variant];
+  return ((n + BP_TILE_W - 1) / BP_TILE_W) * ((n + th - 1) / th);
+}
+
+// variant id -> (ROWS, STAGE); must match kSSBpRows / kSSBpStage
+#define SS_BP_FOR_VARIANTS(X) X(0, 4, 4096) X(1, 2, 2048) X(2, 8, 4096) X(3, 4, 2048) X(4, 2, 4096) X(5, 8, 2048)
+
 hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices, int mode,
                                     float* out, SSEmUpdateArgs em, hipStream_t stream) {
+  if (j1 <= j0 && mode == SS_BP_EM_UPDATE) return hipSuccess;
   const int n = p.g.n;
-  dim3 grid((n + BP_TILE - 1) / BP_TILE, (n + BP_TILE - 1) / BP_TILE, nslices);
-  if (mode == SS_BP_PLAIN)
-    hipLaunchKernelGGL(k_backprojection<SS_BP_PLAIN>, grid, dim3(256), 0, stream, p, sinoT, j0, j1, out, em);
-  else
-    hipLaunchKernelGGL(k_backprojection<SS_BP_EM_UPDATE>, grid, dim3(256), 0, stream, p, sinoT, j0, j1, out, em);
+  const int th = 8 * kSSBpRows[p.bp_variant];
+  dim3 grid((n + BP_TILE_W - 1) / BP_TILE_W, (n + th - 1) / th, nslices);
+#define SS_BP_LAUNCH(V, ROWS, STAGE)                                                                              \
+  case V:                                                                                                         \
+    if (mode == SS_BP_PLAIN)                                                                                      \
+      hipLaunchKernelGGL((k_backprojection<SS_BP_PLAIN, ROWS, STAGE>), grid, dim3(256), 0, stream, p, sinoT, j0, \
+                         j1, out, em);                                                                            \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_backprojection<SS_BP_EM_UPDATE, ROWS, STAGE>), grid, dim3(256), 0, stream, p, sinoT, \
+                         j0, j1, out, em);                                                                        \
+    break;
+  switch (p.bp_variant) {
+    SS_BP_FOR_VARIANTS(SS_BP_LAUNCH)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef SS_BP_LAUNCH
   return hipGetLastError();
 }
 
@@ -803,6 +875,272 @@ hipError_t ss_launch_fill(float* p, float v, size_t n, hipStream_t stream) {
   if (blocks > 65536) blocks = 65536;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, stream, p, v, n);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// batched execution: persistent kernels over the work items of many slices
+//
+// A launch covers entries (slice, angle range) of a batch whose slices differ in
+// n, R and A.  Items are numbered entry by entry; a fixed grid of resident blocks
+// takes items blockIdx.x, blockIdx.x + gridDim.x, ...  The host orders entries by
+// decreasing item cost, so the last items to run are the cheapest.  Each item runs
+// the same device function as the per-plan kernels, on that slice's plan and
+// buffers, so a slice's result does not depend on what it is batched with.
+// =====================================================================================
+
+// largest entry e with entries[e].item0 <= item (entries with no items are skipped
+// because a later entry shares their item0)
+__device__ __forceinline__ int ss_find_entry(const SSBatchEntry* __restrict__ e, int n, int item) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (e[mid].item0 <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_gather(SSBatchLaunch L, const float* __restrict__ images,
+                                                      float2* __restrict__ gA, float2* __restrict__ gB) {
+  __shared__ float T[33][34];
+  for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
+    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
+    const SSBatchSlice& s = L.slices[e.slice];
+    const int n = s.p.g.n;
+    const int tiles = (n + 2 + 31) / 32;
+    const int local = item - e.item0;
+    const int ty = local / tiles, tx = local - ty * tiles;
+    ss_gather_tile(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch, tx * 32, ty * 32, T);
+    __syncthreads();
+  }
+}
+
+// to_t: [R][A] -> [A][R] per slice, else [A][R] -> [R][A]
+__global__ __launch_bounds__(256) void k_batch_transpose(SSBatchLaunch L, const float* __restrict__ in,
+                                                         float* __restrict__ out, int to_t) {
+  __shared__ float tile[32][33];
+  for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
+    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
+    const SSBatchSlice& s = L.slices[e.slice];
+    const int rows = to_t ? s.p.g.nrays : s.p.g.nangles;
+    const int cols = to_t ? s.p.g.nangles : s.p.g.nrays;
+    const int tcols = (cols + 31) / 32;
+    const int local = item - e.item0;
+    const int ty = local / tcols, tx = local - ty * tcols;
+    ss_transpose_tile(in + s.sino_off, out + s.sino_off, rows, cols, tx * 32, ty * 32, tile);
+    __syncthreads();
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(512) void k_batch_radon(SSBatchLaunch L, const float2* __restrict__ gA,
+                                                     const float2* __restrict__ gB, int split, float* __restrict__ out,
+                                                     const float* __restrict__ meas, double* __restrict__ ll) {
+  __shared__ SSRadonLds lds;
+  for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
+    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
+    const SSBatchSlice& s = L.slices[e.slice];
+    const int local = item - e.item0;
+    const int ablk = local / s.p.nwaves, grp = local - ablk * s.p.nwaves;
+    ss_radon_item<MODE>(s.p, gA + s.lay_off, gB + s.lay_off, e.j0, e.j1, split, grp, ablk, out + s.sino_off,
+                        MODE == SS_OUT_EM_RATIO ? meas + s.sino_off : nullptr,
+                        MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, lds);
+    __syncthreads();
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_batch_radon_fixup(SSBatchLaunch L, const float2* __restrict__ gA,
+                                                           const float2* __restrict__ gB, float* __restrict__ out,
+                                                           const float* __restrict__ meas, double* __restrict__ ll) {
+  __shared__ float redf[4];
+  for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
+    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
+    const SSBatchSlice& s = L.slices[e.slice];
+    ss_radon_fixup_item<MODE>(s.p, gA + s.lay_off, gB + s.lay_off, e.j0, e.j1, item - e.item0, out + s.sino_off,
+                              MODE == SS_OUT_EM_RATIO ? meas + s.sino_off : nullptr,
+                              MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, redf);
+    __syncthreads();
+  }
+}
+
+template <int MODE, int ROWS, int STAGE>
+__global__ __launch_bounds__(256) void k_batch_bp(SSBatchLaunch L, const float* __restrict__ sinoT,
+                                                  float* __restrict__ out, float* __restrict__ x,
+                                                  const float* __restrict__ weight) {
+  __shared__ SSBpLds<STAGE> lds;
+  constexpr int TH = 8 * ROWS;
+  for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
+    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
+    const SSBatchSlice& s = L.slices[e.slice];
+    const int tiles_x = (s.p.g.n + BP_TILE_W - 1) / BP_TILE_W;
+    const int local = item - e.item0;
+    const int ty = local / tiles_x, tx = local - ty * tiles_x;
+    ss_bp_item<MODE, ROWS, STAGE>(s.p, sinoT + s.sino_off, e.j0, e.j1, tx * BP_TILE_W, ty * TH,
+                                  MODE == SS_BP_PLAIN ? out + s.img_off : nullptr,
+                                  MODE == SS_BP_PLAIN ? nullptr : x + s.img_off,
+                                  MODE == SS_BP_PLAIN ? nullptr : weight + s.img_off, lds);
+    __syncthreads();
+  }
+}
+
+// resident blocks per CU of each batched kernel, from the runtime's occupancy query
+namespace {
+template <class K>
+int ss_resident_blocks(K kernel, int block) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  return per_cu * cus;
+}
+}  // namespace
+
+int ss_batch_grid(int kind, int variant, int mode, int total) {
+  if (total <= 0) return 0;
+  int g = 0;
+  switch (kind) {
+    case SS_KIND_GATHER:
+      g = ss_resident_blocks(k_batch_gather, 256);
+      break;
+    case SS_KIND_TRANSPOSE:
+      g = ss_resident_blocks(k_batch_transpose, 256);
+      break;
+    case SS_KIND_RADON:
+      g = mode == SS_OUT_EM_RATIO ? ss_resident_blocks(k_batch_radon<SS_OUT_EM_RATIO>, 512)
+                                  : ss_resident_blocks(k_batch_radon<SS_OUT_SINOT>, 512);
+      break;
+    case SS_KIND_FIXUP:
+      g = ss_resident_blocks(k_batch_radon_fixup<SS_OUT_EM_RATIO>, 256);
+      break;
+    default: {
+#define SS_BP_OCC(V, ROWS, STAGE)                                                               \
+  case V:                                                                                       \
+    g = mode == SS_BP_PLAIN ? ss_resident_blocks(k_batch_bp<SS_BP_PLAIN, ROWS, STAGE>, 256)     \
+                            : ss_resident_blocks(k_batch_bp<SS_BP_EM_UPDATE, ROWS, STAGE>, 256); \
+    break;
+      switch (variant) {
+        SS_BP_FOR_VARIANTS(SS_BP_OCC)
+        default:
+          g = 256;
+      }
+#undef SS_BP_OCC
+    }
+  }
+  return g < total ? g : total;
+}
+
+hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, const float* images, float2* gatherA, float2* gatherB,
+                                  hipStream_t stream) {
+  if (L.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_gather, dim3(L.grid), dim3(256), 0, stream, L, images, gatherA, gatherB);
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_batch_transpose(const SSBatchLaunch& L, const float* in, float* out, int to_angle_major,
+                                     hipStream_t stream) {
+  if (L.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_transpose, dim3(L.grid), dim3(256), 0, stream, L, in, out, to_angle_major);
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_batch_radon(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, int split,
+                                 int mode, float* out, const float* meas, double* ll_partials, hipStream_t stream) {
+  if (L.total <= 0) return hipSuccess;
+  const dim3 grid(L.grid), block(RADON_WAVES * SS_WAVE);
+  switch (mode) {
+    case SS_OUT_SINO:
+      hipLaunchKernelGGL(k_batch_radon<SS_OUT_SINO>, grid, block, 0, stream, L, gatherA, gatherB, split, out, meas,
+                         ll_partials);
+      break;
+    case SS_OUT_SINOT:
+      hipLaunchKernelGGL(k_batch_radon<SS_OUT_SINOT>, grid, block, 0, stream, L, gatherA, gatherB, split, out, meas,
+                         ll_partials);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batch_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, L, gatherA, gatherB, split, out,
+                         meas, ll_partials);
+  }
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_batch_radon_fixup(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, int mode,
+                                       float* out, const float* meas, double* ll_partials, hipStream_t stream) {
+  if (L.total <= 0) return hipSuccess;
+  const dim3 grid(L.grid), block(256);
+  switch (mode) {
+    case SS_OUT_SINO:
+      hipLaunchKernelGGL(k_batch_radon_fixup<SS_OUT_SINO>, grid, block, 0, stream, L, gatherA, gatherB, out, meas,
+                         ll_partials);
+      break;
+    case SS_OUT_SINOT:
+      hipLaunchKernelGGL(k_batch_radon_fixup<SS_OUT_SINOT>, grid, block, 0, stream, L, gatherA, gatherB, out, meas,
+                         ll_partials);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batch_radon_fixup<SS_OUT_EM_RATIO>, grid, block, 0, stream, L, gatherA, gatherB, out,
+                         meas, ll_partials);
+  }
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_batch_backprojection(const SSBatchLaunch& L, int variant, const float* sinoT, int mode,
+                                          float* out, float* x, const float* weight, hipStream_t stream) {
+  if (L.total <= 0) return hipSuccess;
+  const dim3 grid(L.grid), block(256);
+#define SS_BP_BATCH(V, ROWS, STAGE)                                                                            \
+  case V:                                                                                                      \
+    if (mode == SS_BP_PLAIN)                                                                                   \
+      hipLaunchKernelGGL((k_batch_bp<SS_BP_PLAIN, ROWS, STAGE>), grid, block, 0, stream, L, sinoT, out, x,     \
+                         weight);                                                                              \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_batch_bp<SS_BP_EM_UPDATE, ROWS, STAGE>), grid, block, 0, stream, L, sinoT, out, x, \
+                         weight);                                                                              \
+    break;
+  switch (variant) {
+    SS_BP_FOR_VARIANTS(SS_BP_BATCH)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef SS_BP_BATCH
+  return hipGetLastError();
+}
+
+// one block per slice: the slice's partial ranges summed in a fixed order
+__global__ __launch_bounds__(256) void k_batch_ll(const int2* __restrict__ ranges, int nranges,
+                                                  const double* __restrict__ partials, double* __restrict__ ll_out) {
+  __shared__ double red[4];
+  const size_t slice = blockIdx.x;
+  double v = 0.0;
+  for (int r = 0; r < nranges; ++r) {
+    const int2 rg = ranges[slice * nranges + r];
+    for (int i = threadIdx.x; i < rg.y; i += 256) v += partials[(size_t)rg.x + i];
+  }
+  const double tot = ss_block_sum_f64(v, red);
+  if (threadIdx.x == 0) ll_out[slice] = tot;
+}
+
+hipError_t ss_launch_batch_ll(const int2* ranges, int nranges, int nslices, const double* partials, double* ll_out,
+                              hipStream_t stream) {
+  if (nslices <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_ll, dim3(nslices), dim3(256), 0, stream, ranges, nranges, partials, ll_out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fill_f64(double* __restrict__ p, double v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
+hipError_t ss_launch_fill_f64(double* p, double v, size_t n, hipStream_t stream) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)blocks), dim3(256), 0, stream, p, v, n);
   return hipGetLastError();
 }
 

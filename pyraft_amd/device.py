@@ -15,7 +15,7 @@ import math
 
 import torch
 
-from .raftypes import check, libraft
+from .raftypes import BP_NVARIANTS, TUNE_FORCE, check, libraft
 
 DEFAULT_IMG_CORNERS = (-1.0, 1.0, 1.0, -1.0)
 DEFAULT_SINO_CORNERS = (0.0, 1.0, math.pi, -1.0)
@@ -87,10 +87,24 @@ class Plan:
         return self._handle
 
     def info(self):
-        out = (ctypes.c_int * 7)()
-        check(libraft.raft_ss_plan_info(self._handle, out, 7), "raft_ss_plan_info")
-        keys = ("n", "nrays", "nangles", "irregular_rays", "degenerate_angles", "bp_window", "bp_angles")
+        out = (ctypes.c_int * 8)()
+        check(libraft.raft_ss_plan_info(self._handle, out, 8), "raft_ss_plan_info")
+        keys = ("n", "nrays", "nangles", "irregular_rays", "degenerate_angles", "bp_window", "bp_angles",
+                "bp_variant")
         return dict(zip(keys, list(out)))
+
+    def set_bp_variant(self, variant):
+        """Use backprojection kernel variant ``variant`` (see ``bp_variant_shape``)."""
+        check(libraft.raft_ss_plan_set_bp_variant(self._handle, int(variant)), "raft_ss_plan_set_bp_variant")
+
+    def tune(self, reps=3, force=False):
+        """Time every backprojection variant on this geometry, keep the fastest.
+        Returns (variant, {variant: microseconds}) (variants that do not fit are left out)."""
+        times = (ctypes.c_double * BP_NVARIANTS)()
+        with torch.cuda.device(self.device):
+            v = libraft.raft_ss_plan_tune(self._handle, int(reps), TUNE_FORCE if force else 0, times, self._stream())
+        check(v if v < 0 else 0, "raft_ss_plan_tune")
+        return v, {i: times[i] for i in range(BP_NVARIANTS) if times[i] >= 0}
 
     def bounds(self):
         import numpy
@@ -188,6 +202,15 @@ class Plan:
         if return_ll:
             return x, [ll[i] for i in range(int(niter))]
         return x
+
+
+def bp_variant_shape(variant):
+    """(pixel rows per thread, LDS stage pairs) of backprojection variant ``variant``;
+    the block tile is 32 columns x 8*rows rows."""
+    rows, stage = ctypes.c_int(), ctypes.c_int()
+    check(libraft.raft_ss_bp_variant_shape(int(variant), ctypes.byref(rows), ctypes.byref(stage)),
+          "raft_ss_bp_variant_shape")
+    return rows.value, stage.value
 
 
 def ramp_kernel(nrays, tmin=-1.0, tmax=1.0):

@@ -139,6 +139,25 @@ _decl("raft_ss_em", [_vp, _vp, _vp, _i, _i, _i, _d4, _vp], _i)
 _decl("raft_ss_phantom", [_d4, _i, _i, _i, _d4, _vp, _vp], _i)
 _decl("raft_ss_ramp_kernel", [_i, ctypes.c_double, ctypes.c_double, _c_float_p], _i)
 _decl("raft_ss_filter_rows", [_vp, _vp, _vp, _i, _i, _vp], _i)
+_decl("raft_ss_plan_host_bp_staging", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i)
+_decl("raft_ss_bp_variant_shape", [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i)
+_decl("raft_ss_plan_set_bp_variant", [_vp, _i], _i)
+_decl("raft_ss_plan_tune", [_vp, _i, _i, _d4, _vp], _i)
+_decl("raft_ss_batch_create", [ctypes.POINTER(_vp), _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                               _d4, _d4, _vp], _i)
+_decl("raft_ss_batch_destroy", [_vp], _i)
+_decl("raft_ss_batch_layout", [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                               ctypes.POINTER(ctypes.c_longlong)], _i)
+_decl("raft_ss_batch_info", [_vp, _i, ctypes.POINTER(_i), _i], _i)
+_decl("raft_ss_batch_set_bp_variant", [_vp, _i, _i], _i)
+_decl("raft_ss_batch_tune", [_vp, _i, _i, _vp], _i)
+_decl("raft_ss_batch_radon", [_vp, _vp, _vp, _i, _i, _vp], _i)
+_decl("raft_ss_batch_backprojection", [_vp, _vp, _vp, _i, _i, _vp], _i)
+_decl("raft_ss_batch_em", [_vp, _vp, _vp, _i, _i, _d4, _i, _vp], _i)
+
+BP_NVARIANTS = 6          # RAFT_SS_BP_NVARIANTS
+TUNE_FORCE = 1            # RAFT_SS_TUNE_FORCE
+BATCH_GRAPH = 1           # RAFT_SS_BATCH_GRAPH
 
 
 class RaftError(RuntimeError):
@@ -303,4 +322,8 @@ EXPORTED_SYMBOLS = (
     "raft_ss_radon", "raft_ss_radon_range", "raft_ss_backprojection", "raft_ss_backprojection_range",
     "raft_ss_radon_transpose", "raft_ss_radon_transpose_range", "raft_ss_fbp_filter", "raft_ss_fbp",
     "raft_ss_em", "raft_ss_phantom", "raft_ss_ramp_kernel", "raft_ss_filter_rows",
+    "raft_ss_plan_host_bp_staging", "raft_ss_bp_variant_shape", "raft_ss_plan_set_bp_variant", "raft_ss_plan_tune",
+    "raft_ss_batch_create", "raft_ss_batch_destroy", "raft_ss_batch_layout", "raft_ss_batch_info",
+    "raft_ss_batch_set_bp_variant", "raft_ss_batch_tune", "raft_ss_batch_radon", "raft_ss_batch_backprojection",
+    "raft_ss_batch_em",
 )

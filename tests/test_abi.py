@@ -118,11 +118,26 @@ def test_large_plan_index_ranges(lib, n, A):
     assert np.all((lo >= 0) & (hi <= n - 1) | ~ok)
 
 
-def _bp_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC):
-    info = (ctypes.c_int * 5)()
-    assert lib.raft_ss_plan_host_info(n, R, A, (ctypes.c_double * 4)(*ic), (ctypes.c_double * 4)(*sc), info) == 0
-    W, J = info[1], info[2]
-    assert 1 <= J <= 64 and J * W <= 4096
+BP_ROWS = (4, 2, 8, 4, 2, 8)          # pixel rows per thread of each backprojection variant
+BP_STAGE = (4096, 2048, 4096, 2048, 4096, 2048)
+
+
+def _bp_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC, variant=0):
+    """Emulate the kernel's FP32 window index for every pixel tile of variant
+    `variant` (32 columns x 8*rows rows) at the tile corners; return the extremes
+    and the window W the library stages."""
+    W6 = (ctypes.c_int * 6)()
+    J6 = (ctypes.c_int * 6)()
+    assert lib.raft_ss_plan_host_bp_staging(n, R, A, (ctypes.c_double * 4)(*ic), (ctypes.c_double * 4)(*sc),
+                                            W6, J6) == 0
+    W, J = W6[variant], J6[variant]
+    if J == 0:
+        return None
+    assert 1 <= J <= 64 and J * W <= BP_STAGE[variant]
+    rows, stage = ctypes.c_int(), ctypes.c_int()
+    assert lib.raft_ss_bp_variant_shape(variant, ctypes.byref(rows), ctypes.byref(stage)) == 0
+    assert (rows.value, stage.value) == (BP_ROWS[variant], BP_STAGE[variant])
+    tile_h = 8 * rows.value
     xmin, ymin = ic[0], ic[3]
     dx = (ic[2] - ic[0]) / (n - 1)
     dy = (ic[1] - ic[3]) / (n - 1)
@@ -134,23 +149,29 @@ def _bp_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC):
     lo, hi = np.inf, -np.inf
     f32 = np.float32
     for c0 in range(0, n, 32):
-        for r0 in range(0, n, 32):
+        for r0 in range(0, n, tile_h):
             U = bpa + c0 * bpb + r0 * bpc
-            umin = U + np.minimum(0, 31 * bpb) + np.minimum(0, 31 * bpc)
+            umin = U + np.minimum(0, 31 * bpb) + np.minimum(0, (tile_h - 1) * bpc)
             base = np.floor(umin) - 1
             f0 = (U - base).astype(f32)
             for tx in (0, 31):
                 t0 = (np.float64(tx) * bpb.astype(f32).astype(np.float64) + f0).astype(f32)
-                for ty in (0, 31):
+                for ty in (0, tile_h - 1):
                     u = (np.float64(ty) * bpc.astype(f32).astype(np.float64) + t0).astype(f32)
                     ii = np.floor(u)
                     lo, hi = min(lo, ii.min()), max(hi, (ii + 1).max())
     return lo, hi, W
 
 
+@pytest.mark.parametrize("variant", range(6))
 @pytest.mark.parametrize("n,R,A", [(64, 64, 45), (65, 70, 37), (512, 512, 720), (2048, 2048, 1500), (100, 300, 50)])
-def test_backprojection_window_in_bounds(lib, n, R, A):
-    lo, hi, W = _bp_window_extremes(lib, n, R, A)
+def test_backprojection_window_in_bounds(lib, n, R, A, variant):
+    """The kernel reads wrow[(int)u] and the pair's difference to ray (int)u + 1: both
+    inside the staged window, and u >= 0 so truncation equals the floor."""
+    r = _bp_window_extremes(lib, n, R, A, variant=variant)
+    if r is None:
+        pytest.skip("window does not fit this variant's stage")
+    lo, hi, W = r
     assert lo >= 0 and hi <= W - 1, (lo, hi, W)
 
 
