@@ -87,10 +87,14 @@ def _tv_grad(x, eps):
     return g
 
 
-def operator_norm_sq(plan, iters=12, angle_major=False, device=None):
-    """||R||_2^2 by power iteration on R^T R (a lower bound that converges from below)."""
+def operator_norm_sq(plan, iters=12, angle_major=False, device=None, seed=0):
+    """||R||_2^2 by power iteration on R^T R (a lower bound that converges from below),
+    from a seeded random start so that repeated calls return the same value."""
+    dev = torch.device(device) if device is not None else plan.device
     with torch.no_grad():
-        v = torch.rand((plan.n, plan.n), device=device or plan.device, dtype=torch.float32)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(seed))
+        v = torch.rand((plan.n, plan.n), device=dev, dtype=torch.float32, generator=gen)
         lam = 0.0
         for _ in range(iters):
             v = v / v.norm()

@@ -195,11 +195,24 @@ __device__ __forceinline__ double ss_block_sum_f64(double v, double* scratch) {
   return r;  // valid in thread 0
 }
 
+// The plan fields the forward kernel reads (kept small: in the batched kernels they
+// are loaded per item into scalar registers).
+struct SSRadonView {
+  const SSAngle* angles;
+  const int2* bounds;
+  const int4* wbounds;
+  int n, R, A, pitch, nwaves;
+};
+
+__device__ __forceinline__ SSRadonView ss_radon_view(const SSDevPlan& p) {
+  return SSRadonView{p.angles, p.bounds, p.wbounds, p.g.n, p.g.nrays, p.g.nangles, p.g.pitch, p.nwaves};
+}
+
 // out / meas point at this slice's sinogram
 template <int MODE>
-__device__ __forceinline__ void ss_radon_store(const SSDevPlan& p, int j, int k, float val, float* out,
+__device__ __forceinline__ void ss_radon_store(const SSRadonView& p, int j, int k, float val, float* out,
                                                const float* meas, double* ll) {
-  const int R = p.g.nrays, A = p.g.nangles;
+  const int R = p.R, A = p.A;
   if (MODE == SS_OUT_SINO) {
     out[(size_t)k * A + j] = val;
   } else if (MODE == SS_OUT_SINOT) {
@@ -237,17 +250,108 @@ __device__ __forceinline__ ss_f32x2 ss_pair(__amdgpu_buffer_rsrc_t rs, uint32_t 
   return ss_struct_load_f32x2(rs, rec, 0, (int)row_off, 0);
 }
 
+#define RADON_LDS_CAP 4096  // staged (tap, difference) records per block (32 KiB)
+#define RADON_CHUNK 32      // image rows (gather rows) per staged chunk, at most
+
 struct SSRadonLds {
   float comb_a[RADON_WAVES][SS_WAVE];
   float comb_d[RADON_WAVES][SS_WAVE];
   double red[RADON_WAVES];
+  int wmeta[RADON_WAVES][4];  // per wave: branch (-1: no samples), first row, last row
+  int wwin[RADON_WAVES][2];   // per wave: lowest and highest cell of the current chunk
+  float2 stage[RADON_LDS_CAP + 1];  // chunk window; the last record stays (0, 0)
 };
+
+// wave-uniform copies (scalar registers) of values the compiler cannot prove uniform
+__device__ __forceinline__ int ss_rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t ss_rfl64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float ss_rflf(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+// Sample readers: (row s, trajectory u) -> (lower tap, difference).
+// Global: u is the Q32.32 trajectory plus one cell, its high word the record index.
+struct SSGatherReader {
+  __amdgpu_buffer_rsrc_t rs;
+  uint32_t row_bytes;
+  __device__ __forceinline__ float2 operator()(int s, int64_t u) const {
+    const ss_f32x2 t = ss_pair(rs, (uint32_t)s * row_bytes, (int)(u >> 32));
+    return make_float2(t.x, t.y);
+  }
+};
+// LDS: u carries the row offset of the staged window in its high word already.
+struct SSStageReader {
+  const float2* st;
+  __device__ __forceinline__ float2 operator()(int, int64_t u) const { return st[(int)(u >> 32)]; }
+};
+
+// Samples [sa, sb) of one lane's ray: predicated before zlo and from zhi1 on (outside
+// the group's common range), unpredicated in between with 4 samples in flight, each
+// on its own trajectory register (one 64-bit add per sample; the empty asm keeps the
+// compiler from re-deriving the four from one base, which costs extra adds).
+template <class Rd>
+__device__ __forceinline__ void ss_radon_rows(const Rd& rd, int sa, int sb, int zlo, int zhi1, int64_t u,
+                                              int64_t beta, int lo, unsigned span, float& acc_a, float& acc_d) {
+  int s = sa;
+  const int h_end = min(sb, zlo);
+  for (; s < h_end; ++s) {
+    if ((unsigned)(s - lo) <= span) {
+      const float2 t = rd(s, u);
+      acc_a += t.x;
+      acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
+    }
+    u += beta;
+  }
+  const int m_end = min(sb, zhi1);
+  if (s + 4 <= m_end) {
+    int64_t u0 = u, u1 = u + beta, u2 = u + 2 * beta, u3 = u + 3 * beta;
+    const int64_t beta4 = 4 * beta;
+    for (; s + 4 <= m_end; s += 4) {
+      const float2 t0 = rd(s, u0), t1 = rd(s + 1, u1), t2 = rd(s + 2, u2), t3 = rd(s + 3, u3);
+      acc_a += (t0.x + t1.x) + (t2.x + t3.x);
+      acc_d = fmaf((float)(uint32_t)u0, t0.y, acc_d);
+      acc_d = fmaf((float)(uint32_t)u1, t1.y, acc_d);
+      acc_d = fmaf((float)(uint32_t)u2, t2.y, acc_d);
+      acc_d = fmaf((float)(uint32_t)u3, t3.y, acc_d);
+      u0 += beta4;
+      u1 += beta4;
+      u2 += beta4;
+      u3 += beta4;
+      __asm__ volatile("" : "+v"(u0), "+v"(u1), "+v"(u2), "+v"(u3));
+    }
+    u = u0;
+  }
+  for (; s < m_end; ++s) {
+    const float2 t = rd(s, u);
+    acc_a += t.x;
+    acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
+    u += beta;
+  }
+  for (; s < sb; ++s) {
+    if ((unsigned)(s - lo) <= span) {
+      const float2 t = rd(s, u);
+      acc_a += t.x;
+      acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
+    }
+    u += beta;
+  }
+}
 
 // One block's work: rays [grp*64, grp*64+64) of the RADON_WAVES/split angles starting
 // at j0 + ablk*(RADON_WAVES/split), clipped to [j0, j1).  gA/gB/out/meas point at this
 // slice's data; in EM mode the block's FP64 log-likelihood partial goes to *ll_slot.
+//
+// split == 1 and all angles of the block on one branch (the usual case): the block
+// walks its sample rows in chunks of up to RADON_CHUNK rows.  For each chunk it
+// bounds the cells every wave can touch (the trajectory is affine in ray and row, so
+// the extremes are at the corners), copies that window of the gather rows into LDS
+// with contiguous loads, and the waves interpolate from LDS.  A window too wide for
+// LDS falls back to direct gathers for that chunk.  Otherwise (split > 1, or a block
+// straddling the branch switch) every wave gathers directly from the layout.
 template <int MODE>
-__device__ __forceinline__ void ss_radon_item(const SSDevPlan& p, const float2* __restrict__ gA,
+__device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2* __restrict__ gA,
                                               const float2* __restrict__ gB, int j0, int j1, int split, int grp,
                                               int ablk, float* __restrict__ out, const float* __restrict__ meas,
                                               double* ll_slot, SSRadonLds& lds) {
@@ -256,21 +360,129 @@ __device__ __forceinline__ void ss_radon_item(const SSDevPlan& p, const float2* 
   const int per_block = RADON_WAVES / split;
   const int aw = wv / split, part = wv - aw * split;
   const int j = j0 + ablk * per_block + aw;
-  const int R = p.g.nrays, n = p.g.n, pitch = p.g.pitch;
+  const int R = p.R, n = p.n, pitch = p.pitch;
   const int k = grp * SS_WAVE + lane;
   const bool active = j < j1;
   float acc_a = 0.f, acc_d = 0.f;
   float delta = 0.f;
+  int branch = -1;
+  int4 wb = make_int4(0, -1, 0, -1);
+  int lo = 0, hi = -1;
+  int64_t alpha = 0, gamma = 0, beta = 0;
   if (active) {
     const SSAngle& a = p.angles[j];
-    delta = a.delta;
-    const int4 wb = p.wbounds[(size_t)j * p.nwaves + grp];
-    int lo = 0, hi = -1;
+    delta = ss_rflf(a.delta);
+    alpha = ss_rfl64(a.alpha0);
+    gamma = ss_rfl64(a.gamma);
+    beta = ss_rfl64(a.beta);
+    const int4 w4 = p.wbounds[(size_t)j * p.nwaves + grp];
+    wb = make_int4(ss_rfl(w4.x), ss_rfl(w4.y), ss_rfl(w4.z), ss_rfl(w4.w));
+    if (wb.y >= wb.x) branch = ss_rfl(a.branch_a);
     if (k < R) {
       const int2 b = p.bounds[(size_t)j * R + k];
       lo = b.x;
       hi = b.y;
     }
+  }
+  const bool empty = hi < lo;
+  const unsigned span = (unsigned)(hi - lo);
+  const uint32_t row_bytes = (uint32_t)pitch * 8u;
+
+  bool staged = false;
+  if (split == 1) {
+    if (lane == 0) {
+      lds.wmeta[wv][0] = branch;
+      lds.wmeta[wv][1] = wb.x;
+      lds.wmeta[wv][2] = wb.y;
+    }
+    if (threadIdx.x == 0) lds.stage[RADON_LDS_CAP] = make_float2(0.f, 0.f);
+    __syncthreads();
+    int bmin = 2, bmax = -1, s_lo = 0x7fffffff, s_hi = -1;
+    for (int w = 0; w < RADON_WAVES; ++w) {
+      const int b = lds.wmeta[w][0];
+      if (b >= 0) {
+        bmin = min(bmin, b);
+        bmax = max(bmax, b);
+        s_lo = min(s_lo, lds.wmeta[w][1]);
+        s_hi = max(s_hi, lds.wmeta[w][2]);
+      }
+    }
+    bmin = ss_rfl(bmin);
+    bmax = ss_rfl(bmax);
+    s_lo = ss_rfl(s_lo);
+    s_hi = ss_rfl(s_hi);
+    staged = bmax < 0 || bmin == bmax;  // block-uniform
+    if (staged && bmax >= 0) {
+      const float2* lay = bmax ? gA : gB;
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)lay, (short)sizeof(float2), (int)((uint32_t)n * (uint32_t)pitch * 8u), SS_RSRC_DWORD3);
+      const SSGatherReader grd{rs, row_bytes};
+      const SSStageReader srd{lds.stage};
+      const int k_first = grp * SS_WAVE, k_last = min(grp * SS_WAVE + SS_WAVE - 1, R - 1);
+      for (int s0 = s_lo; s0 <= s_hi;) {
+        int s1 = min(s0 + RADON_CHUNK, s_hi + 1);
+        // this wave's cells over its rows of the chunk (valid samples have m in [-1, n-1])
+        const int ra = max(s0, wb.x), rb = min(s1 - 1, wb.y);
+        if (lane == 0) {
+          int cmin = 0x7fffffff, cmax = -0x7fffffff;
+          if (branch >= 0 && ra <= rb) {
+            const int64_t c00 = alpha + (int64_t)k_first * gamma + (int64_t)ra * beta;
+            const int64_t c10 = alpha + (int64_t)k_last * gamma + (int64_t)ra * beta;
+            const int64_t ds = (int64_t)(rb - ra) * beta;
+            const int m0 = (int)(c00 >> 32), m1 = (int)(c10 >> 32);
+            const int m2 = (int)((c00 + ds) >> 32), m3 = (int)((c10 + ds) >> 32);
+            cmin = max(min(min(m0, m1), min(m2, m3)), -1);
+            cmax = min(max(max(m0, m1), max(m2, m3)), n - 1);
+          }
+          lds.wwin[wv][0] = cmin;
+          lds.wwin[wv][1] = cmax;
+        }
+        __syncthreads();  // windows published; every wave is done with the previous chunk
+        int c_lo = 0x7fffffff, c_hi = -0x7fffffff;
+        for (int w = 0; w < RADON_WAVES; ++w) {
+          c_lo = min(c_lo, lds.wwin[w][0]);
+          c_hi = max(c_hi, lds.wwin[w][1]);
+        }
+        c_lo = ss_rfl(c_lo);
+        c_hi = ss_rfl(c_hi);
+        const int Wc = c_hi - c_lo + 1;
+        const int rows = Wc > 0 ? min(s1 - s0, RADON_LDS_CAP / Wc) : 0;
+        if (Wc > 0 && rows > 0) {
+          s1 = s0 + rows;
+          for (int idx = threadIdx.x; idx < rows * Wc; idx += RADON_WAVES * SS_WAVE) {
+            const int r = idx / Wc, c = idx - r * Wc;
+            lds.stage[idx] = lay[(size_t)(s0 + r) * pitch + (c_lo + c + 1)];
+          }
+        }
+        __syncthreads();  // window staged
+        const int ra2 = max(s0, wb.x), rb2 = min(s1 - 1, wb.y);
+        if (branch >= 0 && ra2 <= rb2) {
+          if (Wc > 0 && rows > 0) {
+            // high word: staged record index (cell - c_lo) + (row - s0) * Wc
+            const int64_t rowstep = (int64_t)Wc << 32;
+            int64_t u = alpha + (int64_t)k * gamma + (int64_t)ra2 * beta - ((int64_t)c_lo << 32) +
+                        (int64_t)(ra2 - s0) * rowstep;
+            int64_t step = beta + rowstep;
+            if (empty) {
+              u = (int64_t)RADON_LDS_CAP << 32;
+              step = 0;
+            }
+            ss_radon_rows(srd, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
+          } else if (Wc > 0) {
+            int64_t u = alpha + (int64_t)k * gamma + (int64_t)ra2 * beta + ((int64_t)1 << 32);
+            int64_t step = beta;
+            if (empty) {
+              u = 0;
+              step = 0;
+            }
+            ss_radon_rows(grd, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
+          }
+        }
+        s0 = s1;
+      }
+    }
+  }
+  if (!staged && active && branch >= 0) {
     // this wave's share [sa, sb) of the group's sample range [wb.x, wb.y]
     const int len = wb.y - wb.x + 1;
     const int chunk = (len + split - 1) / split;
@@ -278,71 +490,17 @@ __device__ __forceinline__ void ss_radon_item(const SSDevPlan& p, const float2* 
     const int sb = min(wb.y + 1, sa + chunk);
     // u carries the trajectory plus one cell: its high word is the record index m + 1
     // of the lower tap in the gather row, its low word the interpolation weight
-    int64_t beta = a.beta;
-    int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)sa * beta + ((int64_t)1 << 32);
-    if (hi < lo) {
+    int64_t u = alpha + (int64_t)k * gamma + (int64_t)sa * beta + ((int64_t)1 << 32);
+    int64_t step = beta;
+    if (empty) {
       // an empty ray reads record 0 (lower tap 0) with weight 0: exactly 0 in the unpredicated loop
       u = 0;
-      beta = 0;
+      step = 0;
     }
-    // buffer of 8-byte records over this slice's gather layout
-    const float2* lay = a.branch_a ? gA : gB;
+    const float2* lay = branch ? gA : gB;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void*)lay, (short)sizeof(float2), (int)((uint32_t)n * (uint32_t)pitch * 8u), SS_RSRC_DWORD3);
-    const uint32_t row_bytes = (uint32_t)pitch * 8u;
-    const unsigned span = (unsigned)(hi - lo);
-    int s = sa;
-    // head: predicated (before the intersection of the group's ranges)
-    const int h_end = min(sb, wb.z);
-    for (; s < h_end; ++s) {
-      if ((unsigned)(s - lo) <= span) {
-        const ss_f32x2 t = ss_pair(rs, (uint32_t)s * row_bytes, (int)(u >> 32));
-        acc_a += t.x;
-        acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
-      }
-      u += beta;
-    }
-    // middle: every lane valid; 4 samples in flight per iteration, each on its own
-    // trajectory register (one 64-bit add per sample; the empty asm keeps the compiler
-    // from re-deriving the four from one base, which costs extra adds)
-    const int m_end = min(sb, wb.w + 1);
-    if (s + 4 <= m_end) {
-      int64_t u0 = u, u1 = u + beta, u2 = u + 2 * beta, u3 = u + 3 * beta;
-      const int64_t beta4 = 4 * beta;
-      for (; s + 4 <= m_end; s += 4) {
-        const uint32_t off = (uint32_t)s * row_bytes;
-        const ss_f32x2 t0 = ss_pair(rs, off, (int)(u0 >> 32));
-        const ss_f32x2 t1 = ss_pair(rs, off + row_bytes, (int)(u1 >> 32));
-        const ss_f32x2 t2 = ss_pair(rs, off + 2 * row_bytes, (int)(u2 >> 32));
-        const ss_f32x2 t3 = ss_pair(rs, off + 3 * row_bytes, (int)(u3 >> 32));
-        acc_a += (t0.x + t1.x) + (t2.x + t3.x);
-        acc_d = fmaf((float)(uint32_t)u0, t0.y, acc_d);
-        acc_d = fmaf((float)(uint32_t)u1, t1.y, acc_d);
-        acc_d = fmaf((float)(uint32_t)u2, t2.y, acc_d);
-        acc_d = fmaf((float)(uint32_t)u3, t3.y, acc_d);
-        u0 += beta4;
-        u1 += beta4;
-        u2 += beta4;
-        u3 += beta4;
-        __asm__ volatile("" : "+v"(u0), "+v"(u1), "+v"(u2), "+v"(u3));
-      }
-      u = u0;
-    }
-    for (; s < m_end; ++s) {
-      const ss_f32x2 t = ss_pair(rs, (uint32_t)s * row_bytes, (int)(u >> 32));
-      acc_a += t.x;
-      acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
-      u += beta;
-    }
-    // tail: predicated
-    for (; s < sb; ++s) {
-      if ((unsigned)(s - lo) <= span) {
-        const ss_f32x2 t = ss_pair(rs, (uint32_t)s * row_bytes, (int)(u >> 32));
-        acc_a += t.x;
-        acc_d = fmaf((float)(uint32_t)u, t.y, acc_d);
-      }
-      u += beta;
-    }
+    ss_radon_rows(SSGatherReader{rs, row_bytes}, sa, sb, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
   }
   if (split > 1) {
     lds.comb_a[wv][lane] = acc_a;
@@ -376,7 +534,7 @@ __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __rest
   const size_t so = slice * p.g.nangles * p.g.nrays;
   double* slot = nullptr;
   if (MODE == SS_OUT_EM_RATIO) slot = em.ll_partials + (slice * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  ss_radon_item<MODE>(p, gA + lay, gB + lay, j0, j1, split, blockIdx.x, blockIdx.y, out + so,
+  ss_radon_item<MODE>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, split, blockIdx.x, blockIdx.y, out + so,
                       MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr, slot, lds);
 }
 
@@ -414,7 +572,7 @@ __device__ __forceinline__ void ss_radon_fixup_item(const SSDevPlan& p, const fl
     float tot = 0.f;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) tot += redf[i];
     double ll = 0.0;
-    ss_radon_store<MODE>(p, j, k, tot * a.delta, out, meas, &ll);
+    ss_radon_store<MODE>(ss_radon_view(p), j, k, tot * a.delta, out, meas, &ll);
     if (MODE == SS_OUT_EM_RATIO) *ll_slot = ll;
   }
 }
@@ -903,12 +1061,38 @@ __device__ __forceinline__ int ss_find_entry(const SSBatchEntry* __restrict__ e,
   return lo;
 }
 
+// A copy of *src held in scalar registers: the tables live in ordinary global memory,
+// which the kernels also write, so the compiler would otherwise load them per lane.
+template <class T>
+__device__ __forceinline__ T ss_uniform_copy(const T* src) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized type");
+  int w[sizeof(T) / 4];
+  const int* p = reinterpret_cast<const int*>(src);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = __builtin_amdgcn_readfirstlane(p[i]);
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
+// The entry of `item` with every field wave-uniform: the compiler then keeps the
+// slice's plan in scalar registers instead of per-lane copies.
+__device__ __forceinline__ SSBatchEntry ss_entry_of(const SSBatchLaunch& L, int item) {
+  const int idx = __builtin_amdgcn_readfirstlane(ss_find_entry(L.entries, L.nentries, item));
+  SSBatchEntry e = L.entries[idx];
+  e.slice = __builtin_amdgcn_readfirstlane(e.slice);
+  e.j0 = __builtin_amdgcn_readfirstlane(e.j0);
+  e.j1 = __builtin_amdgcn_readfirstlane(e.j1);
+  e.item0 = __builtin_amdgcn_readfirstlane(e.item0);
+  return e;
+}
+
 __global__ __launch_bounds__(256) void k_batch_gather(SSBatchLaunch L, const float* __restrict__ images,
                                                       float2* __restrict__ gA, float2* __restrict__ gB) {
   __shared__ float T[33][34];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
-    const SSBatchSlice& s = L.slices[e.slice];
+    const SSBatchEntry e = ss_entry_of(L, item);
+    const SSBatchSlice s = ss_uniform_copy(L.slices + e.slice);
     const int n = s.p.g.n;
     const int tiles = (n + 2 + 31) / 32;
     const int local = item - e.item0;
@@ -923,8 +1107,8 @@ __global__ __launch_bounds__(256) void k_batch_transpose(SSBatchLaunch L, const 
                                                          float* __restrict__ out, int to_t) {
   __shared__ float tile[32][33];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
-    const SSBatchSlice& s = L.slices[e.slice];
+    const SSBatchEntry e = ss_entry_of(L, item);
+    const SSBatchSlice s = ss_uniform_copy(L.slices + e.slice);
     const int rows = to_t ? s.p.g.nrays : s.p.g.nangles;
     const int cols = to_t ? s.p.g.nangles : s.p.g.nrays;
     const int tcols = (cols + 31) / 32;
@@ -941,12 +1125,22 @@ __global__ __launch_bounds__(512) void k_batch_radon(SSBatchLaunch L, const floa
                                                      const float* __restrict__ meas, double* __restrict__ ll) {
   __shared__ SSRadonLds lds;
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
-    const SSBatchSlice& s = L.slices[e.slice];
+    const SSBatchEntry e = ss_entry_of(L, item);
+    const SSBatchSlice* sp = L.slices + e.slice;
+    SSRadonView v;
+    v.angles = ss_uniform_copy(&sp->p.angles);
+    v.bounds = ss_uniform_copy(&sp->p.bounds);
+    v.wbounds = ss_uniform_copy(&sp->p.wbounds);
+    v.n = ss_uniform_copy(&sp->p.g.n);
+    v.R = ss_uniform_copy(&sp->p.g.nrays);
+    v.A = ss_uniform_copy(&sp->p.g.nangles);
+    v.pitch = ss_uniform_copy(&sp->p.g.pitch);
+    v.nwaves = ss_uniform_copy(&sp->p.nwaves);
+    const int64_t lay_off = ss_uniform_copy(&sp->lay_off), sino_off = ss_uniform_copy(&sp->sino_off);
     const int local = item - e.item0;
-    const int ablk = local / s.p.nwaves, grp = local - ablk * s.p.nwaves;
-    ss_radon_item<MODE>(s.p, gA + s.lay_off, gB + s.lay_off, e.j0, e.j1, split, grp, ablk, out + s.sino_off,
-                        MODE == SS_OUT_EM_RATIO ? meas + s.sino_off : nullptr,
+    const int ablk = local / v.nwaves, grp = local - ablk * v.nwaves;
+    ss_radon_item<MODE>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, split, grp, ablk, out + sino_off,
+                        MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr,
                         MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, lds);
     __syncthreads();
   }
@@ -958,8 +1152,8 @@ __global__ __launch_bounds__(256) void k_batch_radon_fixup(SSBatchLaunch L, cons
                                                            const float* __restrict__ meas, double* __restrict__ ll) {
   __shared__ float redf[4];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
-    const SSBatchSlice& s = L.slices[e.slice];
+    const SSBatchEntry e = ss_entry_of(L, item);
+    const SSBatchSlice s = ss_uniform_copy(L.slices + e.slice);
     ss_radon_fixup_item<MODE>(s.p, gA + s.lay_off, gB + s.lay_off, e.j0, e.j1, item - e.item0, out + s.sino_off,
                               MODE == SS_OUT_EM_RATIO ? meas + s.sino_off : nullptr,
                               MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, redf);
@@ -974,8 +1168,8 @@ __global__ __launch_bounds__(256) void k_batch_bp(SSBatchLaunch L, const float* 
   __shared__ SSBpLds<STAGE> lds;
   constexpr int TH = 8 * ROWS;
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const SSBatchEntry e = L.entries[ss_find_entry(L.entries, L.nentries, item)];
-    const SSBatchSlice& s = L.slices[e.slice];
+    const SSBatchEntry e = ss_entry_of(L, item);
+    const SSBatchSlice s = ss_uniform_copy(L.slices + e.slice);
     const int tiles_x = (s.p.g.n + BP_TILE_W - 1) / BP_TILE_W;
     const int local = item - e.item0;
     const int ty = local / tiles_x, tx = local - ty * tiles_x;
@@ -1000,6 +1194,19 @@ int ss_resident_blocks(K kernel, int block) {
   return per_cu * cus;
 }
 }  // namespace
+
+// Blocks take items blockIdx.x + i*gridDim.x.  Item costs repeat with the ray-group
+// or tile-column count of a slice (often a power of two), so a grid that shares a
+// factor with it would hand some blocks only the long central rays; a prime grid
+// larger than any such period gives every block the full mix.
+static int ss_prime_at_most(int g) {
+  for (; g > 2; --g) {
+    bool prime = (g % 2) != 0;
+    for (int d = 3; prime && d * d <= g; d += 2) prime = (g % d) != 0;
+    if (prime) return g;
+  }
+  return g;
+}
 
 int ss_batch_grid(int kind, int variant, int mode, int total) {
   if (total <= 0) return 0;
@@ -1032,7 +1239,7 @@ int ss_batch_grid(int kind, int variant, int mode, int total) {
 #undef SS_BP_OCC
     }
   }
-  return g < total ? g : total;
+  return g < total ? ss_prime_at_most(g) : total;
 }
 
 hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, const float* images, float2* gatherA, float2* gatherB,

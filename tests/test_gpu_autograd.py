@@ -88,13 +88,16 @@ def test_unrolled_ls_tv_checkpointed(setup):
     g_ck, = torch.autograd.grad(obj, lam)
     assert torch.isfinite(g_ck)
     # same gradient without checkpointing (8 slices to keep stored activations small)
+    from pyraft_amd.autograd import operator_norm_sq
+    alpha = 0.95 / operator_norm_sq(plan)
+    assert alpha == 0.95 / operator_norm_sq(plan)   # seeded power iteration
     y8 = y[:8].contiguous()
     xt8 = x_true[:8]
     lam2 = torch.tensor(1e-3, device="cuda", requires_grad=True)
-    xa, _ = unrolled_ls_tv(y8, plan, steps=20, lam=lam2, checkpoint=True)
+    xa, _ = unrolled_ls_tv(y8, plan, steps=20, step_size=alpha, lam=lam2, checkpoint=True)
     ga, = torch.autograd.grad(((xa - xt8) ** 2).mean(), lam2)
     lam3 = torch.tensor(1e-3, device="cuda", requires_grad=True)
-    xb, _ = unrolled_ls_tv(y8, plan, steps=20, lam=lam3, checkpoint=False)
+    xb, _ = unrolled_ls_tv(y8, plan, steps=20, step_size=alpha, lam=lam3, checkpoint=False)
     gb, = torch.autograd.grad(((xb - xt8) ** 2).mean(), lam3)
     print("lambda grads", float(ga), float(gb))
     assert abs(float(ga) - float(gb)) <= 1e-3 * abs(float(gb)) + 1e-12
