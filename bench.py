@@ -266,6 +266,9 @@ def main():
         del src, dst
     print("bench: plan n=%d R=%d A=%d" % (n, R, A), file=sys.stderr, flush=True)
     plan = Plan(n, R, A)
+    bp_variant, bp_variant_us = plan.tune(reps=3)   # autotuned backprojection tile / LDS stage
+    from pyraft_amd.device import bp_variant_shape
+    bp_rows, bp_stage = bp_variant_shape(bp_variant)
     img = phantom(random_ellipses(32, 2), n)
     sinoT = torch.empty((A, R), dtype=torch.float32, device=dev)
     bp = torch.empty((n, n), dtype=torch.float32, device=dev)
@@ -314,7 +317,7 @@ def main():
     torch.cuda.synchronize()
     fbp_ms = e0.elapsed_time(e1) / 3
 
-    dominant = "k_backprojection<0>" if t_bp >= t_radon else "k_radon<1>"
+    dominant = ("k_backprojection<0, %d, %d>" % (bp_rows, bp_stage)) if t_bp >= t_radon else "k_radon<1>"
     t_dom = max(t_bp, t_radon)
     achieved = BYTES_PER_UPDATE * upd / t_dom / 1e9
     traffic, traffic_src = measured_traffic(dominant)
@@ -338,6 +341,8 @@ def main():
         "radon_ms": round(t_radon * 1e3, 4),
         "backprojection_ms": round(t_bp * 1e3, 4),
         "fbp_ms": round(fbp_ms, 4),
+        "bp_variant": {"chosen": bp_variant, "rows_per_thread": bp_rows, "stage_pairs": bp_stage,
+                       "tuned_us": {str(k): round(v, 1) for k, v in bp_variant_us.items()}},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_src,
@@ -364,11 +369,33 @@ def main():
         torch.cuda.synchronize()
         tc = time.perf_counter()
         it_s = iters / max(1e-9, (tc - tb) - (tb - ta))
+        # the same OSEM through the batch path: one persistent launch per kernel,
+        # the whole call captured once into a HIP graph and replayed
+        from pyraft_amd.batch import Batch
+        eb = Batch([ne], [ne], [Ae])
+        eb.tune(reps=2)
+        sb = eb.pack_sinos([s])
+        xb = torch.ones(eb.img_total, dtype=torch.float32, device=dev)
+        eb.em(sb, iters, S, x=xb, graph=True)          # capture + first replay
+        xb.fill_(1.0)
+        torch.cuda.synchronize()
+        td = time.perf_counter()
+        _, llb = eb.em(sb, iters, S, x=xb, return_ll=True, graph=True)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        it_b = iters / (te - td)
+        xd = eb.image(xb, 0).double()
+        em_diff = float(torch.linalg.vector_norm(xd - x.double()) / torch.linalg.vector_norm(x.double()))
         result["em"] = {"workload": "OSEM 1024x1024, 720 angles, 1024 bins, 16 subsets, %d iterations, "
                                     "Poisson data (mean count 1e3)" % iters,
                         "iters_per_s_plain": round(it_s, 2),
+                        "iters_per_s_batched_graph": round(it_b, 2),
+                        "speedup": round(it_b / it_s, 3),
                         "gups": round(2.0 * ne * ne * Ae * it_s / 1e9, 2),
-                        "ll_first_last": [round(ll[0], 3), round(ll[-1], 3)]}
+                        "gups_batched_graph": round(2.0 * ne * ne * Ae * it_b / 1e9, 2),
+                        "rel_l2_batched_vs_plain": em_diff,
+                        "ll_first_last": [round(ll[0], 3), round(ll[-1], 3)],
+                        "ll_first_last_batched": [round(llb[0][0], 3), round(llb[-1][0], 3)]}
 
     if not args.no_batch:
         result["batched"] = batched_leg(dev, args.batch_slices, args.batch_iters)

@@ -41,24 +41,26 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, defines=()):
+    """Compile and link the library (into `out`, default the in-tree path), with extra
+    preprocessor `defines` ("NAME=VALUE") for the HIP sources."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     objs = []
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        out = _run([hipcc] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
-        if verbose and out.strip():
-            print(out)
+        out_txt = _run([hipcc] + HIPCC_FLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj])
+        if verbose and out_txt.strip():
+            print(out_txt)
         objs.append(obj)
     for src in CXX_SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        out = _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
-        if verbose and out.strip():
-            print(out)
+        out_txt = _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        if verbose and out_txt.strip():
+            print(out_txt)
         objs.append(obj)
-    lib = library_path()
+    lib = out or library_path()
     tmp = lib + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, lib)

@@ -472,7 +472,7 @@ extern "C" int raft_ss_batch_create(raft_ss_batch** out, int nslices, const int*
     b->sino_off[i] = b->sino_total;
     b->sino_total = align_up(b->sino_total + (int64_t)nrays[i] * nangles[i], 64);
     b->lay_off[i] = b->lay_total;
-    b->lay_total = align_up(b->lay_total + (int64_t)n[i] * (n[i] + 2), 32);
+    b->lay_total = align_up(b->lay_total + (int64_t)n[i] * it->second->g.pitch, 32);
   }
   *out = b.release();
   return 0;

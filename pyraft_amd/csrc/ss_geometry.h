@@ -20,7 +20,7 @@ struct SSGeom {
   int n;        // image side
   int nrays;    // R
   int nangles;  // A
-  int pitch;    // padded row length of the Radon gather layouts: n + 2
+  int pitch;    // row length of the Radon gather layouts: n + 2 rounded up to even
   double xmin, ymin, dx, dy;
   double tmin, dt;
   double th0, dth;

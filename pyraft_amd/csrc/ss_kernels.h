@@ -4,7 +4,7 @@
 //   image      [slice][n][n]        row-major, row 0 = ymin (bottom)
 //   sino       [slice][R][A]        the reference layout (angle contiguous)
 //   sinoT      [slice][A][R]        angle-major (detector contiguous), internal
-//   gatherA/B  [slice][n][n+2]      Radon gather layouts of float2 (lower tap, difference to the
+//   gatherA/B  [slice][n][pitch]    Radon gather layouts (pitch = n+2 rounded up to even) of float2 (lower tap, difference to the
 //                                   upper tap) per interpolation cell; A: image columns, B: rows
 #ifndef PYRAFT_AMD_SS_KERNELS_H
 #define PYRAFT_AMD_SS_KERNELS_H

@@ -89,7 +89,7 @@ hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4*
 // layouts
 // =====================================================================================
 
-// image [n][n] -> pair layouts [n][n+2] of float2, q in [0, n+1]:
+// image [n][n] -> pair layouts [n][pitch] of float2, q in [0, n+1] (pitch >= n+2):
 //   gatherB[r][q] = (F(r, q-1), F(r, q) - F(r, q-1))      image rows
 //   gatherA[c][q] = (F(q-1, c), F(q, c) - F(q-1, c))      image columns
 // with F = the image inside [0, n)^2 and 0 outside.  Element m+1 of a row holds the
@@ -259,8 +259,11 @@ struct SSRadonLds {
   double red[RADON_WAVES];
   int wmeta[RADON_WAVES][4];  // per wave: branch (-1: no samples), first row, last row
   int wwin[RADON_WAVES][2];   // per wave: lowest and highest cell of the current chunk
-  float2 stage[RADON_LDS_CAP + 1];  // chunk window; the last record stays (0, 0)
+  __attribute__((aligned(16))) float2 stage[RADON_LDS_CAP + 2];  // chunk window; record CAP stays (0, 0)
 };
+
+typedef __attribute__((address_space(1))) const void* ss_global_cptr;
+typedef __attribute__((address_space(3))) void* ss_lds_ptr;
 
 // wave-uniform copies (scalar registers) of values the compiler cannot prove uniform
 __device__ __forceinline__ int ss_rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -445,22 +448,33 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
         }
         c_lo = ss_rfl(c_lo);
         c_hi = ss_rfl(c_hi);
-        const int Wc = c_hi - c_lo + 1;
+        // the window holds gather records [rec0, rec0 + Wc) of each row: cells
+        // c_lo..c_hi need records c_lo+1..c_hi+1; rec0 is even so that rows of the
+        // window start 16-byte aligned in the layout (even pitch) and in LDS (even Wc)
+        const int rec0 = (c_lo + 1) & ~1;
+        const int Wc = c_hi >= c_lo ? ((c_hi + 2 - rec0) + 1) & ~1 : 0;
         const int rows = Wc > 0 ? min(s1 - s0, RADON_LDS_CAP / Wc) : 0;
         if (Wc > 0 && rows > 0) {
           s1 = s0 + rows;
-          for (int idx = threadIdx.x; idx < rows * Wc; idx += RADON_WAVES * SS_WAVE) {
-            const int r = idx / Wc, c = idx - r * Wc;
-            lds.stage[idx] = lay[(size_t)(s0 + r) * pitch + (c_lo + c + 1)];
+          // direct global -> LDS copies, 16 bytes (two records) per lane: wave wv copies
+          // rows wv, wv+8, ...; the last record read is rec0 + Wc - 1 <= c_hi + 2 <= n + 1
+          for (int r = wv; r < rows; r += RADON_WAVES) {
+            const float2* src = lay + (size_t)(s0 + r) * pitch + rec0;
+            float2* dst = lds.stage + r * Wc;
+            for (int seg = 0; seg < Wc; seg += 2 * SS_WAVE) {
+              if (seg + 2 * lane < Wc)
+                __builtin_amdgcn_global_load_lds((ss_global_cptr)(src + seg + 2 * lane), (ss_lds_ptr)(dst + seg), 16,
+                                                 0, 0);
+            }
           }
         }
         __syncthreads();  // window staged
         const int ra2 = max(s0, wb.x), rb2 = min(s1 - 1, wb.y);
         if (branch >= 0 && ra2 <= rb2) {
           if (Wc > 0 && rows > 0) {
-            // high word: staged record index (cell - c_lo) + (row - s0) * Wc
+            // high word: staged record index (cell + 1 - rec0) + (row - s0) * Wc
             const int64_t rowstep = (int64_t)Wc << 32;
-            int64_t u = alpha + (int64_t)k * gamma + (int64_t)ra2 * beta - ((int64_t)c_lo << 32) +
+            int64_t u = alpha + (int64_t)k * gamma + (int64_t)ra2 * beta + ((int64_t)(1 - rec0) << 32) +
                         (int64_t)(ra2 - s0) * rowstep;
             int64_t step = beta + rowstep;
             if (empty) {

@@ -28,7 +28,8 @@ nthreads = multiprocessing.cpu_count()
 
 _LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 _EXT = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
-LIBRARY_PATH = os.path.join(_LIBDIR, "libraft" + _EXT)
+# PYRAFT_AMD_LIBRARY selects another build of the library (e.g. a locally tuned one)
+LIBRARY_PATH = os.environ.get("PYRAFT_AMD_LIBRARY") or os.path.join(_LIBDIR, "libraft" + _EXT)
 
 if not os.path.exists(LIBRARY_PATH):
     raise ImportError(
