@@ -397,6 +397,41 @@ def main():
                         "ll_first_last": [round(ll[0], 3), round(ll[-1], 3)],
                         "ll_first_last_batched": [round(llb[0][0], 3), round(llb[-1][0], 3)]}
 
+    # the drop-in host path (pyraft.radon / backprojection: FP64 numpy in and out,
+    # PCIe transfers and FP64<->FP32 conversion included), same workload
+    print("bench: host-buffer drop-in path", file=sys.stderr, flush=True)
+    import pyraft_amd as pr
+    host_img = img.double().cpu().numpy()
+    pr.radon(host_img, (R, A), method=pr.radon_method.SLANTSTACK)  # plan cache, warm-up
+    t0 = time.perf_counter()
+    host_sino = pr.radon(host_img, (R, A), method=pr.radon_method.SLANTSTACK)
+    t1 = time.perf_counter()
+    pr.backprojection(host_sino, pr.backprojection_plan(host_sino.shape, (n, n), method=pr.radon_method.SLANTSTACK))
+    t2 = time.perf_counter()
+    result["host_path"] = {"note": "pyraft-compatible radon/backprojection on FP64 host arrays (H2D/D2H, "
+                                   "FP64<->FP32 conversion included); value is device-resident",
+                           "radon_ms": round((t1 - t0) * 1e3, 3), "backprojection_ms": round((t2 - t1) * 1e3, 3),
+                           "gups": round(2 * upd / (t2 - t0) / 1e9, 2)}
+
+    # differentiable projector: forward + backward (exact transpose) on 64 x 512^2, 720 angles
+    print("bench: autograd 64 x 512^2 x 720", file=sys.stderr, flush=True)
+    from pyraft_amd.autograd import radon as ad_radon
+    aplan = Plan(512, 512, 720)
+    xa = torch.rand((64, 512, 512), device=dev, requires_grad=True)
+    ya = torch.rand((64, 512, 720), device=dev)
+    for _ in range(2):
+        ((ad_radon(xa, aplan) * ya).sum()).backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ((ad_radon(xa, aplan) * ya).sum()).backward()
+    torch.cuda.synchronize()
+    ta = (time.perf_counter() - t0) / 3
+    aupd = 64.0 * 512 * 512 * 720
+    result["autograd"] = {"workload": "64 x 512^2, 720 angles: Radon forward + exact-transpose backward "
+                                      "(+ torch elementwise/sum)",
+                          "ms_per_step": round(ta * 1e3, 3), "gups": round(2 * aupd / ta / 1e9, 2)}
+
     if not args.no_batch:
         result["batched"] = batched_leg(dev, args.batch_slices, args.batch_iters)
 
