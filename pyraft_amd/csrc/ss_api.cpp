@@ -85,6 +85,9 @@ static void build_angles(const SSGeom& g, std::vector<SSAngle>& out, int* ndeg) 
     a.bp_a = (g.xmin * a.cost + g.ymin * a.sint - g.tmin) / g.dt;
     a.bp_b = (g.dx * a.cost) / g.dt;
     a.bp_c = (g.dy * a.sint) / g.dt;
+    a.rt_ig = a.gamma != 0 ? SS_FIX_ONE / (double)a.gamma : 0.0;
+    a.rt_a0 = a.gamma != 0 ? (double)a.alpha0 / (double)a.gamma : 0.0;
+    a.rt_b = a.gamma != 0 ? (double)a.beta / (double)a.gamma : 0.0;
     out[j] = a;
   }
 }
@@ -417,13 +420,15 @@ extern "C" int raft_ss_radon_transpose_range(const raft_ss_plan* p, const float*
   hipStream_t stream = as_stream(stream_);
   const SSDevPlan dp = p->dev();
   const float* sT = d_sino;
-  DevBuf ws;
+  DevBuf ws, tr;
   if (!angle_major) {
     HIP_TRY(ws.alloc((size_t)nslices * p->g.nrays * p->g.nangles * sizeof(float), stream));
     HIP_TRY(ss_launch_sino_to_t(d_sino, ws.as<float>(), p->g.nrays, p->g.nangles, nslices, stream));
     sT = ws.as<float>();
   }
-  HIP_TRY(ss_launch_radon_transpose(dp, sT, j0, j1, nslices, d_image, stream));
+  const size_t scratch = ss_radon_transpose_scratch(dp, nslices);
+  if (scratch) HIP_TRY(tr.alloc(scratch, stream));
+  HIP_TRY(ss_launch_radon_transpose(dp, sT, j0, j1, nslices, d_image, tr.as<float>(), stream));
   return 0;
 }
 

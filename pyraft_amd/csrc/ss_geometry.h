@@ -39,6 +39,10 @@ struct SSAngle {
   int branch_a;
   // Backprojection ray coordinate u = (x cos + y sin - tmin)/dt = bp_a + c*bp_b + r*bp_c.
   double bp_a, bp_b, bp_c;
+  // Transpose: the ray index whose Q32.32 trajectory reaches cell position x (in cells)
+  // at sample s is k = x*rt_ig - rt_a0 - s*rt_b  (rt_ig = 2^32/gamma, rt_a0 = alpha0/gamma,
+  // rt_b = beta/gamma), used to bound the candidate rays of a pixel tile.
+  double rt_ig, rt_a0, rt_b;
   // |beta| below this threshold: sample validity is evaluated sample by sample
   // (knife-edge angles where the FP64 predicate need not be monotone).
   int degenerate;

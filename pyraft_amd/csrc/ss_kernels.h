@@ -93,8 +93,10 @@ hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int 
                                     float* out, SSEmUpdateArgs em, hipStream_t stream);
 
 // ---- exact transpose of the Radon (reads angle-major sinogram)
+// scratch bytes the transpose needs (a transposed image per slice; 0 for the tiled form)
+size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices);
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
-                                     float* out, hipStream_t stream);
+                                     float* out, float* scratch, hipStream_t stream);
 
 // ---- batched forms (items: gather = 32x32 tiles; radon = 64-ray groups x angle blocks of
 //      RADON_WAVES/split angles; fixup = irregular rays; bp = pixel tiles of the variant;

@@ -6,11 +6,14 @@
 // backprojection and in the EM log-likelihood sums.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ss_kernels.h"
 #include "ss_plan_logic.h"
 
 #define SS_WAVE 64
 #define SS_INV_FIX 2.3283064365386963e-10f  // 2^-32
+#define SS_FIX_ONE_I64 ((int64_t)1 << 32)    // one cell in Q32.32
 
 // =====================================================================================
 // plan: exact per-ray sample ranges
@@ -880,53 +883,211 @@ hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int 
 }
 
 // =====================================================================================
-// exact transpose of the Radon: pixel-driven gather over the forward's own index map
+// exact transpose of the Radon: image lines accumulated in LDS from the forward's own
+// sample map
 //
-// Pixel (r,c) receives, from ray (j,k), (1-w)*delta*q when the forward's sample at
-// s (= c in branch A, r in branch B) has floor index m == p (= r in A, c in B) and
-// w*delta*q when m == p-1, provided s lies in the ray's exact range.  The candidate
-// rays are those with u in [p-1, p+1); u is recomputed with the forward's integer
-// expression, so indices and weights are bit-identical to the forward's.
+// The forward adds (1-w)*F[m] + w*F[m+1] for every valid sample (ray k, sample s), with
+// m and w from the Q32.32 trajectory u = alpha0 + k*gamma + s*beta.  Its transpose
+// gives pixel m the amount (1-w)*delta*q_k and pixel m+1 the amount w*delta*q_k.
+//
+// k_radon_transpose_lines: one wave owns one image line along the forward's sample
+// axis -- image row s for branch-B angles, image column s for branch-A angles -- as
+// an LDS row of n+2 cells (pixel c at index c+1, the two pads absorb m = -1 and
+// m+1 = n).  For each angle of its branch, the lanes take the rays 64 at a time,
+// step u by 64*gamma between passes, check the ray's exact range and add the two
+// contributions.  Each line belongs to one wave and its terms arrive in a fixed
+// order, so the result is deterministic without global atomics.  Branch-B lines are
+// image rows and are written to the output; branch-A lines are image columns and go
+// to a transposed scratch image that k_transpose_add folds in.
+//
+// k_radon_transpose (tiles): the same sums for images too wide for one LDS line; a
+// block owns a T x T tile and finds the rays crossing each tile line from the
+// angle's inverse slope.  Irregular rays are added by the fixup kernel afterwards.
 // =====================================================================================
 
-__global__ __launch_bounds__(256) void k_radon_transpose(SSDevPlan p, const float* __restrict__ qT, int j0, int j1,
-                                                         float* __restrict__ out) {
+#define RT_LINES 8         // image lines (waves) per block
+#define RT_LDS_MAX 65536   // LDS budget of one block: lines + two ray buffers
+
+// LDS layout of k_radon_transpose_lines: `nl` lines of n+2 cells, then two ray
+// buffers of 3R dwords (the angle's exact ranges as int2, then its sinogram row)
+__host__ __device__ inline size_t ss_rt_line_lds(int n, int R, int nl) {
+  return sizeof(float) * ((size_t)nl * (n + 2) + 2 * (size_t)(3 * R + 4));
+}
+
+// next angle >= j of `branch` (or j1)
+__device__ __forceinline__ int ss_rt_next(const SSAngle* angles, int j, int j1, int branch) {
+  while (j < j1 && ss_rfl(angles[j].branch_a) != branch) ++j;
+  return j;
+}
+
+// copy the ranges and sinogram row of angle j into ray buffer `buf` (3R dwords) with
+// direct global -> LDS loads; wave w copies 64-dword chunks w, w + nw, ...
+__device__ __forceinline__ void ss_rt_stage(const SSDevPlan& p, const float* q, int j, float* buf, int wv, int nw,
+                                            int lane) {
+  const int R = p.g.nrays;
+  const int* bsrc = reinterpret_cast<const int*>(p.bounds + (size_t)j * R);
+  const float* qsrc = q + (size_t)j * R;
+  for (int c = wv * SS_WAVE; c < 3 * R; c += nw * SS_WAVE) {
+    const int d = c + lane;
+    if (d < 3 * R) {
+      const void* src = d < 2 * R ? (const void*)(bsrc + d) : (const void*)(qsrc + (d - 2 * R));
+      __builtin_amdgcn_global_load_lds((ss_global_cptr)src, (ss_lds_ptr)(buf + c), 4, 0, 0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(RT_LINES * SS_WAVE) void k_radon_transpose_lines(SSDevPlan p, const float* __restrict__ qT,
+                                                                             int j0, int j1, float* __restrict__ outB,
+                                                                             float* __restrict__ outA) {
+  extern __shared__ float sh[];
   const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
-  const int c = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int r = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int slice = blockIdx.z;
-  if (r >= n || c >= n) return;
-  const float* q = qT + (size_t)slice * A * R;
-  float acc = 0.f;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = (int)(blockDim.x >> 6);
+  const int groups = (n + nw - 1) / nw;
+  const int branch = blockIdx.x >= (unsigned)groups ? 1 : 0;
+  const int s = ((int)blockIdx.x - branch * groups) * nw + wv;  // image row (B) or column (A)
+  const size_t slice = blockIdx.y;
+  const float* q = qT + slice * A * R;
+  float* line = sh + wv * (n + 2);
+  float* rbuf[2] = {sh + nw * (n + 2), sh + nw * (n + 2) + (3 * R + 4)};
+  for (int i = lane; i < n + 2; i += SS_WAVE) line[i] = 0.f;
+  int j = ss_rt_next(p.angles, j0, j1, branch);
+  if (j < j1) ss_rt_stage(p, q, j, rbuf[0], wv, nw, lane);
+  int cur = 0;
+  while (j < j1) {
+    const int jn = ss_rt_next(p.angles, j + 1, j1, branch);
+    __syncthreads();  // angle j is staged; every wave is done with the other buffer
+    if (jn < j1) ss_rt_stage(p, q, jn, rbuf[cur ^ 1], wv, nw, lane);
+    if (s < n) {
+      const SSAngle& a = p.angles[j];
+      const int64_t gamma = ss_rfl64(a.gamma);
+      const float delta = ss_rflf(a.delta);
+      const int2* bb = reinterpret_cast<const int2*>(rbuf[cur]);
+      const float* qq = rbuf[cur] + 2 * R;
+      // u of ray `lane` at sample s, plus one cell: the high word is the line index m + 1
+      int64_t u = ss_rfl64(a.alpha0) + (int64_t)s * ss_rfl64(a.beta) + (int64_t)lane * gamma + ((int64_t)1 << 32);
+      const int64_t step = (int64_t)SS_WAVE * gamma;
+      // rays at least one cell apart (|gamma| >= 1): the lanes of a pass hit distinct
+      // cells, so plain read-modify-writes in two ordered phases are exact (the empty
+      // asm keeps the compiler from moving the second phase's reads above the first
+      // phase's writes of other lanes); denser rays use LDS atomics
+      const bool distinct = (gamma >= SS_FIX_ONE_I64) || (gamma <= -SS_FIX_ONE_I64);
+      for (int k = lane; k - lane < R; k += SS_WAVE, u += step) {
+        const bool valid = k < R && s >= bb[k].x && s <= bb[k].y;
+        const float qd = valid ? qq[k] * delta : 0.f;
+        const float wq = (float)(uint32_t)u * SS_INV_FIX * qd;
+        const int idx = (int)(u >> 32);  // m + 1, in [0, n] for a valid sample
+        if (distinct) {
+          if (valid) line[idx] += qd - wq;
+          __asm__ volatile("" ::: "memory");
+          if (valid) line[idx + 1] += wq;
+          __asm__ volatile("" ::: "memory");
+        } else if (valid) {
+          atomicAdd(&line[idx], qd - wq);
+          atomicAdd(&line[idx + 1], wq);
+        }
+      }
+    }
+    cur ^= 1;
+    j = jn;
+  }
+  if (s < n) {
+    float* dst = (branch ? outA : outB) + (slice * n + s) * n;
+    for (int c = lane; c < n; c += SS_WAVE) dst[c] = line[c + 1];
+  }
+}
+
+// out[s][r][c] += t[s][c][r], 32x32 tiles
+__global__ __launch_bounds__(256) void k_transpose_add(const float* __restrict__ t, float* __restrict__ out, int n) {
+  __shared__ float tile[32][33];
+  const size_t off = (size_t)blockIdx.z * n * n;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int r = by + yy, c = bx + tx;  // t row r (image column), column c (image row)
+    if (r < n && c < n) tile[yy][tx] = t[off + (size_t)r * n + c];
+  }
+  __syncthreads();
+  for (int xx = ty; xx < 32; xx += 8) {
+    const int r = bx + xx, c = by + tx;  // image row r, column c
+    if (r < n && c < n) out[off + (size_t)r * n + c] += tile[tx][xx];
+  }
+}
+
+#define RT_T 62      // tile side: a tile line is crossed by at most T + 2 rays when |gamma| >= 1 cell
+#define RT_WAVES 8
+
+__global__ __launch_bounds__(RT_WAVES * SS_WAVE) void k_radon_transpose(SSDevPlan p, const float* __restrict__ qT,
+                                                                       int j0, int j1, float* __restrict__ out) {
+  __shared__ float acc[RT_T * (RT_T + 1)];
+  const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = blockIdx.x * RT_T, r0 = blockIdx.y * RT_T;
+  const size_t slice = blockIdx.z;
+  const float* q = qT + slice * A * R;
+  for (int i = threadIdx.x; i < RT_T * (RT_T + 1); i += RT_WAVES * SS_WAVE) acc[i] = 0.f;
+  __syncthreads();
+  int prev_branch = -1;
   for (int j = j0; j < j1; ++j) {
     const SSAngle& a = p.angles[j];
-    const int s = a.branch_a ? c : r;
-    const int pp = a.branch_a ? r : c;
-    const int64_t base = a.alpha0 + (int64_t)s * a.beta;
-    const double g = (double)a.gamma;
-    // k with u = base + k*gamma in [(pp-1), (pp+1)) grid units
-    const double ka = ((double)(pp - 1) * SS_FIX_ONE - (double)base) / g;
-    const double kb = ((double)(pp + 1) * SS_FIX_ONE - (double)base) / g;
-    double klo = fmin(ka, kb), khi = fmax(ka, kb);
-    if (khi < -2.0 || klo > (double)R + 1.0) continue;
-    int k0 = (int)floor(fmax(klo, -2.0)) - 1;
-    int k1 = (int)ceil(fmin(khi, (double)R + 1.0)) + 1;
-    if (k0 < 0) k0 = 0;
-    if (k1 > R - 1) k1 = R - 1;
-    float sum = 0.f;
-    for (int k = k0; k <= k1; ++k) {
-      const int64_t u = base + (int64_t)k * a.gamma;
-      const int m = (int)(u >> 32);
-      if (m != pp && m != pp - 1) continue;
-      const int2 b = p.bounds[(size_t)j * R + k];
-      if (s < b.x || s > b.y) continue;
-      const float w = (float)(uint32_t)u * SS_INV_FIX;
-      const float wt = (m == pp) ? (1.f - w) : w;
-      sum = fmaf(wt, q[(size_t)j * R + k], sum);
+    const int branch = ss_rfl(a.branch_a);
+    if (branch != prev_branch) {
+      __syncthreads();  // every wave is done with the other branch's lines
+      prev_branch = branch;
     }
-    acc = fmaf(sum, a.delta, acc);
+    const int cell0 = branch ? r0 : c0;  // interpolation axis origin
+    const int s0 = branch ? c0 : r0;     // sample axis origin
+    const int64_t alpha = ss_rfl64(a.alpha0), gamma = ss_rfl64(a.gamma), beta = ss_rfl64(a.beta);
+    const float delta = ss_rflf(a.delta);
+    // candidate rays of line s: between x = cell0 - 1 and x = cell0 + T (cells), in
+    // ray units k(x, s) = x*ig - a0 - s*b; two rays of margin against rounding
+    const double ig = a.rt_ig, a0 = a.rt_a0, bb = a.rt_b;
+    const double ka = (double)(cell0 - 1) * ig - a0, kb = (double)(cell0 + RT_T) * ig - a0;
+    const double kmin0 = fmin(ka, kb);
+    const int kcount = (int)ceil(fabs(kb - ka)) + 5;
+    const int64_t lane_gamma = (int64_t)lane * gamma;
+    const int64_t step_gamma = (int64_t)SS_WAVE * gamma;
+    const bool distinct = (gamma >= SS_FIX_ONE_I64) || (gamma <= -SS_FIX_ONE_I64);
+    for (int l = wv; l < RT_T; l += RT_WAVES) {
+      const int s = s0 + l;
+      if (s >= n) break;
+      // candidate rays [k_lo, k_hi) of this line, clipped to the detector
+      const double kd = floor(kmin0 - (double)s * bb) - 2.0;
+      if (kd >= (double)R || kd + kcount <= 0.0) continue;
+      const int k_lo = ss_rfl((int)fmax(kd, 0.0));
+      const int k_hi = ss_rfl((int)fmin(kd + kcount, (double)R));
+      // u of ray k_lo + lane at sample s
+      int64_t u = alpha + (int64_t)s * beta + (int64_t)k_lo * gamma + lane_gamma;
+      for (int k = k_lo + lane; k - lane < k_hi; k += SS_WAVE, u += step_gamma) {
+        if (k >= k_hi) continue;
+        const int2 b = p.bounds[(size_t)j * R + k];
+        if (s < b.x || s > b.y) continue;
+        const int m = (int)(u >> 32) - cell0;
+        const float qd = q[(size_t)j * R + k] * delta;
+        const float wq = (float)(uint32_t)u * SS_INV_FIX * qd;
+        float* p0 = &acc[branch ? m * (RT_T + 1) + l : l * (RT_T + 1) + m];
+        float* p1 = &acc[branch ? (m + 1) * (RT_T + 1) + l : l * (RT_T + 1) + m + 1];
+        if (distinct) {  // lanes hit distinct cells: ordered plain updates (see the line kernel)
+          if ((unsigned)m < (unsigned)RT_T) *p0 += qd - wq;
+          __asm__ volatile("" ::: "memory");
+          if ((unsigned)(m + 1) < (unsigned)RT_T) *p1 += wq;
+          __asm__ volatile("" ::: "memory");
+        } else {
+          if ((unsigned)m < (unsigned)RT_T) atomicAdd(p0, qd - wq);
+          if ((unsigned)(m + 1) < (unsigned)RT_T) atomicAdd(p1, wq);
+        }
+      }
+    }
   }
-  out[((size_t)slice * n + r) * n + c] = acc;
+  __syncthreads();
+  float* img = out + slice * n * n;
+  for (int i = threadIdx.x; i < RT_T * RT_T; i += RT_WAVES * SS_WAVE) {
+    const int rr = i / RT_T, cc = i - rr * RT_T;
+    const int r = r0 + rr, c = c0 + cc;
+    if (r < n && c < n) img[(size_t)r * n + c] = acc[rr * (RT_T + 1) + cc];
+  }
 }
 
 // irregular rays: scatter their exact contributions
@@ -957,11 +1118,45 @@ __global__ __launch_bounds__(256) void k_radon_transpose_fixup(SSDevPlan p, cons
   }
 }
 
+// LDS a block may use on the current device (gfx950: 160 KiB)
+static size_t ss_lds_per_block() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    return RT_LDS_MAX;
+  return (size_t)v;
+}
+
+// lines per block for the line kernel (0: use the tiled kernel); at most half the
+// LDS of a block so that two blocks share a CU
+static int ss_rt_lines(int n, int R) {
+  const size_t budget = std::max<size_t>(RT_LDS_MAX, ss_lds_per_block() / 2);
+  for (int nl = RT_LINES; nl >= 1; nl >>= 1)
+    if (ss_rt_line_lds(n, R, nl) <= budget) return nl;
+  return 0;
+}
+
+size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices) {
+  return ss_rt_lines(p.g.n, p.g.nrays) ? sizeof(float) * (size_t)nslices * p.g.n * p.g.n : 0;
+}
+
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
-                                     float* out, hipStream_t stream) {
+                                     float* out, float* scratch, hipStream_t stream) {
   const int n = p.g.n;
-  dim3 grid((n + 15) / 16, (n + 15) / 16, nslices);
-  hipLaunchKernelGGL(k_radon_transpose, grid, dim3(256), 0, stream, p, sinoT, j0, j1, out);
+  const int nl = ss_rt_lines(n, p.g.nrays);
+  if (nl > 0) {
+    const size_t lds = ss_rt_line_lds(n, p.g.nrays, nl);
+    const int groups = (n + nl - 1) / nl;
+    hipLaunchKernelGGL(k_radon_transpose_lines, dim3(2 * groups, nslices), dim3(nl * SS_WAVE), lds, stream, p, sinoT,
+                       j0, j1, out, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int t = (n + 31) / 32;
+    hipLaunchKernelGGL(k_transpose_add, dim3(t, t, nslices), dim3(256), 0, stream, scratch, out, n);
+  } else {
+    dim3 grid((n + RT_T - 1) / RT_T, (n + RT_T - 1) / RT_T, nslices);
+    hipLaunchKernelGGL(k_radon_transpose, grid, dim3(RT_WAVES * SS_WAVE), 0, stream, p, sinoT, j0, j1, out);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || p.nirregular == 0) return e;
   hipLaunchKernelGGL(k_radon_transpose_fixup, dim3(p.nirregular, nslices), dim3(256), 0, stream, p, sinoT, j0, j1,

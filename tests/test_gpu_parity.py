@@ -199,3 +199,25 @@ def test_phantom_device(torch_mod):
     got = phantom(desc, 200).cpu().numpy()
     ref = O.phantom_fromdesc(desc, 200)
     assert np.array_equal(got, ref.astype(np.float32))
+
+
+def test_transpose_wide_image_tiled_path(torch_mod):
+    """Images too wide for one LDS line per wave take the tiled transpose kernel:
+    same exactness bars (adjoint identity, oracle)."""
+    torch = torch_mod
+    n = R = 3000
+    A = 24
+    print("transpose tiled path", n, R, A, flush=True)
+    rng = np.random.default_rng(17)
+    x = rng.random((n, n)).astype(np.float32)
+    y = rng.random((R, A)).astype(np.float32)
+    p = _plan(n, R, A)
+    Rx = p.radon(torch.from_numpy(x).cuda()).cpu().numpy().astype(np.float64)
+    Rty = p.radon_transpose(torch.from_numpy(y).cuda()).cpu().numpy().astype(np.float64)
+    lhs = float(np.sum(Rx * y.astype(np.float64)))
+    rhs = float(np.sum(x.astype(np.float64) * Rty))
+    mism = abs(lhs - rhs) / abs(lhs)
+    print("adjoint mismatch", mism)
+    assert mism <= ADJOINT
+    ref = O.radon_transpose(y.astype(np.float64), n)
+    assert rel_l2(Rty, ref) <= REL_L2
