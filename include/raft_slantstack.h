@@ -67,6 +67,22 @@ void raft_backprojection_slantstack(raft_image sinogram, raft_image backprojecti
  * (pyraft/backprojection/backprojection.py:229-232); this is its SLANTSTACK case. */
 void raft_radon_transpose_slantstack(raft_image sinogram, raft_image image, int nthreads);
 
+/* Replaces raft_radon_slantstack_withmesh (raft/raft_radon_slantstack_withmesh.c:49-97;
+ * the header raft/raft_radon_slantstack_withmesh.h:10-14 names the first two arguments
+ * the other way round, the definition and pyraft/sinogram/sino.py:274 pass the phantom
+ * first), called by pyraft.radon_withmesh and radon_3d_stack (pyraft/sinogram/sino.py:251-311):
+ * slant-stack ray sums for angle theta[j] and detector coordinate t[k] read from the
+ * vectors, element (k, j) of radon.  Every output is computed (the reference's thread
+ * split leaves the last nviews*nrays mod nthreads outputs unwritten,
+ * raft/raft_radon_slantstack_withmesh.c:65-77). */
+void raft_radon_slantstack_withmesh(raft_image phantom, raft_image radon, raft_vector t, raft_vector theta,
+                                    int nthreads);
+
+/* Replaces raft_radon_slantstack_view (raft/raft_radon_slantstack_view.c:22-38), called
+ * by pyraft.radon_view and snapshot (pyraft/sinogram/sino.py:220-343): one angle, view[k]
+ * = ray sum at t[k]. */
+void raft_radon_slantstack_view(raft_vector view, raft_image phantom, raft_vector t, double theta);
+
 /* Status (0 or RAFT_SS_E*) of the last drop-in call above on this thread: the drop-in
  * symbols return void like the reference's, so callers that want to raise use this. */
 int raft_ss_legacy_status(void);
@@ -123,6 +139,16 @@ int raft_ss_plan_wave_bounds(const raft_ss_plan *plan, int *host_wbounds);
  * the host, for checking index ranges without a GPU */
 int raft_ss_plan_host_bounds(int n, int nrays, int nangles, const double *img_corners, const double *sino_corners,
                              int *host_bounds, long long *host_fix, int *host_branch);
+
+/* Plan on arbitrary tables: ray k at detector coordinate t[k], angle j at theta[j]
+ * (the geometry of raft_radon_slantstack_withmesh).  Only the Radon (raft_ss_radon*)
+ * accepts such a plan; the other operators need the uniform grid.  nrays >= 2. */
+int raft_ss_plan_create_mesh(raft_ss_plan **plan, int n, const double *img_corners, int nrays, const double *t,
+                             int nangles, const double *theta, void *stream);
+/* the mesh plan's ranges ([A][R] x (lo, hi)) and Q32.32 trajectory offsets ([A][R])
+ * computed on the host */
+int raft_ss_plan_host_bounds_mesh(int n, const double *img_corners, int nrays, const double *t, int nangles,
+                                  const double *theta, int *host_bounds, long long *host_alpha);
 
 /* plan summary computed on the host (no device needed): info[0..4] = #degenerate
  * angles, bp window, bp angles per stage, 64-ray groups, 0 */

@@ -15,6 +15,8 @@
  *                                      counterpart; pinned by the dot-product identity)
  *   oracle_phantom_fromdesc            raft/raft_image_phantom_fromdesc.c:4-59
  *   oracle_radon_fromdesc              raft/raft_radon_fromdesc.c:4-67
+ *   oracle_radon_withmesh              raft/raft_radon_slantstack_withmesh.c:49-224
+ *   oracle_radon_view                  raft/raft_radon_slantstack_view.c:22-113
  *
  * Threading: the reference splits the outputs into nthreads+1 contiguous chunks
  * of floor(total/nthreads) (raft/raft_radon_slantstack.c:62-81), which leaves
@@ -93,11 +95,12 @@ typedef struct {
   int j0, j1;
 } radon_ctx;
 
-/* One ray integral.  j = angle column, k = detector row. */
-static double ray_sum(const raft_image *img, const raft_image *sino, int j, int k) {
+/* One ray integral at angle theta and detector coordinate t: the body shared by the
+   uniform-grid routine (raft/raft_radon_slantstack.c:122-214) and the arbitrary-mesh
+   ones (raft/raft_radon_slantstack_withmesh.c:147-224, raft/raft_radon_slantstack_view.c:40-113),
+   which differ only in where theta and t come from. */
+static double ray_sum_at(const raft_image *img, double theta, double t) {
   const int n = img->data.lines;            /* square image assumed (reference :140) */
-  const int nviews = sino->data.columns;
-  const int nrays = sino->data.lines;
   const double tol = 1.0 / sqrt(2);         /* :139 */
 
   const double xmin = img->tl_x, xmax = img->br_x;   /* :143-147 */
@@ -105,16 +108,8 @@ static double ray_sum(const raft_image *img, const raft_image *sino, int j, int 
   const double dx = (xmax - xmin) / (n - 1);          /* :150-151 */
   const double dy = (ymax - ymin) / (n - 1);
 
-  const double th0 = sino->tl_x, thf = sino->br_x;    /* :154-158 */
-  const double dth = (thf - th0) / (nviews);
-  const double theta = th0 + j * dth;
-
   const double c = cos(theta), s = sin(theta), tn = tan(theta); /* :160-164 */
   const double fs = fabs(s), fc = fabs(c);
-
-  const double tmin = sino->br_y, tmax = sino->tl_y;  /* :168-172 */
-  const double dt = (tmax - tmin) / (nrays - 1);
-  const double t = tmin + k * dt;
 
   double acc = 0.0;
   if (fc < tol) {
@@ -147,6 +142,19 @@ static double ray_sum(const raft_image *img, const raft_image *sino, int j, int 
   }
 }
 
+/* j = angle column, k = detector row of the uniform grid */
+static double ray_sum(const raft_image *img, const raft_image *sino, int j, int k) {
+  const int nviews = sino->data.columns;
+  const int nrays = sino->data.lines;
+  const double th0 = sino->tl_x, thf = sino->br_x;    /* :154-158 */
+  const double dth = (thf - th0) / (nviews);
+  const double theta = th0 + j * dth;
+  const double tmin = sino->br_y, tmax = sino->tl_y;  /* :168-172 */
+  const double dt = (tmax - tmin) / (nrays - 1);
+  const double t = tmin + k * dt;
+  return ray_sum_at(img, theta, t);
+}
+
 static void radon_work(void *vctx, long b, long e) {
   radon_ctx *ctx = (radon_ctx *)vctx;
   const int nrays = ctx->sino.data.lines;
@@ -163,6 +171,42 @@ void oracle_radon_slantstack(raft_image phantom, raft_image radon, int j0, int j
   if (j1 <= j0) return;
   radon_ctx ctx = {phantom, radon, j0, j1};
   fan_out(radon_work, &ctx, (long)(j1 - j0) * radon.data.lines, nthreads);
+}
+
+/* ------------------------------------------------------------------ */
+/* arbitrary meshes (raft/raft_radon_slantstack_withmesh.c:49-145)      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  raft_image img;
+  raft_matrix out;
+  raft_vector t, theta;
+} mesh_ctx;
+
+static void mesh_work(void *vctx, long b, long e) {
+  mesh_ctx *ctx = (mesh_ctx *)vctx;
+  const int nrays = ctx->out.lines;
+  for (long w = b; w < e; ++w) {
+    const int j = (int)(w / nrays);  /* :116-117 */
+    const int k = (int)(w % nrays);
+    const double th = ctx->theta.p_data[(long)j * ctx->theta.stride];
+    const double tt = ctx->t.p_data[(long)k * ctx->t.stride];
+    AT(ctx->out, k, j) = ray_sum_at(&ctx->img, th, tt);
+  }
+}
+
+/* radon element (k, j) = ray sum at (theta[j], t[k]); every output is computed (the
+   reference's split of nviews*nrays into nthreads chunks of floor(total/nthreads)
+   leaves the remainder unwritten, :65-77) */
+void oracle_radon_withmesh(raft_image phantom, raft_image radon, raft_vector t, raft_vector theta, int nthreads) {
+  mesh_ctx ctx = {phantom, radon.data, t, theta};
+  fan_out(mesh_work, &ctx, (long)radon.data.lines * radon.data.columns, nthreads);
+}
+
+/* one angle (raft/raft_radon_slantstack_view.c:22-38) */
+void oracle_radon_view(raft_vector view, raft_image phantom, raft_vector t, double theta) {
+  for (int k = 0; k < t.size; ++k)
+    view.p_data[(long)k * view.stride] = ray_sum_at(&phantom, theta, t.p_data[(long)k * t.stride]);
 }
 
 /* ------------------------------------------------------------------ */
@@ -378,31 +422,28 @@ void oracle_radon_fromdesc(raft_image sino, raft_matrix desc) {
 /* sample maps for checking index logic (not a reference routine)       */
 /* ------------------------------------------------------------------ */
 
-/* mask[(j*R + k)*N + s] = 1 iff sample s of ray (j,k) passes the reference's
-   0 <= m <= N-2 test (raft/raft_radon_slantstack.c:184-186, 203-205);
+/* mask[(j*R + k)*N + s] = 1 iff sample s of the ray at t[k], theta[j] passes the
+   reference's 0 <= m <= N-2 test (raft/raft_radon_slantstack_withmesh.c:184-186, 203-205);
    coord[...] = the reference's FP64 interpolation coordinate before floor(). */
-void oracle_radon_sample_map(raft_image img, raft_image sino, unsigned char *mask, double *coord) {
-  const int n = img.data.lines, nviews = sino.data.columns, nrays = sino.data.lines;
+void oracle_radon_sample_map_mesh(raft_image img, int nrays, const double *t, int nviews, const double *theta,
+                                  unsigned char *mask, double *coord) {
+  const int n = img.data.lines;
   const double tol = 1.0 / sqrt(2);
   const double xmin = img.tl_x, xmax = img.br_x, ymin = img.br_y, ymax = img.tl_y;
   const double dx = (xmax - xmin) / (n - 1), dy = (ymax - ymin) / (n - 1);
-  const double th0 = sino.tl_x, thf = sino.br_x, dth = (thf - th0) / (nviews);
-  const double tmin = sino.br_y, tmax = sino.tl_y, dt = (tmax - tmin) / (nrays - 1);
   for (int j = 0; j < nviews; ++j) {
-    const double theta = th0 + j * dth;
-    const double c = cos(theta), s = sin(theta), tn = tan(theta);
+    const double c = cos(theta[j]), s = sin(theta[j]), tn = tan(theta[j]);
     const int branch_a = fabs(c) < tol;
     for (int k = 0; k < nrays; ++k) {
-      const double t = tmin + k * dt;
       for (int i = 0; i < n; ++i) {
         double u;
         if (branch_a) {
           const double x = (double)(xmin + i * dx);
-          const double y = (double)(t / s - x / tn);
+          const double y = (double)(t[k] / s - x / tn);
           u = (y - ymin) / dy;
         } else {
           const double y = (double)(xmin + i * dy);
-          const double x = (double)(t / c - y * tn);
+          const double x = (double)(t[k] / c - y * tn);
           u = (x - xmin) / dx;
         }
         const int m = (int)floor(u);
@@ -412,6 +453,21 @@ void oracle_radon_sample_map(raft_image img, raft_image sino, unsigned char *mas
       }
     }
   }
+}
+
+/* the same map on the uniform grid theta = th0 + j*dth, t = tmin + k*dt
+   (raft/raft_radon_slantstack.c:153-172) */
+void oracle_radon_sample_map(raft_image img, raft_image sino, unsigned char *mask, double *coord) {
+  const int nviews = sino.data.columns, nrays = sino.data.lines;
+  const double th0 = sino.tl_x, thf = sino.br_x, dth = (thf - th0) / (nviews);
+  const double tmin = sino.br_y, tmax = sino.tl_y, dt = (tmax - tmin) / (nrays - 1);
+  double *t = (double *)malloc(sizeof(double) * nrays);
+  double *theta = (double *)malloc(sizeof(double) * nviews);
+  for (int k = 0; k < nrays; ++k) t[k] = tmin + k * dt;
+  for (int j = 0; j < nviews; ++j) theta[j] = th0 + j * dth;
+  oracle_radon_sample_map_mesh(img, nrays, t, nviews, theta, mask, coord);
+  free(t);
+  free(theta);
 }
 
 /* mask[(j*N + r)*N + c] = 1 iff pixel (r,c) takes angle j in the backprojection

@@ -22,6 +22,8 @@ Contents (each restates the cited reference routine):
 * ``loglik`` -- emission log-likelihood sum s*log(p) - sum p (xfct/likelihood.c:83-94).
 * ``phantom_fromdesc`` / ``radon_fromdesc`` -- raft/raft_image_phantom_fromdesc.c:4-59,
   raft/raft_radon_fromdesc.c:4-67.
+* ``radon_withmesh`` / ``radon_view`` -- the arbitrary-mesh forms,
+  raft/raft_radon_slantstack_withmesh.c:49-224 and raft/raft_radon_slantstack_view.c:22-113.
 * ``ref_*`` -- the same calls into ``_ref/libraft_ref.so``, the reference's own C
   compiled from its sources by ``oracle/Makefile`` (present only where it was built).
 
@@ -47,6 +49,15 @@ class RaftMatrix(ctypes.Structure):
 class RaftImage(ctypes.Structure):
     _fields_ = [("data", RaftMatrix), ("tl_x", ctypes.c_double), ("tl_y", ctypes.c_double),
                 ("br_x", ctypes.c_double), ("br_y", ctypes.c_double)]
+
+
+class RaftVector(ctypes.Structure):
+    _fields_ = [("p_data", ctypes.POINTER(ctypes.c_double)), ("size", ctypes.c_int), ("stride", ctypes.c_int)]
+
+
+def _vector(v):
+    assert v.dtype == np.float64 and v.ndim == 1
+    return RaftVector(v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), v.shape[0], v.strides[0] // 8)
 
 
 def _matrix(a):
@@ -79,6 +90,13 @@ def lib():
             f = getattr(_lib, name)
             f.argtypes = [RaftImage, RaftMatrix]
             f.restype = None
+        _lib.oracle_radon_withmesh.argtypes = [RaftImage, RaftImage, RaftVector, RaftVector, ctypes.c_int]
+        _lib.oracle_radon_withmesh.restype = None
+        _lib.oracle_radon_view.argtypes = [RaftVector, RaftImage, RaftVector, ctypes.c_double]
+        _lib.oracle_radon_view.restype = None
+        _lib.oracle_radon_sample_map_mesh.argtypes = [RaftImage, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_radon_sample_map_mesh.restype = None
         _lib.oracle_radon_sample_map.argtypes = [RaftImage, RaftImage, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_radon_sample_map.restype = None
         _lib.oracle_bp_sample_map.argtypes = [RaftImage, RaftImage, ctypes.c_void_p]
@@ -107,6 +125,11 @@ def ref():
             f = getattr(_ref, name)
             f.argtypes = [RaftImage, RaftMatrix]
             f.restype = None
+        # (phantom, radon, t, theta, nthreads): the definition's order, which pyraft uses
+        _ref.raft_radon_slantstack_withmesh.argtypes = [RaftImage, RaftImage, RaftVector, RaftVector, ctypes.c_int]
+        _ref.raft_radon_slantstack_withmesh.restype = None
+        _ref.raft_radon_slantstack_view.argtypes = [RaftVector, RaftImage, RaftVector, ctypes.c_double]
+        _ref.raft_radon_slantstack_view.restype = None
     return _ref
 
 
@@ -165,6 +188,19 @@ def radon_sample_map(n, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SI
     return mask.astype(bool), coord
 
 
+def radon_sample_map_mesh(n, t, theta, img_tl=IMG_TL, img_br=IMG_BR):
+    """radon_sample_map for rays at t[k] and angles theta[j] from tables."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    R, A = t.shape[0], theta.shape[0]
+    mask = np.zeros((A, R, n), np.uint8)
+    coord = np.zeros((A, R, n))
+    dummy = np.zeros((n, n))
+    lib().oracle_radon_sample_map_mesh(_image(dummy, img_tl, img_br), R, t.ctypes.data, A, theta.ctypes.data,
+                                       mask.ctypes.data, coord.ctypes.data)
+    return mask.astype(bool), coord
+
+
 def bp_sample_map(n, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR):
     """mask of shape (A, N, N): which (angle, pixel) pairs the reference backprojection uses."""
     img = np.zeros((n, n))
@@ -191,6 +227,47 @@ def radon_fromdesc(desc, nrays, nangles, tl=SINO_TL, br=SINO_BR):
 
 
 # ---------------------------------------------------------------- reference C
+
+def radon_withmesh(img, t, theta, img_tl=IMG_TL, img_br=IMG_BR, nthreads=None):
+    """Slant-stack Radon on arbitrary tables (raft/raft_radon_slantstack_withmesh.c:49-224):
+    element (k, j) = ray sum at theta[j], t[k]; returns (len(t), len(theta)) FP64."""
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    sino = np.zeros((t.shape[0], theta.shape[0]))
+    lib().oracle_radon_withmesh(_image(img, img_tl, img_br), _image(sino, (0.0, 1.0), (1.0, -1.0)), _vector(t),
+                                _vector(theta), _threads(nthreads))
+    return sino
+
+
+def radon_view(img, t, theta, img_tl=IMG_TL, img_br=IMG_BR):
+    """One angle (raft/raft_radon_slantstack_view.c:22-113); returns len(t) FP64."""
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    view = np.zeros(t.shape[0])
+    lib().oracle_radon_view(_vector(view), _image(img, img_tl, img_br), _vector(t), float(theta))
+    return view
+
+
+def ref_radon_withmesh(img, t, theta, img_tl=IMG_TL, img_br=IMG_BR, nthreads=1):
+    """The reference's withmesh; nthreads must divide len(t)*len(theta) for every output
+    to be written (its thread split drops the remainder)."""
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    sino = np.zeros((t.shape[0], theta.shape[0]))
+    ref().raft_radon_slantstack_withmesh(_image(img, img_tl, img_br), _image(sino, (0.0, 1.0), (1.0, -1.0)),
+                                         _vector(t), _vector(theta), nthreads)
+    return sino
+
+
+def ref_radon_view(img, t, theta, img_tl=IMG_TL, img_br=IMG_BR):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    view = np.zeros(t.shape[0])
+    ref().raft_radon_slantstack_view(_vector(view), _image(img, img_tl, img_br), _vector(t), float(theta))
+    return view
+
 
 def ref_radon(img, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
               nthreads=1):

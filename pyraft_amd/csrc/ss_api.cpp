@@ -61,15 +61,16 @@ static SSGeom make_geom(int n, int nrays, int nangles, const double* ic, const d
 
 static int64_t to_fix(double v) { return (int64_t)llrint(v * SS_FIX_ONE); }
 
-// Host-side angle table (glibc cos/sin/tan, as the reference calls them).
-static void build_angles(const SSGeom& g, std::vector<SSAngle>& out, int* ndeg) {
+// Host-side angle table (glibc cos/sin/tan, as the reference calls them).  `thetas`
+// (optional) gives the angles of a mesh plan (raft/raft_radon_slantstack_withmesh.c:193).
+static void build_angles(const SSGeom& g, std::vector<SSAngle>& out, int* ndeg, const double* thetas = nullptr) {
   const double tol = 1.0 / sqrt(2.0);  // raft/raft_radon_slantstack.c:139
   out.resize(g.nangles);
   *ndeg = 0;
   for (int j = 0; j < g.nangles; ++j) {
     SSAngle a;
     memset(&a, 0, sizeof(a));
-    const double theta = g.th0 + j * g.dth;  // :158
+    const double theta = thetas ? thetas[j] : g.th0 + j * g.dth;  // :158
     a.cost = cos(theta);
     a.sint = sin(theta);
     a.tant = tan(theta);
@@ -140,8 +141,24 @@ static int bp_variants(raft_ss_plan* p) {
   return 0;
 }
 
-extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nangles, const double* img_corners,
-                                   const double* sino_corners, void* stream_) {
+// Mesh plans: every trajectory offset must stay far inside Q32.32's int64 range.
+static int validate_mesh(const SSGeom& g, const std::vector<SSAngle>& angles, const double* t) {
+  double tmax = 0.0;
+  for (int k = 0; k < g.nrays; ++k) {
+    if (!std::isfinite(t[k])) return set_error(RAFT_SS_EINVAL, "non-finite detector coordinate t[%d]", k);
+    tmax = std::max(tmax, fabs(t[k]));
+  }
+  for (const SSAngle& a : angles) {
+    const double u = std::max(fabs(ss_ray_alpha(g, a, tmax)), fabs(ss_ray_alpha(g, a, -tmax)));
+    const double amax = u * SS_FIX_ONE + fabs((double)a.beta) * g.n;
+    if (!(amax < 4.0e18))
+      return set_error(RAFT_SS_EINVAL, "ray coordinates out of fixed-point range; unsupported mesh");
+  }
+  return 0;
+}
+
+static int plan_create(raft_ss_plan** out, int n, int nrays, int nangles, const double* img_corners,
+                       const double* sino_corners, const double* thetas, const double* ts, void* stream_) {
   if (!out || !img_corners || !sino_corners) return set_error(RAFT_SS_EINVAL, "null argument");
   *out = nullptr;
   hipStream_t stream = as_stream(stream_);
@@ -150,24 +167,32 @@ extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nan
   memcpy(p->img_corners, img_corners, sizeof(p->img_corners));
   memcpy(p->sino_corners, sino_corners, sizeof(p->sino_corners));
   p->g = make_geom(n, nrays, nangles, img_corners, sino_corners);
+  p->mesh = ts != nullptr;
   if (n < 2 || nrays < 2 || nangles < 1) return validate_geom(p->g, p->h_angles);
-  build_angles(p->g, p->h_angles, &p->ndegenerate);
+  build_angles(p->g, p->h_angles, &p->ndegenerate, thetas);
   int rc = validate_geom(p->g, p->h_angles);
   if (rc) return rc;
+  if (p->mesh && (rc = validate_mesh(p->g, p->h_angles, ts))) return rc;
 
   const SSGeom& g = p->g;
   p->nwaves = (nrays + 63) / 64;
-  if ((rc = bp_variants(p.get()))) return rc;
+  if (!p->mesh && (rc = bp_variants(p.get()))) return rc;
 
   const int capacity = (p->ndegenerate + 2) * nrays + 4096;
   HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSAngle) * nangles));
   HIP_TRY(hipMalloc(&p->d_bounds, sizeof(int2) * (size_t)nangles * nrays));
   HIP_TRY(hipMalloc(&p->d_wbounds, sizeof(int4) * (size_t)nangles * p->nwaves));
   HIP_TRY(hipMalloc(&p->d_irregular, sizeof(int) * (capacity + 1)));
+  if (p->mesh) {
+    HIP_TRY(hipMalloc(&p->d_ray_t, sizeof(double) * nrays));
+    HIP_TRY(hipMalloc(&p->d_ray_alpha, sizeof(int64_t) * (size_t)nangles * nrays));
+    HIP_TRY(hipMemcpyAsync(p->d_ray_t, ts, sizeof(double) * nrays, hipMemcpyHostToDevice, stream));
+  }
   int* d_count = p->d_irregular + capacity;
   HIP_TRY(hipMemcpyAsync(p->d_angles, p->h_angles.data(), sizeof(SSAngle) * nangles, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), stream));
-  HIP_TRY(ss_launch_plan_bounds(g, p->d_angles, p->d_bounds, p->d_irregular, d_count, capacity, stream));
+  HIP_TRY(ss_launch_plan_bounds(g, p->d_angles, p->d_bounds, p->d_irregular, d_count, capacity, p->d_ray_t,
+                                p->d_ray_alpha, stream));
   HIP_TRY(ss_launch_plan_wave_bounds(g, p->d_bounds, p->d_wbounds, stream));
   int count = 0;
   HIP_TRY(hipMemcpyAsync(&count, d_count, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -176,6 +201,21 @@ extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nan
   p->nirregular = count;
   *out = p.release();
   return 0;
+}
+
+extern "C" int raft_ss_plan_create(raft_ss_plan** out, int n, int nrays, int nangles, const double* img_corners,
+                                   const double* sino_corners, void* stream) {
+  return plan_create(out, n, nrays, nangles, img_corners, sino_corners, nullptr, nullptr, stream);
+}
+
+extern "C" int raft_ss_plan_create_mesh(raft_ss_plan** out, int n, const double* img_corners, int nrays,
+                                        const double* t, int nangles, const double* theta, void* stream) {
+  if (!t || !theta) return set_error(RAFT_SS_EINVAL, "null argument");
+  if (nrays < 2 || nangles < 1) return set_error(RAFT_SS_EINVAL, "a mesh needs nrays >= 2 and nangles >= 1");
+  // the kernels take t and theta from the tables; the uniform-grid fields of the
+  // geometry record are unused and get a valid placeholder extent
+  const double sc[4] = {0.0, 1.0, 1.0, -1.0};
+  return plan_create(out, n, nrays, nangles, img_corners, sc, theta, t, stream);
 }
 
 extern "C" int raft_ss_plan_destroy(raft_ss_plan* plan) {
@@ -250,6 +290,39 @@ extern "C" int raft_ss_plan_host_bounds(int n, int nrays, int nangles, const dou
       host_fix[3 * j + 2] = angles[j].beta;
     }
     if (host_branch) host_branch[j] = angles[j].branch_a;
+  }
+  return 0;
+}
+
+// Host evaluation of a mesh plan's ranges and per-ray Q32.32 trajectory offsets, in the
+// layouts of raft_ss_plan_bounds ([A][R] x (lo, hi)) and [A][R] int64.
+extern "C" int raft_ss_plan_host_bounds_mesh(int n, const double* img_corners, int nrays, const double* t,
+                                             int nangles, const double* theta, int* host_bounds,
+                                             long long* host_alpha) {
+  if (!img_corners || !t || !theta || !host_bounds) return set_error(RAFT_SS_EINVAL, "null argument");
+  if (nrays < 2 || nangles < 1) return set_error(RAFT_SS_EINVAL, "a mesh needs nrays >= 2 and nangles >= 1");
+  const double sc[4] = {0.0, 1.0, 1.0, -1.0};
+  SSGeom g = make_geom(n, nrays, nangles, img_corners, sc);
+  std::vector<SSAngle> angles;
+  int ndeg = 0;
+  if (n < 2) return validate_geom(g, angles);
+  build_angles(g, angles, &ndeg, theta);
+  int rc = validate_geom(g, angles);
+  if (rc || (rc = validate_mesh(g, angles, t))) return rc;
+  for (int j = 0; j < nangles; ++j) {
+    const SSAngle& a = angles[j];
+    const SSTraj tr = ss_trajectory(g, a);
+    for (int k = 0; k < nrays; ++k) {
+      const double u0 = ss_ray_alpha(g, a, t[k]);
+      int lo, hi;
+      if (ss_ray_range_t(g, a, tr.beta, t[k], u0, &lo, &hi)) {
+        lo = 0;
+        hi = -2;
+      }
+      host_bounds[2 * ((size_t)j * nrays + k)] = lo;
+      host_bounds[2 * ((size_t)j * nrays + k) + 1] = hi;
+      if (host_alpha) host_alpha[(size_t)j * nrays + k] = (long long)llrint(u0 * SS_FIX_ONE);
+    }
   }
   return 0;
 }
@@ -389,6 +462,7 @@ extern "C" int raft_ss_backprojection_range(const raft_ss_plan* p, const float* 
                                             int angle_major, int j0, int j1, int nslices, void* stream_) {
   int rc = check_plan(p, nslices);
   if (rc) return rc;
+  if ((rc = check_uniform(p, "backprojection"))) return rc;
   if ((rc = check_range(p, j0, j1))) return rc;
   if (!d_image || !d_sino) return set_error(RAFT_SS_EINVAL, "null device pointer");
   hipStream_t stream = as_stream(stream_);
@@ -415,6 +489,7 @@ extern "C" int raft_ss_radon_transpose_range(const raft_ss_plan* p, const float*
                                              int angle_major, int j0, int j1, int nslices, void* stream_) {
   int rc = check_plan(p, nslices);
   if (rc) return rc;
+  if ((rc = check_uniform(p, "radon_transpose"))) return rc;
   if ((rc = check_range(p, j0, j1))) return rc;
   if (!d_image || !d_sino) return set_error(RAFT_SS_EINVAL, "null device pointer");
   hipStream_t stream = as_stream(stream_);
@@ -442,6 +517,7 @@ extern "C" int raft_ss_fbp_filter(raft_ss_plan* p, const float* d_sino, float* d
                                   void* stream_) {
   int rc = check_plan(p, nslices);
   if (rc) return rc;
+  if ((rc = check_uniform(p, "fbp_filter"))) return rc;
   if (!d_sino || !d_out) return set_error(RAFT_SS_EINVAL, "null device pointer");
   hipStream_t stream = as_stream(stream_);
   if ((rc = ensure_ramp(p, stream))) return rc;
@@ -464,6 +540,7 @@ extern "C" int raft_ss_fbp_filter(raft_ss_plan* p, const float* d_sino, float* d
 extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image, int nslices, void* stream_) {
   int rc = check_plan(p, nslices);
   if (rc) return rc;
+  if ((rc = check_uniform(p, "fbp"))) return rc;
   if (!d_sino || !d_image) return set_error(RAFT_SS_EINVAL, "null device pointer");
   hipStream_t stream = as_stream(stream_);
   if ((rc = ensure_ramp(p, stream))) return rc;
@@ -488,6 +565,7 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
                           double* ll_host, void* stream_) {
   int rc = check_plan(p, nslices);
   if (rc) return rc;
+  if ((rc = check_uniform(p, "em"))) return rc;
   if (!d_sino || !d_x) return set_error(RAFT_SS_EINVAL, "null device pointer");
   if (niter < 0) return set_error(RAFT_SS_EINVAL, "niter must be >= 0");
   const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
@@ -682,6 +760,48 @@ int host_radon(const raft_image& phantom, const raft_image& radon) {
   return 0;
 }
 
+// Radon on arbitrary (t, theta) tables into an nrays x nangles strided matrix, element
+// (k, j) = ray k, angle j (raft/raft_radon_slantstack_withmesh.c:118-121)
+int host_radon_mesh(const raft_image& phantom, const raft_matrix& out, const raft_vector& t,
+                    const raft_vector& theta) {
+  if (!phantom.data.p_data || !out.p_data || !t.p_data || !theta.p_data)
+    return set_error(RAFT_SS_EINVAL, "empty image or vector");
+  const int n = phantom.data.lines;
+  if (phantom.data.columns != n)
+    return set_error(RAFT_SS_EINVAL, "slant-stack needs a square image (got %dx%d)", n, phantom.data.columns);
+  const int R = out.lines, A = out.columns;
+  if (t.size < R || theta.size < A)
+    return set_error(RAFT_SS_EINVAL, "mesh tables too short (t: %d < %d rays or theta: %d < %d angles)", t.size, R,
+                     theta.size, A);
+  if (R < 1 || A < 1) return 0;
+  // a one-ray table is run as two rays at the same t
+  const int Rp = std::max(R, 2);
+  std::vector<double> ht(Rp), hth(A);
+  for (int k = 0; k < Rp; ++k) ht[k] = t.p_data[(long)std::min(k, R - 1) * t.stride];
+  for (int j = 0; j < A; ++j) hth[j] = theta.p_data[(long)j * theta.stride];
+  double ic[4];
+  corners_of(phantom, ic);
+  raft_ss_plan* raw = nullptr;
+  int rc = raft_ss_plan_create_mesh(&raw, n, ic, Rp, ht.data(), A, hth.data(), nullptr);
+  if (rc) return rc;
+  std::unique_ptr<raft_ss_plan, int (*)(raft_ss_plan*)> plan(raw, raft_ss_plan_destroy);
+  PinnedF32 hi, ho;
+  HIP_TRY(hi.alloc((size_t)n * n));
+  HIP_TRY(ho.alloc((size_t)Rp * A));
+  gather_f64(phantom.data, hi.p);
+  DevBuf di, dso;
+  HIP_TRY(di.alloc(sizeof(float) * n * n, nullptr));
+  HIP_TRY(dso.alloc(sizeof(float) * (size_t)Rp * A, nullptr));
+  HIP_TRY(hipMemcpyAsync(di.p, hi.p, sizeof(float) * n * n, hipMemcpyHostToDevice, nullptr));
+  rc = raft_ss_radon(plan.get(), di.as<float>(), dso.as<float>(), 1, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ho.p, dso.p, sizeof(float) * (size_t)Rp * A, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  for (int k = 0; k < R; ++k)
+    for (int j = 0; j < A; ++j) RAFT_MATRIX_AT(out, k, j) = (double)ho.p[(size_t)k * A + j];
+  return 0;
+}
+
 int host_backprojection(const raft_image& sino, const raft_image& img) {
   if (!sino.data.p_data || !img.data.p_data) return set_error(RAFT_SS_EINVAL, "empty image");
   const int n = img.data.lines;
@@ -793,6 +913,30 @@ extern "C" void raft_backprojection_slantstack(raft_image sinogram, raft_image b
 extern "C" void raft_radon_transpose_slantstack(raft_image sinogram, raft_image image, int nthreads) {
   (void)nthreads;
   report("raft_radon_transpose_slantstack", host_radon_transpose(sinogram, image));
+}
+
+extern "C" void raft_radon_slantstack_withmesh(raft_image phantom, raft_image radon, raft_vector t, raft_vector theta,
+                                               int nthreads) {
+  (void)nthreads;
+  report("raft_radon_slantstack_withmesh", host_radon_mesh(phantom, radon.data, t, theta));
+}
+
+extern "C" void raft_radon_slantstack_view(raft_vector view, raft_image phantom, raft_vector t, double theta) {
+  // one angle: the view is a t.size x 1 matrix over the view vector
+  raft_matrix out;
+  out.p_data = view.p_data;
+  out.lines = t.size;
+  out.columns = 1;
+  out.column_stride = view.stride;
+  out.line_stride = 1;
+  raft_vector th;
+  th.p_data = &theta;
+  th.size = 1;
+  th.stride = 1;
+  int rc = 0;
+  if (view.size < t.size) rc = set_error(RAFT_SS_EINVAL, "view vector shorter than t (%d < %d)", view.size, t.size);
+  if (!rc) rc = host_radon_mesh(phantom, out, t, th);
+  report("raft_radon_slantstack_view", rc);
 }
 
 extern "C" void raft_radon_slantstack_gpu(float* h_output, float* h_input, int sizeImage, int nrays, int nangles) {

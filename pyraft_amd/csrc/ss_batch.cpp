@@ -41,7 +41,9 @@ static int check_variant(const raft_ss_plan* p, int v) {
 
 extern "C" int raft_ss_plan_set_bp_variant(raft_ss_plan* p, int v) {
   if (!p) return ss_set_error(RAFT_SS_EINVAL, "null plan");
-  int rc = check_variant(p, v);
+  int rc = check_uniform(p, "set_bp_variant");
+  if (rc) return rc;
+  rc = check_variant(p, v);
   if (rc) return rc;
   p->bp_variant = v;
   return 0;
@@ -106,6 +108,7 @@ std::string plan_key(const raft_ss_plan* p) {
 extern "C" int raft_ss_plan_tune(raft_ss_plan* p, int reps, int flags, double* times_us, void* stream_) {
   int rc = check_plan(p, 1);
   if (rc) return rc;
+  if ((rc = check_uniform(p, "plan_tune"))) return rc;
   hipStream_t stream = as_stream(stream_);
   const std::string key = plan_key(p);
   if (!(flags & RAFT_SS_TUNE_FORCE)) {

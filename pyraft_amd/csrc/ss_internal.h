@@ -37,6 +37,9 @@ struct raft_ss_plan {
   int2* d_bounds = nullptr;
   int4* d_wbounds = nullptr;
   int* d_irregular = nullptr;
+  double* d_ray_t = nullptr;       // mesh plans: detector coordinates [R]
+  int64_t* d_ray_alpha = nullptr;  // mesh plans: trajectory offsets [A][R]
+  bool mesh = false;               // arbitrary (theta, t) tables: Radon only
   int nirregular = 0;
   int ndegenerate = 0;
   int nwaves = 0;
@@ -54,6 +57,8 @@ struct raft_ss_plan {
     p.bounds = d_bounds;
     p.wbounds = d_wbounds;
     p.irregular = d_irregular;
+    p.ray_t = d_ray_t;
+    p.ray_alpha = d_ray_alpha;
     p.nirregular = nirregular;
     p.nwaves = nwaves;
     p.bp_variant = v;
@@ -66,6 +71,8 @@ struct raft_ss_plan {
     if (d_bounds) (void)hipFree(d_bounds);
     if (d_wbounds) (void)hipFree(d_wbounds);
     if (d_irregular) (void)hipFree(d_irregular);
+    if (d_ray_t) (void)hipFree(d_ray_t);
+    if (d_ray_alpha) (void)hipFree(d_ray_alpha);
     if (d_ramp) (void)hipFree(d_ramp);
   }
 };
@@ -100,6 +107,12 @@ inline int check_plan(const raft_ss_plan* p, int nslices) {
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
     return ss_set_error(RAFT_SS_EINVAL, "plan was built on device %d but device %d is current", p->device, dev);
+  return 0;
+}
+
+// operators other than the Radon need the uniform detector grid
+inline int check_uniform(const raft_ss_plan* p, const char* what) {
+  if (p && p->mesh) return ss_set_error(RAFT_SS_EINVAL, "%s needs a uniform-grid plan (mesh plans do the Radon only)", what);
   return 0;
 }
 

@@ -35,6 +35,8 @@ struct SSDevPlan {
   const int2* bounds;      // [A][R]   exact sample range per ray; (0,-1) empty, (0,-2) irregular
   const int4* wbounds;     // [A][ceil(R/64)] (union lo, union hi, inter lo, inter hi)
   const int* irregular;    // list of j*R + k
+  const double* ray_t;       // mesh plans: detector coordinate of each ray [R]; nullptr: uniform grid
+  const int64_t* ray_alpha;  // mesh plans: Q32.32 trajectory offset per ray [A][R]; nullptr: alpha0 + k*gamma
   int nirregular;
   int nwaves;              // ceil(R/64)
   int bp_variant;          // index into kSSBpRows / kSSBpStage
@@ -66,7 +68,8 @@ struct SSBatchLaunch {
 
 // ---- plan construction
 hipError_t ss_launch_plan_bounds(const SSGeom& g, const SSAngle* angles, int2* bounds, int* irregular,
-                                 int* nirregular, int capacity, hipStream_t stream);
+                                 int* nirregular, int capacity, const double* ray_t, int64_t* ray_alpha,
+                                 hipStream_t stream);
 hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4* wbounds, hipStream_t stream);
 
 // ---- layouts

@@ -19,16 +19,26 @@
 // plan: exact per-ray sample ranges
 // =====================================================================================
 
+// ray_t == nullptr: the uniform detector grid.  Otherwise ray k sits at ray_t[k] (the
+// arbitrary-mesh form) and its Q32.32 trajectory offset goes to ray_alpha[j][k].
 __global__ __launch_bounds__(256) void k_plan_bounds(SSGeom g, const SSAngle* __restrict__ angles,
                                                      int2* __restrict__ bounds, int* __restrict__ irregular,
-                                                     int* __restrict__ nirregular, int capacity) {
+                                                     int* __restrict__ nirregular, int capacity,
+                                                     const double* __restrict__ ray_t, int64_t* __restrict__ ray_alpha) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (k >= g.nrays) return;
   const SSAngle a = angles[j];
   const SSTraj tr = ss_trajectory(g, a);
-  int lo, hi;
-  const int irr = ss_ray_range(g, a, tr, k, &lo, &hi);
+  int lo, hi, irr;
+  if (ray_t) {
+    const double t = ray_t[k];
+    const double u0 = ss_ray_alpha(g, a, t);
+    ray_alpha[(size_t)j * g.nrays + k] = __double2ll_rn(u0 * SS_FIX_ONE);
+    irr = ss_ray_range_t(g, a, tr.beta, t, u0, &lo, &hi);
+  } else {
+    irr = ss_ray_range(g, a, tr, k, &lo, &hi);
+  }
   if (irr) {
     lo = 0;
     hi = -2;  // empty for the main kernels; the fixup kernels evaluate it sample by sample
@@ -76,9 +86,11 @@ __global__ __launch_bounds__(64) void k_plan_wave_bounds(SSGeom g, const int2* _
 }
 
 hipError_t ss_launch_plan_bounds(const SSGeom& g, const SSAngle* angles, int2* bounds, int* irregular,
-                                 int* nirregular, int capacity, hipStream_t stream) {
+                                 int* nirregular, int capacity, const double* ray_t, int64_t* ray_alpha,
+                                 hipStream_t stream) {
   dim3 grid((g.nrays + 255) / 256, g.nangles);
-  hipLaunchKernelGGL(k_plan_bounds, grid, dim3(256), 0, stream, g, angles, bounds, irregular, nirregular, capacity);
+  hipLaunchKernelGGL(k_plan_bounds, grid, dim3(256), 0, stream, g, angles, bounds, irregular, nirregular, capacity,
+                     ray_t, ray_alpha);
   return hipGetLastError();
 }
 
@@ -204,11 +216,12 @@ struct SSRadonView {
   const SSAngle* angles;
   const int2* bounds;
   const int4* wbounds;
+  const int64_t* ray_alpha;  // per-ray trajectory offsets of a mesh plan, else nullptr
   int n, R, A, pitch, nwaves;
 };
 
 __device__ __forceinline__ SSRadonView ss_radon_view(const SSDevPlan& p) {
-  return SSRadonView{p.angles, p.bounds, p.wbounds, p.g.n, p.g.nrays, p.g.nangles, p.g.pitch, p.nwaves};
+  return SSRadonView{p.angles, p.bounds, p.wbounds, p.ray_alpha, p.g.n, p.g.nrays, p.g.nangles, p.g.pitch, p.nwaves};
 }
 
 // out / meas point at this slice's sinogram
@@ -393,6 +406,9 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
   const bool empty = hi < lo;
   const unsigned span = (unsigned)(hi - lo);
   const uint32_t row_bytes = (uint32_t)pitch * 8u;
+  // this lane's trajectory offset: u = a_k + s*beta
+  int64_t a_k = alpha + (int64_t)k * gamma;
+  if (p.ray_alpha && active && k < R) a_k = p.ray_alpha[(size_t)j * R + k];
 
   bool staged = false;
   if (split == 1) {
@@ -424,7 +440,14 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
           (void*)lay, (short)sizeof(float2), (int)((uint32_t)n * (uint32_t)pitch * 8u), SS_RSRC_DWORD3);
       const SSGatherReader grd{rs, row_bytes};
       const SSStageReader srd{lds.stage};
-      const int k_first = grp * SS_WAVE, k_last = min(grp * SS_WAVE + SS_WAVE - 1, R - 1);
+      // the wave's extreme trajectory offsets (over its rays on the detector)
+      int64_t amin = (k < R) ? a_k : INT64_MAX, amax = (k < R) ? a_k : INT64_MIN;
+      for (int off = 32; off > 0; off >>= 1) {
+        amin = min(amin, (int64_t)__shfl_xor((long long)amin, off));
+        amax = max(amax, (int64_t)__shfl_xor((long long)amax, off));
+      }
+      amin = ss_rfl64(amin);
+      amax = ss_rfl64(amax);
       for (int s0 = s_lo; s0 <= s_hi;) {
         int s1 = min(s0 + RADON_CHUNK, s_hi + 1);
         // this wave's cells over its rows of the chunk (valid samples have m in [-1, n-1])
@@ -432,8 +455,8 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
         if (lane == 0) {
           int cmin = 0x7fffffff, cmax = -0x7fffffff;
           if (branch >= 0 && ra <= rb) {
-            const int64_t c00 = alpha + (int64_t)k_first * gamma + (int64_t)ra * beta;
-            const int64_t c10 = alpha + (int64_t)k_last * gamma + (int64_t)ra * beta;
+            const int64_t c00 = amin + (int64_t)ra * beta;
+            const int64_t c10 = amax + (int64_t)ra * beta;
             const int64_t ds = (int64_t)(rb - ra) * beta;
             const int m0 = (int)(c00 >> 32), m1 = (int)(c10 >> 32);
             const int m2 = (int)((c00 + ds) >> 32), m3 = (int)((c10 + ds) >> 32);
@@ -477,8 +500,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
           if (Wc > 0 && rows > 0) {
             // high word: staged record index (cell + 1 - rec0) + (row - s0) * Wc
             const int64_t rowstep = (int64_t)Wc << 32;
-            int64_t u = alpha + (int64_t)k * gamma + (int64_t)ra2 * beta + ((int64_t)(1 - rec0) << 32) +
-                        (int64_t)(ra2 - s0) * rowstep;
+            int64_t u = a_k + (int64_t)ra2 * beta + ((int64_t)(1 - rec0) << 32) + (int64_t)(ra2 - s0) * rowstep;
             int64_t step = beta + rowstep;
             if (empty) {
               u = (int64_t)RADON_LDS_CAP << 32;
@@ -486,7 +508,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
             }
             ss_radon_rows(srd, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
           } else if (Wc > 0) {
-            int64_t u = alpha + (int64_t)k * gamma + (int64_t)ra2 * beta + ((int64_t)1 << 32);
+            int64_t u = a_k + (int64_t)ra2 * beta + ((int64_t)1 << 32);
             int64_t step = beta;
             if (empty) {
               u = 0;
@@ -507,7 +529,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
     const int sb = min(wb.y + 1, sa + chunk);
     // u carries the trajectory plus one cell: its high word is the record index m + 1
     // of the lower tap in the gather row, its low word the interpolation weight
-    int64_t u = alpha + (int64_t)k * gamma + (int64_t)sa * beta + ((int64_t)1 << 32);
+    int64_t u = a_k + (int64_t)sa * beta + ((int64_t)1 << 32);
     int64_t step = beta;
     if (empty) {
       // an empty ray reads record 0 (lower tap 0) with weight 0: exactly 0 in the unpredicated loop
@@ -572,14 +594,16 @@ __device__ __forceinline__ void ss_radon_fixup_item(const SSDevPlan& p, const fl
   }
   const SSAngle& a = p.angles[j];
   const float2* img = (a.branch_a ? gA : gB) + 1;
+  const double t = p.ray_t ? p.ray_t[k] : ss_ray_t(p.g, k);
+  const int64_t a_k = p.ray_alpha ? p.ray_alpha[(size_t)j * R + k] : a.alpha0 + (int64_t)k * a.gamma;
   float acc = 0.f;
   for (int s = threadIdx.x; s < n; s += blockDim.x) {
-    if (ss_radon_sample_valid(p.g, a, k, s)) {
-      const int64_t u = a.alpha0 + (int64_t)k * a.gamma + (int64_t)s * a.beta;
+    if (ss_radon_sample_valid_t(p.g, a, t, s)) {
+      const int64_t u = a_k + (int64_t)s * a.beta;
       const int m = (int)(u >> 32);
       const float w = (float)(uint32_t)u * SS_INV_FIX;
-      const float2 t = img[(size_t)s * pitch + m];
-      acc += fmaf(w, t.y, t.x);
+      const float2 rec = img[(size_t)s * pitch + m];
+      acc += fmaf(w, rec.y, rec.x);
     }
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
@@ -1340,6 +1364,7 @@ __global__ __launch_bounds__(512) void k_batch_radon(SSBatchLaunch L, const floa
     v.angles = ss_uniform_copy(&sp->p.angles);
     v.bounds = ss_uniform_copy(&sp->p.bounds);
     v.wbounds = ss_uniform_copy(&sp->p.wbounds);
+    v.ray_alpha = ss_uniform_copy(&sp->p.ray_alpha);
     v.n = ss_uniform_copy(&sp->p.g.n);
     v.R = ss_uniform_copy(&sp->p.g.nrays);
     v.A = ss_uniform_copy(&sp->p.g.nangles);

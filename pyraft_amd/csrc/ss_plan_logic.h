@@ -29,12 +29,19 @@
 #define SS_NOCONTRACT
 #endif
 
+// the detector coordinate of ray k on the uniform grid (raft/raft_radon_slantstack.c:170-172)
+static inline SS_HD double ss_ray_t(const SSGeom& g, int k) {
+  SS_NOCONTRACT
+  return g.tmin + k * g.dt;
+}
+
+// sample s of the ray at detector coordinate t; also the rule of the arbitrary-mesh
+// forms, which take t and theta from tables (raft/raft_radon_slantstack_withmesh.c:176-224).
 // floor() result compared in double: equivalent to the reference's (int) cast plus
 // integer compare for every in-range value, and excludes NaN / out-of-range values
 // (the reference's cvttsd2si yields INT_MIN for those, which also fails the test).
-static inline SS_HD bool ss_radon_sample_valid(const SSGeom& g, const SSAngle& a, int k, int s) {
+static inline SS_HD bool ss_radon_sample_valid_t(const SSGeom& g, const SSAngle& a, double t, int s) {
   SS_NOCONTRACT
-  const double t = g.tmin + k * g.dt;
   double v;
   if (a.branch_a) {
     const double x = g.xmin + s * g.dx;
@@ -46,6 +53,10 @@ static inline SS_HD bool ss_radon_sample_valid(const SSGeom& g, const SSAngle& a
     v = floor((x - g.xmin) / g.dx);
   }
   return v >= 0.0 && v <= (double)(g.n - 2);
+}
+
+static inline SS_HD bool ss_radon_sample_valid(const SSGeom& g, const SSAngle& a, int k, int s) {
+  return ss_radon_sample_valid_t(g, a, ss_ray_t(g, k), s);
 }
 
 static inline SS_HD bool ss_bp_sample_valid(const SSGeom& g, double cost, double sint, int row, int col) {
@@ -62,6 +73,13 @@ static inline SS_HD bool ss_bp_sample_valid(const SSGeom& g, double cost, double
 struct SSTraj {
   double alpha0, gamma, beta;
 };
+
+// u at s = 0 of the ray at detector coordinate t (any t: the mesh forms)
+static inline SS_HD double ss_ray_alpha(const SSGeom& g, const SSAngle& a, double t) {
+  SS_NOCONTRACT
+  if (a.branch_a) return (t / a.sint - g.xmin / a.tant - g.ymin) / g.dy;
+  return (t / a.cost - g.xmin * a.tant - g.xmin) / g.dx;
+}
 
 static inline SS_HD SSTraj ss_trajectory(const SSGeom& g, const SSAngle& a) {
   SS_NOCONTRACT
@@ -82,18 +100,27 @@ static inline SS_HD SSTraj ss_trajectory(const SSGeom& g, const SSAngle& a) {
 // rounding noise (~1e-12 grid units) can then make the valid set non-contiguous.
 #define SS_DEGENERATE_SLOPE 1e-7
 
-// Exact contiguous valid range [lo, hi] of ray (j,k).  Returns 0 on success with
-// hi < lo for an empty ray; returns 1 when the valid set is not a single interval
-// (the ray is then handled sample by sample).
+// Exact contiguous valid range [lo, hi] of the ray at detector coordinate t whose
+// real trajectory is u = u0 + s*beta.  Returns 0 on success with hi < lo for an
+// empty ray; returns 1 when the valid set is not a single interval (the ray is then
+// handled sample by sample).
+static inline SS_HD int ss_ray_range_t(const SSGeom& g, const SSAngle& a, double beta, double t, double u0,
+                                       int* lo_out, int* hi_out);
+
 static inline SS_HD int ss_ray_range(const SSGeom& g, const SSAngle& a, const SSTraj& tr, int k, int* lo_out,
                                      int* hi_out) {
+  return ss_ray_range_t(g, a, tr.beta, ss_ray_t(g, k), tr.alpha0 + k * tr.gamma, lo_out, hi_out);
+}
+
+static inline SS_HD int ss_ray_range_t(const SSGeom& g, const SSAngle& a, double beta, double t, double u0,
+                                       int* lo_out, int* hi_out) {
   const int n = g.n;
   *lo_out = 0;
   *hi_out = -1;
   if (a.degenerate) {
     int first = -1, last = -1, count = 0;
     for (int s = 0; s < n; ++s) {
-      if (ss_radon_sample_valid(g, a, k, s)) {
+      if (ss_radon_sample_valid_t(g, a, t, s)) {
         if (first < 0) first = s;
         last = s;
         ++count;
@@ -106,9 +133,8 @@ static inline SS_HD int ss_ray_range(const SSGeom& g, const SSAngle& a, const SS
     return 0;
   }
   // valid iff 0 <= u < n-1 with u = u0 + s*beta (beta != 0); real-valued interval first
-  const double u0 = tr.alpha0 + k * tr.gamma;
-  const double sa = (0.0 - u0) / tr.beta;
-  const double sb = ((double)(n - 1) - u0) / tr.beta;
+  const double sa = (0.0 - u0) / beta;
+  const double sb = ((double)(n - 1) - u0) / beta;
   double s_lo = sa < sb ? sa : sb;
   double s_hi = sa < sb ? sb : sa;
   if (!(s_hi >= -3.0) || !(s_lo <= (double)n + 2.0)) return 0;  // misses the image entirely
@@ -124,7 +150,7 @@ static inline SS_HD int ss_ray_range(const SSGeom& g, const SSAngle& a, const SS
     int b = glo - 3 < 0 ? 0 : glo - 3;
     int e = glo + 3 > n - 1 ? n - 1 : glo + 3;
     for (int s = b; s <= e; ++s)
-      if (ss_radon_sample_valid(g, a, k, s)) {
+      if (ss_radon_sample_valid_t(g, a, t, s)) {
         lo = s;
         break;
       }
@@ -133,7 +159,7 @@ static inline SS_HD int ss_ray_range(const SSGeom& g, const SSAngle& a, const SS
     int b = ghi + 3 > n - 1 ? n - 1 : ghi + 3;
     int e = ghi - 3 < 0 ? 0 : ghi - 3;
     for (int s = b; s >= e; --s)
-      if (ss_radon_sample_valid(g, a, k, s)) {
+      if (ss_radon_sample_valid_t(g, a, t, s)) {
         hi = s;
         break;
       }
@@ -141,14 +167,14 @@ static inline SS_HD int ss_ray_range(const SSGeom& g, const SSAngle& a, const SS
   if (lo < 0 && hi < 0) {
     // nothing found near either end: empty unless the estimate is badly off
     const int mid = (glo + ghi) / 2;
-    if (glo <= ghi && mid >= 0 && mid < n && ss_radon_sample_valid(g, a, k, mid)) return 1;
+    if (glo <= ghi && mid >= 0 && mid < n && ss_radon_sample_valid_t(g, a, t, mid)) return 1;
     return 0;
   }
   if (lo < 0 || hi < 0 || hi < lo) return 1;
   // verify the interval is maximal and its interior holds
-  if (lo > 0 && ss_radon_sample_valid(g, a, k, lo - 1)) return 1;
-  if (hi < n - 1 && ss_radon_sample_valid(g, a, k, hi + 1)) return 1;
-  if (!ss_radon_sample_valid(g, a, k, (lo + hi) / 2)) return 1;
+  if (lo > 0 && ss_radon_sample_valid_t(g, a, t, lo - 1)) return 1;
+  if (hi < n - 1 && ss_radon_sample_valid_t(g, a, t, hi + 1)) return 1;
+  if (!ss_radon_sample_valid_t(g, a, t, (lo + hi) / 2)) return 1;
   *lo_out = lo;
   *hi_out = hi;
   return 0;

@@ -72,6 +72,35 @@ class Plan:
             check(libraft.raft_ss_plan_create(ctypes.byref(self._handle), self.n, self.nrays, self.nangles,
                                               ic, sc, current_stream_handle()), "raft_ss_plan_create")
 
+    @classmethod
+    def mesh(cls, n, t, theta, img_corners=DEFAULT_IMG_CORNERS, device=None):
+        """Plan on arbitrary tables: ray k at detector coordinate ``t[k]``, angle j at
+        ``theta[j]`` (the geometry of raft_radon_slantstack_withmesh,
+        raft/raft_radon_slantstack_withmesh.c:176-224).  Only ``radon`` accepts it."""
+        _require_gpu()
+        import numpy
+        ts = numpy.ascontiguousarray(t, dtype=numpy.float64).reshape(-1)
+        ths = numpy.ascontiguousarray(theta, dtype=numpy.float64).reshape(-1)
+        self = cls.__new__(cls)
+        self.n, self.nrays, self.nangles = int(n), int(ts.shape[0]), int(ths.shape[0])
+        self.img_corners = tuple(float(v) for v in img_corners)
+        self.sino_corners = None
+        self.t, self.theta = ts, ths
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else
+                                   torch.device(device).index or 0)
+        self._handle = ctypes.c_void_p()
+        ic = (ctypes.c_double * 4)(*self.img_corners)
+        dp = ctypes.POINTER(ctypes.c_double)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_ss_plan_create_mesh(ctypes.byref(self._handle), self.n, ic, self.nrays,
+                                                   ts.ctypes.data_as(dp), self.nangles, ths.ctypes.data_as(dp),
+                                                   current_stream_handle()), "raft_ss_plan_create_mesh")
+        return self
+
+    @property
+    def is_mesh(self):
+        return self.sino_corners is None
+
     # -- bookkeeping
     def __del__(self):
         h = getattr(self, "_handle", None)

@@ -115,6 +115,8 @@ _decl("raft_radon_slantstack_gpu", [_c_float_p, _c_float_p, _i, _i, _i], None)
 _decl("raft_backprojection_slantstack_gpu", [_c_float_p, _c_float_p, _i, _i, _i], None)
 _decl("raft_em_gpu", [_c_float_p, _c_float_p, _i, _i, _i, _i], None)
 _decl("raft_radon_transpose_slantstack", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
+_decl("raft_radon_slantstack_withmesh", [RAFT_IMAGE, RAFT_IMAGE, RAFT_VECTOR, RAFT_VECTOR, _i], None)
+_decl("raft_radon_slantstack_view", [RAFT_VECTOR, RAFT_IMAGE, RAFT_VECTOR, ctypes.c_double], None)
 _decl("raft_ss_legacy_status", [], _i)
 
 # new entry points
@@ -128,6 +130,9 @@ _decl("raft_ss_plan_wave_bounds", [_vp, ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_host_bounds", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i),
                                    ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_host_info", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i)], _i)
+_decl("raft_ss_plan_create_mesh", [ctypes.POINTER(_vp), _i, _d4, _i, _d4, _i, _d4, _vp], _i)
+_decl("raft_ss_plan_host_bounds_mesh", [_i, _d4, _i, _d4, _i, _d4, ctypes.POINTER(_i),
+                                        ctypes.POINTER(ctypes.c_longlong)], _i)
 _decl("raft_ss_radon", [_vp, _vp, _vp, _i, _vp], _i)
 _decl("raft_ss_radon_range", [_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i)
 _decl("raft_ss_backprojection", [_vp, _vp, _vp, _i, _vp], _i)
@@ -326,5 +331,6 @@ EXPORTED_SYMBOLS = (
     "raft_ss_plan_host_bp_staging", "raft_ss_bp_variant_shape", "raft_ss_plan_set_bp_variant", "raft_ss_plan_tune",
     "raft_ss_batch_create", "raft_ss_batch_destroy", "raft_ss_batch_layout", "raft_ss_batch_info",
     "raft_ss_batch_set_bp_variant", "raft_ss_batch_tune", "raft_ss_batch_radon", "raft_ss_batch_backprojection",
-    "raft_ss_batch_em",
+    "raft_ss_batch_em", "raft_radon_slantstack_withmesh", "raft_radon_slantstack_view",
+    "raft_ss_plan_create_mesh", "raft_ss_plan_host_bounds_mesh",
 )

@@ -2,6 +2,7 @@
 
 Runs oracle/_ref/libraft_ref.so, which oracle/Makefile compiles from the reference's
 sources (raft/raft_radon_slantstack.c, raft/raft_backprojection_slantstack.c,
+raft/raft_radon_slantstack_withmesh.c, raft/raft_radon_slantstack_view.c,
 raft/raft_image_phantom_fromdesc.c, raft/raft_radon_fromdesc.c and their helpers),
 on small seeded inputs.  The fixture holds inputs and outputs only.
 
@@ -45,6 +46,25 @@ SHEPP = np.array([[1.0, 0.69, 0.92, 0.0, 0.0, 0.0],
                   [0.1, 0.0230, 0.046, 0.06, -0.605, 0.0]])
 
 
+# (name, n, image corners, #rays, #angles): arbitrary (t, theta) tables for withmesh
+MESH_CASES = [
+    ("mesh_n33_sorted", 33, DEF_IMG, 29, 23),
+    ("mesh_n40_unsorted", 40, DEF_IMG, 37, 19),
+    ("mesh_n24_fov", 24, (-0.7, 1.3, 1.2, -0.9), 30, 16),
+]
+# angles every mesh case also holds: the branch switch at pi/4, exact axes, wrap-around
+SPECIAL_THETA = [0.0, math.pi / 4, math.pi / 2, 3 * math.pi / 4, math.pi, -math.pi / 2, 2 * math.pi]
+
+
+def mesh_tables(rng, R, A, sort):
+    t = rng.uniform(-1.6, 1.6, R)           # some rays miss the image
+    t[:3] = (-1.0, 0.0, 1.0)
+    if sort:
+        t.sort()
+    theta = np.concatenate([rng.uniform(-4.0, 7.0, A - len(SPECIAL_THETA)), SPECIAL_THETA])
+    return t, theta
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libraft_ref.so is missing: run make -C oracle ref")
@@ -65,6 +85,16 @@ def main():
     out["shepp/phantom33_fov"] = O.ref_phantom_fromdesc(SHEPP, 33, tl=(-0.9, 1.1), br=(1.2, -1.0))
     out["shepp/radon_fromdesc_96x60"] = O.ref_radon_fromdesc(SHEPP, 96, 60)
     out["shepp/radon_fromdesc_40x7_fov"] = O.ref_radon_fromdesc(SHEPP, 40, 7, tl=(0.2, 1.2), br=(2.9, -0.7))
+    # arbitrary meshes (raft_radon_slantstack_withmesh) and single views
+    for name, n, ic, R, A in MESH_CASES:
+        img = rng.random((n, n))
+        t, theta = mesh_tables(rng, R, A, sort=name.endswith("sorted") and "unsorted" not in name)
+        out[name + "/img"] = img
+        out[name + "/corners"] = np.array(ic, dtype=np.float64)
+        out[name + "/t"] = t
+        out[name + "/theta"] = theta
+        out[name + "/radon"] = O.ref_radon_withmesh(img, t, theta, img_tl=ic[:2], img_br=ic[2:], nthreads=1)
+        out[name + "/view"] = O.ref_radon_view(img, t, float(theta[0]), img_tl=ic[:2], img_br=ic[2:])
     path = os.path.join(HERE, "slantstack_golden.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
