@@ -490,3 +490,309 @@ void oracle_bp_sample_map(raft_image sino, raft_image img, unsigned char *mask) 
       }
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* line-traversal (Bresenham / DDA) projector pair                      */
+/*   oracle_radon_bresenham            raft/raft_radon_bresenham.cpp:17-105  */
+/*   oracle_radon_transpose_bresenham  raft/raft_radon_transpose_bresenham.cpp */
+/*   oracle_backprojection_bresenham   raft/raft_backprojection_bresenham.cpp:17-251 */
+/* with the traversal of raft/raft_ddaiterator.c:16-257.                 */
+/*                                                                      */
+/* Geometry (raft_image corners, signed pitches):                        */
+/*   line i <-> y = tl_y + i*hy, hy = (br_y - tl_y)/(lines - 1)          */
+/*   column j <-> x = tl_x + j*hx, hx = (br_x - tl_x)/(columns - 1)      */
+/*   angle a <-> theta = stl_x + a*(sbr_x - stl_x)/(A - 1)               */
+/*   ray r   <-> t = stl_y + r*(sbr_y - stl_y)/(R - 1)                   */
+/* The ray {x cos + y sin = t} walks the pixel grid; each visited pixel  */
+/* gets its intersection length.                                         */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int lines, cols;
+  double hx, hy;                    /* signed pixel pitches */
+  double ox, oy;                    /* tl corner (pixel centre of line 0 / column 0) */
+  double bx0, bx1, by0, by1;        /* pixel-edge bounding box: tl_x - hx/2, br_x + hx/2, tl_y - hy/2, br_y + hy/2 */
+  /* per angle */
+  double c, s;
+  double xe, ye;                    /* edge lines a ray may enter through */
+  int di, dj;                       /* index steps along the ray */
+  int i0, j0;                       /* entry line / column on those edges */
+  int fast_i;                       /* 1: the line index advances every step */
+  double e_step, e_cap, len_full;   /* slow-axis drift per step, pixel width across, full length */
+  int dfast, dslow;
+  /* per ray */
+  int i, j;
+  double len, e;
+  int slow_next;
+} trace_t;
+
+static double sgn(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
+
+/* double -> int as the reference's assignment compiles on x86-64 (cvttsd2si):
+   NaN and out-of-range values give INT_MIN, i.e. an index outside the image */
+static int to_index(double v) {
+  if (!(v >= -2147483648.0 && v < 2147483648.0)) return (int)0x80000000u;
+  return (int)v;
+}
+
+/* pixel index of coordinate v on an axis starting at v0 with pitch h
+   (raft/raft_ddaiterator.c:10-14) */
+static int axis_index(double v, double v0, double h, int dir) {
+  if (dir >= 0) return to_index(floor(((v - v0) / h) - 0.5) + 1);
+  return to_index(ceil(((v - v0) / h) - 0.5));
+}
+
+static void trace_grid(trace_t *tr, const raft_image *img) {
+  tr->lines = img->data.lines;
+  tr->cols = img->data.columns;
+  tr->hx = (img->br_x - img->tl_x) / (tr->cols - 1);
+  tr->hy = (img->br_y - img->tl_y) / (tr->lines - 1);
+  tr->bx0 = img->tl_x - (tr->hx / 2.0);
+  tr->bx1 = img->br_x + (tr->hx / 2.0);
+  tr->by0 = img->tl_y - (tr->hy / 2.0);
+  tr->by1 = img->br_y + (tr->hy / 2.0);
+  tr->ox = img->tl_x;
+  tr->oy = img->tl_y;
+}
+
+/* raft/raft_ddaiterator.c:38-106 */
+static void trace_angle(trace_t *tr, double theta) {
+  tr->c = cos(theta);
+  tr->s = sin(theta);
+  int sg = (int)sgn(tr->s);
+  double p = sg * tr->bx1, q = sg * tr->bx0;
+  tr->xe = sg * (p > q ? p : q);
+  tr->dj = (int)(-sgn(tr->hx) * sg);
+  sg = (int)sgn(tr->c);
+  p = sg * tr->by1;
+  q = sg * tr->by0;
+  tr->ye = sg * (p < q ? p : q);
+  tr->di = (int)(sgn(tr->hy) * sg);
+  tr->i0 = tr->di < 0 ? tr->lines - 1 : 0;
+  tr->j0 = tr->dj < 0 ? tr->cols - 1 : 0;
+  tr->fast_i = fabs(tr->hx * tr->s) <= fabs(tr->hy * tr->c);
+  if (tr->fast_i) {
+    tr->e_step = fabs(tr->hy * tr->s / tr->c);
+    tr->e_cap = fabs(tr->hx);
+    tr->dslow = tr->dj;
+    tr->dfast = tr->di;
+    tr->len_full = fabs(tr->hy / tr->c);
+  } else {
+    tr->e_step = fabs(tr->hx * tr->c / tr->s);
+    tr->e_cap = fabs(tr->hy);
+    tr->dslow = tr->di;
+    tr->dfast = tr->dj;
+    tr->len_full = fabs(tr->hx / tr->s);
+  }
+}
+
+/* one move along the ray (raft/raft_ddaiterator.c:196-248) */
+static void trace_step(trace_t *tr) {
+  if (tr->slow_next) {
+    if (tr->fast_i) tr->j += tr->dslow; else tr->i += tr->dslow;
+    tr->len = tr->len_full - tr->len;
+    tr->slow_next = 0;
+    return;
+  }
+  if (tr->fast_i) tr->i += tr->dfast; else tr->j += tr->dfast;
+  const double e_prev = tr->e;
+  tr->e += tr->e_step;
+  tr->slow_next = tr->e > tr->e_cap;
+  if (tr->slow_next) {
+    double a;
+    tr->e -= tr->e_cap;
+    if (tr->e) {
+      a = (tr->e_cap - e_prev) / tr->e;
+      a /= (1.0 + a);
+    } else {
+      a = 1.0;
+    }
+    tr->len = a * tr->len_full;
+  } else {
+    tr->len = tr->len_full;
+  }
+}
+
+static int trace_outside(const trace_t *tr) {
+  return tr->i < 0 || tr->i >= tr->lines || tr->j < 0 || tr->j >= tr->cols;
+}
+
+/* entry pixel and first intersection of the ray at t (raft/raft_ddaiterator.c:108-194).
+   Axis-parallel rays start with the full pixel length: the reference leaves the
+   length of their first pixel as the previous ray left it (uninitialised for a
+   thread's first ray) and carries len_full into every later ray of the angle. */
+static void trace_ray(trace_t *tr, double t) {
+  tr->slow_next = 0;
+  double sx, sy, xy;
+  if (tr->s == 0.0) {
+    tr->j = axis_index(t, tr->ox, tr->hx, tr->dj);
+    tr->i = tr->i0;
+    tr->e = 0.0;
+    tr->e_step = 0.0;
+    tr->len = tr->len_full;
+  } else if (tr->c == 0.0) {
+    tr->i = axis_index(t, tr->oy, tr->hy, tr->di);
+    tr->j = tr->j0;
+    tr->e = 0.0;
+    tr->e_step = 0.0;
+    tr->len = tr->len_full;
+  } else if ((sx = ((t * tr->c - tr->xe) / tr->s)) > (sy = ((tr->ye - t * tr->s) / tr->c))) {
+    /* enters through the vertical edge x = xe */
+    xy = (sx * tr->c) + (t * tr->s);
+    tr->i = axis_index(xy, tr->oy, tr->hy, tr->di);
+    tr->j = tr->j0;
+    tr->e = fabs(xy - (tr->oy + (tr->i - (0.5 * tr->di)) * tr->hy));
+    if (!tr->fast_i) {
+      tr->j -= tr->dj;
+      trace_step(tr);
+    } else {
+      tr->e -= fabs(tr->hy);
+      tr->e /= tr->c;
+      tr->len = fabs(tr->e);
+      tr->e = fabs(tr->len * tr->s);
+    }
+  } else {
+    /* enters through the horizontal edge y = ye */
+    xy = (-sy * tr->s) + (t * tr->c);
+    tr->j = axis_index(xy, tr->ox, tr->hx, tr->dj);
+    tr->i = tr->i0;
+    tr->e = fabs(xy - (tr->ox + (tr->j - (0.5 * tr->dj)) * tr->hx));
+    if (tr->fast_i) {
+      tr->i -= tr->di;
+      trace_step(tr);
+    } else {
+      tr->e -= fabs(tr->hx);
+      tr->e /= tr->s;
+      tr->len = fabs(tr->e);
+      tr->e = fabs(tr->len * tr->c);
+    }
+  }
+}
+
+/* a ray never visits more than lines + columns pixels (every move advances one
+   index monotonically); the cap only matters for non-finite geometry */
+#define TRACE_CAP(tr) ((long)(tr)->lines + (tr)->cols + 4)
+
+typedef struct {
+  raft_image img, sino;
+  double *partial, *scale; /* per-chunk accumulators (transpose / backprojection) */
+  int mode;                /* 0 radon, 1 transpose, 2 backprojection */
+  long nchunks;
+  int base, extra;         /* angle partition of the reference's threads */
+} dda_ctx;
+
+static void dda_angles(const dda_ctx *ctx, long chunk, int *a0, int *a1) {
+  *a0 = (int)(chunk * ctx->base + (chunk < ctx->extra ? chunk : ctx->extra));
+  *a1 = *a0 + ctx->base + (chunk < ctx->extra);
+}
+
+static void dda_work(void *vctx, long b, long e) {
+  dda_ctx *ctx = (dda_ctx *)vctx;
+  const raft_image *sino = &ctx->sino;
+  const int R = sino->data.lines, A = sino->data.columns;
+  const double dth = (sino->br_x - sino->tl_x) / (A - 1);
+  const double dt = (sino->br_y - sino->tl_y) / (R - 1);
+  trace_t tr;
+  memset(&tr, 0, sizeof tr);
+  trace_grid(&tr, &ctx->img);
+  const long npix = (long)tr.lines * tr.cols;
+  for (long ch = b; ch < e; ++ch) {
+    int a0, a1;
+    dda_angles(ctx, ch, &a0, &a1);
+    double *acc = ctx->mode ? ctx->partial + ch * npix : NULL;
+    double *sc = ctx->mode == 2 ? ctx->scale + ch * npix : NULL;
+    for (int a = a0; a < a1; ++a) {
+      trace_angle(&tr, sino->tl_x + (a * dth));
+      for (int r = 0; r < R; ++r) {
+        trace_ray(&tr, sino->tl_y + (r * dt));
+        if (ctx->mode == 0) {
+          double sum = 0.0;
+          for (long it = 0; it < TRACE_CAP(&tr) && !trace_outside(&tr); ++it) {
+            sum += AT(ctx->img.data, tr.i, tr.j) * tr.len;
+            trace_step(&tr);
+          }
+          AT(sino->data, r, a) = sum;
+        } else {
+          const double q = AT(sino->data, r, a);
+          for (long it = 0; it < TRACE_CAP(&tr) && !trace_outside(&tr); ++it) {
+            acc[(long)tr.i * tr.cols + tr.j] += q * tr.len;
+            if (sc) sc[(long)tr.i * tr.cols + tr.j] += tr.len;
+            trace_step(&tr);
+          }
+        }
+      }
+    }
+  }
+}
+
+/* nthreads chunks of consecutive angles, the first A mod nthreads one angle longer
+   (raft/raft_radon_bresenham.cpp:68-101); the partial images are summed into the
+   output in chunk order, as the reference's per-column daxpy does
+   (raft/raft_backprojection_bresenham.cpp:17-56). */
+static void dda_run(raft_image img, raft_image sino, int mode, int nthreads) {
+  if (img.data.lines < 2 || img.data.columns < 2 || sino.data.lines < 2 || sino.data.columns < 2) return;
+  if (nthreads < 1) nthreads = 1;
+  dda_ctx ctx;
+  memset(&ctx, 0, sizeof ctx);
+  ctx.img = img;
+  ctx.sino = sino;
+  ctx.mode = mode;
+  ctx.nchunks = nthreads;
+  ctx.base = sino.data.columns / nthreads;
+  ctx.extra = sino.data.columns % nthreads;
+  const long npix = (long)img.data.lines * img.data.columns;
+  if (mode) ctx.partial = (double *)calloc((size_t)nthreads * npix, sizeof(double));
+  if (mode == 2) ctx.scale = (double *)calloc((size_t)nthreads * npix, sizeof(double));
+  fan_out(dda_work, &ctx, nthreads, nthreads);
+  if (mode) {
+    for (long p = 0; p < npix; ++p) {
+      const long i = p / img.data.columns, j = p % img.data.columns;
+      double v = AT(img.data, i, j), w = 0.0;
+      for (long ch = 0; ch < nthreads; ++ch) {
+        v += ctx.partial[ch * npix + p];
+        if (mode == 2) w += ctx.scale[ch * npix + p];
+      }
+      AT(img.data, i, j) = mode == 2 ? v / w : v;
+    }
+  }
+  free(ctx.partial);
+  free(ctx.scale);
+}
+
+/* sinogram (R x A) of the image; every element overwritten */
+void oracle_radon_bresenham(raft_image image, raft_image sino, int nthreads) {
+  dda_run(image, sino, 0, nthreads);
+}
+
+/* image += R^T sino (adds into the caller's image, as the reference does) */
+void oracle_radon_transpose_bresenham(raft_image sino, raft_image image, int nthreads) {
+  dda_run(image, sino, 1, nthreads);
+}
+
+/* image = (image + R^T sino) / R^T 1, unguarded: pixels no ray reaches get 0/0 */
+void oracle_backprojection_bresenham(raft_image sino, raft_image image, int nthreads) {
+  dda_run(image, sino, 2, nthreads);
+}
+
+/* the pixels ray r of angle a visits, in order: ij[2k], ij[2k+1] = line, column and
+   len[k] = intersection length; returns the count (at most cap written) */
+int oracle_dda_ray_path(raft_image img, raft_image sino, int a, int r, int *ij, double *len, int cap) {
+  const int R = sino.data.lines, A = sino.data.columns;
+  if (img.data.lines < 2 || img.data.columns < 2 || R < 2 || A < 2) return 0;
+  trace_t tr;
+  memset(&tr, 0, sizeof tr);
+  trace_grid(&tr, &img);
+  trace_angle(&tr, sino.tl_x + (a * ((sino.br_x - sino.tl_x) / (A - 1))));
+  trace_ray(&tr, sino.tl_y + (r * ((sino.br_y - sino.tl_y) / (R - 1))));
+  int count = 0;
+  for (long it = 0; it < TRACE_CAP(&tr) && !trace_outside(&tr); ++it) {
+    if (count < cap) {
+      ij[2 * count] = tr.i;
+      ij[2 * count + 1] = tr.j;
+      len[count] = tr.len;
+    }
+    ++count;
+    trace_step(&tr);
+  }
+  return count;
+}

@@ -22,6 +22,9 @@ Contents (each restates the cited reference routine):
 * ``loglik`` -- emission log-likelihood sum s*log(p) - sum p (xfct/likelihood.c:83-94).
 * ``phantom_fromdesc`` / ``radon_fromdesc`` -- raft/raft_image_phantom_fromdesc.c:4-59,
   raft/raft_radon_fromdesc.c:4-67.
+* ``radon_bresenham`` / ``radon_transpose_bresenham`` / ``backprojection_bresenham`` --
+  the line-traversal pair, raft/raft_radon_bresenham.cpp, raft/raft_radon_transpose_bresenham.cpp,
+  raft/raft_backprojection_bresenham.cpp over raft/raft_ddaiterator.c.
 * ``radon_withmesh`` / ``radon_view`` -- the arbitrary-mesh forms,
   raft/raft_radon_slantstack_withmesh.c:49-224 and raft/raft_radon_slantstack_view.c:22-113.
 * ``ref_*`` -- the same calls into ``_ref/libraft_ref.so``, the reference's own C
@@ -94,6 +97,15 @@ def lib():
         _lib.oracle_radon_withmesh.restype = None
         _lib.oracle_radon_view.argtypes = [RaftVector, RaftImage, RaftVector, ctypes.c_double]
         _lib.oracle_radon_view.restype = None
+        for name in ("oracle_radon_bresenham",):
+            getattr(_lib, name).argtypes = [RaftImage, RaftImage, ctypes.c_int]
+            getattr(_lib, name).restype = None
+        for name in ("oracle_radon_transpose_bresenham", "oracle_backprojection_bresenham"):
+            getattr(_lib, name).argtypes = [RaftImage, RaftImage, ctypes.c_int]
+            getattr(_lib, name).restype = None
+        _lib.oracle_dda_ray_path.argtypes = [RaftImage, RaftImage, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_int]
+        _lib.oracle_dda_ray_path.restype = ctypes.c_int
         _lib.oracle_radon_sample_map_mesh.argtypes = [RaftImage, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_radon_sample_map_mesh.restype = None
@@ -130,6 +142,12 @@ def ref():
         _ref.raft_radon_slantstack_withmesh.restype = None
         _ref.raft_radon_slantstack_view.argtypes = [RaftVector, RaftImage, RaftVector, ctypes.c_double]
         _ref.raft_radon_slantstack_view.restype = None
+        _ref.harness_radon_bresenham.argtypes = [RaftImage, RaftImage]
+        _ref.harness_radon_bresenham.restype = None
+        _ref.harness_radon_transpose_bresenham.argtypes = [RaftImage, RaftImage]
+        _ref.harness_radon_transpose_bresenham.restype = None
+        _ref.harness_backprojection_bresenham.argtypes = [RaftImage, RaftImage, ctypes.c_void_p]
+        _ref.harness_backprojection_bresenham.restype = None
     return _ref
 
 
@@ -267,6 +285,85 @@ def ref_radon_view(img, t, theta, img_tl=IMG_TL, img_br=IMG_BR):
     view = np.zeros(t.shape[0])
     ref().raft_radon_slantstack_view(_vector(view), _image(img, img_tl, img_br), _vector(t), float(theta))
     return view
+
+
+# Line-traversal (Bresenham / DDA) pair.  Sinogram corners as pyraft.radon sets them by
+# default (theta_extent (0, pi), t from +1 at row 0 to -1, pyraft/sinogram/sino.py:179-193);
+# the angle pitch divides by A - 1 here (raft/raft_image.h:56-61).
+DDA_SINO_TL, DDA_SINO_BR = (0.0, 1.0), (np.pi, -1.0)
+
+
+def radon_bresenham(img, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=DDA_SINO_TL,
+                    sino_br=DDA_SINO_BR, nthreads=None):
+    """raft/raft_radon_bresenham.cpp:60-105 (traversal raft/raft_ddaiterator.c)."""
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    sino = np.zeros((nrays, nangles))
+    lib().oracle_radon_bresenham(_image(img, img_tl, img_br), _image(sino, sino_tl, sino_br), _threads(nthreads))
+    return sino
+
+
+def radon_transpose_bresenham(sino, shape, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=DDA_SINO_TL,
+                              sino_br=DDA_SINO_BR, nthreads=None, out=None):
+    """raft/raft_radon_transpose_bresenham.cpp: out += R^T sino (out defaults to zeros)."""
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    img = np.zeros(shape) if out is None else out
+    lib().oracle_radon_transpose_bresenham(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br),
+                                           _threads(nthreads))
+    return img
+
+
+def backprojection_bresenham(sino, shape, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=DDA_SINO_TL,
+                             sino_br=DDA_SINO_BR, nthreads=None):
+    """raft/raft_backprojection_bresenham.cpp:120-251: R^T sino / R^T 1 (0/0 where no ray passes)."""
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    img = np.zeros(shape)
+    with np.errstate(all="ignore"):
+        lib().oracle_backprojection_bresenham(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br),
+                                              _threads(nthreads))
+    return img
+
+
+def dda_ray_path(shape, nrays, nangles, a, r, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=DDA_SINO_TL,
+                 sino_br=DDA_SINO_BR):
+    """(pixels (k, 2) as (line, column), lengths (k,)) of ray r of angle a, in visit order."""
+    cap = 2 * (shape[0] + shape[1]) + 8
+    ij = np.zeros((cap, 2), np.int32)
+    ln = np.zeros(cap)
+    dummy_img = np.zeros((2, 2))
+    dummy_sino = np.zeros((2, 2))
+    img = _image(dummy_img, img_tl, img_br)
+    img.data.lines, img.data.columns = shape[0], shape[1]
+    sino = _image(dummy_sino, sino_tl, sino_br)
+    sino.data.lines, sino.data.columns = nrays, nangles
+    k = lib().oracle_dda_ray_path(img, sino, a, r, ij.ctypes.data, ln.ctypes.data, cap)
+    return ij[:k].copy(), ln[:k].copy()
+
+
+def ref_radon_bresenham(img, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=DDA_SINO_TL,
+                        sino_br=DDA_SINO_BR):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    sino = np.zeros((nrays, nangles))
+    ref().harness_radon_bresenham(_image(img, img_tl, img_br), _image(sino, sino_tl, sino_br))
+    return sino
+
+
+def ref_radon_transpose_bresenham(sino, shape, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=DDA_SINO_TL,
+                                  sino_br=DDA_SINO_BR):
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    img = np.zeros(shape)
+    ref().harness_radon_transpose_bresenham(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br))
+    return img
+
+
+def ref_backprojection_bresenham(sino, shape, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=DDA_SINO_TL,
+                                 sino_br=DDA_SINO_BR):
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    img = np.zeros(shape)
+    scale = np.zeros(shape)
+    with np.errstate(all="ignore"):
+        ref().harness_backprojection_bresenham(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br),
+                                               scale.ctypes.data)
+    return img
 
 
 def ref_radon(img, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,

@@ -1,11 +1,13 @@
 """Backprojection (pyraft/backprojection/backprojection.py).
 
 ``backprojection_plan`` / ``backprojection`` keep pyraft's plan-tuple protocol
-(pyraft/backprojection/backprojection.py:35-213); the SLANTSTACK plan is
-``(method, shape, padding)`` and runs the drop-in ``raft_backprojection_slantstack``
-symbol on the GPU.  ``radon_transpose(..., method=SLANTSTACK)`` is the exact
-matched transpose of the slant-stack Radon (the reference offers only the
-Bresenham transpose, pyraft/backprojection/backprojection.py:220-233).
+(pyraft/backprojection/backprojection.py:35-213); SLANTSTACK and BRESENHAM plans are
+``(method, shape, padding)`` and run the drop-in ``raft_backprojection_slantstack`` /
+``raft_backprojection_bresenham`` symbols on the GPU.  (pyraft's BRESENHAM plan is an
+empty list, which its ``backprojection`` cannot index; here it is a working plan.)
+``radon_transpose`` defaults to the Bresenham transpose as in pyraft
+(pyraft/backprojection/backprojection.py:220-233); ``method=SLANTSTACK`` gives the
+exact matched transpose of the slant-stack Radon, which pyraft does not offer.
 """
 import ctypes
 
@@ -31,41 +33,48 @@ def _float64(arr):
 
 def backprojection_plan(sino_shape=(256, 180), shape=(256, 256), padding=2, method=radon_method.BST):
     """Plan tuple for ``backprojection`` (pyraft/backprojection/backprojection.py:35-166)."""
-    if method == radon_method.SLANTSTACK:
+    if method in (radon_method.SLANTSTACK, radon_method.BRESENHAM):
         return (method, shape, padding, )
-    if method in (radon_method.BRESENHAM, radon_method.BST, radon_method.ANDERSSON, radon_method.ANDERSSONPB):
-        raise NotImplementedError("backprojection_plan: only radon_method.SLANTSTACK is part of this build")
+    if method in (radon_method.BST, radon_method.ANDERSSON, radon_method.ANDERSSONPB):
+        raise NotImplementedError("backprojection_plan: only the SLANTSTACK and BRESENHAM methods are part "
+                                  "of this build")
     raise TypeError('Unsupported method for Backprojection!')
 
 
 def backprojection(sino, plan, **kwargs):
-    """Slant-stack backprojection into an ``image`` of ``plan[1]`` (corners from kwargs)."""
+    """Backprojection into an ``image`` of ``plan[1]`` (corners from kwargs): slant-stack,
+    or the line-traversal one normalised by R^T 1 (0/0 where no ray passes a pixel)."""
     method, shape = plan[0], plan[1]
-    if method != radon_method.SLANTSTACK:
+    if method == radon_method.SLANTSTACK:
+        fn, name = libraft.raft_backprojection_slantstack, "raft_backprojection_slantstack"
+    elif method == radon_method.BRESENHAM:
+        fn, name = libraft.raft_backprojection_bresenham, "raft_backprojection_bresenham"
+    else:
         raise TypeError('Unsupported method for Backprojection!')
     img = image(shape, **kwargs)
     IMG = make_RAFT_IMAGE(img, img.top_left, img.bottom_right)
     tl, br = _sino_corners(sino)
     s64 = _float64(sino)
     SINO = make_RAFT_IMAGE(s64, tl, br)
-    libraft.raft_backprojection_slantstack(SINO, IMG, ctypes.c_int(nthreads))
-    check_legacy("raft_backprojection_slantstack")
+    fn(SINO, IMG, ctypes.c_int(nthreads))
+    check_legacy(name)
     return img
 
 
-def radon_transpose(sino, shape=(256, 256), method=radon_method.SLANTSTACK, **kwargs):
-    """Exact transpose of the slant-stack ``radon`` (adjoint to rounding)."""
+def radon_transpose(sino, shape=(256, 256), method=radon_method.BRESENHAM, **kwargs):
+    """Transpose of ``radon`` for the method (adjoint to rounding)."""
     if method == radon_method.BRESENHAM:
-        raise NotImplementedError("radon_transpose: the Bresenham transpose is not part of this build; "
-                                  "pass method=radon_method.SLANTSTACK")
-    if method != radon_method.SLANTSTACK:
+        fn, name = libraft.raft_radon_transpose_bresenham, "raft_radon_transpose_bresenham"
+    elif method == radon_method.SLANTSTACK:
+        fn, name = libraft.raft_radon_transpose_slantstack, "raft_radon_transpose_slantstack"
+    else:
         raise TypeError('Unsupported method for transpose Radon!')
     img = image(shape, **kwargs)
     IMG = make_RAFT_IMAGE(img, img.top_left, img.bottom_right)
     tl, br = _sino_corners(sino)
     s64 = _float64(sino)
-    libraft.raft_radon_transpose_slantstack(make_RAFT_IMAGE(s64, tl, br), IMG, ctypes.c_int(nthreads))
-    check_legacy("raft_radon_transpose_slantstack")
+    fn(make_RAFT_IMAGE(s64, tl, br), IMG, ctypes.c_int(nthreads))
+    check_legacy(name)
     return img
 
 

@@ -1,0 +1,428 @@
+// dda_api.cpp -- plans, device-pointer operators and drop-in symbols of the
+// line-traversal (Bresenham / DDA) projector pair.
+//
+// The per-angle table repeats raft_ddaiterator_set_image / set_theta
+// (raft/raft_ddaiterator.c:16-106) in FP64 with glibc cos/sin and no fused
+// multiply-adds (this file is compiled with -ffp-contract=off), so the device traces
+// from the same numbers the reference computes.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <list>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/raft_bresenham.h"
+#include "dda_kernels.h"
+#include "ss_internal.h"
+
+struct raft_dda_plan {
+  int device = 0;
+  SSDdaGeom g{};
+  double img_corners[4] = {0, 0, 0, 0};
+  double sino_corners[4] = {0, 0, 0, 0};
+  std::vector<SSDdaAngle> h_angles;
+  SSDdaAngle* d_angles = nullptr;
+  double* d_weights = nullptr;  // R^T 1 in FP64, built on first use
+  std::mutex weights_mutex;
+  ~raft_dda_plan() {
+    if (d_angles) (void)hipFree(d_angles);
+    if (d_weights) (void)hipFree(d_weights);
+  }
+};
+
+namespace {
+
+double sgn(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
+
+SSDdaGeom make_geom(int lines, int cols, int nrays, int nangles, const double* ic, const double* sc) {
+  SSDdaGeom g;
+  g.lines = lines;
+  g.cols = cols;
+  g.nrays = nrays;
+  g.nangles = nangles;
+  g.hx = (ic[2] - ic[0]) / (cols - 1);
+  g.hy = (ic[3] - ic[1]) / (lines - 1);
+  g.ox = ic[0];
+  g.oy = ic[1];
+  g.bx0 = ic[0] - (g.hx / 2.0);
+  g.bx1 = ic[2] + (g.hx / 2.0);
+  g.by0 = ic[1] - (g.hy / 2.0);
+  g.by1 = ic[3] + (g.hy / 2.0);
+  g.t0 = sc[1];
+  g.dt = (sc[3] - sc[1]) / (nrays - 1);
+  return g;
+}
+
+// raft/raft_ddaiterator.c:38-106 for theta
+SSDdaAngle make_angle(const SSDdaGeom& g, double theta) {
+  SSDdaAngle a;
+  a.c = cos(theta);
+  a.s = sin(theta);
+  int sg = (int)sgn(a.s);
+  double p = sg * g.bx1, q = sg * g.bx0;
+  a.xe = sg * (p > q ? p : q);
+  a.dj = (int)(-sgn(g.hx) * sg);
+  sg = (int)sgn(a.c);
+  p = sg * g.by1;
+  q = sg * g.by0;
+  a.ye = sg * (p < q ? p : q);
+  a.di = (int)(sgn(g.hy) * sg);
+  a.i0 = a.di < 0 ? g.lines - 1 : 0;
+  a.j0 = a.dj < 0 ? g.cols - 1 : 0;
+  // the reference's fast-axis test (fast = line when |hx sin| <= |hy cos|): for
+  // square pixels the axis crossed more often; kept as is for other pixel shapes
+  a.fast_i = fabs(g.hx * a.s) <= fabs(g.hy * a.c);
+  if (a.fast_i) {
+    a.e_step = fabs(g.hy * a.s / a.c);
+    a.e_cap = fabs(g.hx);
+    a.dslow = a.dj;
+    a.dfast = a.di;
+    a.len_full = fabs(g.hy / a.c);
+  } else {
+    a.e_step = fabs(g.hx * a.c / a.s);
+    a.e_cap = fabs(g.hy);
+    a.dslow = a.di;
+    a.dfast = a.dj;
+    a.len_full = fabs(g.hx / a.s);
+  }
+  a.axis = a.s == 0.0 ? 1 : (a.c == 0.0 ? 2 : 0);
+  a.k_len = a.e_step > 0.0 ? a.len_full / a.e_step : 0.0;
+  return a;
+}
+
+int check_dda(const raft_dda_plan* p, int nslices) {
+  if (!p) return ss_set_error(RAFT_SS_EINVAL, "null plan");
+  if (nslices < 1 || nslices > 65535)
+    return ss_set_error(RAFT_SS_EINVAL, "nslices must be in [1, 65535] (got %d)", nslices);
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
+    return ss_set_error(RAFT_SS_EINVAL, "plan was built on device %d but device %d is current", p->device, dev);
+  return 0;
+}
+
+// R^T 1 for the plan in FP64, built once (atomics: the first call fixes its rounding)
+int ensure_weights(raft_dda_plan* p, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(p->weights_mutex);
+  if (p->d_weights) return 0;
+  const size_t npix = (size_t)p->g.lines * p->g.cols;
+  double* w = nullptr;
+  HIP_TRY(hipMalloc(&w, sizeof(double) * npix));
+  DevBuf acc, accT;
+  hipError_t e = acc.alloc(sizeof(double) * npix, stream);
+  if (e == hipSuccess) e = accT.alloc(sizeof(double) * npix, stream);
+  if (e == hipSuccess)
+    e = dda_launch_scatter<double>(p->g, p->d_angles, nullptr, true, acc.as<double>(), accT.as<double>(), 1, stream);
+  if (e == hipSuccess)
+    e = dda_launch_fold<double, double>(p->g, acc.as<double>(), accT.as<double>(), nullptr, w, 1, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    (void)hipFree(w);
+    return ss_set_error(RAFT_SS_EHIP, "building the backprojection weights failed: %s", hipGetErrorString(e));
+  }
+  p->d_weights = w;
+  return 0;
+}
+
+// num[s] = R^T sino[s] accumulated in FP64 (the backprojection's numerator)
+int numerator_f64(const raft_dda_plan* p, const float* sino, double* num, int nslices, hipStream_t stream) {
+  const size_t n = (size_t)p->g.lines * p->g.cols * nslices;
+  DevBuf acc, accT;
+  HIP_TRY(acc.alloc(sizeof(double) * n, stream));
+  HIP_TRY(accT.alloc(sizeof(double) * n, stream));
+  HIP_TRY(dda_launch_scatter<double>(p->g, p->d_angles, sino, false, acc.as<double>(), accT.as<double>(), nslices,
+                                     stream));
+  HIP_TRY((dda_launch_fold<double, double>(p->g, acc.as<double>(), accT.as<double>(), nullptr, num, nslices, stream)));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, int nrays, int nangles,
+                                    const double* img_corners, const double* sino_corners, void* stream_) {
+  if (!out || !img_corners || !sino_corners) return ss_set_error(RAFT_SS_EINVAL, "null argument");
+  *out = nullptr;
+  if (lines < 2 || cols < 2 || nrays < 2 || nangles < 2)
+    return ss_set_error(RAFT_SS_EINVAL, "line traversal needs lines, cols, nrays, nangles >= 2 (got %d, %d, %d, %d)",
+                        lines, cols, nrays, nangles);
+  if (lines > 65536 || cols > 65536 || nrays > (1 << 24) || nangles > 65535)
+    return ss_set_error(RAFT_SS_EINVAL, "geometry too large (%d x %d image, %d x %d sinogram)", lines, cols, nrays,
+                        nangles);
+  for (int i = 0; i < 4; ++i)
+    if (!isfinite(img_corners[i]) || !isfinite(sino_corners[i]))
+      return ss_set_error(RAFT_SS_EINVAL, "non-finite corner");
+  hipStream_t stream = as_stream(stream_);
+  std::unique_ptr<raft_dda_plan> p(new raft_dda_plan());
+  HIP_TRY(hipGetDevice(&p->device));
+  memcpy(p->img_corners, img_corners, sizeof(p->img_corners));
+  memcpy(p->sino_corners, sino_corners, sizeof(p->sino_corners));
+  p->g = make_geom(lines, cols, nrays, nangles, img_corners, sino_corners);
+  const SSDdaGeom& g = p->g;
+  if (!(g.hx != 0.0) || !(g.hy != 0.0) || !isfinite(g.hx) || !isfinite(g.hy) || !isfinite(g.dt))
+    return ss_set_error(RAFT_SS_EINVAL, "degenerate image or detector extent");
+  const double dth = (sino_corners[2] - sino_corners[0]) / (nangles - 1);
+  p->h_angles.resize(nangles);
+  for (int a = 0; a < nangles; ++a) {
+    p->h_angles[a] = make_angle(g, sino_corners[0] + (a * dth));
+    if (!isfinite(p->h_angles[a].c) || !isfinite(p->h_angles[a].s))
+      return ss_set_error(RAFT_SS_EINVAL, "non-finite angle %d", a);
+  }
+  HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSDdaAngle) * nangles));
+  HIP_TRY(hipMemcpyAsync(p->d_angles, p->h_angles.data(), sizeof(SSDdaAngle) * nangles, hipMemcpyHostToDevice,
+                         stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  *out = p.release();
+  return 0;
+}
+
+extern "C" int raft_dda_plan_destroy(raft_dda_plan* p) {
+  delete p;
+  return 0;
+}
+
+extern "C" int raft_dda_plan_info(const raft_dda_plan* p, int* info, int ninfo) {
+  if (!p || !info) return ss_set_error(RAFT_SS_EINVAL, "null argument");
+  int nfast = 0;
+  for (const SSDdaAngle& a : p->h_angles) nfast += a.fast_i;
+  const int vals[] = {p->g.lines, p->g.cols, p->g.nrays, p->g.nangles, nfast};
+  for (int i = 0; i < ninfo && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
+  return 0;
+}
+
+extern "C" int raft_dda_radon(const raft_dda_plan* p, const float* img, float* sino, int nslices, void* stream_) {
+  int rc = check_dda(p, nslices);
+  if (rc) return rc;
+  if (!img || !sino) return ss_set_error(RAFT_SS_EINVAL, "null buffer");
+  hipStream_t stream = as_stream(stream_);
+  DevBuf imgT;
+  HIP_TRY(imgT.alloc(sizeof(float) * (size_t)p->g.lines * p->g.cols * nslices, stream));
+  HIP_TRY(dda_launch_radon(p->g, p->d_angles, img, imgT.as<float>(), sino, nslices, stream));
+  return 0;
+}
+
+extern "C" int raft_dda_radon_transpose(const raft_dda_plan* p, const float* sino, float* img, int nslices,
+                                        void* stream_) {
+  int rc = check_dda(p, nslices);
+  if (rc) return rc;
+  if (!img || !sino) return ss_set_error(RAFT_SS_EINVAL, "null buffer");
+  hipStream_t stream = as_stream(stream_);
+  const size_t n = (size_t)p->g.lines * p->g.cols * nslices;
+  DevBuf acc, accT;
+  HIP_TRY(acc.alloc(sizeof(float) * n, stream));
+  HIP_TRY(accT.alloc(sizeof(float) * n, stream));
+  HIP_TRY(dda_launch_scatter<float>(p->g, p->d_angles, sino, false, acc.as<float>(), accT.as<float>(), nslices,
+                                    stream));
+  HIP_TRY((dda_launch_fold<float, float>(p->g, acc.as<float>(), accT.as<float>(), nullptr, img, nslices, stream)));
+  return 0;
+}
+
+extern "C" int raft_dda_backprojection(raft_dda_plan* p, const float* sino, float* img, int nslices, void* stream_) {
+  int rc = check_dda(p, nslices);
+  if (rc) return rc;
+  if (!img || !sino) return ss_set_error(RAFT_SS_EINVAL, "null buffer");
+  hipStream_t stream = as_stream(stream_);
+  if ((rc = ensure_weights(p, stream))) return rc;
+  const size_t n = (size_t)p->g.lines * p->g.cols * nslices;
+  DevBuf acc, accT;
+  HIP_TRY(acc.alloc(sizeof(double) * n, stream));
+  HIP_TRY(accT.alloc(sizeof(double) * n, stream));
+  HIP_TRY(dda_launch_scatter<double>(p->g, p->d_angles, sino, false, acc.as<double>(), accT.as<double>(), nslices,
+                                     stream));
+  HIP_TRY((dda_launch_fold<double, float>(p->g, acc.as<double>(), accT.as<double>(), p->d_weights, img, nslices,
+                                         stream)));
+  return 0;
+}
+
+extern "C" int raft_dda_weights(raft_dda_plan* p, float* out, void* stream_) {
+  int rc = check_dda(p, 1);
+  if (rc) return rc;
+  if (!out) return ss_set_error(RAFT_SS_EINVAL, "null buffer");
+  hipStream_t stream = as_stream(stream_);
+  if ((rc = ensure_weights(p, stream))) return rc;
+  HIP_TRY(dda_launch_to_f32(p->d_weights, out, (size_t)p->g.lines * p->g.cols, stream));
+  return 0;
+}
+
+// The traversal of one ray exactly as k_dda_radon runs it, evaluated on the host (for
+// checking index logic without a device): (line, column) pairs and lengths of the
+// visited pixels in order.  Returns the pixel count (at most cap are written).
+extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, const double* img_corners,
+                                 const double* sino_corners, int angle, int ray, int* ij, double* len, int cap) {
+  if (!img_corners || !sino_corners || (cap > 0 && (!ij || !len))) return ss_set_error(RAFT_SS_EINVAL, "null argument");
+  if (lines < 2 || cols < 2 || nrays < 2 || nangles < 2 || angle < 0 || angle >= nangles || ray < 0 || ray >= nrays)
+    return ss_set_error(RAFT_SS_EINVAL, "bad geometry or ray index");
+  const SSDdaGeom g = make_geom(lines, cols, nrays, nangles, img_corners, sino_corners);
+  const double dth = (sino_corners[2] - sino_corners[0]) / (nangles - 1);
+  const SSDdaAngle a = make_angle(g, sino_corners[0] + (angle * dth));
+  const int nf = a.fast_i ? lines : cols, ns = a.fast_i ? cols : lines;
+  const double t = g.t0 + (ray * g.dt);
+  SSDdaRay r = dda_ray_start(g, a, t);
+  int count = 0;
+  auto visit = [&](void) {
+    if (count < cap) {
+      ij[2 * count] = a.fast_i ? r.f : r.s;
+      ij[2 * count + 1] = a.fast_i ? r.s : r.f;
+      len[count] = r.len;
+    }
+    ++count;
+  };
+  auto inside = [&](void) { return (unsigned)r.f < (unsigned)nf && (unsigned)r.s < (unsigned)ns; };
+  for (int it = 0; it <= nf; ++it) {
+    if (!inside()) break;
+    visit();
+    if (r.slow_next) {
+      r.s = dda_step(r.s, a.dslow);
+      r.len = a.len_full - r.len;
+      r.slow_next = 0;
+      if (!inside()) break;
+      visit();
+    }
+    dda_fast_move(a, r);
+  }
+  return count;
+}
+
+// ------------------------------------------------------------------ drop-in symbols
+
+namespace {
+
+struct DdaKey {
+  int device, lines, cols, R, A;
+  double c[8];
+  bool operator==(const DdaKey& o) const {
+    return device == o.device && lines == o.lines && cols == o.cols && R == o.R && A == o.A &&
+           memcmp(c, o.c, sizeof(c)) == 0;
+  }
+};
+std::mutex g_dda_cache_mutex;
+std::list<std::pair<DdaKey, std::shared_ptr<raft_dda_plan>>> g_dda_cache;
+
+// plans (with their weights) of recent geometries, shared by the drop-in calls
+int cached_dda_plan(const raft_image& img, const raft_image& sino, std::shared_ptr<raft_dda_plan>* out) {
+  DdaKey key;
+  memset(&key, 0, sizeof(key));
+  HIP_TRY(hipGetDevice(&key.device));
+  key.lines = img.data.lines;
+  key.cols = img.data.columns;
+  key.R = sino.data.lines;
+  key.A = sino.data.columns;
+  corners_of(img, key.c);
+  corners_of(sino, key.c + 4);
+  {
+    std::lock_guard<std::mutex> lock(g_dda_cache_mutex);
+    for (auto it = g_dda_cache.begin(); it != g_dda_cache.end(); ++it)
+      if (it->first == key) {
+        g_dda_cache.splice(g_dda_cache.begin(), g_dda_cache, it);
+        *out = g_dda_cache.front().second;
+        return 0;
+      }
+  }
+  raft_dda_plan* raw = nullptr;
+  int rc = raft_dda_plan_create(&raw, key.lines, key.cols, key.R, key.A, key.c, key.c + 4, nullptr);
+  if (rc) return rc;
+  std::shared_ptr<raft_dda_plan> sp(raw);
+  std::lock_guard<std::mutex> lock(g_dda_cache_mutex);
+  g_dda_cache.emplace_front(key, sp);
+  while (g_dda_cache.size() > 8) g_dda_cache.pop_back();
+  *out = sp;
+  return 0;
+}
+
+int check_images(const raft_image& img, const raft_image& sino) {
+  if (!img.data.p_data || !sino.data.p_data) return ss_set_error(RAFT_SS_EINVAL, "empty image");
+  return 0;
+}
+
+int host_dda_radon(const raft_image& img, const raft_image& sino) {
+  int rc = check_images(img, sino);
+  if (rc) return rc;
+  std::shared_ptr<raft_dda_plan> plan;
+  if ((rc = cached_dda_plan(img, sino, &plan))) return rc;
+  const size_t npix = (size_t)img.data.lines * img.data.columns, nsino = (size_t)sino.data.lines * sino.data.columns;
+  PinnedF32 hi, hs;
+  HIP_TRY(hi.alloc(npix));
+  HIP_TRY(hs.alloc(nsino));
+  gather_f64(img.data, hi.p);
+  DevBuf di, ds;
+  HIP_TRY(di.alloc(sizeof(float) * npix, nullptr));
+  HIP_TRY(ds.alloc(sizeof(float) * nsino, nullptr));
+  HIP_TRY(hipMemcpyAsync(di.p, hi.p, sizeof(float) * npix, hipMemcpyHostToDevice, nullptr));
+  if ((rc = raft_dda_radon(plan.get(), di.as<float>(), ds.as<float>(), 1, nullptr))) return rc;
+  HIP_TRY(hipMemcpyAsync(hs.p, ds.p, sizeof(float) * nsino, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  scatter_f64(hs.p, sino.data);
+  return 0;
+}
+
+// image += R^T sino
+int host_dda_transpose(const raft_image& sino, const raft_image& img) {
+  int rc = check_images(img, sino);
+  if (rc) return rc;
+  std::shared_ptr<raft_dda_plan> plan;
+  if ((rc = cached_dda_plan(img, sino, &plan))) return rc;
+  const size_t npix = (size_t)img.data.lines * img.data.columns, nsino = (size_t)sino.data.lines * sino.data.columns;
+  PinnedF32 hs, ho;
+  HIP_TRY(hs.alloc(nsino));
+  HIP_TRY(ho.alloc(npix));
+  gather_f64(sino.data, hs.p);
+  DevBuf ds, dout;
+  HIP_TRY(ds.alloc(sizeof(float) * nsino, nullptr));
+  HIP_TRY(dout.alloc(sizeof(float) * npix, nullptr));
+  HIP_TRY(hipMemcpyAsync(ds.p, hs.p, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
+  if ((rc = raft_dda_radon_transpose(plan.get(), ds.as<float>(), dout.as<float>(), 1, nullptr))) return rc;
+  HIP_TRY(hipMemcpyAsync(ho.p, dout.p, sizeof(float) * npix, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  const raft_matrix& m = img.data;
+  for (int i = 0; i < m.lines; ++i)
+    for (int j = 0; j < m.columns; ++j) RAFT_MATRIX_AT(m, i, j) += (double)ho.p[(size_t)i * m.columns + j];
+  return 0;
+}
+
+// image = (image + R^T sino) / R^T 1, both sums in FP64
+int host_dda_backprojection(const raft_image& sino, const raft_image& img) {
+  int rc = check_images(img, sino);
+  if (rc) return rc;
+  std::shared_ptr<raft_dda_plan> plan;
+  if ((rc = cached_dda_plan(img, sino, &plan))) return rc;
+  if ((rc = check_dda(plan.get(), 1))) return rc;
+  if ((rc = ensure_weights(plan.get(), nullptr))) return rc;
+  const size_t npix = (size_t)img.data.lines * img.data.columns, nsino = (size_t)sino.data.lines * sino.data.columns;
+  PinnedF32 hs;
+  HIP_TRY(hs.alloc(nsino));
+  gather_f64(sino.data, hs.p);
+  std::vector<double> num(npix), w(npix);
+  DevBuf ds, dnum;
+  HIP_TRY(ds.alloc(sizeof(float) * nsino, nullptr));
+  HIP_TRY(dnum.alloc(sizeof(double) * npix, nullptr));
+  HIP_TRY(hipMemcpyAsync(ds.p, hs.p, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
+  if ((rc = numerator_f64(plan.get(), ds.as<float>(), dnum.as<double>(), 1, nullptr))) return rc;
+  HIP_TRY(hipMemcpyAsync(num.data(), dnum.p, sizeof(double) * npix, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipMemcpyAsync(w.data(), plan->d_weights, sizeof(double) * npix, hipMemcpyDeviceToHost, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  const raft_matrix& m = img.data;
+  for (int i = 0; i < m.lines; ++i)
+    for (int j = 0; j < m.columns; ++j) {
+      const size_t k = (size_t)i * m.columns + j;
+      RAFT_MATRIX_AT(m, i, j) = (RAFT_MATRIX_AT(m, i, j) + num[k]) / w[k];
+    }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" void raft_radon_bresenham(raft_image image, raft_image sino, int nthreads) {
+  (void)nthreads;
+  ss_report_legacy("raft_radon_bresenham", host_dda_radon(image, sino));
+}
+
+extern "C" void raft_radon_transpose_bresenham(raft_image sino, raft_image image, int nthreads) {
+  (void)nthreads;
+  ss_report_legacy("raft_radon_transpose_bresenham", host_dda_transpose(sino, image));
+}
+
+extern "C" void raft_backprojection_bresenham(raft_image sino, raft_image image, int nthreads) {
+  (void)nthreads;
+  ss_report_legacy("raft_backprojection_bresenham", host_dda_backprojection(sino, image));
+}

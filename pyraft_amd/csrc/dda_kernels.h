@@ -1,0 +1,184 @@
+// dda_kernels.h -- line-traversal (Bresenham / DDA) projector pair: geometry records,
+// the FP64 ray-entry logic shared by host and device, and the kernel launchers.
+//
+// Reference: raft/raft_ddaiterator.c:16-257 (traversal), raft/raft_radon_bresenham.cpp,
+// raft/raft_radon_transpose_bresenham.cpp, raft/raft_backprojection_bresenham.cpp.
+//
+// Geometry (raft_image corners, signed pitches; raft/raft_image.h:56-61):
+//   image line i <-> y = tl_y + i*hy, hy = (br_y - tl_y)/(lines - 1)
+//   image column j <-> x = tl_x + j*hx, hx = (br_x - tl_x)/(cols - 1)
+//   sinogram column a <-> theta = stl_x + a*(sbr_x - stl_x)/(A - 1)
+//   sinogram row r    <-> t     = stl_y + r*(sbr_y - stl_y)/(R - 1)
+// Ray {x cos(theta) + y sin(theta) = t} visits pixels along a "fast" axis (one index
+// step per move) and a "slow" axis (a step whenever the accumulated drift e passes
+// the pixel width across); each visited pixel receives its intersection length.
+//
+// Kernels work in (fast, slow) coordinates: a per-angle choice of the row-major image
+// (fast = line) or its transpose (fast = column), so the 64 parallel rays of a wave
+// read and write along a contiguous line of pixels.
+#ifndef PYRAFT_AMD_DDA_KERNELS_H
+#define PYRAFT_AMD_DDA_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define DDA_HD __host__ __device__
+#else
+#define DDA_HD
+#endif
+
+#if defined(__clang__)
+#define DDA_NOCONTRACT _Pragma("clang fp contract(off)")
+#else
+#define DDA_NOCONTRACT
+#endif
+
+struct SSDdaGeom {
+  int lines, cols, nrays, nangles;
+  double hx, hy;            // signed pixel pitches
+  double ox, oy;            // tl corner: centre of line 0 / column 0
+  double bx0, bx1, by0, by1;  // pixel-edge box: tl_x - hx/2, br_x + hx/2, tl_y - hy/2, br_y + hy/2
+  double t0, dt;            // ray r at t = t0 + r*dt
+};
+
+struct SSDdaAngle {
+  double c, s;              // glibc cos / sin of theta (host)
+  double xe, ye;            // edges a ray may enter through (x = xe, y = ye)
+  double e_step;            // slow-axis drift per fast move
+  double e_cap;             // pixel width across the slow axis
+  double len_full;          // intersection length of a fast move without slow move
+  double k_len;             // len_full / e_step (0 for axis-parallel angles)
+  int di, dj;               // index steps along the ray
+  int i0, j0;               // entry line / column on those edges
+  int fast_i;               // 1: the line index is the fast axis
+  int dfast, dslow;
+  int axis;                 // 1: sin == 0, 2: cos == 0, 0: oblique
+};
+
+// ray state in (fast, slow) coordinates
+struct SSDdaRay {
+  int f, s;                 // fast / slow index
+  double len;               // intersection length of the current pixel
+  double e;                 // drift since the last slow-axis boundary
+  int slow_next;            // the next move is along the slow axis
+};
+
+// double -> int as the reference's assignments compile on x86-64: NaN and values
+// outside the int range give INT_MIN, an index outside the image
+static inline DDA_HD int dda_to_index(double v) {
+  if (!(v >= -2147483648.0 && v < 2147483648.0)) return (int)0x80000000u;
+  return (int)v;
+}
+
+// index step with two's-complement wrap-around: an entry index of INT_MIN (a ray that
+// misses the image) stays far outside the image instead of overflowing
+static inline DDA_HD int dda_step(int v, int d) { return (int)((unsigned)v + (unsigned)d); }
+
+// pixel index of coordinate v on an axis from v0 with pitch h, moving in direction
+// dir (raft/raft_ddaiterator.c:10-14)
+static inline DDA_HD int dda_axis_index(double v, double v0, double h, int dir) {
+  DDA_NOCONTRACT
+  if (dir >= 0) return dda_to_index(floor(((v - v0) / h) - 0.5) + 1);
+  return dda_to_index(ceil(((v - v0) / h) - 0.5));
+}
+
+// one fast move (raft/raft_ddaiterator.c:212-247).  The intersection length of a move
+// that crosses a slow-axis boundary is len_full*(e_cap - e_prev)/e_step, the
+// reference's alpha/(1+alpha) with alpha = (e_cap - e_prev)/e written without its
+// two divisions (same value up to rounding of the length; the move sequence, which
+// depends on e only, is identical).
+static inline DDA_HD void dda_fast_move(const SSDdaAngle& a, SSDdaRay& r) {
+  DDA_NOCONTRACT
+  r.f = dda_step(r.f, a.dfast);
+  const double e_prev = r.e;
+  r.e += a.e_step;
+  r.slow_next = r.e > a.e_cap;
+  if (r.slow_next) {
+    r.e -= a.e_cap;
+    r.len = r.e != 0.0 ? (a.e_cap - e_prev) * a.k_len : a.len_full;
+  } else {
+    r.len = a.len_full;
+  }
+}
+
+// entry pixel and first intersection length of the ray at t
+// (raft/raft_ddaiterator.c:108-194).  Axis-parallel rays start with len_full: the
+// reference leaves that first length as the previous ray left it (uninitialised for a
+// thread's first ray) and carries len_full into every later ray of the angle.
+static inline DDA_HD SSDdaRay dda_ray_start(const SSDdaGeom& g, const SSDdaAngle& a, double t) {
+  DDA_NOCONTRACT
+  SSDdaRay r;
+  int i, j;
+  r.slow_next = 0;
+  r.len = a.len_full;
+  r.e = 0.0;
+  if (a.axis == 1) {
+    j = dda_axis_index(t, g.ox, g.hx, a.dj);
+    i = a.i0;
+  } else if (a.axis == 2) {
+    i = dda_axis_index(t, g.oy, g.hy, a.di);
+    j = a.j0;
+  } else {
+    const double sx = (t * a.c - a.xe) / a.s;
+    const double sy = (a.ye - t * a.s) / a.c;
+    if (sx > sy) {
+      // enters through the vertical edge x = xe
+      const double xy = (sx * a.c) + (t * a.s);
+      i = dda_axis_index(xy, g.oy, g.hy, a.di);
+      j = a.j0;
+      r.e = fabs(xy - (g.oy + (i - (0.5 * a.di)) * g.hy));
+      if (!a.fast_i) {
+        // fast = column: step back one column and take a fast move into the image
+        r.f = dda_step(j, -a.dj);
+        r.s = i;
+        dda_fast_move(a, r);
+        return r;
+      }
+      r.e -= fabs(g.hy);
+      r.e /= a.c;
+      r.len = fabs(r.e);
+      r.e = fabs(r.len * a.s);
+    } else {
+      // enters through the horizontal edge y = ye
+      const double xy = (-sy * a.s) + (t * a.c);
+      j = dda_axis_index(xy, g.ox, g.hx, a.dj);
+      i = a.i0;
+      r.e = fabs(xy - (g.ox + (j - (0.5 * a.dj)) * g.hx));
+      if (a.fast_i) {
+        r.f = dda_step(i, -a.di);
+        r.s = j;
+        dda_fast_move(a, r);
+        return r;
+      }
+      r.e -= fabs(g.hx);
+      r.e /= a.s;
+      r.len = fabs(r.e);
+      r.e = fabs(r.len * a.c);
+    }
+  }
+  r.f = a.fast_i ? i : j;
+  r.s = a.fast_i ? j : i;
+  return r;
+}
+
+// ---- launchers (dda_kernels.hip)
+
+// Radon: sino[slice][R][A] (overwritten) of img[slice][lines][cols]; imgT[slice][cols][lines]
+// is scratch that the launcher fills with the transposed image.
+hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* imgT, float* sino,
+                            int nslices, hipStream_t stream);
+// acc / accT ([nslices][lines*cols] each, T = float or double) = R^T sino[slice] split
+// by fast axis (zeroed first); `ones`: R^T 1 (sino unused)
+template <typename T>
+hipError_t dda_launch_scatter(const SSDdaGeom& g, const SSDdaAngle* angles, const float* sino, bool ones, T* acc,
+                              T* accT, int nslices, hipStream_t stream);
+// out[slice] = (acc + accT^T) [/ w]; instantiated for (float, float), (double, float),
+// (double, double)
+template <typename T, typename Out>
+hipError_t dda_launch_fold(const SSDdaGeom& g, const T* acc, const T* accT, const double* w, Out* out, int nslices,
+                           hipStream_t stream);
+hipError_t dda_launch_to_f32(const double* src, float* dst, size_t n, hipStream_t stream);
+
+#endif

@@ -667,23 +667,6 @@ extern "C" int raft_ss_phantom(const double* desc_rows, int nell, int lines, int
 // ------------------------------------------------------------------ host <-> raft_image staging
 
 namespace {
-struct PinnedF32 {
-  float* p = nullptr;
-  ~PinnedF32() {
-    if (p) (void)hipHostFree(p);
-  }
-  hipError_t alloc(size_t n) { return hipHostMalloc(reinterpret_cast<void**>(&p), n * sizeof(float) + 4, 0); }
-};
-
-void gather_f64(const raft_matrix& m, float* dst) {
-  for (int i = 0; i < m.lines; ++i)
-    for (int j = 0; j < m.columns; ++j) dst[(size_t)i * m.columns + j] = (float)RAFT_MATRIX_AT(m, i, j);
-}
-
-void scatter_f64(const float* src, const raft_matrix& m) {
-  for (int i = 0; i < m.lines; ++i)
-    for (int j = 0; j < m.columns; ++j) RAFT_MATRIX_AT(m, i, j) = (double)src[(size_t)i * m.columns + j];
-}
 
 // plan cache for the host-pointer entry points
 struct PlanKey {
@@ -725,12 +708,6 @@ int cached_plan(int n, int R, int A, const double* ic, const double* sc, std::sh
   return 0;
 }
 
-void corners_of(const raft_image& im, double* c) {
-  c[0] = im.tl_x;
-  c[1] = im.tl_y;
-  c[2] = im.br_x;
-  c[3] = im.br_y;
-}
 
 int host_radon(const raft_image& phantom, const raft_image& radon) {
   if (!phantom.data.p_data || !radon.data.p_data) return set_error(RAFT_SS_EINVAL, "empty image");
@@ -890,11 +867,13 @@ int host_f32_op(int which, float* h_out, const float* h_in, int n, int R, int A,
 
 thread_local int g_legacy_status = 0;
 
-void report(const char* fn, int rc) {
+void report(const char* fn, int rc) { ss_report_legacy(fn, rc); }
+}  // namespace
+
+void ss_report_legacy(const char* fn, int rc) {
   g_legacy_status = rc;
   if (rc) fprintf(stderr, "libraft %s: %s\n", fn, g_last_error.c_str());
 }
-}  // namespace
 
 extern "C" int raft_ss_legacy_status(void) { return g_legacy_status; }
 

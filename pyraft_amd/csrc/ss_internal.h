@@ -79,6 +79,40 @@ struct raft_ss_plan {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// status of a drop-in (void) symbol: kept for raft_ss_legacy_status(), printed on failure
+void ss_report_legacy(const char* fn, int rc);
+
+// ------------------------------------------------------------------ host <-> raft_image staging
+
+struct PinnedF32 {
+  float* p = nullptr;
+  PinnedF32() = default;
+  PinnedF32(const PinnedF32&) = delete;
+  PinnedF32& operator=(const PinnedF32&) = delete;
+  ~PinnedF32() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipHostMalloc(reinterpret_cast<void**>(&p), n * sizeof(float) + 4, 0); }
+};
+
+// strided FP64 matrix -> dense row-major FP32
+inline void gather_f64(const raft_matrix& m, float* dst) {
+  for (int i = 0; i < m.lines; ++i)
+    for (int j = 0; j < m.columns; ++j) dst[(size_t)i * m.columns + j] = (float)RAFT_MATRIX_AT(m, i, j);
+}
+
+inline void scatter_f64(const float* src, const raft_matrix& m) {
+  for (int i = 0; i < m.lines; ++i)
+    for (int j = 0; j < m.columns; ++j) RAFT_MATRIX_AT(m, i, j) = (double)src[(size_t)i * m.columns + j];
+}
+
+inline void corners_of(const raft_image& im, double* c) {
+  c[0] = im.tl_x;
+  c[1] = im.tl_y;
+  c[2] = im.br_x;
+  c[3] = im.br_y;
+}
+
 // ------------------------------------------------------------------ workspace
 
 // stream-ordered scratch allocation, freed on the same stream

@@ -274,3 +274,82 @@ def phantom(desc, n, corners=DEFAULT_IMG_CORNERS, device=None):
         check(libraft.raft_ss_phantom(d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), d.shape[0], int(n), int(n),
                                       c, out.data_ptr(), current_stream_handle(dev)), "raft_ss_phantom")
     return out
+
+
+class DdaPlan:
+    """Line-traversal (Bresenham / DDA) geometry of a lines x cols image and an R x A
+    sinogram on one device (raft/raft_ddaiterator.c, raft/raft_radon_bresenham.cpp).
+
+    ``img_corners`` / ``sino_corners`` are (tl_x, tl_y, br_x, br_y): image line i at
+    y = tl_y + i*(br_y - tl_y)/(lines - 1) (line 0 at the top for the default corners),
+    angle a at theta = tl_x + a*(br_x - tl_x)/(A - 1), ray r at
+    t = tl_y + r*(br_y - tl_y)/(R - 1)."""
+
+    def __init__(self, shape, nrays, nangles, img_corners=DEFAULT_IMG_CORNERS,
+                 sino_corners=DEFAULT_SINO_CORNERS, device=None):
+        _require_gpu()
+        if isinstance(shape, int):
+            shape = (shape, shape)
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.nrays, self.nangles = int(nrays), int(nangles)
+        self.img_corners = tuple(float(v) for v in img_corners)
+        self.sino_corners = tuple(float(v) for v in sino_corners)
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else
+                                   torch.device(device).index or 0)
+        self._handle = ctypes.c_void_p()
+        ic = (ctypes.c_double * 4)(*self.img_corners)
+        sc = (ctypes.c_double * 4)(*self.sino_corners)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_dda_plan_create(ctypes.byref(self._handle), self.shape[0], self.shape[1], self.nrays,
+                                               self.nangles, ic, sc, current_stream_handle()), "raft_dda_plan_create")
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                libraft.raft_dda_plan_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+            self._handle = None
+
+    def info(self):
+        out = (ctypes.c_int * 5)()
+        check(libraft.raft_dda_plan_info(self._handle, out, 5), "raft_dda_plan_info")
+        return dict(zip(("lines", "cols", "nrays", "nangles", "line_fast_angles"), list(out)))
+
+    def _stream(self):
+        return current_stream_handle(self.device)
+
+    def radon(self, image):
+        """(..., lines, cols) -> (..., R, A)."""
+        x = _check_tensor(image, self.shape, "image")
+        out = torch.empty(x.shape[:-2] + (self.nrays, self.nangles), dtype=torch.float32, device=x.device)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_dda_radon(self._handle, x.data_ptr(), out.data_ptr(), _nslices(x, 2), self._stream()),
+                  "raft_dda_radon")
+        return out
+
+    def radon_transpose(self, sino):
+        """(..., R, A) -> (..., lines, cols), the transpose of ``radon``."""
+        s = _check_tensor(sino, (self.nrays, self.nangles), "sinogram")
+        out = torch.empty(s.shape[:-2] + self.shape, dtype=torch.float32, device=s.device)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_dda_radon_transpose(self._handle, s.data_ptr(), out.data_ptr(), _nslices(s, 2),
+                                                   self._stream()), "raft_dda_radon_transpose")
+        return out
+
+    def backprojection(self, sino):
+        """(..., R, A) -> (..., lines, cols): R^T sino / R^T 1 (NaN where no ray passes)."""
+        s = _check_tensor(sino, (self.nrays, self.nangles), "sinogram")
+        out = torch.empty(s.shape[:-2] + self.shape, dtype=torch.float32, device=s.device)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_dda_backprojection(self._handle, s.data_ptr(), out.data_ptr(), _nslices(s, 2),
+                                                  self._stream()), "raft_dda_backprojection")
+        return out
+
+    def weights(self):
+        """R^T 1, the backprojection's normalisation, (lines, cols)."""
+        out = torch.empty(self.shape, dtype=torch.float32, device=self.device)
+        with torch.cuda.device(self.device):
+            check(libraft.raft_dda_weights(self._handle, out.data_ptr(), self._stream()), "raft_dda_weights")
+        return out

@@ -1,16 +1,19 @@
 """Emission EM and OSEM (pyraft/iterative/em.py).
 
-All variants run the multiplicative recurrence of raft/raft_em_gpu.cu:41-69,
+``em`` and ``make_em`` keep pyraft's defaults: the line-traversal (Bresenham) pair,
+composed exactly as pyraft/iterative/em.py:109-169 composes it (x <- x * R^T(s / R x)
+/ R^T 1 with unguarded numpy divisions), each projection a GPU call.
+
+``make_em(..., projection_method=SLANTSTACK, backprojection_method=SLANTSTACK)``,
+``osem`` and ``em_gpu`` run the multiplicative recurrence of raft/raft_em_gpu.cu:41-69,
 x <- x * BP_b(s / R_b x) / BP_b(1), with the slant-stack Radon R and the slant-stack
 backprojection BP, as fused HIP kernels: the forward projection writes the
 guarded ratio (0 where R x <= 0; the reference divides unguarded,
 raft/raft_cuda_aux.cu:21) and FP64 log-likelihood partials, the backprojection
 applies the update in its epilogue.  OSEM subset b covers the contiguous angle
 columns [b*A/S, (b+1)*A/S) (pyraft/iterative/em.py:284-312) on the full
-sinogram's angle grid.  Pixels with BP_b(1) = 0 keep their value.
-
-The reference's CPU ``em``/``make_em`` default to the Bresenham pair and its
-``osem`` to the NFFT slice pair; those projectors are not part of this build.
+sinogram's angle grid.  Pixels with BP_b(1) = 0 keep their value.  (pyraft's
+``osem`` uses the NFFT slice pair, which is not part of this build.)
 """
 import ctypes
 import math
@@ -45,12 +48,34 @@ def _device(a):
     return torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(a, dtype=numpy.float32))).cuda()
 
 
-def make_em(sino, shape, projection_method=radon_method.SLANTSTACK, backprojection_method=radon_method.SLANTSTACK,
+def _composed_em(sino, shape, projection_method, backprojection_method, weight, **kwargs):
+    """pyraft/iterative/em.py:109-142 with this build's radon / radon_transpose."""
+    from .backprojection import radon_transpose
+    from .sinogram import radon
+    if weight is None:
+        one = image(numpy.ones(sino.shape), top_left=sino.top_left, bottom_right=sino.bottom_right)
+        if isinstance(shape, image):
+            weight = radon_transpose(one, shape.shape, top_left=shape.top_left, bottom_right=shape.bottom_right,
+                                     method=backprojection_method, **kwargs)
+        else:
+            weight = radon_transpose(one, shape, method=backprojection_method, **kwargs)
+
+    def em_iter(x):
+        Ax = radon(x, sino.shape, top_left=sino.top_left, bottom_right=sino.bottom_right, method=projection_method)
+        Ax = sino / Ax
+        return x * radon_transpose(Ax, x.shape, top_left=x.top_left, bottom_right=x.bottom_right,
+                                   method=backprojection_method) / weight
+
+    return em_iter
+
+
+def make_em(sino, shape, projection_method=radon_method.BRESENHAM, backprojection_method=radon_method.BRESENHAM,
             weight=None, **kwargs):
-    """One-iteration EM closure (pyraft/iterative/em.py:109-144) on the slant-stack pair."""
-    for m in (projection_method, backprojection_method):
-        if m != radon_method.SLANTSTACK:
-            raise NotImplementedError("make_em: only radon_method.SLANTSTACK is part of this build")
+    """One-iteration EM closure (pyraft/iterative/em.py:109-144).  The slant-stack pair
+    runs as one fused device iteration; any other pair is composed as in pyraft."""
+    if (projection_method, backprojection_method) != (radon_method.SLANTSTACK, radon_method.SLANTSTACK):
+        with numpy.errstate(divide="ignore", invalid="ignore"):
+            return _composed_em(sino, shape, projection_method, backprojection_method, weight, **kwargs)
     if weight is not None:
         raise NotImplementedError("make_em: the weight is BP(1) of the slant-stack backprojection")
     if isinstance(shape, image):
@@ -69,7 +94,9 @@ def make_em(sino, shape, projection_method=radon_method.SLANTSTACK, backprojecti
 
 
 def em(sino, shape=None, niter=5, **kwargs):
-    """EM from ones (or from ``shape`` when it is an image), pyraft/iterative/em.py:151-169."""
+    """EM from ones (or from ``shape`` when it is an image) with pyraft's default
+    (line-traversal) pair, pyraft/iterative/em.py:151-169; the default image is
+    A x A for an R x A sinogram, as in pyraft."""
     if shape is None:
         shape = [sino.shape[1], sino.shape[1]]
     if isinstance(shape, image):
@@ -79,10 +106,11 @@ def em(sino, shape=None, niter=5, **kwargs):
     else:
         x = image(shape, **kwargs)
         x[:] = 1.0
-    xc = _corners(x, (-1.0, 1.0), (1.0, -1.0))
-    plan = _plan_for(x.shape, xc, sino)
-    out = plan.em(_device(sino), int(niter), 1, x0=_device(x)).cpu().numpy().astype(numpy.float64)
-    return image(out, top_left=(xc[0], xc[1]), bottom_right=(xc[2], xc[3]))
+    step = make_em(sino, x)
+    with numpy.errstate(divide="ignore", invalid="ignore"):
+        for _ in range(0, int(niter)):
+            x = step(x)
+    return x
 
 
 def em_gpu(*args):

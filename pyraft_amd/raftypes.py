@@ -115,6 +115,9 @@ _decl("raft_radon_slantstack_gpu", [_c_float_p, _c_float_p, _i, _i, _i], None)
 _decl("raft_backprojection_slantstack_gpu", [_c_float_p, _c_float_p, _i, _i, _i], None)
 _decl("raft_em_gpu", [_c_float_p, _c_float_p, _i, _i, _i, _i], None)
 _decl("raft_radon_transpose_slantstack", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
+_decl("raft_radon_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
+_decl("raft_radon_transpose_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
+_decl("raft_backprojection_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
 _decl("raft_radon_slantstack_withmesh", [RAFT_IMAGE, RAFT_IMAGE, RAFT_VECTOR, RAFT_VECTOR, _i], None)
 _decl("raft_radon_slantstack_view", [RAFT_VECTOR, RAFT_IMAGE, RAFT_VECTOR, ctypes.c_double], None)
 _decl("raft_ss_legacy_status", [], _i)
@@ -130,6 +133,14 @@ _decl("raft_ss_plan_wave_bounds", [_vp, ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_host_bounds", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i),
                                    ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_host_info", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i)], _i)
+_decl("raft_dda_plan_create", [ctypes.POINTER(_vp), _i, _i, _i, _i, _d4, _d4, _vp], _i)
+_decl("raft_dda_plan_destroy", [_vp], _i)
+_decl("raft_dda_plan_info", [_vp, ctypes.POINTER(_i), _i], _i)
+_decl("raft_dda_radon", [_vp, _vp, _vp, _i, _vp], _i)
+_decl("raft_dda_radon_transpose", [_vp, _vp, _vp, _i, _vp], _i)
+_decl("raft_dda_backprojection", [_vp, _vp, _vp, _i, _vp], _i)
+_decl("raft_dda_weights", [_vp, _vp, _vp], _i)
+_decl("raft_dda_host_ray", [_i, _i, _i, _i, _d4, _d4, _i, _i, ctypes.POINTER(_i), _d4, _i], _i)
 _decl("raft_ss_plan_create_mesh", [ctypes.POINTER(_vp), _i, _d4, _i, _d4, _i, _d4, _vp], _i)
 _decl("raft_ss_plan_host_bounds_mesh", [_i, _d4, _i, _d4, _i, _d4, ctypes.POINTER(_i),
                                         ctypes.POINTER(ctypes.c_longlong)], _i)
@@ -333,4 +344,7 @@ EXPORTED_SYMBOLS = (
     "raft_ss_batch_set_bp_variant", "raft_ss_batch_tune", "raft_ss_batch_radon", "raft_ss_batch_backprojection",
     "raft_ss_batch_em", "raft_radon_slantstack_withmesh", "raft_radon_slantstack_view",
     "raft_ss_plan_create_mesh", "raft_ss_plan_host_bounds_mesh",
+    "raft_radon_bresenham", "raft_radon_transpose_bresenham", "raft_backprojection_bresenham",
+    "raft_dda_plan_create", "raft_dda_plan_destroy", "raft_dda_plan_info", "raft_dda_radon",
+    "raft_dda_radon_transpose", "raft_dda_backprojection", "raft_dda_weights", "raft_dda_host_ray",
 )

@@ -5,8 +5,9 @@ signature and sinogram layout (rows = rays, columns = angles, default corners
 theta in (0, pi), t in (-1, 1); pyraft/sinogram/sino.py:175-212) and calls the
 drop-in ``raft_radon_slantstack`` symbol of libraft, which runs on the GPU.
 
-Only the slant-stack discretisation is provided: ``radon_method.BRESENHAM`` (the
-reference's default) and the Fourier methods raise ``NotImplementedError``.
+``radon_method.BRESENHAM`` (the default, the line-traversal projector of
+raft/raft_radon_bresenham.cpp) and ``radon_method.SLANTSTACK`` run on the GPU; the
+Fourier methods raise ``NotImplementedError``.
 
 The arbitrary-mesh forms ``radon_view`` / ``radon_withmesh`` / ``radon_3d_stack`` /
 ``snapshot`` (pyraft/sinogram/sino.py:220-343) take t and theta from tables.  As in
@@ -47,7 +48,8 @@ def _float64_image(img):
 
 
 def radon(img, shape=(367, 180), method=radon_method.BRESENHAM, **kwargs):
-    """Radon transform of a pyraft ``image`` (slant-stack) or of an ellipse description.
+    """Radon transform of a pyraft ``image`` (line traversal or slant-stack) or of an
+    ellipse description.
 
     pyraft/sinogram/sino.py:175-212.  With no extent keywords the sinogram spans
     theta in (0, pi) (``theta_extent`` overrides) and t in (-1, 1) (``t_extent``)."""
@@ -67,8 +69,10 @@ def radon(img, shape=(367, 180), method=radon_method.BRESENHAM, **kwargs):
             libraft.raft_radon_slantstack(IMG, SINO, ctypes.c_int(nthreads))
             check_legacy("raft_radon_slantstack")
         elif method == radon_method.BRESENHAM:
-            raise NotImplementedError("radon: the Bresenham projector is not part of this build; "
-                                      "pass method=radon_method.SLANTSTACK")
+            src = _float64_image(img)
+            IMG = make_RAFT_IMAGE(src, src.top_left, src.bottom_right)
+            libraft.raft_radon_bresenham(IMG, SINO, ctypes.c_int(nthreads))
+            check_legacy("raft_radon_bresenham")
         else:
             raise TypeError('Unsupported method for Radon Transform!')
         return sino

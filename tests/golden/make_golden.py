@@ -3,8 +3,8 @@
 Runs oracle/_ref/libraft_ref.so, which oracle/Makefile compiles from the reference's
 sources (raft/raft_radon_slantstack.c, raft/raft_backprojection_slantstack.c,
 raft/raft_radon_slantstack_withmesh.c, raft/raft_radon_slantstack_view.c,
-raft/raft_image_phantom_fromdesc.c, raft/raft_radon_fromdesc.c and their helpers),
-on small seeded inputs.  The fixture holds inputs and outputs only.
+raft/raft_image_phantom_fromdesc.c, raft/raft_radon_fromdesc.c, raft/raft_ddaiterator.c and
+their helpers), on small seeded inputs.  The fixture holds inputs and outputs only.
 
     make -C oracle ref && python tests/golden/make_golden.py
 """
@@ -56,6 +56,16 @@ MESH_CASES = [
 SPECIAL_THETA = [0.0, math.pi / 4, math.pi / 2, 3 * math.pi / 4, math.pi, -math.pi / 2, 2 * math.pi]
 
 
+# line-traversal (Bresenham / DDA) pair: (name, (lines, cols), R, A, image corners, sinogram
+# corners); outputs from the reference's own iterator raft/raft_ddaiterator.c driven by
+# oracle/ref_dda_harness.c (the reference's worker files need a BLAS this image lacks)
+DDA_CASES = [
+    ("dda_24x24_r30_a20", (24, 24), 30, 20, DEF_IMG, (0.0, 1.0, math.pi, -1.0)),
+    ("dda_20x31_r25_a17_rect", (20, 31), 25, 17, (-0.7, 1.3, 1.2, -0.9), (-math.pi / 2, 1.1, math.pi / 2, -0.8)),
+    ("dda_16x16_r21_a9_flip", (16, 16), 21, 9, (1.0, -1.0, -1.0, 1.0), (math.pi, -1.2, 0.0, 1.2)),
+]
+
+
 def mesh_tables(rng, R, A, sort):
     t = rng.uniform(-1.6, 1.6, R)           # some rays miss the image
     t[:3] = (-1.0, 0.0, 1.0)
@@ -95,6 +105,16 @@ def main():
         out[name + "/theta"] = theta
         out[name + "/radon"] = O.ref_radon_withmesh(img, t, theta, img_tl=ic[:2], img_br=ic[2:], nthreads=1)
         out[name + "/view"] = O.ref_radon_view(img, t, float(theta[0]), img_tl=ic[:2], img_br=ic[2:])
+    for name, shape, R, A, ic, sc in DDA_CASES:
+        geo = dict(img_tl=ic[:2], img_br=ic[2:], sino_tl=sc[:2], sino_br=sc[2:])
+        img = rng.random(shape)
+        sino = rng.random((R, A))
+        out[name + "/img"] = img
+        out[name + "/sino"] = sino
+        out[name + "/geom"] = np.array([shape[0], shape[1], R, A] + list(ic) + list(sc), dtype=np.float64)
+        out[name + "/radon"] = O.ref_radon_bresenham(img, R, A, **geo)
+        out[name + "/transpose"] = O.ref_radon_transpose_bresenham(sino, shape, **geo)
+        out[name + "/backprojection"] = O.ref_backprojection_bresenham(sino, shape, **geo)
     path = os.path.join(HERE, "slantstack_golden.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -42,16 +42,31 @@ def test_radon_method_members_are_tuples():
 def test_plans_and_unsupported_methods():
     assert pr.backprojection_plan((16, 8), (16, 16), method=pr.radon_method.SLANTSTACK) == \
         (pr.radon_method.SLANTSTACK, (16, 16), 2)
+    assert pr.backprojection_plan((16, 8), (16, 16), method=pr.radon_method.BRESENHAM) == \
+        (pr.radon_method.BRESENHAM, (16, 16), 2)
     with pytest.raises(NotImplementedError):
         pr.backprojection_plan((16, 8), (16, 16))  # BST default
     with pytest.raises(TypeError):
         pr.backprojection_plan((16, 8), (16, 16), method=(99,))
-    with pytest.raises(NotImplementedError):
-        pr.radon(pr.image((8, 8)), (8, 4))  # BRESENHAM default
-    with pytest.raises(NotImplementedError):
-        pr.radon_transpose(np.zeros((8, 4)), (8, 8), method=pr.radon_method.BRESENHAM)
-    with pytest.raises(NotImplementedError):
-        pr.make_em(np.zeros((8, 4)), (8, 8), projection_method=pr.radon_method.BRESENHAM)
+    with pytest.raises(TypeError):
+        pr.radon(pr.image((8, 8)), (8, 4), method=pr.radon_method.BST)
+    with pytest.raises(TypeError):
+        pr.radon_transpose(np.zeros((8, 4)), (8, 8), method=pr.radon_method.BST)
+    # defaults as in pyraft: the line-traversal pair
+    assert pr.radon.__defaults__[1] == pr.radon_method.BRESENHAM
+    assert pr.radon_transpose.__defaults__[1] == pr.radon_method.BRESENHAM
+    assert pr.make_em.__defaults__[:2] == (pr.radon_method.BRESENHAM, pr.radon_method.BRESENHAM)
+
+
+def test_operators_fail_loudly_without_a_device():
+    """No CPU fallback: with no GPU the drop-in symbols report a HIP error."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(pr.RaftError):
+        pr.radon(pr.image(np.ones((8, 8))), (8, 4))
+    with pytest.raises(pr.RaftError):
+        pr.radon(pr.image(np.ones((8, 8))), (8, 4), method=pr.radon_method.SLANTSTACK)
 
 
 def test_make_raft_matrix_strides_and_dtype():
