@@ -25,10 +25,19 @@ struct raft_dda_plan {
   double sino_corners[4] = {0, 0, 0, 0};
   std::vector<SSDdaAngle> h_angles;
   SSDdaAngle* d_angles = nullptr;
+  std::vector<SSDdaGatherAngle> h_gather;  // angles of the pixel-driven transpose
+  std::vector<int> h_walk;                 // angles traced ray by ray in the transpose
+  SSDdaGatherAngle* d_gather = nullptr;
+  int* d_walk = nullptr;
   double* d_weights = nullptr;  // R^T 1 in FP64, built on first use
   std::mutex weights_mutex;
+  SSDdaTransposePlan tplan() const {
+    return SSDdaTransposePlan{d_angles, d_gather, (int)h_gather.size(), d_walk, (int)h_walk.size()};
+  }
   ~raft_dda_plan() {
     if (d_angles) (void)hipFree(d_angles);
+    if (d_gather) (void)hipFree(d_gather);
+    if (d_walk) (void)hipFree(d_walk);
     if (d_weights) (void)hipFree(d_weights);
   }
 };
@@ -93,6 +102,31 @@ SSDdaAngle make_angle(const SSDdaGeom& g, double theta) {
   return a;
 }
 
+// Splits the transpose's angles: the traversal equals exact w x h pixel geometry when
+// its slow-axis drift per move does not exceed the pixel width across (always for
+// square pixels; for other shapes the reference's fast-axis test can pick the axis
+// crossed less often) and the angle is not within 1e-7 of an axis, where the
+// traversal's tie rule at pixel edges decides which pixel a ray on an edge belongs to.
+bool pixel_driven(const SSDdaGeom& g, const SSDdaAngle& a, SSDdaGatherAngle* out, int index) {
+  const double ac = fabs(a.c), as = fabs(a.s);
+  if (ac < 1e-7 || as < 1e-7 || !(a.e_step <= a.e_cap)) return false;
+  const double w = fabs(g.hx), h = fabs(g.hy);
+  SSDdaGatherAngle ga;
+  ga.c = a.c;
+  ga.s = a.s;
+  ga.hys = g.hy * a.s;
+  ga.p = 0.5 * (w * ac + h * as);
+  const double q = 0.5 * fabs(w * ac - h * as);
+  ga.inv_pq = 1.0 / (ga.p - q);
+  ga.lmax = fmin(w / as, h / ac);
+  ga.half = ga.p / fabs(g.dt);
+  ga.a = index;
+  ga.k = (int)ceil(2.0 * ga.half) + 2;
+  if (!isfinite(ga.inv_pq) || !isfinite(ga.half) || ga.k > 1 << 20) return false;
+  *out = ga;
+  return true;
+}
+
 int check_dda(const raft_dda_plan* p, int nslices) {
   if (!p) return ss_set_error(RAFT_SS_EINVAL, "null plan");
   if (nslices < 1 || nslices > 65535)
@@ -103,20 +137,21 @@ int check_dda(const raft_dda_plan* p, int nslices) {
   return 0;
 }
 
-// R^T 1 for the plan in FP64, built once (atomics: the first call fixes its rounding)
+// R^T 1 for the plan in FP64, built once (atomics of the ray-traced angles: the first
+// call fixes their rounding)
 int ensure_weights(raft_dda_plan* p, hipStream_t stream) {
   std::lock_guard<std::mutex> lock(p->weights_mutex);
   if (p->d_weights) return 0;
   const size_t npix = (size_t)p->g.lines * p->g.cols;
   double* w = nullptr;
   HIP_TRY(hipMalloc(&w, sizeof(double) * npix));
-  DevBuf acc, accT;
-  hipError_t e = acc.alloc(sizeof(double) * npix, stream);
+  DevBuf sinoT, acc, accT;
+  hipError_t e = sinoT.alloc(sizeof(float) * (size_t)p->g.nrays * p->g.nangles, stream);
+  if (e == hipSuccess) e = acc.alloc(sizeof(double) * npix, stream);
   if (e == hipSuccess) e = accT.alloc(sizeof(double) * npix, stream);
   if (e == hipSuccess)
-    e = dda_launch_scatter<double>(p->g, p->d_angles, nullptr, true, acc.as<double>(), accT.as<double>(), 1, stream);
-  if (e == hipSuccess)
-    e = dda_launch_fold<double, double>(p->g, acc.as<double>(), accT.as<double>(), nullptr, w, 1, stream);
+    e = dda_launch_transpose<double, double>(p->g, p->tplan(), sinoT.as<float>(), true, sinoT.as<float>(),
+                                             acc.as<double>(), accT.as<double>(), nullptr, w, 1, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
     (void)hipFree(w);
@@ -126,15 +161,17 @@ int ensure_weights(raft_dda_plan* p, hipStream_t stream) {
   return 0;
 }
 
-// num[s] = R^T sino[s] accumulated in FP64 (the backprojection's numerator)
-int numerator_f64(const raft_dda_plan* p, const float* sino, double* num, int nslices, hipStream_t stream) {
+// out[s] = R^T sino[s] [/ w] with accumulator T
+template <typename T, typename Out>
+int transpose_into(const raft_dda_plan* p, const float* sino, const double* w, Out* out, int nslices,
+                   hipStream_t stream) {
   const size_t n = (size_t)p->g.lines * p->g.cols * nslices;
-  DevBuf acc, accT;
-  HIP_TRY(acc.alloc(sizeof(double) * n, stream));
-  HIP_TRY(accT.alloc(sizeof(double) * n, stream));
-  HIP_TRY(dda_launch_scatter<double>(p->g, p->d_angles, sino, false, acc.as<double>(), accT.as<double>(), nslices,
-                                     stream));
-  HIP_TRY((dda_launch_fold<double, double>(p->g, acc.as<double>(), accT.as<double>(), nullptr, num, nslices, stream)));
+  DevBuf sinoT, acc, accT;
+  HIP_TRY(sinoT.alloc(sizeof(float) * (size_t)p->g.nrays * p->g.nangles * nslices, stream));
+  HIP_TRY(acc.alloc(sizeof(T) * n, stream));
+  HIP_TRY(accT.alloc(sizeof(T) * n, stream));
+  HIP_TRY((dda_launch_transpose<T, Out>(p->g, p->tplan(), sino, false, sinoT.as<float>(), acc.as<T>(), accT.as<T>(),
+                                        w, out, nslices, stream)));
   return 0;
 }
 
@@ -169,9 +206,24 @@ extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, in
     if (!isfinite(p->h_angles[a].c) || !isfinite(p->h_angles[a].s))
       return ss_set_error(RAFT_SS_EINVAL, "non-finite angle %d", a);
   }
+  for (int a = 0; a < nangles; ++a) {
+    SSDdaGatherAngle ga;
+    if (pixel_driven(g, p->h_angles[a], &ga, a))
+      p->h_gather.push_back(ga);
+    else
+      p->h_walk.push_back(a);
+  }
   HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSDdaAngle) * nangles));
+  HIP_TRY(hipMalloc(&p->d_gather, sizeof(SSDdaGatherAngle) * (p->h_gather.size() + 1)));
+  HIP_TRY(hipMalloc(&p->d_walk, sizeof(int) * (p->h_walk.size() + 1)));
   HIP_TRY(hipMemcpyAsync(p->d_angles, p->h_angles.data(), sizeof(SSDdaAngle) * nangles, hipMemcpyHostToDevice,
                          stream));
+  if (!p->h_gather.empty())
+    HIP_TRY(hipMemcpyAsync(p->d_gather, p->h_gather.data(), sizeof(SSDdaGatherAngle) * p->h_gather.size(),
+                           hipMemcpyHostToDevice, stream));
+  if (!p->h_walk.empty())
+    HIP_TRY(hipMemcpyAsync(p->d_walk, p->h_walk.data(), sizeof(int) * p->h_walk.size(), hipMemcpyHostToDevice,
+                           stream));
   HIP_TRY(hipStreamSynchronize(stream));
   *out = p.release();
   return 0;
@@ -186,7 +238,7 @@ extern "C" int raft_dda_plan_info(const raft_dda_plan* p, int* info, int ninfo) 
   if (!p || !info) return ss_set_error(RAFT_SS_EINVAL, "null argument");
   int nfast = 0;
   for (const SSDdaAngle& a : p->h_angles) nfast += a.fast_i;
-  const int vals[] = {p->g.lines, p->g.cols, p->g.nrays, p->g.nangles, nfast};
+  const int vals[] = {p->g.lines, p->g.cols, p->g.nrays, p->g.nangles, nfast, (int)p->h_walk.size()};
   for (int i = 0; i < ninfo && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
   return 0;
 }
@@ -207,15 +259,7 @@ extern "C" int raft_dda_radon_transpose(const raft_dda_plan* p, const float* sin
   int rc = check_dda(p, nslices);
   if (rc) return rc;
   if (!img || !sino) return ss_set_error(RAFT_SS_EINVAL, "null buffer");
-  hipStream_t stream = as_stream(stream_);
-  const size_t n = (size_t)p->g.lines * p->g.cols * nslices;
-  DevBuf acc, accT;
-  HIP_TRY(acc.alloc(sizeof(float) * n, stream));
-  HIP_TRY(accT.alloc(sizeof(float) * n, stream));
-  HIP_TRY(dda_launch_scatter<float>(p->g, p->d_angles, sino, false, acc.as<float>(), accT.as<float>(), nslices,
-                                    stream));
-  HIP_TRY((dda_launch_fold<float, float>(p->g, acc.as<float>(), accT.as<float>(), nullptr, img, nslices, stream)));
-  return 0;
+  return transpose_into<float, float>(p, sino, nullptr, img, nslices, as_stream(stream_));
 }
 
 extern "C" int raft_dda_backprojection(raft_dda_plan* p, const float* sino, float* img, int nslices, void* stream_) {
@@ -224,15 +268,7 @@ extern "C" int raft_dda_backprojection(raft_dda_plan* p, const float* sino, floa
   if (!img || !sino) return ss_set_error(RAFT_SS_EINVAL, "null buffer");
   hipStream_t stream = as_stream(stream_);
   if ((rc = ensure_weights(p, stream))) return rc;
-  const size_t n = (size_t)p->g.lines * p->g.cols * nslices;
-  DevBuf acc, accT;
-  HIP_TRY(acc.alloc(sizeof(double) * n, stream));
-  HIP_TRY(accT.alloc(sizeof(double) * n, stream));
-  HIP_TRY(dda_launch_scatter<double>(p->g, p->d_angles, sino, false, acc.as<double>(), accT.as<double>(), nslices,
-                                     stream));
-  HIP_TRY((dda_launch_fold<double, float>(p->g, acc.as<double>(), accT.as<double>(), p->d_weights, img, nslices,
-                                         stream)));
-  return 0;
+  return transpose_into<double, float>(p, sino, p->d_weights, img, nslices, stream);
 }
 
 extern "C" int raft_dda_weights(raft_dda_plan* p, float* out, void* stream_) {
@@ -397,7 +433,8 @@ int host_dda_backprojection(const raft_image& sino, const raft_image& img) {
   HIP_TRY(ds.alloc(sizeof(float) * nsino, nullptr));
   HIP_TRY(dnum.alloc(sizeof(double) * npix, nullptr));
   HIP_TRY(hipMemcpyAsync(ds.p, hs.p, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
-  if ((rc = numerator_f64(plan.get(), ds.as<float>(), dnum.as<double>(), 1, nullptr))) return rc;
+  if ((rc = transpose_into<double, double>(plan.get(), ds.as<float>(), nullptr, dnum.as<double>(), 1, nullptr)))
+    return rc;
   HIP_TRY(hipMemcpyAsync(num.data(), dnum.p, sizeof(double) * npix, hipMemcpyDeviceToHost, nullptr));
   HIP_TRY(hipMemcpyAsync(w.data(), plan->d_weights, sizeof(double) * npix, hipMemcpyDeviceToHost, nullptr));
   HIP_TRY(hipStreamSynchronize(nullptr));

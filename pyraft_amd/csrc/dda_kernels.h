@@ -57,6 +57,25 @@ struct SSDdaAngle {
   int axis;                 // 1: sin == 0, 2: cos == 0, 0: oblique
 };
 
+// Pixel-driven form of the transpose for one angle.  Where the traversal is exact
+// geometry (its slow-axis drift per move e_step <= e_cap, and the angle is not within
+// 1e-7 of an axis, where the reference's tie rule at pixel edges decides), the length
+// of ray t in pixel (i, j) is the trapezoid L(|t - t_c|) of the line's distance from the
+// pixel centre, t_c = x_j cos + y_i sin:
+//   L = Lmax                          for |d| <= q
+//   L = Lmax (p - |d|) / (p - q)      for q < |d| < p
+//   L = 0                             for |d| >= p
+// with p = (w|cos| + h|sin|)/2, q = |w|cos| - h|sin||/2, Lmax = min(w/|sin|, h/|cos|)
+// for w x h pixels.  Other angles are traced ray by ray.
+struct SSDdaGatherAngle {
+  double c, s;
+  double hys;               // hy * sin: t_c step per image line
+  double p, inv_pq, lmax;
+  double half;              // p / |dt|: candidate rays either side of t_c
+  int a;                    // sinogram column
+  int k;                    // candidate rays per pixel: ceil(2 half) + 1
+};
+
 // ray state in (fast, slow) coordinates
 struct SSDdaRay {
   int f, s;                 // fast / slow index
@@ -169,16 +188,23 @@ static inline DDA_HD SSDdaRay dda_ray_start(const SSDdaGeom& g, const SSDdaAngle
 // is scratch that the launcher fills with the transposed image.
 hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* imgT, float* sino,
                             int nslices, hipStream_t stream);
-// acc / accT ([nslices][lines*cols] each, T = float or double) = R^T sino[slice] split
-// by fast axis (zeroed first); `ones`: R^T 1 (sino unused)
-template <typename T>
-hipError_t dda_launch_scatter(const SSDdaGeom& g, const SSDdaAngle* angles, const float* sino, bool ones, T* acc,
-                              T* accT, int nslices, hipStream_t stream);
-// out[slice] = (acc + accT^T) [/ w]; instantiated for (float, float), (double, float),
-// (double, double)
+// The transpose's angles split into the pixel-driven set (gather table, ngather) and the
+// ray-traced set (walk list of sinogram columns, nwalk).
+struct SSDdaTransposePlan {
+  const SSDdaAngle* angles;
+  const SSDdaGatherAngle* gather;
+  int ngather;
+  const int* walk;
+  int nwalk;
+};
+
+// out[slice] = R^T sino[slice] [/ w] (T: accumulator type of the ray-traced part, float or
+// double).  sinoT ([nslices][A][R] floats), acc, accT ([nslices][lines*cols] T) are
+// scratch.  `ones`: R^T 1 for one slice (sino unused).
 template <typename T, typename Out>
-hipError_t dda_launch_fold(const SSDdaGeom& g, const T* acc, const T* accT, const double* w, Out* out, int nslices,
-                           hipStream_t stream);
+hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp, const float* sino, bool ones,
+                                float* sinoT, T* acc, T* accT, const double* w, Out* out, int nslices,
+                                hipStream_t stream);
 hipError_t dda_launch_to_f32(const double* src, float* dst, size_t n, hipStream_t stream);
 
 #endif

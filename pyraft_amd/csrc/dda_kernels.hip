@@ -68,10 +68,10 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
 // reference's non-square-pixel traversal yields negative lengths).
 template <typename T>
 __global__ __launch_bounds__(256) void k_dda_scatter(SSDdaGeom g, const SSDdaAngle* __restrict__ angles,
-                                                     const float* __restrict__ sino, int ones, T* __restrict__ acc,
-                                                     T* __restrict__ accT) {
+                                                     const int* __restrict__ walk, const float* __restrict__ sino,
+                                                     int ones, T* __restrict__ acc, T* __restrict__ accT) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ai = blockIdx.y;
+  const int ai = walk[blockIdx.y];
   const size_t slice = blockIdx.z;
   if (r >= g.nrays) return;
   const T q = ones ? T(1) : T(sino[(slice * g.nrays + r) * g.nangles + ai]);
@@ -95,6 +95,55 @@ __global__ __launch_bounds__(256) void k_dda_scatter(SSDdaGeom g, const SSDdaAng
   }
 }
 
+// ---------------------------------------------------------------- transpose (gather)
+
+#define DDA_GX 64     // image columns per block (one per lane)
+#define DDA_GROWS 4   // image lines per thread
+#define DDA_GY (4 * DDA_GROWS)
+
+// acc[i][j] = sum over the table's angles and the rays crossing pixel (i, j) of
+// q(r, a) * L(t_r - t_c) (trapezoid lengths, FP64 geometry and sums).  Overwrites acc.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGatherAngle* __restrict__ ga, int nga,
+                                                    const float* __restrict__ sinoT, int ones, T* __restrict__ acc) {
+  const int j = blockIdx.x * DDA_GX + (threadIdx.x & (DDA_GX - 1));
+  const int i0 = blockIdx.y * DDA_GY + (threadIdx.x / DDA_GX) * DDA_GROWS;
+  const size_t slice = blockIdx.z;
+  if (j >= g.cols || i0 >= g.lines) return;
+  const double x = g.ox + j * g.hx;
+  const double y0 = g.oy + i0 * g.hy;
+  const double inv_dt = 1.0 / g.dt;
+  const float* __restrict__ srows = sinoT + slice * (size_t)g.nangles * g.nrays;
+  double sum[DDA_GROWS];
+#pragma unroll
+  for (int rr = 0; rr < DDA_GROWS; ++rr) sum[rr] = 0.0;
+  for (int k = 0; k < nga; ++k) {
+    const SSDdaGatherAngle A = ga[k];
+    const float* __restrict__ q = srows + (size_t)A.a * g.nrays;
+    double tc = fma(x, A.c, y0 * A.s);
+#pragma unroll
+    for (int rr = 0; rr < DDA_GROWS; ++rr) {
+      const double rc = (tc - g.t0) * inv_dt;
+      const double lo = fmin(fmax(floor(rc - A.half), -2.0), (double)g.nrays);
+      const int r0 = (int)lo;
+      for (int m = 0; m < A.k; ++m) {
+        const int r = r0 + m;
+        if ((unsigned)r < (unsigned)g.nrays) {
+          const double d = fma((double)r, g.dt, g.t0) - tc;
+          const double z = fmin(1.0, fmax(0.0, (A.p - fabs(d)) * A.inv_pq));
+          const double v = ones ? 1.0 : (double)q[r];
+          sum[rr] = fma(v * A.lmax, z, sum[rr]);
+        }
+      }
+      tc += A.hys;
+    }
+  }
+  T* __restrict__ out = acc + slice * (size_t)g.lines * g.cols;
+#pragma unroll
+  for (int rr = 0; rr < DDA_GROWS; ++rr)
+    if (i0 + rr < g.lines) out[(size_t)(i0 + rr) * g.cols + j] = T(sum[rr]);
+}
+
 // ---------------------------------------------------------------- layouts
 
 template <typename T>
@@ -102,7 +151,8 @@ __global__ __launch_bounds__(256) void k_dda_fill(T* __restrict__ p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = T(0);
 }
 
-// out[lines][cols] = acc[lines][cols] + accT[cols][lines]^T, divided by w when given.
+// out[lines][cols] = acc[lines][cols] + accT[cols][lines]^T (accT may be null), divided by
+// w when given.
 // Block (32, 8) handles a 32 x 32 tile; accT is read along its lines through LDS.
 template <typename T, typename Out>
 __global__ __launch_bounds__(256) void k_dda_fold(int lines, int cols, const T* __restrict__ acc,
@@ -112,10 +162,10 @@ __global__ __launch_bounds__(256) void k_dda_fold(int lines, int cols, const T* 
   const size_t slice = blockIdx.z;
   const size_t npix = (size_t)lines * cols;
   const int i0 = blockIdx.y * DDA_TILE, j0 = blockIdx.x * DDA_TILE;
-  const T* aT = accT + slice * npix;
+  const T* aT = accT ? accT + slice * npix : nullptr;
   for (int jj = threadIdx.y; jj < DDA_TILE; jj += blockDim.y) {
     const int j = j0 + jj, i = i0 + threadIdx.x;
-    tile[jj][threadIdx.x] = (j < cols && i < lines) ? aT[(size_t)j * lines + i] : T(0);
+    tile[jj][threadIdx.x] = (accT && j < cols && i < lines) ? aT[(size_t)j * lines + i] : T(0);
   }
   __syncthreads();
   for (int ii = threadIdx.y; ii < DDA_TILE; ii += blockDim.y) {
@@ -161,22 +211,27 @@ hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const 
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t dda_launch_scatter(const SSDdaGeom& g, const SSDdaAngle* angles, const float* sino, bool ones, T* acc,
-                              T* accT, int nslices, hipStream_t stream) {
-  const size_t n = (size_t)g.lines * g.cols * nslices;
-  hipLaunchKernelGGL(k_dda_fill<T>, dim3(1024), dim3(256), 0, stream, acc, n);
-  hipLaunchKernelGGL(k_dda_fill<T>, dim3(1024), dim3(256), 0, stream, accT, n);
-  hipLaunchKernelGGL(k_dda_scatter<T>, dim3((g.nrays + 255) / 256, g.nangles, nslices), dim3(256), 0, stream, g,
-                     angles, sino, ones ? 1 : 0, acc, accT);
-  return hipGetLastError();
-}
-
 template <typename T, typename Out>
-hipError_t dda_launch_fold(const SSDdaGeom& g, const T* acc, const T* accT, const double* w, Out* out, int nslices,
-                           hipStream_t stream) {
+hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp, const float* sino, bool ones,
+                                float* sinoT, T* acc, T* accT, const double* w, Out* out, int nslices,
+                                hipStream_t stream) {
+  const size_t n = (size_t)g.lines * g.cols * nslices;
+  if (tp.ngather > 0) {
+    if (!ones)  // angle-major copy of the sinogram: rays of one angle contiguous
+      hipLaunchKernelGGL(k_dda_transpose_image, dda_tiles(g.nrays, g.nangles, nslices), dim3(DDA_TILE, 8), 0, stream,
+                         g.nrays, g.nangles, sino, sinoT);
+    hipLaunchKernelGGL(k_dda_gather<T>, dim3((g.cols + DDA_GX - 1) / DDA_GX, (g.lines + DDA_GY - 1) / DDA_GY, nslices),
+                       dim3(256), 0, stream, g, tp.gather, tp.ngather, (const float*)sinoT, ones ? 1 : 0, acc);
+  } else {
+    hipLaunchKernelGGL(k_dda_fill<T>, dim3(1024), dim3(256), 0, stream, acc, n);
+  }
+  if (tp.nwalk > 0) {
+    hipLaunchKernelGGL(k_dda_fill<T>, dim3(1024), dim3(256), 0, stream, accT, n);
+    hipLaunchKernelGGL(k_dda_scatter<T>, dim3((g.nrays + 255) / 256, tp.nwalk, nslices), dim3(256), 0, stream, g,
+                       tp.angles, tp.walk, sino, ones ? 1 : 0, acc, accT);
+  }
   hipLaunchKernelGGL((k_dda_fold<T, Out>), dda_tiles(g.lines, g.cols, nslices), dim3(DDA_TILE, 8), 0, stream, g.lines,
-                     g.cols, acc, accT, w, out);
+                     g.cols, (const T*)acc, (const T*)(tp.nwalk > 0 ? accT : nullptr), w, out);
   return hipGetLastError();
 }
 
@@ -190,13 +245,12 @@ hipError_t dda_launch_to_f32(const double* src, float* dst, size_t n, hipStream_
   return hipGetLastError();
 }
 
-template hipError_t dda_launch_scatter<float>(const SSDdaGeom&, const SSDdaAngle*, const float*, bool, float*, float*,
-                                              int, hipStream_t);
-template hipError_t dda_launch_scatter<double>(const SSDdaGeom&, const SSDdaAngle*, const float*, bool, double*,
-                                               double*, int, hipStream_t);
-template hipError_t dda_launch_fold<float, float>(const SSDdaGeom&, const float*, const float*, const double*, float*,
-                                                  int, hipStream_t);
-template hipError_t dda_launch_fold<double, float>(const SSDdaGeom&, const double*, const double*, const double*,
-                                                   float*, int, hipStream_t);
-template hipError_t dda_launch_fold<double, double>(const SSDdaGeom&, const double*, const double*, const double*,
-                                                    double*, int, hipStream_t);
+template hipError_t dda_launch_transpose<float, float>(const SSDdaGeom&, const SSDdaTransposePlan&, const float*,
+                                                      bool, float*, float*, float*, const double*, float*, int,
+                                                      hipStream_t);
+template hipError_t dda_launch_transpose<double, float>(const SSDdaGeom&, const SSDdaTransposePlan&, const float*,
+                                                       bool, float*, double*, double*, const double*, float*, int,
+                                                       hipStream_t);
+template hipError_t dda_launch_transpose<double, double>(const SSDdaGeom&, const SSDdaTransposePlan&, const float*,
+                                                        bool, float*, double*, double*, const double*, double*, int,
+                                                        hipStream_t);

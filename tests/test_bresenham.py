@@ -13,7 +13,10 @@ CPU (oracle and host logic):
 * the transpose is the adjoint of the Radon, the backprojection of a constant
   sinogram is that constant wherever a ray passes;
 * the library's host build of the device traversal visits exactly the oracle's
-  pixels, in order, with the same lengths (1e-13).
+  pixels, in order, with the same lengths (1e-13);
+* on the angles the transpose evaluates pixel by pixel, every traced length equals
+  the pixel's trapezoid in the ray's distance from the pixel centre (1e-12), the
+  closed form that transpose uses.
 
 GPU (device API, drop-in symbols, pyraft wrappers): rel-L2 <= 1e-5 against the FP64
 oracle (golden and random geometries, rectangular images with non-square pixels,
@@ -145,6 +148,47 @@ def test_radon_of_constant_is_chord_length():
         for r in range(R):
             t = 1.0 + r * (-2.0 / (R - 1))
             assert abs(s[r, a] - clip_len(math.cos(th), math.sin(th), t, *box)) <= 1e-12, (r, a)
+
+
+def _pixel_driven(c, s, hx, hy):
+    """The angles the library's transpose evaluates pixel by pixel: exact geometry
+    (slow-axis drift per move within the pixel width) and not within 1e-7 of an axis."""
+    fast_i = abs(hx * s) <= abs(hy * c)
+    e_step = abs(hy * s / c) if fast_i else abs(hx * c / s)
+    e_cap = abs(hx) if fast_i else abs(hy)
+    return min(abs(c), abs(s)) >= 1e-7 and e_step <= e_cap
+
+
+@pytest.mark.parametrize("geom", [GEOMS[0], GEOMS[2], GEOMS[3], ((17, 29), 19, 37, DEF_IC, (0.0, 1.5, 2 * math.pi, -1.5)),
+                                  ((30, 17), 33, 60, DEF_IC, DEF_SC)],
+                         ids=lambda g: "%dx%d_A%d" % (g[0] + g[2:3]))
+def test_trapezoid_lengths_of_pixel_driven_angles(geom):
+    """On those angles every traced length is the w x h pixel's trapezoid in the
+    distance d = t - (x cos + y sin): Lmax for |d| <= q, linear to 0 at |d| = p."""
+    shape, R, A, ic, sc = geom
+    L, C = shape
+    hx, hy = (ic[2] - ic[0]) / (C - 1), (ic[3] - ic[1]) / (L - 1)
+    w, h = abs(hx), abs(hy)
+    xs, ys = ic[0] + np.arange(C) * hx, ic[1] + np.arange(L) * hy
+    checked = 0
+    for a in range(A):
+        th = sc[0] + a * ((sc[2] - sc[0]) / (A - 1))
+        c, s = math.cos(th), math.sin(th)
+        if not _pixel_driven(c, s, hx, hy):
+            continue
+        p = 0.5 * (w * abs(c) + h * abs(s))
+        q = 0.5 * abs(w * abs(c) - h * abs(s))
+        lmax = min(w / abs(s), h / abs(c))
+        for r in range(R):
+            t = sc[1] + r * ((sc[3] - sc[1]) / (R - 1))
+            ij, ln = O.dda_ray_path(shape, R, A, a, r, ic[:2], ic[2:], sc[:2], sc[2:])
+            got = np.zeros(shape)
+            np.add.at(got, (ij[:, 0], ij[:, 1]), ln)
+            d = np.abs(t - (xs[None, :] * c + ys[:, None] * s))
+            trap = lmax * np.clip((p - d) / (p - q), 0.0, 1.0)
+            assert np.abs(got - trap).max() <= 1e-12, (a, r)
+        checked += 1
+    assert checked >= 3
 
 
 @pytest.mark.parametrize("geom", GEOMS[:4], ids=lambda g: "%dx%d" % g[0])
