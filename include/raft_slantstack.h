@@ -187,6 +187,12 @@ int raft_ss_fbp_filter(raft_ss_plan *plan, const float *d_sino, float *d_out, in
 /* The FP32 convolution kernel g (length R) of lowpassino_fbp for a detector grid
  * t = linspace(tmin, tmax, R), computed in FP64 on the host. */
 int raft_ss_ramp_kernel(int R, double tmin, double tmax, float *host_g);
+/* Kernel of either numpy FBP filter: RAFT_SS_FILTER_COS_RAMP, h = |w| cos(pi w dt)
+ * (pyraft.lowpassino_fbp); RAFT_SS_FILTER_TIKHONOV_RAMP, h = |w| / (1 + param |w|^2)
+ * (pyraft.lowpassino_reg_fbp, pyraft/filters/filters.py:73-111, param = L). */
+#define RAFT_SS_FILTER_COS_RAMP 0
+#define RAFT_SS_FILTER_TIKHONOV_RAMP 1
+int raft_ss_filter_kernel(int R, double tmin, double tmax, int kind, double param, float *host_g);
 /* Circular convolution of nrows contiguous length-R rows with the device kernel d_g. */
 int raft_ss_filter_rows(const float *d_in, float *d_out, const float *d_g, int R, int nrows, void *stream);
 /* filter + backprojection (pyraft/inversion/inversion.py:48-56) */

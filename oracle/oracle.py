@@ -10,7 +10,11 @@ Contents (each restates the cited reference routine):
   ``liboracle.so`` (oracle.c), restating raft/raft_radon_slantstack.c:122-214 and
   raft/raft_backprojection_slantstack.c:121-181, plus the exact adjoint of the
   forward (no reference counterpart).
-* ``lowpassino_fbp`` -- numpy restatement of pyraft/filters/filters.py:36-68.
+* ``lowpassino_fbp`` / ``lowpassino_reg_fbp`` -- numpy restatements of
+  pyraft/filters/filters.py:36-68 and 73-111.
+* ``filter1d_ramp`` -- numpy restatement of raft/raft_filter.cpp:118-213 (the reference
+  filters through FFTW, which this image lacks: parity unpinned beyond the FFT
+  conventions, which tests/test_filters.py checks against a brute-force DFT).
 * ``fbp`` -- filter then backprojection, the composition pyraft/inversion/inversion.py:48-56
   performs (with the CPU slant-stack backprojection in place of the CUDA one).
 * ``em`` / ``osem`` -- the multiplicative EM recurrence of raft/raft_em_gpu.cu:41-69
@@ -424,6 +428,43 @@ def ramp_kernel(R, tl=SINO_TL, br=SINO_BR):
     w = np.linspace(-wc, wc, R)
     h = np.abs(w) * np.cos(np.pi * w * dt)
     return np.fft.ifft(np.fft.fftshift(h)).real
+
+
+def lowpassino_reg_fbp(sino, L, tl=SINO_TL, br=SINO_BR):
+    """pyraft/filters/filters.py:73-111: as lowpassino_fbp with h = |w| / (1 + L |w|^2)."""
+    sino = np.asarray(sino, dtype=np.float64)
+    tmin, tmax = br[1], tl[1]
+    t = np.linspace(tmin, tmax, sino.shape[0])
+    R, V = int(sino.shape[0]), int(sino.shape[1])
+    dt = (t[R - 1] - t[0]) / (R - 1)
+    wc = 1 / (2 * dt)
+    w = np.linspace(-wc, wc, R)
+    h = np.abs(w) / (1.0 + L * np.abs(w) ** 2.0)
+    G = np.fft.fftshift(np.transpose(np.kron(np.ones((V, 1)), h)))
+    return np.fft.ifft(np.fft.fft(sino, axis=0) * G, axis=0).real
+
+
+def filter1d_ramp(sino, cutoff, tl=SINO_TL, br=SINO_BR):
+    """raft/raft_filter.cpp:118-213 restated with numpy's FFT (FFTW is absent here): per
+    column, zero-pad to Z = 2R - 1, real FFT, multiply bins 0..R/2 by ramp/Z (bins
+    R/2+1..R-1 untouched, as the reference's loop bound osize = R/2 + 1 leaves them),
+    unnormalised inverse (FFTW's c2r: numpy's irfft times Z), keep the first R samples."""
+    sino = np.asarray(sino, dtype=np.float64)
+    R = sino.shape[0]
+    F = R // 2 + 1
+    cutoff = cutoff if cutoff <= 1.0 else 1.0
+    delta = 3.14159265358979323846264338327 / (F - 1)
+    delta /= abs((br[1] - tl[1]) / (R - 1))
+    length = int(F * cutoff)
+    filt = np.zeros(F)
+    if length > 0:
+        filt[:min(length, F)] = delta * np.arange(min(length, F))
+    Z = 2 * R - 1
+    xz = np.zeros((Z, sino.shape[1]))
+    xz[:R] = sino
+    X = np.fft.rfft(xz, axis=0)
+    X[:F] *= (filt / Z)[:, None]
+    return (np.fft.irfft(X, n=Z, axis=0) * Z)[:R]
 
 
 def fbp(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR, nthreads=None):

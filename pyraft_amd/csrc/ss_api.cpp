@@ -377,7 +377,15 @@ extern "C" int raft_ss_plan_host_bp_staging(int n, int nrays, int nangles, const
 namespace {
 // g = Re(ifft(fftshift(h))) with h = |w| cos(pi w dt), w = linspace(-wc, wc, R), wc = 1/(2 dt)
 // (pyraft/filters/filters.py:44-55), evaluated in FP64 by a direct real DFT.
-void compute_ramp(int R, double tmin, double tmax, float* gk) {
+void compute_ramp(int R, double tmin, double tmax, float* gk) { ss_filter_kernel(R, tmin, tmax, 0, 0.0, gk); }
+}  // namespace
+
+// The circular-convolution kernel g = Re(ifft(fftshift(h))) of the numpy FBP filters:
+// kind 0, h = |w| cos(pi w dt) (lowpassino_fbp, pyraft/filters/filters.py:49-52); kind 1,
+// h = |w| / (1 + L |w|^2) (lowpassino_reg_fbp, pyraft/filters/filters.py:93-96), with
+// w = linspace(-wc, wc, R), wc = 1/(2 dt).  g is real because s is real: only the even
+// part of the (non-Hermitian) fftshift(h) acts.
+void ss_filter_kernel(int R, double tmin, double tmax, int kind, double param, float* gk) {
   // numpy.linspace(tmin, tmax, R) stores the endpoint exactly, so t[R-1] - t[0] = tmax - tmin
   const double dt = (tmax - tmin) / (R - 1);
   const double wc = 1 / (2 * dt);
@@ -386,7 +394,7 @@ void compute_ramp(int R, double tmin, double tmax, float* gk) {
   const double pi = 3.141592653589793;
   for (int i = 0; i < R; ++i) {
     const double w = (i == R - 1) ? wc : (i * step + (-wc));
-    h[i] = fabs(w) * cos(pi * w * dt);
+    h[i] = kind == 1 ? fabs(w) / (1.0 + param * (fabs(w) * fabs(w))) : fabs(w) * cos(pi * w * dt);
   }
   const int shift = R / 2;  // numpy.fft.fftshift = roll by R//2
   for (int f = 0; f < R; ++f) G[f] = h[((f - shift) % R + R) % R];
@@ -399,6 +407,8 @@ void compute_ramp(int R, double tmin, double tmax, float* gk) {
     gk[m] = (float)(acc / R);
   }
 }
+
+namespace {
 
 int ensure_ramp(raft_ss_plan* p, hipStream_t stream) {
   std::lock_guard<std::mutex> lock(p->ramp_mutex);
@@ -418,10 +428,16 @@ int ensure_ramp(raft_ss_plan* p, hipStream_t stream) {
 // ------------------------------------------------------------------ device-pointer entry points
 
 extern "C" int raft_ss_ramp_kernel(int R, double tmin, double tmax, float* host_g) {
-  if (R < 2 || !host_g) return set_error(RAFT_SS_EINVAL, "ramp kernel needs R >= 2 and an output buffer");
+  return raft_ss_filter_kernel(R, tmin, tmax, RAFT_SS_FILTER_COS_RAMP, 0.0, host_g);
+}
+
+extern "C" int raft_ss_filter_kernel(int R, double tmin, double tmax, int kind, double param, float* host_g) {
+  if (R < 2 || !host_g) return set_error(RAFT_SS_EINVAL, "filter kernel needs R >= 2 and an output buffer");
   if (!std::isfinite(tmin) || !std::isfinite(tmax) || tmin == tmax)
     return set_error(RAFT_SS_EINVAL, "degenerate detector extent");
-  compute_ramp(R, tmin, tmax, host_g);
+  if (kind != RAFT_SS_FILTER_COS_RAMP && kind != RAFT_SS_FILTER_TIKHONOV_RAMP)
+    return set_error(RAFT_SS_EINVAL, "unknown filter kind %d", kind);
+  ss_filter_kernel(R, tmin, tmax, kind, param, host_g);
   return 0;
 }
 

@@ -82,6 +82,9 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // status of a drop-in (void) symbol: kept for raft_ss_legacy_status(), printed on failure
 void ss_report_legacy(const char* fn, int rc);
 
+// FP32 circular-convolution kernel of the numpy FBP filters (kind RAFT_SS_FILTER_*)
+void ss_filter_kernel(int R, double tmin, double tmax, int kind, double param, float* g);
+
 // ------------------------------------------------------------------ host <-> raft_image staging
 
 struct PinnedF32 {

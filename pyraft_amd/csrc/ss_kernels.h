@@ -126,7 +126,8 @@ int ss_batch_grid(int kind, int variant, int mode, int total);
 enum { SS_KIND_GATHER = 0, SS_KIND_TRANSPOSE, SS_KIND_RADON, SS_KIND_FIXUP, SS_KIND_BP };
 
 // ---- FBP row filter (rows of length R, circular convolution with g)
-hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream);
+hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream,
+                                 bool toeplitz = false);
 
 // ---- FP64 reductions / EM helpers
 hipError_t ss_launch_sum_partials(const double* partials, int n, double* dst, hipStream_t stream);

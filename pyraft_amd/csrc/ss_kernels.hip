@@ -1195,16 +1195,23 @@ hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int
 // =====================================================================================
 
 #define RF_ROWS 4
+// out[row][k] = sum_m in[row][m] * taps[k - m + R].  toeplitz = 0: g holds R taps of a
+// circular convolution (taps[j] = g[j mod R]); toeplitz = 1: g holds the 2R Toeplitz
+// taps themselves (entry 0 unused).
 __global__ __launch_bounds__(256) void k_ramp_filter(const float* __restrict__ in, float* __restrict__ out,
-                                                     const float* __restrict__ g, int R, int nrows) {
+                                                     const float* __restrict__ g, int R, int nrows, int toeplitz) {
   extern __shared__ float lds[];
   float* g2 = lds;                          // 2R
   float* rows = lds + ((2 * R + 3) & ~3);   // R x RF_ROWS, interleaved [m][row], 16-byte aligned
   const int row0 = blockIdx.x * RF_ROWS;
-  for (int i = threadIdx.x; i < R; i += 256) {
-    const float v = g[i];
-    g2[i] = v;
-    g2[i + R] = v;
+  if (toeplitz) {
+    for (int i = threadIdx.x; i < 2 * R; i += 256) g2[i] = g[i];
+  } else {
+    for (int i = threadIdx.x; i < R; i += 256) {
+      const float v = g[i];
+      g2[i] = v;
+      g2[i + R] = v;
+    }
   }
   for (int i = threadIdx.x; i < R * RF_ROWS; i += 256) {
     const int rr = i / R, m = i - rr * R;
@@ -1230,11 +1237,12 @@ __global__ __launch_bounds__(256) void k_ramp_filter(const float* __restrict__ i
   }
 }
 
-hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream) {
+hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream,
+                                 bool toeplitz) {
   const size_t lds = (size_t)(((2 * R + 3) & ~3) + RF_ROWS * R) * sizeof(float);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_ramp_filter, dim3((nrows + RF_ROWS - 1) / RF_ROWS), dim3(256), lds, stream, in, out, g, R,
-                     nrows);
+                     nrows, toeplitz ? 1 : 0);
   return hipGetLastError();
 }
 

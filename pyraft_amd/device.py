@@ -242,12 +242,13 @@ def bp_variant_shape(variant):
     return rows.value, stage.value
 
 
-def ramp_kernel(nrays, tmin=-1.0, tmax=1.0):
-    """FP32 convolution kernel (length nrays) of lowpassino_fbp for t = linspace(tmin, tmax, nrays)."""
+def ramp_kernel(nrays, tmin=-1.0, tmax=1.0, kind=0, param=0.0):
+    """FP32 circular-convolution kernel (length nrays) for t = linspace(tmin, tmax, nrays) of
+    lowpassino_fbp (kind 0) or lowpassino_reg_fbp with L = param (kind 1)."""
     import numpy
     g = numpy.zeros(int(nrays), numpy.float32)
-    check(libraft.raft_ss_ramp_kernel(int(nrays), float(tmin), float(tmax),
-                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "raft_ss_ramp_kernel")
+    check(libraft.raft_ss_filter_kernel(int(nrays), float(tmin), float(tmax), int(kind), float(param),
+                                        g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "raft_ss_filter_kernel")
     return g
 
 

@@ -118,6 +118,8 @@ _decl("raft_radon_transpose_slantstack", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
 _decl("raft_radon_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
 _decl("raft_radon_transpose_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
 _decl("raft_backprojection_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
+_decl("raft_filter1d_ramp", [ctypes.c_double, RAFT_IMAGE, _i], None)
+_decl("raft_fbp", [ctypes.c_double, RAFT_IMAGE, RAFT_IMAGE, _i, _i], None)
 _decl("raft_radon_slantstack_withmesh", [RAFT_IMAGE, RAFT_IMAGE, RAFT_VECTOR, RAFT_VECTOR, _i], None)
 _decl("raft_radon_slantstack_view", [RAFT_VECTOR, RAFT_IMAGE, RAFT_VECTOR, ctypes.c_double], None)
 _decl("raft_ss_legacy_status", [], _i)
@@ -156,6 +158,9 @@ _decl("raft_ss_em", [_vp, _vp, _vp, _i, _i, _i, _d4, _vp], _i)
 _decl("raft_ss_phantom", [_d4, _i, _i, _i, _d4, _vp, _vp], _i)
 _decl("raft_ss_ramp_kernel", [_i, ctypes.c_double, ctypes.c_double, _c_float_p], _i)
 _decl("raft_ss_filter_rows", [_vp, _vp, _vp, _i, _i, _vp], _i)
+_decl("raft_ss_filter_kernel", [_i, ctypes.c_double, ctypes.c_double, _i, ctypes.c_double, _c_float_p], _i)
+_decl("raft_ss_filter1d_ramp_taps", [_i, ctypes.c_double, ctypes.c_double, _c_float_p], _i)
+_decl("raft_ss_filter_rows_toeplitz", [_vp, _vp, _vp, _i, _i, _vp], _i)
 _decl("raft_ss_plan_host_bp_staging", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i)
 _decl("raft_ss_bp_variant_shape", [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_set_bp_variant", [_vp, _i], _i)
@@ -347,4 +352,8 @@ EXPORTED_SYMBOLS = (
     "raft_radon_bresenham", "raft_radon_transpose_bresenham", "raft_backprojection_bresenham",
     "raft_dda_plan_create", "raft_dda_plan_destroy", "raft_dda_plan_info", "raft_dda_radon",
     "raft_dda_radon_transpose", "raft_dda_backprojection", "raft_dda_weights", "raft_dda_host_ray",
+    "raft_filter1d_ramp", "raft_fbp", "raft_ss_filter_kernel", "raft_ss_filter1d_ramp_taps",
+    "raft_ss_filter_rows_toeplitz",
 )
+RAFT_BRESENHAM, RAFT_SLANTSTACK = 0, 1          # raft_radon_method (raft/raft_enums.h:4-9)
+FILTER_COS_RAMP, FILTER_TIKHONOV_RAMP = 0, 1    # raft_ss_filter_kernel kinds

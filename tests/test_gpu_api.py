@@ -3,7 +3,7 @@
 Covers the reference's CPU-runnable workload (256 x 256 Shepp-Logan, 180 angles,
 256 bins: Radon, backprojection, FBP) and every drop-in symbol: radon/backprojection
 through raft_image structs (including strided views), radon_gpu, backprojection_gpu,
-fbp_gpu, em_gpu, lowpassino_fbp, radon_transpose, em/make_em/osem.
+fbp_gpu, em_gpu, lowpassino_fbp, radon_transpose, make_em (slant-stack) / osem.
 Tolerance: relative L2 <= 1e-5 against the oracle on the same (FP32-rounded) inputs.
 """
 import math
@@ -111,13 +111,18 @@ def test_em_osem_make_em(pr):
     A = 48
     f = pr.ellipses_phantom(pr.random_ellipses(10, 9), (n, n))
     s = pr.radon(f, (R, A), method=pr.radon_method.SLANTSTACK) * 40.0
-    x = pr.em(s, shape=(n, n), niter=3)
-    ref, _ = O.em(_f32(s), n, 3)
-    assert rel_l2(x, ref) <= REL_L2
-    step = pr.make_em(s, (n, n))
+    # the fused slant-stack iteration (pyraft's em() defaults to the Bresenham pair,
+    # covered in test_bresenham.py)
+    ss = pr.radon_method.SLANTSTACK
+    step = pr.make_em(s, (n, n), projection_method=ss, backprojection_method=ss)
     x1 = step(pr.image(np.ones((n, n))))
     ref1, _ = O.em(_f32(s), n, 1)
     assert rel_l2(x1, ref1) <= REL_L2
+    x = x1
+    for _ in range(2):
+        x = step(x)
+    ref, _ = O.em(_f32(s), n, 3)
+    assert rel_l2(x, ref) <= REL_L2
     xo = pr.osem(s, niter=2, nblocks=4)
     refo, _ = O.osem(_f32(s), n, 2, nsubsets=4)
     assert rel_l2(xo, refo) <= REL_L2
