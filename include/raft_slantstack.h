@@ -105,6 +105,14 @@ void raft_backprojection_slantstack_gpu(float *image, float *sino, int sizeImage
  * with the slant-stack pair; zero projections give a zero ratio instead of Inf/NaN. */
 void raft_em_gpu(float *output, float *sino, int sizeImage, int nrays, int nangles, int niter);
 
+/* Replaces raft_tr_em_gpu (raft/raft_transmission_em.cu:16-66; pyraft/raftypes.py:212-213),
+ * called by pyraft.em_transmission_gpu (pyraft/iterative/em.py:212-248).  Transmission EM
+ * from x0 = 1 with the slant-stack pair and default corners: W = BP(rawsino), then per
+ * iteration x <- x * (BP(flat * exp(-R x)) / W)^2 -- the reference applies its
+ * multiplicative update twice (raft/raft_transmission_em.cu:57-59), kept here; pixels with
+ * W = 0 keep their value instead of the reference's unguarded division. */
+void raft_tr_em_gpu(float *output, float *rawsino, float *flat, int sizeImage, int nrays, int nangles, int niter);
+
 /* Replaces raft_image_phantom_fromdesc (raft/raft_image_phantom_fromdesc.h:26;
  * pyraft/raftypes.py:138) used by pyraft.shepplogan (pyraft/phantom/phantom.py:49).
  * Adds into the image.  A description with < 6 columns is an error (the reference
@@ -197,6 +205,11 @@ int raft_ss_filter_kernel(int R, double tmin, double tmax, int kind, double para
 int raft_ss_filter_rows(const float *d_in, float *d_out, const float *d_g, int R, int nrows, void *stream);
 /* filter + backprojection (pyraft/inversion/inversion.py:48-56) */
 int raft_ss_fbp(raft_ss_plan *plan, const float *d_sino, float *d_image, int nslices, void *stream);
+
+/* Transmission EM of raft_tr_em_gpu on device buffers: d_raw and d_flat [s][R][A], d_x the
+ * starting image on entry and the estimate on exit. */
+int raft_ss_tr_em(raft_ss_plan *plan, const float *d_raw, const float *d_flat, float *d_x, int niter, int nslices,
+                  void *stream);
 
 /* Ordered-subsets EM (nsubsets = 1: the EM of raft/raft_em_gpu.cu:41-69).  d_x is the
  * starting image on entry and the estimate on exit.  Per subset: forward projection

@@ -146,6 +146,11 @@ def ref():
         _ref.raft_radon_slantstack_withmesh.restype = None
         _ref.raft_radon_slantstack_view.argtypes = [RaftVector, RaftImage, RaftVector, ctypes.c_double]
         _ref.raft_radon_slantstack_view.restype = None
+        _ref.harness_image_write.argtypes = [ctypes.c_char_p, RaftImage]
+        _ref.harness_image_write.restype = ctypes.c_int
+        _ref.harness_image_read.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+        _ref.harness_image_read.restype = ctypes.c_int
         _ref.harness_radon_bresenham.argtypes = [RaftImage, RaftImage]
         _ref.harness_radon_bresenham.restype = None
         _ref.harness_radon_transpose_bresenham.argtypes = [RaftImage, RaftImage]
@@ -370,6 +375,23 @@ def ref_backprojection_bresenham(sino, shape, img_tl=IMG_TL, img_br=IMG_BR, sino
     return img
 
 
+def ref_image_write(path, img, tl, br):
+    """The reference's raft_image_write (raft/raft_image.c:135-151) to a file."""
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    return ref().harness_image_write(str(path).encode(), _image(img, tl, br))
+
+
+def ref_image_read(path, cap=1 << 22):
+    """The reference's raft_image_read (raft/raft_image.c:5-38): (values, corners) or None."""
+    out = np.zeros(cap)
+    shape = np.zeros(2, np.int32)
+    corners = np.zeros(4)
+    rc = ref().harness_image_read(str(path).encode(), out.ctypes.data, cap, shape.ctypes.data, corners.ctypes.data)
+    if rc:
+        return None
+    return out[:shape[0] * shape[1]].reshape(shape[0], shape[1]).copy(), tuple(corners)
+
+
 def ref_radon(img, nrays, nangles, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
               nthreads=1):
     img = np.ascontiguousarray(img, dtype=np.float64)
@@ -470,6 +492,25 @@ def filter1d_ramp(sino, cutoff, tl=SINO_TL, br=SINO_BR):
 def fbp(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR, nthreads=None):
     filt = lowpassino_fbp(sino, sino_tl, sino_br)
     return backprojection(filt, n, img_tl, img_br, sino_tl, sino_br, nthreads=nthreads)
+
+
+# ---------------------------------------------------------------- transmission EM
+
+def tr_em(raw, flat, n, niter, x0=None, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR):
+    """raft/raft_transmission_em.cu:16-66 on the slant-stack pair: W = BP(raw), then
+    x <- (x r) r with r = BP(flat exp(-R x)) / W (the update applied twice, as the
+    reference does); pixels with W = 0 keep their value."""
+    R, A = raw.shape
+    geo = dict(img_tl=img_tl, img_br=img_br, sino_tl=sino_tl, sino_br=sino_br)
+    x = np.ones((n, n)) if x0 is None else np.array(x0, dtype=np.float64)
+    w = backprojection(np.asarray(raw, np.float64), n, **geo)
+    ok = w != 0
+    for _ in range(niter):
+        p = np.asarray(flat, np.float64) * np.exp(-radon(x, R, A, **geo))
+        b = backprojection(p, n, **geo)
+        r = np.where(ok, b / np.where(ok, w, 1.0), 1.0)
+        x = (x * r) * r
+    return x
 
 
 # ---------------------------------------------------------------- EM / OSEM

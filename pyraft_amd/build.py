@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("PYRAFT_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["ss_kernels.hip", "dda_kernels.hip"]
-CXX_SOURCES = ["ss_api.cpp", "ss_batch.cpp", "dda_api.cpp", "filter_api.cpp"]
+CXX_SOURCES = ["ss_api.cpp", "ss_batch.cpp", "dda_api.cpp", "filter_api.cpp", "io_api.cpp"]
 
 HIPCC_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

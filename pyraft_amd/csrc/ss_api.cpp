@@ -639,6 +639,45 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   return 0;
 }
 
+// Transmission EM (raft/raft_transmission_em.cu:16-66): W = BP(raw), then per iteration
+// x <- x (BP(flat exp(-R x)) / W)^2.
+extern "C" int raft_ss_tr_em(raft_ss_plan* p, const float* d_raw, const float* d_flat, float* d_x, int niter,
+                             int nslices, void* stream_) {
+  int rc = check_plan(p, nslices);
+  if (rc) return rc;
+  if ((rc = check_uniform(p, "tr_em"))) return rc;
+  if (!d_raw || !d_flat || !d_x) return set_error(RAFT_SS_EINVAL, "null device pointer");
+  if (niter < 0) return set_error(RAFT_SS_EINVAL, "niter must be >= 0");
+  const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
+  hipStream_t stream = as_stream(stream_);
+  const SSDevPlan dp = p->dev();
+  const size_t sino_sz = (size_t)nslices * R * A;
+  const size_t img_sz = (size_t)nslices * n * n;
+  const size_t layout = (size_t)nslices * n * p->g.pitch;
+  DevBuf ws;
+  HIP_TRY(ws.alloc(sizeof(float2) * 2 * layout + sizeof(float) * (3 * sino_sz + 2 * img_sz), stream));
+  float2* gA = ws.as<float2>();
+  float2* gB = gA + layout;
+  float* rawT = reinterpret_cast<float*>(gB + layout);
+  float* flatT = rawT + sino_sz;
+  float* pT = flatT + sino_sz;
+  float* w = pT + sino_sz;
+  float* b = w + img_sz;
+  SSEmUpdateArgs none{nullptr, nullptr};
+  SSEmRatioArgs no_ratio{nullptr, nullptr};
+  HIP_TRY(ss_launch_sino_to_t(d_raw, rawT, R, A, nslices, stream));
+  HIP_TRY(ss_launch_sino_to_t(d_flat, flatT, R, A, nslices, stream));
+  HIP_TRY(ss_launch_backprojection(dp, rawT, 0, A, nslices, SS_BP_PLAIN, w, none, stream));
+  for (int it = 0; it < niter; ++it) {
+    HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
+    HIP_TRY(ss_launch_radon(dp, gA, gB, 0, A, nslices, SS_OUT_SINOT, pT, no_ratio, stream));
+    HIP_TRY(ss_launch_tr_expected(pT, flatT, sino_sz, stream));
+    HIP_TRY(ss_launch_backprojection(dp, pT, 0, A, nslices, SS_BP_PLAIN, b, none, stream));
+    HIP_TRY(ss_launch_tr_update(d_x, b, w, img_sz, stream));
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------------ phantoms
 
 static int upload_ellipses(const raft_matrix& desc, std::vector<SSEllipse>& ell) {
@@ -854,7 +893,8 @@ int host_radon_transpose(const raft_image& sino, const raft_image& img) {
 const double kDefaultImg[4] = {-1.0, 1.0, 1.0, -1.0};
 const double kDefaultSino[4] = {0.0, 1.0, 3.14159265358979323846, -1.0};
 
-int host_f32_op(int which, float* h_out, const float* h_in, int n, int R, int A, int niter) {
+int host_f32_op(int which, float* h_out, const float* h_in, int n, int R, int A, int niter,
+                const float* h_in2 = nullptr) {
   if (!h_out || !h_in) return set_error(RAFT_SS_EINVAL, "null host pointer");
   std::shared_ptr<raft_ss_plan> plan;
   int rc = cached_plan(n, R, A, kDefaultImg, kDefaultSino, &plan);
@@ -871,10 +911,20 @@ int host_f32_op(int which, float* h_out, const float* h_in, int n, int R, int A,
     HIP_TRY(hipMemcpyAsync(db.p, h_in, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
     if ((rc = raft_ss_backprojection(plan.get(), db.as<float>(), da.as<float>(), 1, nullptr))) return rc;
     HIP_TRY(hipMemcpyAsync(h_out, da.p, sizeof(float) * nimg, hipMemcpyDeviceToHost, nullptr));
-  } else {  // em from ones
+  } else if (which == 2) {  // em from ones
     HIP_TRY(hipMemcpyAsync(db.p, h_in, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
     HIP_TRY(ss_launch_fill(da.as<float>(), 1.0f, nimg, nullptr));
     if ((rc = raft_ss_em(plan.get(), db.as<float>(), da.as<float>(), niter, 1, 1, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, da.p, sizeof(float) * nimg, hipMemcpyDeviceToHost, nullptr));
+  } else {  // transmission em from ones: h_in = raw sinogram, h_in2 = flat field
+    if (!h_in2) return set_error(RAFT_SS_EINVAL, "null host pointer");
+    DevBuf dflat;
+    HIP_TRY(dflat.alloc(sizeof(float) * nsino, nullptr));
+    HIP_TRY(hipMemcpyAsync(db.p, h_in, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
+    HIP_TRY(hipMemcpyAsync(dflat.p, h_in2, sizeof(float) * nsino, hipMemcpyHostToDevice, nullptr));
+    HIP_TRY(ss_launch_fill(da.as<float>(), 1.0f, nimg, nullptr));
+    if ((rc = raft_ss_tr_em(plan.get(), db.as<float>(), dflat.as<float>(), da.as<float>(), niter, 1, nullptr)))
+      return rc;
     HIP_TRY(hipMemcpyAsync(h_out, da.p, sizeof(float) * nimg, hipMemcpyDeviceToHost, nullptr));
   }
   HIP_TRY(hipStreamSynchronize(nullptr));
@@ -932,6 +982,11 @@ extern "C" void raft_radon_slantstack_view(raft_vector view, raft_image phantom,
   if (view.size < t.size) rc = set_error(RAFT_SS_EINVAL, "view vector shorter than t (%d < %d)", view.size, t.size);
   if (!rc) rc = host_radon_mesh(phantom, out, t, th);
   report("raft_radon_slantstack_view", rc);
+}
+
+extern "C" void raft_tr_em_gpu(float* output, float* rawsino, float* flat, int sizeImage, int nrays, int nangles,
+                               int niter) {
+  report("raft_tr_em_gpu", host_f32_op(3, output, rawsino, sizeImage, nrays, nangles, niter, flat));
 }
 
 extern "C" void raft_radon_slantstack_gpu(float* h_output, float* h_input, int sizeImage, int nrays, int nangles) {

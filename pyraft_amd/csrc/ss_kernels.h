@@ -132,6 +132,9 @@ hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, in
 // ---- FP64 reductions / EM helpers
 hipError_t ss_launch_sum_partials(const double* partials, int n, double* dst, hipStream_t stream);
 hipError_t ss_launch_fill(float* p, float v, size_t n, hipStream_t stream);
+// transmission EM elementwise steps
+hipError_t ss_launch_tr_expected(float* p, const float* flat, size_t n, hipStream_t stream);
+hipError_t ss_launch_tr_update(float* x, const float* b, const float* w, size_t n, hipStream_t stream);
 
 // ---- phantoms (FP64)
 struct SSEllipse {  // value, a, b, x0, y0, cos(rot), sin(rot)

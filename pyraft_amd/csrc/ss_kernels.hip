@@ -1247,6 +1247,40 @@ hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, in
 }
 
 // =====================================================================================
+// transmission EM (raft/raft_transmission_em.cu:52-61)
+// =====================================================================================
+
+// expected counts: p <- flat * exp(-p) (raft/raft_radon_gpu_function.cu:31-39 with expo = -1,
+// then the product with the flat field)
+__global__ __launch_bounds__(256) void k_tr_expected(float* __restrict__ p, const float* __restrict__ flat, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = flat[i] * expf(-p[i]);
+}
+
+// x <- (x * r) * r, r = b / w: the reference applies its multiplicative update twice;
+// pixels with w = 0 keep their value (the reference divides unguarded)
+__global__ __launch_bounds__(256) void k_tr_update(float* __restrict__ x, const float* __restrict__ b,
+                                                   const float* __restrict__ w, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float wi = w[i];
+    if (wi != 0.f) {
+      const float r = b[i] / wi;
+      x[i] = (x[i] * r) * r;
+    }
+  }
+}
+
+hipError_t ss_launch_tr_expected(float* p, const float* flat, size_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tr_expected, dim3(2048), dim3(256), 0, stream, p, flat, n);
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_tr_update(float* x, const float* b, const float* w, size_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tr_update, dim3(2048), dim3(256), 0, stream, x, b, w, n);
+  return hipGetLastError();
+}
+
+// =====================================================================================
 // small helpers
 // =====================================================================================
 

@@ -23,7 +23,7 @@ import numpy
 from .raftypes import check_legacy, image, libraft
 from .sinogram import radon_method
 
-__all__ = ["make_em", "em", "em_gpu", "osem"]
+__all__ = ["make_em", "em", "em_gpu", "em_transmission_gpu", "osem"]
 
 
 def _corners(arr, default_tl, default_br):
@@ -127,6 +127,30 @@ def em_gpu(*args):
                         sino_buff.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), img_size, rays, angles, niter)
     check_legacy("raft_em_gpu")
     return img.reshape([img_size, img_size])
+
+
+def em_transmission_gpu(*args):
+    """em_transmission_gpu(s, flat, dark, img_size=R, niter=20) -> float64 image
+    (pyraft/iterative/em.py:212-248): raft_tr_em_gpu's transmission EM from x = 1.  pyraft
+    passes the dark field to a function that takes none (raft/raft_transmission_em.cu:16),
+    so the call cannot succeed there; here dark is accepted and unused, as that function
+    would use it."""
+    if len(args) < 3:
+        raise TypeError('Not enough arguments!')
+    s, f = args[0], args[1]
+    rays, angles = s.shape[0], s.shape[1]
+    img_size = int(args[3]) if len(args) > 3 else rays
+    niter = int(args[4]) if len(args) > 4 else 20
+    sino_buff = numpy.ascontiguousarray(s, dtype=numpy.float32)
+    flat_buff = numpy.ascontiguousarray(f, dtype=numpy.float32)
+    if flat_buff.shape != sino_buff.shape:
+        raise ValueError("flat field must have the sinogram's shape %s" % (sino_buff.shape,))
+    img = numpy.zeros(img_size * img_size, dtype=numpy.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    libraft.raft_tr_em_gpu(img.ctypes.data_as(fp), sino_buff.ctypes.data_as(fp), flat_buff.ctypes.data_as(fp),
+                           img_size, rays, angles, niter)
+    check_legacy("raft_tr_em_gpu")
+    return img.reshape([img_size, img_size]).astype('float64')
 
 
 def osem(sino, shape=None, niter=5, nblocks=4, **kwargs):

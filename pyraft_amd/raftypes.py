@@ -20,7 +20,7 @@ import sysconfig
 
 import numpy
 
-__all__ = ["nthreads", "libraft", "LIBRARY_PATH", "HIP_RUNTIME", "RAFT_MATRIX", "RAFT_IMAGE", "RAFT_VECTOR", "RaftError",
+__all__ = ["image_read", "image_write", "nthreads", "libraft", "LIBRARY_PATH", "HIP_RUNTIME", "RAFT_MATRIX", "RAFT_IMAGE", "RAFT_VECTOR", "RaftError",
            "check", "check_legacy", "image", "make_RAFT_MATRIX", "make_RAFT_IMAGE", "make_RAFT_VECTOR",
            "EXPORTED_SYMBOLS"]
 
@@ -118,6 +118,13 @@ _decl("raft_radon_transpose_slantstack", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
 _decl("raft_radon_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
 _decl("raft_radon_transpose_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
 _decl("raft_backprojection_bresenham", [RAFT_IMAGE, RAFT_IMAGE, _i], None)
+_decl("raft_tr_em_gpu", [_c_float_p, _c_float_p, _c_float_p, _i, _i, _i, _i], None)
+_decl("raft_matrix_read", [_vp], RAFT_MATRIX)
+_decl("raft_matrix_write", [RAFT_MATRIX, _vp], _i)
+_decl("raft_matrix_destroy", [ctypes.POINTER(RAFT_MATRIX)], None)
+_decl("raft_image_read", [_vp], RAFT_IMAGE)
+_decl("raft_image_write", [RAFT_IMAGE, _vp], _i)
+_decl("raft_image_destroy", [ctypes.POINTER(RAFT_IMAGE)], None)
 _decl("raft_filter1d_ramp", [ctypes.c_double, RAFT_IMAGE, _i], None)
 _decl("raft_fbp", [ctypes.c_double, RAFT_IMAGE, RAFT_IMAGE, _i, _i], None)
 _decl("raft_radon_slantstack_withmesh", [RAFT_IMAGE, RAFT_IMAGE, RAFT_VECTOR, RAFT_VECTOR, _i], None)
@@ -155,6 +162,7 @@ _decl("raft_ss_radon_transpose_range", [_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i)
 _decl("raft_ss_fbp_filter", [_vp, _vp, _vp, _i, _i, _vp], _i)
 _decl("raft_ss_fbp", [_vp, _vp, _vp, _i, _vp], _i)
 _decl("raft_ss_em", [_vp, _vp, _vp, _i, _i, _i, _d4, _vp], _i)
+_decl("raft_ss_tr_em", [_vp, _vp, _vp, _vp, _i, _i, _vp], _i)
 _decl("raft_ss_phantom", [_d4, _i, _i, _i, _d4, _vp, _vp], _i)
 _decl("raft_ss_ramp_kernel", [_i, ctypes.c_double, ctypes.c_double, _c_float_p], _i)
 _decl("raft_ss_filter_rows", [_vp, _vp, _vp, _i, _i, _vp], _i)
@@ -353,7 +361,54 @@ EXPORTED_SYMBOLS = (
     "raft_dda_plan_create", "raft_dda_plan_destroy", "raft_dda_plan_info", "raft_dda_radon",
     "raft_dda_radon_transpose", "raft_dda_backprojection", "raft_dda_weights", "raft_dda_host_ray",
     "raft_filter1d_ramp", "raft_fbp", "raft_ss_filter_kernel", "raft_ss_filter1d_ramp_taps",
-    "raft_ss_filter_rows_toeplitz",
+    "raft_ss_filter_rows_toeplitz", "raft_tr_em_gpu", "raft_ss_tr_em", "raft_matrix_read", "raft_matrix_write",
+    "raft_matrix_destroy", "raft_image_read", "raft_image_write", "raft_image_destroy",
 )
 RAFT_BRESENHAM, RAFT_SLANTSTACK = 0, 1          # raft_radon_method (raft/raft_enums.h:4-9)
 FILTER_COS_RAMP, FILTER_TIKHONOV_RAMP = 0, 1    # raft_ss_filter_kernel kinds
+
+
+# ---------------------------------------------------------------- binary image files
+
+_libc = ctypes.CDLL(None)
+_libc.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+_libc.fopen.restype = ctypes.c_void_p
+_libc.fclose.argtypes = [ctypes.c_void_p]
+_libc.fclose.restype = ctypes.c_int
+
+
+def image_write(img, path):
+    """Write a pyraft image in the reference's binary layout (raft_image_write,
+    raft/raft_image.c:135-151): int32 lines, int32 columns, FP64 values, 4 FP64 corners."""
+    a = numpy.asarray(img, dtype=numpy.float64)
+    tl = tuple(getattr(img, 'top_left', (-1.0, 1.0)))
+    br = tuple(getattr(img, 'bottom_right', (1.0, -1.0)))
+    fp = _libc.fopen(os.fsencode(path), b"wb")
+    if not fp:
+        raise OSError("cannot open %s for writing" % path)
+    try:
+        rc = libraft.raft_image_write(make_RAFT_IMAGE(a, tl, br), fp)
+    finally:
+        _libc.fclose(fp)
+    if rc:
+        raise OSError("raft_image_write failed (%d) on %s" % (rc, path))
+
+
+def image_read(path):
+    """Read a binary raft image file (raft_image_read, raft/raft_image.c:5-38) into a
+    pyraft image with its corners."""
+    fp = _libc.fopen(os.fsencode(path), b"rb")
+    if not fp:
+        raise OSError("cannot open %s" % path)
+    try:
+        im = libraft.raft_image_read(fp)
+    finally:
+        _libc.fclose(fp)
+    if not im.data.p_data:
+        raise OSError("%s is not a complete raft image file" % path)
+    try:
+        n = im.data.lines * im.data.columns
+        vals = numpy.ctypeslib.as_array(im.data.p_data, shape=(n,)).reshape(im.data.lines, im.data.columns).copy()
+        return image(vals, top_left=(im.tl_x, im.tl_y), bottom_right=(im.br_x, im.br_y))
+    finally:
+        libraft.raft_image_destroy(ctypes.byref(im))

@@ -7,6 +7,8 @@ raft/raft_image_phantom_fromdesc.c, raft/raft_radon_fromdesc.c, raft/raft_ddaite
 their helpers), on small seeded inputs.  The fixture holds inputs and outputs only.
 
     make -C oracle ref && python tests/golden/make_golden.py
+
+It also writes image_written_by_reference.raft with the reference's raft_image_write.
 """
 import math
 import os
@@ -115,6 +117,12 @@ def main():
         out[name + "/radon"] = O.ref_radon_bresenham(img, R, A, **geo)
         out[name + "/transpose"] = O.ref_radon_transpose_bresenham(sino, shape, **geo)
         out[name + "/backprojection"] = O.ref_backprojection_bresenham(sino, shape, **geo)
+    # a binary image file written by the reference's raft_image_write (raft/raft_image.c:135-151)
+    io_img = rng.random((5, 7))
+    out["io/values"] = io_img
+    out["io/corners"] = np.array([-0.5, 2.0, 1.5, -1.0])
+    rc = O.ref_image_write(os.path.join(HERE, "image_written_by_reference.raft"), io_img, (-0.5, 2.0), (1.5, -1.0))
+    assert rc == 0
     path = os.path.join(HERE, "slantstack_golden.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
