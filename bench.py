@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-em", action="store_true", help="skip the EM leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-slices leg")
+    ap.add_argument("--no-dda", action="store_true", help="skip the line-traversal (Bresenham) leg")
     ap.add_argument("--batch-slices", type=int, default=256, help="slices in the batched leg")
     ap.add_argument("--batch-iters", type=int, default=20, help="OSEM iterations in the batched leg")
     ap.add_argument("--calibration-copy", action="store_true",
@@ -236,6 +237,36 @@ def batched_leg(dev, nslices, niter, nsubsets=8, seed=4):
     }
 
 
+def dda_leg(dev, img, n, R, A, reps=3):
+    """pyraft's default projector pair (line traversal) on the headline image and angles:
+    event-timed Radon, transpose and normalised backprojection (plan and weights built
+    beforehand)."""
+    print("bench: line-traversal pair", file=sys.stderr, flush=True)
+    import torch
+    from pyraft_amd.device import DdaPlan
+    plan = DdaPlan((n, n), R, A)
+    y = torch.rand((R, A), device=dev, generator=torch.Generator(dev).manual_seed(9))
+    out = {"workload": "%d^2 image, %d rays x %d angles, theta in [0, pi] (A - 1 steps), FP64 traversal, "
+                       "FP32 taps" % (n, R, A)}
+    info = plan.info()
+    out["traced_angles"] = info["traced_angles"]
+    ups = float(n) * n * A
+    for name, fn in (("radon", lambda: plan.radon(img)), ("transpose", lambda: plan.radon_transpose(y)),
+                     ("backprojection", lambda: plan.backprojection(y))):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name + "_ms"] = round(ms, 3)
+        out[name + "_gups"] = round(ups / ms / 1e6, 1)
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -431,6 +462,9 @@ def main():
     result["autograd"] = {"workload": "64 x 512^2, 720 angles: Radon forward + exact-transpose backward "
                                       "(+ torch elementwise/sum)",
                           "ms_per_step": round(ta * 1e3, 3), "gups": round(2 * aupd / ta / 1e9, 2)}
+
+    if not args.no_dda:
+        result["bresenham"] = dda_leg(dev, img, n, R, A)
 
     if not args.no_batch:
         result["batched"] = batched_leg(dev, args.batch_slices, args.batch_iters)
