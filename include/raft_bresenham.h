@@ -74,8 +74,9 @@ int raft_dda_backprojection(raft_dda_plan *plan, const float *sino, float *img, 
  * plan and kept) */
 int raft_dda_weights(raft_dda_plan *plan, float *out, void *stream);
 
-/* The pixels ray `ray` of angle `angle` visits, in order, as the device kernels trace
- * them, evaluated on the host (no device needed; for checking index logic):
+/* The pixels ray `ray` of angle `angle` visits, in order, as the device Radon traces
+ * them (FP64 drift where the transpose traces rays too, Q32 drift on the other angles),
+ * evaluated on the host (no device needed; for checking index logic):
  * ij[2k], ij[2k+1] = line, column and len[k] = intersection length of the k-th pixel.
  * Returns the number of pixels (at most cap are written) or a RAFT_SS_E* code. */
 int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, const double *img_corners,

@@ -99,6 +99,10 @@ SSDdaAngle make_angle(const SSDdaGeom& g, double theta) {
   }
   a.axis = a.s == 0.0 ? 1 : (a.c == 0.0 ? 2 : 0);
   a.k_len = a.e_step > 0.0 ? a.len_full / a.e_step : 0.0;
+  a.fixed = 0;
+  a.step_fix = 0;
+  a.inv_cap = 0.0;
+  a.len_step = 0.0;
   return a;
 }
 
@@ -125,6 +129,17 @@ bool pixel_driven(const SSDdaGeom& g, const SSDdaAngle& a, SSDdaGatherAngle* out
   if (!isfinite(ga.inv_pq) || !isfinite(ga.half) || ga.k > 1 << 20) return false;
   *out = ga;
   return true;
+}
+
+// Q32 drift of an angle whose traversal is exact geometry (e_step <= e_cap)
+void set_fixed_drift(SSDdaAngle& a) {
+  const double frac = a.e_step / a.e_cap * 4294967296.0;
+  a.fixed = 0;
+  if (!(frac >= 1.0) || !(frac < 4294967295.0)) return;
+  a.step_fix = (unsigned)llround(frac);
+  a.inv_cap = 4294967296.0 / a.e_cap;
+  a.len_step = a.len_full / (double)a.step_fix;
+  a.fixed = 1;
 }
 
 int check_dda(const raft_dda_plan* p, int nslices) {
@@ -208,10 +223,13 @@ extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, in
   }
   for (int a = 0; a < nangles; ++a) {
     SSDdaGatherAngle ga;
-    if (pixel_driven(g, p->h_angles[a], &ga, a))
+    SSDdaAngle& an = p->h_angles[a];
+    if (pixel_driven(g, an, &ga, a)) {
       p->h_gather.push_back(ga);
-    else
+      set_fixed_drift(an);
+    } else {
       p->h_walk.push_back(a);
+    }
   }
   HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSDdaAngle) * nangles));
   HIP_TRY(hipMalloc(&p->d_gather, sizeof(SSDdaGatherAngle) * (p->h_gather.size() + 1)));
@@ -281,7 +299,8 @@ extern "C" int raft_dda_weights(raft_dda_plan* p, float* out, void* stream_) {
   return 0;
 }
 
-// The traversal of one ray exactly as k_dda_radon runs it, evaluated on the host (for
+// The traversal of one ray exactly as k_dda_radon runs it (Q32 drift on the angles where
+// the traversal is exact geometry, FP64 drift on the others), evaluated on the host (for
 // checking index logic without a device): (line, column) pairs and lengths of the
 // visited pixels in order.  Returns the pixel count (at most cap are written).
 extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, const double* img_corners,
@@ -291,10 +310,13 @@ extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, co
     return ss_set_error(RAFT_SS_EINVAL, "bad geometry or ray index");
   const SSDdaGeom g = make_geom(lines, cols, nrays, nangles, img_corners, sino_corners);
   const double dth = (sino_corners[2] - sino_corners[0]) / (nangles - 1);
-  const SSDdaAngle a = make_angle(g, sino_corners[0] + (angle * dth));
+  SSDdaAngle a = make_angle(g, sino_corners[0] + (angle * dth));
+  SSDdaGatherAngle ga;
+  if (pixel_driven(g, a, &ga, angle)) set_fixed_drift(a);
   const int nf = a.fast_i ? lines : cols, ns = a.fast_i ? cols : lines;
   const double t = g.t0 + (ray * g.dt);
   SSDdaRay r = dda_ray_start(g, a, t);
+  unsigned frac = a.fixed ? (unsigned)fmin(fmax(r.e * a.inv_cap, 0.0), 4294967295.0) : 0u;
   int count = 0;
   auto visit = [&](void) {
     if (count < cap) {
@@ -315,7 +337,15 @@ extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, co
       if (!inside()) break;
       visit();
     }
-    dda_fast_move(a, r);
+    if (a.fixed) {
+      r.f = dda_step(r.f, a.dfast);
+      const unsigned prev = frac;
+      frac += a.step_fix;
+      r.slow_next = frac < prev;
+      r.len = r.slow_next ? (double)(0u - prev) * a.len_step : a.len_full;
+    } else {
+      dda_fast_move(a, r);
+    }
   }
   return count;
 }

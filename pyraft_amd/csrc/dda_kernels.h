@@ -55,6 +55,15 @@ struct SSDdaAngle {
   int fast_i;               // 1: the line index is the fast axis
   int dfast, dslow;
   int axis;                 // 1: sin == 0, 2: cos == 0, 0: oblique
+  // Q32 drift for the Radon on angles where the traversal is exact pixel geometry (see
+  // SSDdaGatherAngle): the drift e / e_cap as a 32-bit fraction, advanced by step_fix per
+  // fast move; a carry out of the fraction is the slow move, and the crossing length is
+  // (2^32 - fraction before the move) * len_step.  Decisions agree with the FP64 drift
+  // except within ~N 2^-32 of a pixel edge, where the lengths moved are ~N 2^-32 len_full.
+  int fixed;
+  unsigned step_fix;        // round(e_step / e_cap * 2^32), <= 2^32 - 1
+  double inv_cap;           // 2^32 / e_cap
+  double len_step;          // len_full / step_fix
 };
 
 // Pixel-driven form of the transpose for one angle.  Where the traversal is exact

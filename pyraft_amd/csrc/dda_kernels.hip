@@ -29,35 +29,63 @@ __device__ __forceinline__ double dda_ray_t(const SSDdaGeom& g, int r) {
 
 // ---------------------------------------------------------------- Radon (gather)
 
+// The wave's fast index at which its first ray enters the image (all lanes share the
+// angle, so the direction dfast): lanes wait until the wave's index reaches theirs and
+// then advance with it, so at every step the active lanes sit on one line of the
+// layout and their 64 taps fall in a few cache lines.
+__device__ __forceinline__ int dda_wave_start(bool alive, int f, int dfast) {
+  int key = alive ? (dfast > 0 ? f : -f) : 0x7fffffff;
+  for (int off = 32; off > 0; off >>= 1) key = min(key, __shfl_xor(key, off));
+  return dfast > 0 ? key : -key;
+}
+
 __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle* __restrict__ angles,
                                                    const float* __restrict__ img, const float* __restrict__ imgT,
                                                    float* __restrict__ sino) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int ai = blockIdx.y;
   const size_t slice = blockIdx.z;
-  if (r >= g.nrays) return;
+  const bool lane_ok = r < g.nrays;
   const SSDdaAngle a = angles[ai];
   const int nf = a.fast_i ? g.lines : g.cols;
   const int ns = a.fast_i ? g.cols : g.lines;
   const float* __restrict__ src = (a.fast_i ? img : imgT) + slice * (size_t)g.lines * g.cols;
-  SSDdaRay ray = dda_ray_start(g, a, dda_ray_t(g, r));
+  SSDdaRay ray = dda_ray_start(g, a, dda_ray_t(g, lane_ok ? r : 0));
+  bool alive = lane_ok && dda_inside(ray.f, ray.s, nf, ns);
+  // Q32 drift (exact-geometry angles): the fraction's carry is the slow move
+  unsigned frac = a.fixed ? (unsigned)fmin(fmax(ray.e * a.inv_cap, 0.0), 4294967295.0) : 0u;
   // FP64 sum: the lengths are FP64 already, so a tap costs one convert and one FMA as
   // in FP32, without FP32's N*eps growth over a ray of N equal terms
   double acc = 0.0;
-  // every iteration advances the fast index by one: at most nf + 1 of them see the image
-  for (int it = 0; it <= nf; ++it) {
-    if (!dda_inside(ray.f, ray.s, nf, ns)) break;
-    acc = fma((double)src[(size_t)ray.f * ns + ray.s], ray.len, acc);
-    if (ray.slow_next) {
-      ray.s = dda_step(ray.s, a.dslow);
-      ray.len = a.len_full - ray.len;
-      ray.slow_next = 0;
-      if (!dda_inside(ray.f, ray.s, nf, ns)) break;
+  int F = dda_wave_start(alive, ray.f, a.dfast);
+  // lanes start at most nf moves apart and then take at most nf + 1 moves each
+  for (int it = 0; it <= 2 * nf + 1; ++it) {
+    if (__ballot(alive) == 0) break;
+    if (alive && ray.f == F) {
       acc = fma((double)src[(size_t)ray.f * ns + ray.s], ray.len, acc);
+      if (ray.slow_next) {
+        ray.s = dda_step(ray.s, a.dslow);
+        ray.len = a.len_full - ray.len;
+        ray.slow_next = 0;
+        alive = dda_inside(ray.f, ray.s, nf, ns);
+        if (alive) acc = fma((double)src[(size_t)ray.f * ns + ray.s], ray.len, acc);
+      }
+      if (alive) {
+        if (a.fixed) {
+          ray.f = dda_step(ray.f, a.dfast);
+          const unsigned prev = frac;
+          frac += a.step_fix;
+          ray.slow_next = frac < prev;  // carry
+          ray.len = ray.slow_next ? (double)(0u - prev) * a.len_step : a.len_full;
+        } else {
+          dda_fast_move(a, ray);
+        }
+        alive = dda_inside(ray.f, ray.s, nf, ns);
+      }
     }
-    dda_fast_move(a, ray);
+    F += a.dfast;
   }
-  sino[(slice * g.nrays + r) * g.nangles + ai] = (float)acc;
+  if (lane_ok) sino[(slice * g.nrays + r) * g.nangles + ai] = (float)acc;
 }
 
 // ---------------------------------------------------------------- transpose (scatter)

@@ -224,16 +224,33 @@ def host_ray(lib, shape, R, A, a, r, ic, sc):
 
 @pytest.mark.parametrize("geom", GEOMS + [((64, 64), 64, 181, DEF_IC, DEF_SC)], ids=lambda g: "%dx%d_A%d" % (g[0] + g[2:3]))
 def test_device_traversal_matches_oracle(lib, geom):
-    """The kernel's traversal (host build) against the oracle, every ray."""
+    """The kernel's traversal (host build) against the oracle, every ray: the same pixel
+    sequence where the kernel keeps the FP64 drift; on the Q32-drift angles the same
+    lengths per pixel to 1e-6 of a full length (a decision within ~N 2^-32 of a pixel edge
+    may move by one step, which moves a length of that size)."""
     shape, R, A, ic, sc = geom
+    L, C = shape
+    hx, hy = (ic[2] - ic[0]) / (C - 1), (ic[3] - ic[1]) / (L - 1)
+    moved = 0
     for a in range(A):
+        th = sc[0] + a * ((sc[2] - sc[0]) / (A - 1))
+        fixed = _pixel_driven(math.cos(th), math.sin(th), hx, hy)
         for r in range(R):
             p1, l1 = host_ray(lib, shape, R, A, a, r, ic, sc)
             p2, l2 = O.dda_ray_path(shape, R, A, a, r, ic[:2], ic[2:], sc[:2], sc[2:])
-            assert np.array_equal(p1, p2), (a, r)
-            if len(l2):
-                assert np.abs(l1 - l2).max() <= 1e-13 * np.abs(l2).max(), (a, r)
-            assert np.all((p1[:, 0] >= 0) & (p1[:, 0] < shape[0]) & (p1[:, 1] >= 0) & (p1[:, 1] < shape[1]))
+            assert np.all((p1[:, 0] >= 0) & (p1[:, 0] < L) & (p1[:, 1] >= 0) & (p1[:, 1] < C))
+            if not fixed:
+                assert np.array_equal(p1, p2), (a, r)
+                if len(l2):
+                    assert np.abs(l1 - l2).max() <= 1e-13 * np.abs(l2).max(), (a, r)
+                continue
+            m1, m2 = np.zeros(shape), np.zeros(shape)
+            np.add.at(m1, (p1[:, 0], p1[:, 1]), l1)
+            np.add.at(m2, (p2[:, 0], p2[:, 1]), l2)
+            full = max(abs(hx), abs(hy)) * 1.5
+            assert np.abs(m1 - m2).max() <= 1e-6 * full, (a, r)
+            moved += not np.array_equal(p1, p2)
+    print("rays whose pixel sequence differs only by near-zero lengths:", moved, "of", R * A)
 
 
 def test_host_ray_rejects_bad_geometry(lib):
