@@ -125,7 +125,9 @@ bool pixel_driven(const SSDdaGeom& g, const SSDdaAngle& a, SSDdaGatherAngle* out
   ga.lmax = fmin(w / as, h / ac);
   ga.half = ga.p / fabs(g.dt);
   ga.a = index;
-  ga.k = (int)ceil(2.0 * ga.half) + 2;
+  // rays with a nonzero length have |t - t_c| < p: the integers of the open interval
+  // (rc - half, rc + half), at most ceil(2 half) of them, from floor(rc - half) + 1
+  ga.k = (int)fmax(1.0, ceil(2.0 * ga.half));
   if (!isfinite(ga.inv_pq) || !isfinite(ga.half) || ga.k > 1 << 20) return false;
   *out = ga;
   return true;

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGath
 #pragma unroll
     for (int rr = 0; rr < DDA_GROWS; ++rr) {
       const double rc = (tc - g.t0) * inv_dt;
-      const double lo = fmin(fmax(floor(rc - A.half), -2.0), (double)g.nrays);
+      const double lo = fmin(fmax(floor(rc - A.half) + 1.0, -2.0), (double)g.nrays);
       const int r0 = (int)lo;
       for (int m = 0; m < A.k; ++m) {
         const int r = r0 + m;
