@@ -9,6 +9,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <list>
 #include <memory>
 #include <mutex>
@@ -29,13 +30,17 @@ struct raft_dda_plan {
   std::vector<int> h_walk;                 // angles traced ray by ray in the transpose
   SSDdaGatherAngle* d_gather = nullptr;
   int* d_walk = nullptr;
+  float* d_row_scale = nullptr;  // [A] lmax of the gather angles
+  int kmax = 0;
   double* d_weights = nullptr;  // R^T 1 in FP64, built on first use
   std::mutex weights_mutex;
   SSDdaTransposePlan tplan() const {
-    return SSDdaTransposePlan{d_angles, d_gather, (int)h_gather.size(), d_walk, (int)h_walk.size()};
+    return SSDdaTransposePlan{d_angles, d_gather, (int)h_gather.size(), kmax, d_row_scale, d_walk,
+                              (int)h_walk.size()};
   }
   ~raft_dda_plan() {
     if (d_angles) (void)hipFree(d_angles);
+    if (d_row_scale) (void)hipFree(d_row_scale);
     if (d_gather) (void)hipFree(d_gather);
     if (d_walk) (void)hipFree(d_walk);
     if (d_weights) (void)hipFree(d_weights);
@@ -163,7 +168,7 @@ int ensure_weights(raft_dda_plan* p, hipStream_t stream) {
   double* w = nullptr;
   HIP_TRY(hipMalloc(&w, sizeof(double) * npix));
   DevBuf sinoT, acc, accT;
-  hipError_t e = sinoT.alloc(sizeof(float) * (size_t)p->g.nrays * p->g.nangles, stream);
+  hipError_t e = sinoT.alloc(sizeof(float) * (size_t)(p->g.nrays + 4) * p->g.nangles, stream);
   if (e == hipSuccess) e = acc.alloc(sizeof(double) * npix, stream);
   if (e == hipSuccess) e = accT.alloc(sizeof(double) * npix, stream);
   if (e == hipSuccess)
@@ -184,7 +189,7 @@ int transpose_into(const raft_dda_plan* p, const float* sino, const double* w, O
                    hipStream_t stream) {
   const size_t n = (size_t)p->g.lines * p->g.cols * nslices;
   DevBuf sinoT, acc, accT;
-  HIP_TRY(sinoT.alloc(sizeof(float) * (size_t)p->g.nrays * p->g.nangles * nslices, stream));
+  HIP_TRY(sinoT.alloc(sizeof(float) * (size_t)(p->g.nrays + 4) * p->g.nangles * nslices, stream));
   HIP_TRY(acc.alloc(sizeof(T) * n, stream));
   HIP_TRY(accT.alloc(sizeof(T) * n, stream));
   HIP_TRY((dda_launch_transpose<T, Out>(p->g, p->tplan(), sino, false, sinoT.as<float>(), acc.as<T>(), accT.as<T>(),
@@ -233,6 +238,13 @@ extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, in
       p->h_walk.push_back(a);
     }
   }
+  std::vector<float> row_scale(nangles, 0.f);
+  for (const SSDdaGatherAngle& ga : p->h_gather) {
+    row_scale[ga.a] = (float)ga.lmax;
+    p->kmax = std::max(p->kmax, ga.k);
+  }
+  HIP_TRY(hipMalloc(&p->d_row_scale, sizeof(float) * nangles));
+  HIP_TRY(hipMemcpyAsync(p->d_row_scale, row_scale.data(), sizeof(float) * nangles, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSDdaAngle) * nangles));
   HIP_TRY(hipMalloc(&p->d_gather, sizeof(SSDdaGatherAngle) * (p->h_gather.size() + 1)));
   HIP_TRY(hipMalloc(&p->d_walk, sizeof(int) * (p->h_walk.size() + 1)));

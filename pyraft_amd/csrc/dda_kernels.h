@@ -203,12 +203,14 @@ struct SSDdaTransposePlan {
   const SSDdaAngle* angles;
   const SSDdaGatherAngle* gather;
   int ngather;
+  int kmax;                 // largest candidate count of the gather table
+  const float* row_scale;   // [A]: lmax of the gather angles (0 for traced ones)
   const int* walk;
   int nwalk;
 };
 
 // out[slice] = R^T sino[slice] [/ w] (T: accumulator type of the ray-traced part, float or
-// double).  sinoT ([nslices][A][R] floats), acc, accT ([nslices][lines*cols] T) are
+// double).  sinoT ([nslices][A][R + 4] floats), acc, accT ([nslices][lines*cols] T) are
 // scratch.  `ones`: R^T 1 for one slice (sino unused).
 template <typename T, typename Out>
 hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp, const float* sino, bool ones,

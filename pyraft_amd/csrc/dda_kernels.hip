@@ -131,9 +131,13 @@ __global__ __launch_bounds__(256) void k_dda_scatter(SSDdaGeom g, const SSDdaAng
 
 // acc[i][j] = sum over the table's angles and the rays crossing pixel (i, j) of
 // q(r, a) * L(t_r - t_c) (trapezoid lengths, FP64 geometry and sums).  Overwrites acc.
-template <typename T>
+// rows: the angle-major sinogram, each row scaled by its angle's lmax and padded with
+// two zeros either side (row a starts at rows + a * (R + 4) + 2), so the K = 2 path reads
+// its candidates r0, r0 + 1 in [-2, R + 1] without bounds tests.  K = 0: any number of
+// candidates (A.k), with bounds tests.  ones: q = 1 (the normalisation R^T 1).
+template <typename T, int K>
 __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGatherAngle* __restrict__ ga, int nga,
-                                                    const float* __restrict__ sinoT, int ones, T* __restrict__ acc) {
+                                                    const float* __restrict__ rows, int ones, T* __restrict__ acc) {
   const int j = blockIdx.x * DDA_GX + (threadIdx.x & (DDA_GX - 1));
   const int i0 = blockIdx.y * DDA_GY + (threadIdx.x / DDA_GX) * DDA_GROWS;
   const size_t slice = blockIdx.z;
@@ -141,26 +145,43 @@ __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGath
   const double x = g.ox + j * g.hx;
   const double y0 = g.oy + i0 * g.hy;
   const double inv_dt = 1.0 / g.dt;
-  const float* __restrict__ srows = sinoT + slice * (size_t)g.nangles * g.nrays;
+  const int pitch = g.nrays + 4;
+  const float* __restrict__ srows = rows + slice * (size_t)g.nangles * pitch + 2;
   double sum[DDA_GROWS];
 #pragma unroll
   for (int rr = 0; rr < DDA_GROWS; ++rr) sum[rr] = 0.0;
   for (int k = 0; k < nga; ++k) {
     const SSDdaGatherAngle A = ga[k];
-    const float* __restrict__ q = srows + (size_t)A.a * g.nrays;
+    const float* __restrict__ q = srows + (size_t)A.a * pitch;
     double tc = fma(x, A.c, y0 * A.s);
+    const double p_inv = A.p * A.inv_pq;
 #pragma unroll
     for (int rr = 0; rr < DDA_GROWS; ++rr) {
       const double rc = (tc - g.t0) * inv_dt;
       const double lo = fmin(fmax(floor(rc - A.half) + 1.0, -2.0), (double)g.nrays);
       const int r0 = (int)lo;
-      for (int m = 0; m < A.k; ++m) {
-        const int r = r0 + m;
-        if ((unsigned)r < (unsigned)g.nrays) {
-          const double d = fma((double)r, g.dt, g.t0) - tc;
-          const double z = fmin(1.0, fmax(0.0, (A.p - fabs(d)) * A.inv_pq));
-          const double v = ones ? 1.0 : (double)q[r];
-          sum[rr] = fma(v * A.lmax, z, sum[rr]);
+      if (K == 2) {
+        // d = t_r - t_c for r0 and r0 + 1; z = clamp((p - |d|) / (p - q))
+        const double d0 = fma(lo, g.dt, g.t0) - tc;
+        const double d1 = d0 + g.dt;
+        const double z0 = fmin(1.0, fmax(0.0, fma(-fabs(d0), A.inv_pq, p_inv)));
+        const double z1 = fmin(1.0, fmax(0.0, fma(-fabs(d1), A.inv_pq, p_inv)));
+        if (ones) {  // rays that exist only
+          const double w0 = (unsigned)r0 < (unsigned)g.nrays ? z0 : 0.0;
+          const double w1 = (unsigned)(r0 + 1) < (unsigned)g.nrays ? z1 : 0.0;
+          sum[rr] += A.lmax * (w0 + w1);
+        } else {
+          sum[rr] = fma((double)q[r0], z0, sum[rr]);
+          sum[rr] = fma((double)q[r0 + 1], z1, sum[rr]);
+        }
+      } else {
+        for (int m = 0; m < A.k; ++m) {
+          const int r = r0 + m;
+          if ((unsigned)r < (unsigned)g.nrays) {
+            const double d = fma((double)r, g.dt, g.t0) - tc;
+            const double z = fmin(1.0, fmax(0.0, fma(-fabs(d), A.inv_pq, p_inv)));
+            sum[rr] = fma(ones ? A.lmax : (double)q[r], z, sum[rr]);
+          }
         }
       }
       tc += A.hys;
@@ -170,6 +191,30 @@ __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGath
 #pragma unroll
   for (int rr = 0; rr < DDA_GROWS; ++rr)
     if (i0 + rr < g.lines) out[(size_t)(i0 + rr) * g.cols + j] = T(sum[rr]);
+}
+
+// rows[slice][a][2 + r] = sino[slice][r][a] * scale[a], two zeros either side of each row
+__global__ __launch_bounds__(256) void k_dda_sino_rows(int R, int A, const float* __restrict__ sino,
+                                                       const float* __restrict__ scale, float* __restrict__ rows) {
+  __shared__ float tile[DDA_TILE][DDA_TILE + 1];
+  const size_t slice = blockIdx.z;
+  const int r0 = blockIdx.y * DDA_TILE, a0 = blockIdx.x * DDA_TILE;
+  const int pitch = R + 4;
+  const float* __restrict__ s = sino + slice * (size_t)R * A;
+  float* __restrict__ o = rows + slice * (size_t)A * pitch;
+  for (int rr = threadIdx.y; rr < DDA_TILE; rr += blockDim.y) {
+    const int r = r0 + rr, a = a0 + threadIdx.x;
+    tile[rr][threadIdx.x] = (r < R && a < A) ? s[(size_t)r * A + a] : 0.f;
+  }
+  __syncthreads();
+  for (int aa = threadIdx.y; aa < DDA_TILE; aa += blockDim.y) {
+    const int a = a0 + aa, r = r0 + threadIdx.x;
+    if (a < A && r < R) o[(size_t)a * pitch + 2 + r] = tile[threadIdx.x][aa] * scale[a];
+    if (a < A && blockIdx.y == 0 && threadIdx.x < 4) {
+      const int pad = threadIdx.x < 2 ? threadIdx.x : R + threadIdx.x;  // 0, 1, R + 2, R + 3
+      o[(size_t)a * pitch + pad] = 0.f;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- layouts
@@ -245,11 +290,16 @@ hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp
                                 hipStream_t stream) {
   const size_t n = (size_t)g.lines * g.cols * nslices;
   if (tp.ngather > 0) {
-    if (!ones)  // angle-major copy of the sinogram: rays of one angle contiguous
-      hipLaunchKernelGGL(k_dda_transpose_image, dda_tiles(g.nrays, g.nangles, nslices), dim3(DDA_TILE, 8), 0, stream,
-                         g.nrays, g.nangles, sino, sinoT);
-    hipLaunchKernelGGL(k_dda_gather<T>, dim3((g.cols + DDA_GX - 1) / DDA_GX, (g.lines + DDA_GY - 1) / DDA_GY, nslices),
-                       dim3(256), 0, stream, g, tp.gather, tp.ngather, (const float*)sinoT, ones ? 1 : 0, acc);
+    if (!ones)  // angle-major, lmax-scaled, zero-padded copy of the sinogram
+      hipLaunchKernelGGL(k_dda_sino_rows, dda_tiles(g.nrays, g.nangles, nslices), dim3(DDA_TILE, 8), 0, stream,
+                         g.nrays, g.nangles, sino, tp.row_scale, sinoT);
+    const dim3 grid((g.cols + DDA_GX - 1) / DDA_GX, (g.lines + DDA_GY - 1) / DDA_GY, nslices);
+    if (tp.kmax <= 2)
+      hipLaunchKernelGGL((k_dda_gather<T, 2>), grid, dim3(256), 0, stream, g, tp.gather, tp.ngather,
+                         (const float*)sinoT, ones ? 1 : 0, acc);
+    else
+      hipLaunchKernelGGL((k_dda_gather<T, 0>), grid, dim3(256), 0, stream, g, tp.gather, tp.ngather,
+                         (const float*)sinoT, ones ? 1 : 0, acc);
   } else {
     hipLaunchKernelGGL(k_dda_fill<T>, dim3(1024), dim3(256), 0, stream, acc, n);
   }
