@@ -348,6 +348,16 @@ def main():
     torch.cuda.synchronize()
     fbp_ms = e0.elapsed_time(e1) / 3
 
+    # exact transpose of the Radon (the autograd backward) on the same sinogram
+    for _ in range(2):
+        plan.radon_transpose(sino)
+    e0.record(stream)
+    for _ in range(3):
+        plan.radon_transpose(sino)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    transpose_ms = e0.elapsed_time(e1) / 3
+
     dominant = ("k_backprojection<0, %d, %d>" % (bp_rows, bp_stage)) if t_bp >= t_radon else "k_radon<1>"
     t_dom = max(t_bp, t_radon)
     achieved = BYTES_PER_UPDATE * upd / t_dom / 1e9
@@ -372,6 +382,7 @@ def main():
         "radon_ms": round(t_radon * 1e3, 4),
         "backprojection_ms": round(t_bp * 1e3, 4),
         "fbp_ms": round(fbp_ms, 4),
+        "exact_transpose_ms": round(transpose_ms, 4),
         "bp_variant": {"chosen": bp_variant, "rows_per_thread": bp_rows, "stage_pairs": bp_stage,
                        "tuned_us": {str(k): round(v, 1) for k, v in bp_variant_us.items()}},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -82,6 +82,7 @@ static void build_angles(const SSGeom& g, std::vector<SSAngle>& out, int* ndeg, 
     a.gamma = to_fix(tr.gamma);
     a.beta = to_fix(tr.beta);
     a.degenerate = fabs(tr.beta) < SS_DEGENERATE_SLOPE ? 1 : 0;
+    a.regular = !a.degenerate;
     *ndeg += a.degenerate;
     a.bp_a = (g.xmin * a.cost + g.ymin * a.sint - g.tmin) / g.dt;
     a.bp_b = (g.dx * a.cost) / g.dt;
@@ -89,6 +90,10 @@ static void build_angles(const SSGeom& g, std::vector<SSAngle>& out, int* ndeg, 
     a.rt_ig = a.gamma != 0 ? SS_FIX_ONE / (double)a.gamma : 0.0;
     a.rt_a0 = a.gamma != 0 ? (double)a.alpha0 / (double)a.gamma : 0.0;
     a.rt_b = a.gamma != 0 ? (double)a.beta / (double)a.gamma : 0.0;
+    a.rt_w = fmin(-a.rt_ig, g.n * a.rt_ig);
+    a.rt_wn = (int)ceil((g.n + 1) * fabs(a.rt_ig));
+    a.rt_i0 = fmin(a.rt_ig, (g.n - 2) * a.rt_ig);
+    a.rt_i1 = fmax(a.rt_ig, (g.n - 2) * a.rt_ig);
     out[j] = a;
   }
 }
@@ -199,6 +204,13 @@ static int plan_create(raft_ss_plan** out, int n, int nrays, int nangles, const 
   HIP_TRY(hipStreamSynchronize(stream));
   if (count > capacity) return set_error(RAFT_SS_EINTERNAL, "irregular ray list overflow (%d > %d)", count, capacity);
   p->nirregular = count;
+  if (count > 0) {
+    // angles with irregular rays are not regular
+    std::vector<int> irr(count);
+    HIP_TRY(hipMemcpy(irr.data(), p->d_irregular, sizeof(int) * count, hipMemcpyDeviceToHost));
+    for (int id : irr) p->h_angles[id / nrays].regular = 0;
+    HIP_TRY(hipMemcpy(p->d_angles, p->h_angles.data(), sizeof(SSAngle) * nangles, hipMemcpyHostToDevice));
+  }
   *out = p.release();
   return 0;
 }

@@ -43,10 +43,18 @@ struct SSAngle {
   // at sample s is k = x*rt_ig - rt_a0 - s*rt_b  (rt_ig = 2^32/gamma, rt_a0 = alpha0/gamma,
   // rt_b = beta/gamma), used to bound the candidate rays of a pixel tile.
   double rt_ig, rt_a0, rt_b;
+  // Candidate rays of image line s, with a0(s) = rt_a0 + s*rt_b: the rays reaching cells
+  // [-1, n] lie in [rt_w - a0(s), rt_w - a0(s) + rt_wn], those reaching cells [1, n-2] in
+  // [rt_i0 - a0(s), rt_i1 - a0(s)] (real-valued; zero when gamma = 0)
+  double rt_w, rt_i0, rt_i1;
+  int rt_wn;
+  int rt_pad_;
   // |beta| below this threshold: sample validity is evaluated sample by sample
   // (knife-edge angles where the FP64 predicate need not be monotone).
   int degenerate;
-  int pad_;
+  // 1: not degenerate and every ray has a contiguous exact range (no irregular rays);
+  // set when a plan's ranges are built
+  int regular;
 };
 
 // Q32.32 scale

@@ -917,46 +917,181 @@ hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int 
 // k_radon_transpose_lines: one wave owns one image line along the forward's sample
 // axis -- image row s for branch-B angles, image column s for branch-A angles -- as
 // an LDS row of n+2 cells (pixel c at index c+1, the two pads absorb m = -1 and
-// m+1 = n).  For each angle of its branch, the lanes take the rays 64 at a time,
-// step u by 64*gamma between passes, check the ray's exact range and add the two
-// contributions.  Each line belongs to one wave and its terms arrive in a fixed
-// order, so the result is deterministic without global atomics.  Branch-B lines are
-// image rows and are written to the output; branch-A lines are image columns and go
-// to a transposed scratch image that k_transpose_add folds in.
-//
-// k_radon_transpose (tiles): the same sums for images too wide for one LDS line; a
-// block owns a T x T tile and finds the rays crossing each tile line from the
-// angle's inverse slope.  Irregular rays are added by the fixup kernel afterwards.
+// m+1 = n).  For each angle of its branch, the lanes take the rays that can cross the
+// line 64 at a time, step u by 64*gamma between passes, check the sample's validity
+// and add the two contributions.  Each line belongs to one wave, so no global atomics
+// or block barriers are needed; the LDS holds only the lines, so up to 16384-pixel
+// lines fit and many waves share a CU.  The rays' exact ranges and sinogram values are
+// read from global memory (the angle's rows are shared by every line, so they stay in
+// cache).  Branch-B lines are image rows and are written to the output; branch-A lines
+// are image columns and go to a transposed scratch image that k_transpose_add folds
+// in.  Irregular rays are added by the fixup kernel afterwards.
 // =====================================================================================
 
 #define RT_LINES 8         // image lines (waves) per block
-#define RT_LDS_MAX 65536   // LDS budget of one block: lines + two ray buffers
+#define RT_PHASES_MAX 8    // dense rays: lane phases before falling back to LDS atomics
 
-// LDS layout of k_radon_transpose_lines: `nl` lines of n+2 cells, then two ray
-// buffers of 3R dwords (the angle's exact ranges as int2, then its sinogram row)
-__host__ __device__ inline size_t ss_rt_line_lds(int n, int R, int nl) {
-  return sizeof(float) * ((size_t)nl * (n + 2) + 2 * (size_t)(3 * R + 4));
+// LDS of k_radon_transpose_lines: one line of n+2 cells per wave, plus two spare cells
+// that lanes without a sample update with zero
+__host__ __device__ inline int ss_rt_line_stride(int n) { return n + 4; }
+__host__ __device__ inline size_t ss_rt_line_lds(int n, int nl) {
+  return sizeof(float) * (size_t)nl * ss_rt_line_stride(n);
 }
 
+// The angle table read through the constant address space: the plan's angles are not
+// written while a kernel runs, so the reads are scalar loads into wave-uniform registers.
+typedef __attribute__((address_space(4))) const SSAngle* ss_const_angles;
+
 // next angle >= j of `branch` (or j1)
-__device__ __forceinline__ int ss_rt_next(const SSAngle* angles, int j, int j1, int branch) {
-  while (j < j1 && ss_rfl(angles[j].branch_a) != branch) ++j;
+__device__ __forceinline__ int ss_rt_next(ss_const_angles angles, int j, int j1, int branch) {
+  while (j < j1 && angles[j].branch_a != branch) ++j;
   return j;
 }
 
-// copy the ranges and sinogram row of angle j into ray buffer `buf` (3R dwords) with
-// direct global -> LDS loads; wave w copies 64-dword chunks w, w + nw, ...
-__device__ __forceinline__ void ss_rt_stage(const SSDevPlan& p, const float* q, int j, float* buf, int wv, int nw,
-                                            int lane) {
-  const int R = p.g.nrays;
-  const int* bsrc = reinterpret_cast<const int*>(p.bounds + (size_t)j * R);
-  const float* qsrc = q + (size_t)j * R;
-  for (int c = wv * SS_WAVE; c < 3 * R; c += nw * SS_WAVE) {
-    const int d = c + lane;
-    if (d < 3 * R) {
-      const void* src = d < 2 * R ? (const void*)(bsrc + d) : (const void*)(qsrc + (d - 2 * R));
-      __builtin_amdgcn_global_load_lds((ss_global_cptr)src, (ss_lds_ptr)(buf + c), 4, 0, 0);
+// structured buffer loads of one ray's record (address = base + vindex * stride)
+typedef int ss_i32x2 __attribute__((ext_vector_type(2)));
+__device__ ss_i32x2 ss_struct_load_i32x2(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                         int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+__device__ float ss_struct_load_f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                    int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
+
+// How the lanes of a pass add their two terms to the LDS line:
+//   RT_DISTINCT  rays at least one cell apart (|gamma| >= 1): the lanes hit distinct cells,
+//                so plain read-modify-writes, first cell m, then cell m+1, in program order
+//                (the empty asm statements keep the compiler from moving an update across
+//                the previous one); lanes without a sample add zero to the spare cells;
+//   RT_PHASED    denser rays: the same, one phase of lanes `phases` apart at a time;
+//   RT_ATOMIC    rays denser than 1/RT_PHASES_MAX cell: LDS atomics.
+// In the first two every cell takes its terms in a fixed order (deterministic sums).
+enum { RT_DISTINCT = 0, RT_PHASED = 1, RT_ATOMIC = 2 };
+
+struct SSRtLine {
+  float* line;               // LDS line: cell c + 1 holds pixel c; cells n+2, n+3 are spare
+  int n, s;                  // line length, line index (the forward's sample)
+  int k_lo, k_hi;            // candidate rays
+  bool all_edge;             // every sample reads its ray's exact range
+  int p_first, p_last;       // passes p_first .. p_last - 1 hold only valid samples
+  int phases, my_phase;      // RT_PHASED
+  __amdgpu_buffer_rsrc_t rb, rq;  // the angle's range row (int2) and sinogram row (float)
+  int64_t gamma;             // trajectory step per ray
+  int64_t u0, step;          // trajectory (+1 cell) of ray k_lo, and its step per pass
+  float delta;
+};
+
+// RT_PHASED / RT_ATOMIC update of one pass: ray k (this lane) at trajectory u, its term
+// qd = q * delta (0 if the sample is not valid)
+template <int MODE>
+__device__ __forceinline__ void ss_rt_update(const SSRtLine& L, bool valid, int64_t u, float qd) {
+  const int idx = (int)(u >> 32);
+  const float wq = (float)(uint32_t)u * SS_INV_FIX * qd;
+  if (MODE == RT_PHASED) {
+    float* cell = L.line + idx;
+    for (int ph = 0; ph < L.phases; ++ph) {
+      const bool mine = valid && L.my_phase == ph;
+      if (mine) cell[0] += qd - wq;
+      __asm__ volatile("" ::: "memory");
+      if (mine) cell[1] += wq;
+      __asm__ volatile("" ::: "memory");
     }
+  } else if (valid) {
+    atomicAdd(L.line + idx, qd - wq);
+    atomicAdd(L.line + idx + 1, wq);
+  }
+}
+
+// One iteration over two passes (pass, pass + 1) of the candidate rays: lane 0's ray kb and
+// trajectory ub (+1 cell), the passes' sinogram values q0, q1; loads the values of the
+// next iteration's passes into n0, n1 (indices clamped to the last candidate: the loads
+// past the last pass stay in bounds and go unused).
+//
+// Passes p_first .. p_last - 1 hold only valid samples (see the kernel); a pair of them
+// takes a path without range checks.  Other samples near the line's ends (every sample
+// of other angles) read the ray's exact range.
+template <int MODE>
+__device__ __forceinline__ void ss_rt_pair(const SSRtLine& L, int lane, int pass, int npass, int kb, int64_t ub,
+                                           int64_t lane_u, float q0, float q1, float& n0, float& n1) {
+  const int n = L.n, s = L.s, k_hi = L.k_hi, last = k_hi - 1;
+  const int64_t ub1 = ub + L.step;
+  const int64_t u = ub + lane_u, u1 = ub1 + lane_u;
+  const int idx0 = (int)(u >> 32), idx1 = (int)(u1 >> 32);  // m + 1, in [0, n] if valid
+  const int k = kb + lane;
+  if (MODE == RT_DISTINCT && pass >= L.p_first && pass + 1 < L.p_last) {
+    // every sample of both passes is valid
+    n0 = ss_struct_load_f32(L.rq, min(k + 2 * SS_WAVE, last), 0, 0, 0);
+    n1 = ss_struct_load_f32(L.rq, min(k + 3 * SS_WAVE, last), 0, 0, 0);
+    const float ds = L.delta * SS_INV_FIX;
+    const float w0 = (float)(uint32_t)u * (q0 * ds);
+    const float w1 = (float)(uint32_t)u1 * (q1 * ds);
+    float* c0 = L.line + idx0;
+    float* c1 = L.line + idx1;
+    const float x0 = c0[0], x1 = c1[0];
+    c0[0] = fmaf(q0, L.delta, x0) - w0;
+    c1[0] = fmaf(q1, L.delta, x1) - w1;
+    __asm__ volatile("" ::: "memory");
+    const float y0 = c0[1], y1 = c1[1];
+    c0[1] = y0 + w0;
+    c1[1] = y1 + w1;
+    __asm__ volatile("" ::: "memory");
+    return;
+  }
+  const bool two = pass + 1 < npass;  // a second pass this iteration (uniform)
+  const bool e0 = L.all_edge || (unsigned)(idx0 - 2) > (unsigned)(n - 4);
+  const bool e1 = two && (L.all_edge || (unsigned)(idx1 - 2) > (unsigned)(n - 4));
+  ss_i32x2 b0 = {0, -1}, b1 = {0, -1};
+  if (e0) b0 = ss_struct_load_i32x2(L.rb, min(k, last), 0, 0, 0);
+  if (e1) b1 = ss_struct_load_i32x2(L.rb, min(k + SS_WAVE, last), 0, 0, 0);
+  // issued after the ranges, so that waiting for the ranges does not wait for these
+  n0 = ss_struct_load_f32(L.rq, min(k + 2 * SS_WAVE, last), 0, 0, 0);
+  n1 = ss_struct_load_f32(L.rq, min(k + 3 * SS_WAVE, last), 0, 0, 0);
+  bool v0 = k < k_hi, v1 = k + SS_WAVE < k_hi;
+  if (e0) v0 = v0 && s >= b0.x && s <= b0.y;
+  if (e1) v1 = v1 && s >= b1.x && s <= b1.y;
+  const float qd0 = v0 ? q0 * L.delta : 0.f, qd1 = v1 ? q1 * L.delta : 0.f;
+  if (MODE == RT_DISTINCT) {
+    // the first cells of both passes are distinct, and so are their second cells: read
+    // both, then write both, one cell phase at a time; lanes without a sample add zero to
+    // the spare cells
+    const float w0 = (float)(uint32_t)u * SS_INV_FIX * qd0;
+    const float w1 = (float)(uint32_t)u1 * SS_INV_FIX * qd1;
+    float* c0 = L.line + (v0 ? idx0 : n + 2);
+    float* c1 = L.line + (v1 ? idx1 : n + 2);
+    const float x0 = c0[0], x1 = c1[0];
+    c0[0] = x0 + (qd0 - w0);
+    c1[0] = x1 + (qd1 - w1);
+    __asm__ volatile("" ::: "memory");
+    const float y0 = c0[1], y1 = c1[1];
+    c0[1] = y0 + w0;
+    c1[1] = y1 + w1;
+    __asm__ volatile("" ::: "memory");
+  } else {
+    ss_rt_update<MODE>(L, v0, u, qd0);
+    if (two) ss_rt_update<MODE>(L, v1, u1, qd1);
+  }
+}
+
+// the passes over the candidate rays, two per ss_rt_pair; the sinogram values alternate
+// between two register pairs (each half loads the other half's values)
+template <int MODE>
+__device__ __forceinline__ void ss_rt_passes(const SSRtLine& L, int lane) {
+  const int npass = (L.k_hi - L.k_lo + SS_WAVE - 1) / SS_WAVE;
+  const int last = L.k_hi - 1;
+  const int64_t lane_u = (int64_t)lane * L.gamma;
+  int64_t ub = L.u0;  // lane 0's trajectory (+1 cell)
+  int kb = L.k_lo;    // lane 0's ray
+  float qa = ss_struct_load_f32(L.rq, min(kb + lane, last), 0, 0, 0);
+  float qb = ss_struct_load_f32(L.rq, min(kb + SS_WAVE + lane, last), 0, 0, 0);
+  float qc, qd;
+  for (int pass = 0;;) {
+    ss_rt_pair<MODE>(L, lane, pass, npass, kb, ub, lane_u, qa, qb, qc, qd);
+    pass += 2;
+    kb += 2 * SS_WAVE;
+    ub += 2 * L.step;
+    if (pass >= npass) break;
+    ss_rt_pair<MODE>(L, lane, pass, npass, kb, ub, lane_u, qc, qd, qa, qb);
+    pass += 2;
+    kb += 2 * SS_WAVE;
+    ub += 2 * L.step;
+    if (pass >= npass) break;
   }
 }
 
@@ -971,55 +1106,75 @@ __global__ __launch_bounds__(RT_LINES * SS_WAVE) void k_radon_transpose_lines(SS
   const int groups = (n + nw - 1) / nw;
   const int branch = blockIdx.x >= (unsigned)groups ? 1 : 0;
   const int s = ((int)blockIdx.x - branch * groups) * nw + wv;  // image row (B) or column (A)
+  if (s >= n) return;  // waves work independently: no block barrier follows
   const size_t slice = blockIdx.y;
   const float* q = qT + slice * A * R;
-  float* line = sh + wv * (n + 2);
-  float* rbuf[2] = {sh + nw * (n + 2), sh + nw * (n + 2) + (3 * R + 4)};
-  for (int i = lane; i < n + 2; i += SS_WAVE) line[i] = 0.f;
-  int j = ss_rt_next(p.angles, j0, j1, branch);
-  if (j < j1) ss_rt_stage(p, q, j, rbuf[0], wv, nw, lane);
-  int cur = 0;
-  while (j < j1) {
-    const int jn = ss_rt_next(p.angles, j + 1, j1, branch);
-    __syncthreads();  // angle j is staged; every wave is done with the other buffer
-    if (jn < j1) ss_rt_stage(p, q, jn, rbuf[cur ^ 1], wv, nw, lane);
-    if (s < n) {
-      const SSAngle& a = p.angles[j];
-      const int64_t gamma = ss_rfl64(a.gamma);
-      const float delta = ss_rflf(a.delta);
-      const int2* bb = reinterpret_cast<const int2*>(rbuf[cur]);
-      const float* qq = rbuf[cur] + 2 * R;
-      // u of ray `lane` at sample s, plus one cell: the high word is the line index m + 1
-      int64_t u = ss_rfl64(a.alpha0) + (int64_t)s * ss_rfl64(a.beta) + (int64_t)lane * gamma + ((int64_t)1 << 32);
-      const int64_t step = (int64_t)SS_WAVE * gamma;
-      // rays at least one cell apart (|gamma| >= 1): the lanes of a pass hit distinct
-      // cells, so plain read-modify-writes in two ordered phases are exact (the empty
-      // asm keeps the compiler from moving the second phase's reads above the first
-      // phase's writes of other lanes); denser rays use LDS atomics
-      const bool distinct = (gamma >= SS_FIX_ONE_I64) || (gamma <= -SS_FIX_ONE_I64);
-      for (int k = lane; k - lane < R; k += SS_WAVE, u += step) {
-        const bool valid = k < R && s >= bb[k].x && s <= bb[k].y;
-        const float qd = valid ? qq[k] * delta : 0.f;
-        const float wq = (float)(uint32_t)u * SS_INV_FIX * qd;
-        const int idx = (int)(u >> 32);  // m + 1, in [0, n] for a valid sample
-        if (distinct) {
-          if (valid) line[idx] += qd - wq;
-          __asm__ volatile("" ::: "memory");
-          if (valid) line[idx + 1] += wq;
-          __asm__ volatile("" ::: "memory");
-        } else if (valid) {
-          atomicAdd(&line[idx], qd - wq);
-          atomicAdd(&line[idx + 1], wq);
-        }
+  SSRtLine L;
+  // LDS is addressed as sh + offset, so that every access compiles to an LDS instruction
+  L.line = sh + wv * ss_rt_line_stride(n);
+  L.n = n;
+  L.s = s;
+  for (int i = lane; i < ss_rt_line_stride(n); i += SS_WAVE) L.line[i] = 0.f;
+  const ss_const_angles angles = (ss_const_angles)p.angles;
+  for (int j = ss_rt_next(angles, j0, j1, branch); j < j1; j = ss_rt_next(angles, j + 1, j1, branch)) {
+    const __attribute__((address_space(4))) SSAngle& a = angles[j];
+    const int64_t gamma = a.gamma;
+    // candidate rays [k_lo, k_hi) of line s: ray k sits at cell x = (k + a0(s)) / rt_ig on it,
+    // and a valid sample has x in [-1, n]; two rays of margin against rounding (every
+    // candidate is still checked)
+    const double a0 = a.rt_a0 + (double)s * a.rt_b;
+    int k_lo = 0, k_hi = R;
+    if (gamma != 0) {
+      const double kd = floor(a.rt_w - a0) - 2.0;
+      k_lo = ss_rfl((int)fmin(fmax(kd, 0.0), (double)R));
+      k_hi = ss_rfl((int)fmax(fmin(kd + (double)(a.rt_wn + 5), (double)R), 0.0));
+    }
+    if (k_lo >= k_hi) continue;
+    L.k_lo = k_lo;
+    L.k_hi = k_hi;
+    L.all_edge = !(a.regular && n >= 4);
+    // On a regular angle a sample whose Q32.32 cell m lies in [1, n-3] is valid: the FP64
+    // coordinate of the exact predicate is within ~(k + s) 2^-33 cells of the trajectory,
+    // so its floor is in [0, n-2].  Those samples are the rays reaching cells [1, n-2]; the
+    // passes whose 64 rays lie at least one ray inside that interval (and below k_hi) hold
+    // only valid samples.
+    L.p_first = 0;
+    L.p_last = 0;
+    if (!L.all_edge && gamma != 0) {
+      const int kf = ss_rfl((int)fmin(fmax(floor(a.rt_i0 - a0) + 2.0, -1.0), (double)R));  // first fast ray
+      const int kl = min(ss_rfl((int)fmin(fmax(ceil(a.rt_i1 - a0) - 2.0, -1.0), (double)R)), k_hi - 1);
+      // pass p covers rays k_lo + 64 p .. k_lo + 64 p + 63
+      const int pf = kf <= k_lo ? 0 : (kf - k_lo + SS_WAVE - 1) / SS_WAVE;
+      const int span = kl - k_lo - (SS_WAVE - 1);
+      if (span >= 0) {
+        L.p_first = pf;
+        L.p_last = span / SS_WAVE + 1;  // one past the last fast pass
       }
     }
-    cur ^= 1;
-    j = jn;
+    L.delta = a.delta;
+    // record k of the angle's range row (8 bytes) and sinogram row (4 bytes); every index
+    // used is clamped to [0, R-1]
+    L.rb = __builtin_amdgcn_make_buffer_rsrc((void*)(p.bounds + (size_t)j * R), (short)sizeof(int2),
+                                             (int)((uint32_t)R * 8u), SS_RSRC_DWORD3);
+    L.rq = __builtin_amdgcn_make_buffer_rsrc((void*)(q + (size_t)j * R), (short)sizeof(float),
+                                             (int)((uint32_t)R * 4u), SS_RSRC_DWORD3);
+    L.gamma = gamma;
+    L.u0 = a.alpha0 + (int64_t)s * a.beta + (int64_t)k_lo * gamma + ((int64_t)1 << 32);
+    L.step = (int64_t)SS_WAVE * gamma;
+    const uint64_t ag = (uint64_t)(gamma < 0 ? -gamma : gamma);
+    if (ag >= (uint64_t)SS_FIX_ONE_I64) {
+      ss_rt_passes<RT_DISTINCT>(L, lane);
+    } else if (ag * RT_PHASES_MAX >= (uint64_t)SS_FIX_ONE_I64) {
+      // lanes `phases` apart are at least one cell apart
+      L.phases = ss_rfl((int)(((uint64_t)SS_FIX_ONE_I64 + ag - 1) / ag));
+      L.my_phase = lane % L.phases;
+      ss_rt_passes<RT_PHASED>(L, lane);
+    } else {
+      ss_rt_passes<RT_ATOMIC>(L, lane);
+    }
   }
-  if (s < n) {
-    float* dst = (branch ? outA : outB) + (slice * n + s) * n;
-    for (int c = lane; c < n; c += SS_WAVE) dst[c] = line[c + 1];
-  }
+  float* dst = (branch ? outA : outB) + (slice * n + s) * n;
+  for (int c = lane; c < n; c += SS_WAVE) dst[c] = L.line[c + 1];
 }
 
 // out[s][r][c] += t[s][c][r], 32x32 tiles
@@ -1036,81 +1191,6 @@ __global__ __launch_bounds__(256) void k_transpose_add(const float* __restrict__
   for (int xx = ty; xx < 32; xx += 8) {
     const int r = bx + xx, c = by + tx;  // image row r, column c
     if (r < n && c < n) out[off + (size_t)r * n + c] += tile[tx][xx];
-  }
-}
-
-#define RT_T 62      // tile side: a tile line is crossed by at most T + 2 rays when |gamma| >= 1 cell
-#define RT_WAVES 8
-
-__global__ __launch_bounds__(RT_WAVES * SS_WAVE) void k_radon_transpose(SSDevPlan p, const float* __restrict__ qT,
-                                                                       int j0, int j1, float* __restrict__ out) {
-  __shared__ float acc[RT_T * (RT_T + 1)];
-  const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int c0 = blockIdx.x * RT_T, r0 = blockIdx.y * RT_T;
-  const size_t slice = blockIdx.z;
-  const float* q = qT + slice * A * R;
-  for (int i = threadIdx.x; i < RT_T * (RT_T + 1); i += RT_WAVES * SS_WAVE) acc[i] = 0.f;
-  __syncthreads();
-  int prev_branch = -1;
-  for (int j = j0; j < j1; ++j) {
-    const SSAngle& a = p.angles[j];
-    const int branch = ss_rfl(a.branch_a);
-    if (branch != prev_branch) {
-      __syncthreads();  // every wave is done with the other branch's lines
-      prev_branch = branch;
-    }
-    const int cell0 = branch ? r0 : c0;  // interpolation axis origin
-    const int s0 = branch ? c0 : r0;     // sample axis origin
-    const int64_t alpha = ss_rfl64(a.alpha0), gamma = ss_rfl64(a.gamma), beta = ss_rfl64(a.beta);
-    const float delta = ss_rflf(a.delta);
-    // candidate rays of line s: between x = cell0 - 1 and x = cell0 + T (cells), in
-    // ray units k(x, s) = x*ig - a0 - s*b; two rays of margin against rounding
-    const double ig = a.rt_ig, a0 = a.rt_a0, bb = a.rt_b;
-    const double ka = (double)(cell0 - 1) * ig - a0, kb = (double)(cell0 + RT_T) * ig - a0;
-    const double kmin0 = fmin(ka, kb);
-    const int kcount = (int)ceil(fabs(kb - ka)) + 5;
-    const int64_t lane_gamma = (int64_t)lane * gamma;
-    const int64_t step_gamma = (int64_t)SS_WAVE * gamma;
-    const bool distinct = (gamma >= SS_FIX_ONE_I64) || (gamma <= -SS_FIX_ONE_I64);
-    for (int l = wv; l < RT_T; l += RT_WAVES) {
-      const int s = s0 + l;
-      if (s >= n) break;
-      // candidate rays [k_lo, k_hi) of this line, clipped to the detector
-      const double kd = floor(kmin0 - (double)s * bb) - 2.0;
-      if (kd >= (double)R || kd + kcount <= 0.0) continue;
-      const int k_lo = ss_rfl((int)fmax(kd, 0.0));
-      const int k_hi = ss_rfl((int)fmin(kd + kcount, (double)R));
-      // u of ray k_lo + lane at sample s
-      int64_t u = alpha + (int64_t)s * beta + (int64_t)k_lo * gamma + lane_gamma;
-      for (int k = k_lo + lane; k - lane < k_hi; k += SS_WAVE, u += step_gamma) {
-        if (k >= k_hi) continue;
-        const int2 b = p.bounds[(size_t)j * R + k];
-        if (s < b.x || s > b.y) continue;
-        const int m = (int)(u >> 32) - cell0;
-        const float qd = q[(size_t)j * R + k] * delta;
-        const float wq = (float)(uint32_t)u * SS_INV_FIX * qd;
-        float* p0 = &acc[branch ? m * (RT_T + 1) + l : l * (RT_T + 1) + m];
-        float* p1 = &acc[branch ? (m + 1) * (RT_T + 1) + l : l * (RT_T + 1) + m + 1];
-        if (distinct) {  // lanes hit distinct cells: ordered plain updates (see the line kernel)
-          if ((unsigned)m < (unsigned)RT_T) *p0 += qd - wq;
-          __asm__ volatile("" ::: "memory");
-          if ((unsigned)(m + 1) < (unsigned)RT_T) *p1 += wq;
-          __asm__ volatile("" ::: "memory");
-        } else {
-          if ((unsigned)m < (unsigned)RT_T) atomicAdd(p0, qd - wq);
-          if ((unsigned)(m + 1) < (unsigned)RT_T) atomicAdd(p1, wq);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  float* img = out + slice * n * n;
-  for (int i = threadIdx.x; i < RT_T * RT_T; i += RT_WAVES * SS_WAVE) {
-    const int rr = i / RT_T, cc = i - rr * RT_T;
-    const int r = r0 + rr, c = c0 + cc;
-    if (r < n && c < n) img[(size_t)r * n + c] = acc[rr * (RT_T + 1) + cc];
   }
 }
 
@@ -1142,46 +1222,43 @@ __global__ __launch_bounds__(256) void k_radon_transpose_fixup(SSDevPlan p, cons
   }
 }
 
-// LDS a block may use on the current device (gfx950: 160 KiB)
+// LDS a block may use on the current device (gfx950: 160 KiB, also assumed if the query
+// fails)
 static size_t ss_lds_per_block() {
   int dev = 0, v = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
-    return RT_LDS_MAX;
+    return 160 * 1024;
   return (size_t)v;
 }
 
-// lines per block for the line kernel (0: use the tiled kernel); at most half the
-// LDS of a block so that two blocks share a CU
-static int ss_rt_lines(int n, int R) {
-  const size_t budget = std::max<size_t>(RT_LDS_MAX, ss_lds_per_block() / 2);
+// lines (waves) per block for the line kernel: the most whose lines fit one block's LDS
+// (one line of the largest image, 16384 pixels, takes 64 KiB)
+static int ss_rt_lines(int n) {
+  const size_t budget = ss_lds_per_block();
   for (int nl = RT_LINES; nl >= 1; nl >>= 1)
-    if (ss_rt_line_lds(n, R, nl) <= budget) return nl;
+    if (ss_rt_line_lds(n, nl) <= budget) return nl;
   return 0;
 }
 
 size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices) {
-  return ss_rt_lines(p.g.n, p.g.nrays) ? sizeof(float) * (size_t)nslices * p.g.n * p.g.n : 0;
+  return sizeof(float) * (size_t)nslices * p.g.n * p.g.n;
 }
 
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, float* scratch, hipStream_t stream) {
   const int n = p.g.n;
-  const int nl = ss_rt_lines(n, p.g.nrays);
-  if (nl > 0) {
-    const size_t lds = ss_rt_line_lds(n, p.g.nrays, nl);
-    const int groups = (n + nl - 1) / nl;
-    hipLaunchKernelGGL(k_radon_transpose_lines, dim3(2 * groups, nslices), dim3(nl * SS_WAVE), lds, stream, p, sinoT,
-                       j0, j1, out, scratch);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int t = (n + 31) / 32;
-    hipLaunchKernelGGL(k_transpose_add, dim3(t, t, nslices), dim3(256), 0, stream, scratch, out, n);
-  } else {
-    dim3 grid((n + RT_T - 1) / RT_T, (n + RT_T - 1) / RT_T, nslices);
-    hipLaunchKernelGGL(k_radon_transpose, grid, dim3(RT_WAVES * SS_WAVE), 0, stream, p, sinoT, j0, j1, out);
-  }
+  const int nl = ss_rt_lines(n);
+  if (nl == 0) return hipErrorInvalidConfiguration;  // a line does not fit the LDS
+  const size_t lds = ss_rt_line_lds(n, nl);
+  const int groups = (n + nl - 1) / nl;
+  hipLaunchKernelGGL(k_radon_transpose_lines, dim3(2 * groups, nslices), dim3(nl * SS_WAVE), lds, stream, p, sinoT,
+                     j0, j1, out, scratch);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int t = (n + 31) / 32;
+  hipLaunchKernelGGL(k_transpose_add, dim3(t, t, nslices), dim3(256), 0, stream, scratch, out, n);
+  e = hipGetLastError();
   if (e != hipSuccess || p.nirregular == 0) return e;
   hipLaunchKernelGGL(k_radon_transpose_fixup, dim3(p.nirregular, nslices), dim3(256), 0, stream, p, sinoT, j0, j1,
                      out);

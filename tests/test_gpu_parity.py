@@ -134,6 +134,36 @@ def test_transpose_adjoint(torch_mod, geom):
     assert rel_l2(Rty, ref) <= REL_L2
 
 
+DENSE_GEOMS = [
+    # rays 0.32-0.45 cells apart: lanes 3-4 apart update the line in turn
+    (48, 150, 60, (-1.0, 1.0, 1.0, -1.0), (0.0, 1.0, math.pi, -1.0)),
+    # rays 0.08-0.11 cells apart: below 1/8 cell the LDS atomic form
+    (32, 400, 40, (-1.0, 1.0, 1.0, -1.0), (0.0, 1.0, math.pi, -1.0)),
+    # rays 0.9-1.3 cells apart over 2 pi, long lines: interior passes without range reads
+    (300, 200, 90, (-1.0, 1.0, 1.0, -1.0), (0.0, 0.6, 2 * math.pi, -0.6)),
+]
+
+
+@pytest.mark.parametrize("geom", DENSE_GEOMS, ids=lambda g: "n%d_R%d_A%d" % g[:3])
+def test_transpose_dense_rays_and_interior_passes(torch_mod, geom):
+    torch = torch_mod
+    n, R, A, ic, sc = geom
+    print("transpose", geom, flush=True)
+    rng = np.random.default_rng(29)
+    x = rng.random((n, n)).astype(np.float32)
+    y = rng.random((R, A)).astype(np.float32)
+    p = _plan(n, R, A, ic, sc)
+    Rx = p.radon(torch.from_numpy(x).cuda()).cpu().numpy().astype(np.float64)
+    Rty = p.radon_transpose(torch.from_numpy(y).cuda()).cpu().numpy().astype(np.float64)
+    lhs = float(np.sum(Rx * y.astype(np.float64)))
+    rhs = float(np.sum(x.astype(np.float64) * Rty))
+    mism = abs(lhs - rhs) / abs(lhs)
+    print("adjoint mismatch", mism)
+    assert mism <= ADJOINT
+    ref = O.radon_transpose(y.astype(np.float64), n, **_geo(ic, sc))
+    assert rel_l2(Rty, ref) <= REL_L2
+
+
 def test_radon_random_ellipses_512(torch_mod):
     torch = torch_mod
     n = R = 512
@@ -201,13 +231,13 @@ def test_phantom_device(torch_mod):
     assert np.array_equal(got, ref.astype(np.float32))
 
 
-def test_transpose_wide_image_tiled_path(torch_mod):
-    """Images too wide for one LDS line per wave take the tiled transpose kernel:
-    same exactness bars (adjoint identity, oracle)."""
+def test_transpose_wide_image(torch_mod):
+    """A wide image (3000-pixel lines in LDS, the ray ranges and sinogram values read
+    from global memory): same exactness bars (adjoint identity, oracle)."""
     torch = torch_mod
     n = R = 3000
     A = 24
-    print("transpose tiled path", n, R, A, flush=True)
+    print("transpose wide image", n, R, A, flush=True)
     rng = np.random.default_rng(17)
     x = rng.random((n, n)).astype(np.float32)
     y = rng.random((R, A)).astype(np.float32)
