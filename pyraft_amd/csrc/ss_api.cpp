@@ -431,9 +431,19 @@ int ensure_ramp(raft_ss_plan* p, hipStream_t stream) {
   float* d = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(float) * R));
   HIP_TRY(hipMemcpyAsync(d, gk.data(), sizeof(float) * R, hipMemcpyHostToDevice, stream));
+  if (const size_t bytes = ss_fft_taps_bytes(R)) {
+    HIP_TRY(hipMalloc(&p->d_ramp_fft, bytes));
+    HIP_TRY(ss_fft_taps_build(d, R, false, p->d_ramp_fft, &p->ramp_fft, stream));
+  }
   HIP_TRY(hipStreamSynchronize(stream));
   p->d_ramp = d;
   return 0;
+}
+
+// the plan's FBP filter over `nrows` angle rows: the cached FFT form where it fits
+hipError_t plan_filter(const raft_ss_plan* p, const float* in, float* out, int nrows, hipStream_t stream) {
+  if (p->d_ramp_fft) return ss_launch_fft_filter(in, out, p->ramp_fft, nrows, stream);
+  return ss_launch_ramp_filter(in, out, p->d_ramp, p->g.nrays, nrows, stream);
 }
 }  // namespace
 
@@ -552,7 +562,7 @@ extern "C" int raft_ss_fbp_filter(raft_ss_plan* p, const float* d_sino, float* d
   const int R = p->g.nrays, A = p->g.nangles;
   const size_t total = (size_t)nslices * R * A;
   if (angle_major) {
-    HIP_TRY(ss_launch_ramp_filter(d_sino, d_out, p->d_ramp, R, nslices * A, stream));
+    HIP_TRY(plan_filter(p, d_sino, d_out, nslices * A, stream));
     return 0;
   }
   DevBuf ws;
@@ -560,7 +570,7 @@ extern "C" int raft_ss_fbp_filter(raft_ss_plan* p, const float* d_sino, float* d
   float* t0 = ws.as<float>();
   float* t1 = t0 + total;
   HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
-  HIP_TRY(ss_launch_ramp_filter(t0, t1, p->d_ramp, R, nslices * A, stream));
+  HIP_TRY(plan_filter(p, t0, t1, nslices * A, stream));
   HIP_TRY(ss_launch_t_to_sino(t1, d_out, R, A, nslices, stream));
   return 0;
 }
@@ -579,7 +589,7 @@ extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image,
   float* t0 = ws.as<float>();
   float* t1 = t0 + total;
   HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
-  HIP_TRY(ss_launch_ramp_filter(t0, t1, p->d_ramp, R, nslices * A, stream));
+  HIP_TRY(plan_filter(p, t0, t1, nslices * A, stream));
   SSEmUpdateArgs none{nullptr, nullptr};
   HIP_TRY(ss_launch_backprojection(p->dev(), t1, 0, A, nslices, SS_BP_PLAIN, d_image, none, stream));
   return 0;

@@ -47,6 +47,8 @@ struct raft_ss_plan {
   int bp_window_v[SS_BP_NVARIANTS] = {0};  // per backprojection variant: staged window
   int bp_angles_v[SS_BP_NVARIANTS] = {0};  //   and angles per stage (0: not runnable)
   float* d_ramp = nullptr;  // FBP convolution kernel, built on first use
+  void* d_ramp_fft = nullptr;  // its FFT-form spectrum and twiddles (if R fits the FFT form)
+  SSFftTaps ramp_fft{};
   std::mutex ramp_mutex;
   SSDevPlan dev() const { return dev(bp_variant); }
   // the plan as the kernels see it, with backprojection variant v
@@ -73,6 +75,7 @@ struct raft_ss_plan {
     if (d_irregular) (void)hipFree(d_irregular);
     if (d_ray_t) (void)hipFree(d_ray_t);
     if (d_ray_alpha) (void)hipFree(d_ray_alpha);
+    if (d_ramp_fft) (void)hipFree(d_ramp_fft);
     if (d_ramp) (void)hipFree(d_ramp);
   }
 };

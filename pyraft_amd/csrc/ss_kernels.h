@@ -126,6 +126,19 @@ int ss_batch_grid(int kind, int variant, int mode, int total);
 enum { SS_KIND_GATHER = 0, SS_KIND_TRANSPOSE, SS_KIND_RADON, SS_KIND_FIXUP, SS_KIND_BP };
 
 // ---- FBP row filter (rows of length R, circular convolution with g)
+// FFT form of the row filter (ss_kernels.hip): the spectrum of a tap set, built on the
+// device, and the filter launch.  Sizes up to R = SS_FFT_MAX / 2 (two M-point FP64
+// complex buffers in LDS); ss_launch_ramp_filter takes this form where it fits.
+#define SS_FFT_MAX 4096
+struct SSFftTaps {
+  const double2* G;  // [M] FFT of the zero-padded taps
+  const double2* W;  // [M] exp(-2 pi i k / M)
+  int R, M, lgM;
+};
+int ss_fft_size(int R, int* lg);     // M = the power of two >= 2R (0 if above SS_FFT_MAX)
+size_t ss_fft_taps_bytes(int R);     // device bytes of W and G (0 if the FFT form does not fit)
+hipError_t ss_fft_taps_build(const float* g, int R, bool toeplitz, void* mem, SSFftTaps* t, hipStream_t stream);
+hipError_t ss_launch_fft_filter(const float* in, float* out, const SSFftTaps& t, int nrows, hipStream_t stream);
 hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream,
                                  bool toeplitz = false);
 

@@ -1316,10 +1316,193 @@ __global__ __launch_bounds__(256) void k_ramp_filter(const float* __restrict__ i
 
 hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream,
                                  bool toeplitz) {
+  if (nrows <= 0) return hipSuccess;
+  if (const size_t bytes = ss_fft_taps_bytes(R)) {
+    // FFT form, its spectrum in stream-ordered scratch
+    void* mem = nullptr;
+    hipError_t e = hipMallocAsync(&mem, bytes, stream);
+    if (e != hipSuccess) return e;
+    SSFftTaps t;
+    e = ss_fft_taps_build(g, R, toeplitz, mem, &t, stream);
+    if (e == hipSuccess) e = ss_launch_fft_filter(in, out, t, nrows, stream);
+    const hipError_t f = hipFreeAsync(mem, stream);
+    return e != hipSuccess ? e : f;
+  }
   const size_t lds = (size_t)(((2 * R + 3) & ~3) + RF_ROWS * R) * sizeof(float);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_ramp_filter, dim3((nrows + RF_ROWS - 1) / RF_ROWS), dim3(256), lds, stream, in, out, g, R,
                      nrows, toeplitz ? 1 : 0);
+  return hipGetLastError();
+}
+
+// =====================================================================================
+// FFT form of the row filter.  With taps2[j] (j in [0, 2R)) the filter is
+//   y[n] = sum_m x[m] taps2[n - m + R],   n, m in [0, R),
+// where n - m + R stays in [1, 2R - 1].  Zero-padded to a power of two M >= 2R, the
+// M-point circular convolution c = x (*) taps2 holds y[n] = c[n + R]: no term wraps.
+// Two rows go into one complex sequence (row 2i real, row 2i + 1 imaginary); the taps
+// are real, so the real and imaginary parts of c are the two rows' convolutions.
+// A block computes c = IFFT(FFT(z) * G) in LDS with G = FFT(taps2), both transforms
+// Stockham autosort passes (radix 4, one radix-2 pass first when log2 M is odd) over
+// a table W[k] = exp(-2 pi i k / M).  The transforms run in FP64: FP32 rounding in the
+// spectrum of a smooth row (large low-frequency bins, which the ramp suppresses) would
+// reach 1e-5 of the filtered row at R = 2048.
+// =====================================================================================
+
+#define FF_THREADS 512
+
+__device__ __forceinline__ double2 ss_zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 ss_zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 ss_zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// The M-point DFT (INV: with exp(+2 pi i k / M), unnormalised) of the sequence in `a`,
+// with `b` as the second buffer; returns the buffer holding the result.  Every thread
+// of the block takes part (the passes end with block barriers).
+template <bool INV>
+__device__ double2* ss_fft_lds(double2* a, double2* b, const double2* __restrict__ W, int M, int lgM) {
+  const int tid = threadIdx.x;
+  int Ns = 1;
+  if (lgM & 1) {
+    const int h = M >> 1;
+    for (int j = tid; j < h; j += FF_THREADS) {
+      const double2 v0 = a[j], v1 = a[j + h];
+      b[2 * j] = ss_zadd(v0, v1);
+      b[2 * j + 1] = ss_zsub(v0, v1);
+    }
+    __syncthreads();
+    double2* t = a;
+    a = b;
+    b = t;
+    Ns = 2;
+  }
+  const int q = M >> 2;
+  for (; Ns < M; Ns <<= 2) {
+    const int ts = q / Ns;  // twiddle stride: exp(-2 pi i r k / (4 Ns)) = W[r k ts]
+    for (int j = tid; j < q; j += FF_THREADS) {
+      const int k = j & (Ns - 1);
+      double2 w1 = W[k * ts], w2 = W[2 * k * ts], w3 = W[3 * k * ts];
+      if (INV) {
+        w1.y = -w1.y;
+        w2.y = -w2.y;
+        w3.y = -w3.y;
+      }
+      const double2 v0 = a[j], v1 = ss_zmul(a[j + q], w1), v2 = ss_zmul(a[j + 2 * q], w2),
+                    v3 = ss_zmul(a[j + 3 * q], w3);
+      const double2 a0 = ss_zadd(v0, v2), a1 = ss_zsub(v0, v2), a2 = ss_zadd(v1, v3), d = ss_zsub(v1, v3);
+      const double2 a3 = INV ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);  // d * (+-i)
+      const int o = (j - k) * 4 + k;
+      b[o] = ss_zadd(a0, a2);
+      b[o + Ns] = ss_zadd(a1, a3);
+      b[o + 2 * Ns] = ss_zsub(a0, a2);
+      b[o + 3 * Ns] = ss_zsub(a1, a3);
+    }
+    __syncthreads();
+    double2* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+__global__ __launch_bounds__(256) void k_fft_twiddles(double2* __restrict__ W, int M) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  double sv, cv;
+  sincospi(-2.0 * (double)k / (double)M, &sv, &cv);
+  W[k] = make_double2(cv, sv);
+}
+
+// G = FFT(taps2 zero-padded to M); taps2[j] = g[j] (toeplitz) or g[j mod R]
+__global__ __launch_bounds__(FF_THREADS) void k_fft_taps(const float* __restrict__ g, int R, int toeplitz,
+                                                         const double2* __restrict__ W, int M, int lgM,
+                                                         double2* __restrict__ G) {
+  extern __shared__ double2 zb[];
+  double2* a = zb;
+  for (int m = threadIdx.x; m < M; m += FF_THREADS) {
+    double v = 0.0;
+    if (m < 2 * R) v = toeplitz ? (m > 0 ? g[m] : 0.f) : g[m < R ? m : m - R];
+    a[m] = make_double2(v, 0.0);
+  }
+  __syncthreads();
+  const double2* f = ss_fft_lds<false>(a, zb + M, W, M, lgM);
+  for (int m = threadIdx.x; m < M; m += FF_THREADS) G[m] = f[m];
+}
+
+__global__ __launch_bounds__(FF_THREADS) void k_fft_filter(const float* __restrict__ in, float* __restrict__ out,
+                                                           const double2* __restrict__ G,
+                                                           const double2* __restrict__ W, int R, int M, int lgM,
+                                                           int nrows) {
+  extern __shared__ double2 zb[];
+  double2* a = zb;
+  double2* b = zb + M;
+  const int r0 = 2 * blockIdx.x;
+  const bool two = r0 + 1 < nrows;
+  const float* x0 = in + (size_t)r0 * R;
+  for (int m = threadIdx.x; m < M; m += FF_THREADS) {
+    double2 z = make_double2(0.0, 0.0);
+    if (m < R) {
+      z.x = x0[m];
+      if (two) z.y = x0[R + m];
+    }
+    a[m] = z;
+  }
+  __syncthreads();
+  double2* f = ss_fft_lds<false>(a, b, W, M, lgM);
+  for (int m = threadIdx.x; m < M; m += FF_THREADS) f[m] = ss_zmul(f[m], G[m]);
+  __syncthreads();
+  const double2* c = ss_fft_lds<true>(f, f == a ? b : a, W, M, lgM);
+  const double scale = 1.0 / (double)M;
+  float* y0 = out + (size_t)r0 * R;
+  for (int n = threadIdx.x; n < R; n += FF_THREADS) {
+    const double2 v = c[n + R];
+    y0[n] = (float)(v.x * scale);
+    if (two) y0[R + n] = (float)(v.y * scale);
+  }
+}
+
+int ss_fft_size(int R, int* lg) {
+  int M = 4, l = 2;
+  while (M < 2 * R) {
+    M <<= 1;
+    ++l;
+  }
+  if (M > SS_FFT_MAX) return 0;
+  if (lg) *lg = l;
+  return M;
+}
+
+size_t ss_fft_taps_bytes(int R) {
+  const int M = ss_fft_size(R, nullptr);
+  return M ? 2 * sizeof(double2) * (size_t)M : 0;
+}
+
+hipError_t ss_fft_taps_build(const float* g, int R, bool toeplitz, void* mem, SSFftTaps* t, hipStream_t stream) {
+  int lg = 0;
+  const int M = ss_fft_size(R, &lg);
+  if (!M) return hipErrorInvalidValue;
+  double2* W = reinterpret_cast<double2*>(mem);
+  double2* G = W + M;
+  hipLaunchKernelGGL(k_fft_twiddles, dim3((M + 255) / 256), dim3(256), 0, stream, W, M);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fft_taps, dim3(1), dim3(FF_THREADS), 2 * sizeof(double2) * (size_t)M, stream, g, R,
+                     toeplitz ? 1 : 0, W, M, lg, G);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  t->W = W;
+  t->G = G;
+  t->R = R;
+  t->M = M;
+  t->lgM = lg;
+  return hipSuccess;
+}
+
+hipError_t ss_launch_fft_filter(const float* in, float* out, const SSFftTaps& t, int nrows, hipStream_t stream) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fft_filter, dim3((nrows + 1) / 2), dim3(FF_THREADS), 2 * sizeof(double2) * (size_t)t.M,
+                     stream, in, out, t.G, t.W, t.R, t.M, t.lgM, nrows);
   return hipGetLastError();
 }
 

@@ -156,3 +156,33 @@ def test_gpu_filter1d_ramp_and_fbp():
     for L in (1e-4, 1e-2):
         out = pr.lowpassino_reg_fbp(y, L)
         assert out.top_left == y.top_left and rel_l2(out, O.lowpassino_reg_fbp(base, L, tl, br)) <= REL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("R,nrows", [(2, 3), (7, 5), (96, 1), (333, 9), (1024, 6), (2048, 3), (2049, 2)])
+def test_gpu_row_filter_forms(R, nrows):
+    """The device row filter (FFT convolution up to R = 2048, the direct sum above) against
+    an FP64 direct evaluation of the same taps: circular (lowpassino kernels) and Toeplitz
+    (raft_filter1d_ramp), odd sizes and odd row counts (a block holding one row)."""
+    import torch
+    assert torch.cuda.is_available()
+    from pyraft_amd.device import current_stream_handle, filter_rows
+    from pyraft_amd.raftypes import check, libraft
+    print("row filter", R, nrows, flush=True)
+    rng = np.random.default_rng(R)
+    x = rng.random((nrows, R)).astype(np.float32)
+    g = np.zeros(R, np.float32)
+    assert libraft.raft_ss_filter_kernel(R, -1.0, 1.0, 0, 0.0, g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+    got = filter_rows(torch.from_numpy(x).cuda(), torch.from_numpy(g).cuda()).cpu().numpy()
+    n = np.arange(R)
+    C = g.astype(np.float64)[(n[None, :] - n[:, None]) % R]        # C[m][n] = g[(n - m) mod R]
+    assert rel_l2(got, x.astype(np.float64) @ C) <= REL
+    taps = np.zeros(2 * R, np.float32)
+    assert libraft.raft_ss_filter1d_ramp_taps(R, 0.8, -2.0 / (R - 1),
+                                              taps.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+    xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(taps).cuda()
+    out = torch.empty_like(xd)
+    check(libraft.raft_ss_filter_rows_toeplitz(xd.data_ptr(), out.data_ptr(), td.data_ptr(), R, nrows,
+                                               current_stream_handle(xd.device)), "raft_ss_filter_rows_toeplitz")
+    T = taps.astype(np.float64)[(n[None, :] - n[:, None]) + R]       # T[m][n] = taps[n - m + R]
+    assert rel_l2(out.cpu().numpy(), x.astype(np.float64) @ T) <= REL
