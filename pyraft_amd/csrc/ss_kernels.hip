@@ -965,27 +965,30 @@ __device__ float ss_struct_load_f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int
 // In the first two every cell takes its terms in a fixed order (deterministic sums).
 enum { RT_DISTINCT = 0, RT_PHASED = 1, RT_ATOMIC = 2 };
 
+// A wave owns image line s of SL slices (SL = 2 shares the geometry work of a pass
+// between two slices of a batch; every slice has its own LDS line and sinogram row).
 struct SSRtLine {
-  float* line;               // LDS line: cell c + 1 holds pixel c; cells n+2, n+3 are spare
+  float* line[2];            // LDS lines: cell c + 1 holds pixel c; cells n+2, n+3 are spare
   int n, s;                  // line length, line index (the forward's sample)
   int k_lo, k_hi;            // candidate rays
   bool all_edge;             // every sample reads its ray's exact range
   int p_first, p_last;       // passes p_first .. p_last - 1 hold only valid samples
   int phases, my_phase;      // RT_PHASED
-  __amdgpu_buffer_rsrc_t rb, rq;  // the angle's range row (int2) and sinogram row (float)
+  __amdgpu_buffer_rsrc_t rb;     // the angle's range row (int2)
+  __amdgpu_buffer_rsrc_t rq[2];  // the angle's sinogram row (float) of each slice
   int64_t gamma;             // trajectory step per ray
   int64_t u0, step;          // trajectory (+1 cell) of ray k_lo, and its step per pass
   float delta;
 };
 
-// RT_PHASED / RT_ATOMIC update of one pass: ray k (this lane) at trajectory u, its term
-// qd = q * delta (0 if the sample is not valid)
+// RT_PHASED / RT_ATOMIC update of one pass on one line: trajectory u, term qd = q * delta
+// (0 if the sample is not valid)
 template <int MODE>
-__device__ __forceinline__ void ss_rt_update(const SSRtLine& L, bool valid, int64_t u, float qd) {
+__device__ __forceinline__ void ss_rt_update(const SSRtLine& L, float* line, bool valid, int64_t u, float qd) {
   const int idx = (int)(u >> 32);
   const float wq = (float)(uint32_t)u * SS_INV_FIX * qd;
   if (MODE == RT_PHASED) {
-    float* cell = L.line + idx;
+    float* cell = line + idx;
     for (int ph = 0; ph < L.phases; ++ph) {
       const bool mine = valid && L.my_phase == ph;
       if (mine) cell[0] += qd - wq;
@@ -994,22 +997,22 @@ __device__ __forceinline__ void ss_rt_update(const SSRtLine& L, bool valid, int6
       __asm__ volatile("" ::: "memory");
     }
   } else if (valid) {
-    atomicAdd(L.line + idx, qd - wq);
-    atomicAdd(L.line + idx + 1, wq);
+    atomicAdd(line + idx, qd - wq);
+    atomicAdd(line + idx + 1, wq);
   }
 }
 
 // One iteration over two passes (pass, pass + 1) of the candidate rays: lane 0's ray kb and
-// trajectory ub (+1 cell), the passes' sinogram values q0, q1; loads the values of the
-// next iteration's passes into n0, n1 (indices clamped to the last candidate: the loads
-// past the last pass stay in bounds and go unused).
+// trajectory ub (+1 cell), the passes' sinogram values q0[], q1[] per slice; loads the
+// values of the next iteration's passes into n0[], n1[] (indices clamped to the last
+// candidate: the loads past the last pass stay in bounds and go unused).
 //
 // Passes p_first .. p_last - 1 hold only valid samples (see the kernel); a pair of them
 // takes a path without range checks.  Other samples near the line's ends (every sample
 // of other angles) read the ray's exact range.
-template <int MODE>
+template <int MODE, int SL>
 __device__ __forceinline__ void ss_rt_pair(const SSRtLine& L, int lane, int pass, int npass, int kb, int64_t ub,
-                                           int64_t lane_u, float q0, float q1, float& n0, float& n1) {
+                                           int64_t lane_u, const float* q0, const float* q1, float* n0, float* n1) {
   const int n = L.n, s = L.s, k_hi = L.k_hi, last = k_hi - 1;
   const int64_t ub1 = ub + L.step;
   const int64_t u = ub + lane_u, u1 = ub1 + lane_u;
@@ -1017,20 +1020,40 @@ __device__ __forceinline__ void ss_rt_pair(const SSRtLine& L, int lane, int pass
   const int k = kb + lane;
   if (MODE == RT_DISTINCT && pass >= L.p_first && pass + 1 < L.p_last) {
     // every sample of both passes is valid
-    n0 = ss_struct_load_f32(L.rq, min(k + 2 * SS_WAVE, last), 0, 0, 0);
-    n1 = ss_struct_load_f32(L.rq, min(k + 3 * SS_WAVE, last), 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      n0[i] = ss_struct_load_f32(L.rq[i], min(k + 2 * SS_WAVE, last), 0, 0, 0);
+      n1[i] = ss_struct_load_f32(L.rq[i], min(k + 3 * SS_WAVE, last), 0, 0, 0);
+    }
     const float ds = L.delta * SS_INV_FIX;
-    const float w0 = (float)(uint32_t)u * (q0 * ds);
-    const float w1 = (float)(uint32_t)u1 * (q1 * ds);
-    float* c0 = L.line + idx0;
-    float* c1 = L.line + idx1;
-    const float x0 = c0[0], x1 = c1[0];
-    c0[0] = fmaf(q0, L.delta, x0) - w0;
-    c1[0] = fmaf(q1, L.delta, x1) - w1;
+    const float f0 = (float)(uint32_t)u, f1 = (float)(uint32_t)u1;
+    float w0[SL], w1[SL], x0[SL], x1[SL];
+    float *c0[SL], *c1[SL];
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      w0[i] = f0 * (q0[i] * ds);
+      w1[i] = f1 * (q1[i] * ds);
+      c0[i] = L.line[i] + idx0;
+      c1[i] = L.line[i] + idx1;
+      x0[i] = c0[i][0];
+      x1[i] = c1[i][0];
+    }
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      c0[i][0] = fmaf(q0[i], L.delta, x0[i]) - w0[i];
+      c1[i][0] = fmaf(q1[i], L.delta, x1[i]) - w1[i];
+    }
     __asm__ volatile("" ::: "memory");
-    const float y0 = c0[1], y1 = c1[1];
-    c0[1] = y0 + w0;
-    c1[1] = y1 + w1;
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      x0[i] = c0[i][1];
+      x1[i] = c1[i][1];
+    }
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      c0[i][1] = x0[i] + w0[i];
+      c1[i][1] = x1[i] + w1[i];
+    }
     __asm__ volatile("" ::: "memory");
     return;
   }
@@ -1041,53 +1064,81 @@ __device__ __forceinline__ void ss_rt_pair(const SSRtLine& L, int lane, int pass
   if (e0) b0 = ss_struct_load_i32x2(L.rb, min(k, last), 0, 0, 0);
   if (e1) b1 = ss_struct_load_i32x2(L.rb, min(k + SS_WAVE, last), 0, 0, 0);
   // issued after the ranges, so that waiting for the ranges does not wait for these
-  n0 = ss_struct_load_f32(L.rq, min(k + 2 * SS_WAVE, last), 0, 0, 0);
-  n1 = ss_struct_load_f32(L.rq, min(k + 3 * SS_WAVE, last), 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < SL; ++i) {
+    n0[i] = ss_struct_load_f32(L.rq[i], min(k + 2 * SS_WAVE, last), 0, 0, 0);
+    n1[i] = ss_struct_load_f32(L.rq[i], min(k + 3 * SS_WAVE, last), 0, 0, 0);
+  }
   bool v0 = k < k_hi, v1 = k + SS_WAVE < k_hi;
   if (e0) v0 = v0 && s >= b0.x && s <= b0.y;
   if (e1) v1 = v1 && s >= b1.x && s <= b1.y;
-  const float qd0 = v0 ? q0 * L.delta : 0.f, qd1 = v1 ? q1 * L.delta : 0.f;
   if (MODE == RT_DISTINCT) {
     // the first cells of both passes are distinct, and so are their second cells: read
     // both, then write both, one cell phase at a time; lanes without a sample add zero to
     // the spare cells
-    const float w0 = (float)(uint32_t)u * SS_INV_FIX * qd0;
-    const float w1 = (float)(uint32_t)u1 * SS_INV_FIX * qd1;
-    float* c0 = L.line + (v0 ? idx0 : n + 2);
-    float* c1 = L.line + (v1 ? idx1 : n + 2);
-    const float x0 = c0[0], x1 = c1[0];
-    c0[0] = x0 + (qd0 - w0);
-    c1[0] = x1 + (qd1 - w1);
+    const float f0 = (float)(uint32_t)u * SS_INV_FIX, f1 = (float)(uint32_t)u1 * SS_INV_FIX;
+    const int i0 = v0 ? idx0 : n + 2, i1 = v1 ? idx1 : n + 2;
+    float qd0[SL], qd1[SL], w0[SL], w1[SL], x0[SL], x1[SL];
+    float *c0[SL], *c1[SL];
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      qd0[i] = v0 ? q0[i] * L.delta : 0.f;
+      qd1[i] = v1 ? q1[i] * L.delta : 0.f;
+      w0[i] = f0 * qd0[i];
+      w1[i] = f1 * qd1[i];
+      c0[i] = L.line[i] + i0;
+      c1[i] = L.line[i] + i1;
+      x0[i] = c0[i][0];
+      x1[i] = c1[i][0];
+    }
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      c0[i][0] = x0[i] + (qd0[i] - w0[i]);
+      c1[i][0] = x1[i] + (qd1[i] - w1[i]);
+    }
     __asm__ volatile("" ::: "memory");
-    const float y0 = c0[1], y1 = c1[1];
-    c0[1] = y0 + w0;
-    c1[1] = y1 + w1;
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      x0[i] = c0[i][1];
+      x1[i] = c1[i][1];
+    }
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      c0[i][1] = x0[i] + w0[i];
+      c1[i][1] = x1[i] + w1[i];
+    }
     __asm__ volatile("" ::: "memory");
   } else {
-    ss_rt_update<MODE>(L, v0, u, qd0);
-    if (two) ss_rt_update<MODE>(L, v1, u1, qd1);
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      ss_rt_update<MODE>(L, L.line[i], v0, u, v0 ? q0[i] * L.delta : 0.f);
+      if (two) ss_rt_update<MODE>(L, L.line[i], v1, u1, v1 ? q1[i] * L.delta : 0.f);
+    }
   }
 }
 
 // the passes over the candidate rays, two per ss_rt_pair; the sinogram values alternate
-// between two register pairs (each half loads the other half's values)
-template <int MODE>
+// between two register sets (each half loads the other half's values)
+template <int MODE, int SL>
 __device__ __forceinline__ void ss_rt_passes(const SSRtLine& L, int lane) {
   const int npass = (L.k_hi - L.k_lo + SS_WAVE - 1) / SS_WAVE;
   const int last = L.k_hi - 1;
   const int64_t lane_u = (int64_t)lane * L.gamma;
   int64_t ub = L.u0;  // lane 0's trajectory (+1 cell)
   int kb = L.k_lo;    // lane 0's ray
-  float qa = ss_struct_load_f32(L.rq, min(kb + lane, last), 0, 0, 0);
-  float qb = ss_struct_load_f32(L.rq, min(kb + SS_WAVE + lane, last), 0, 0, 0);
-  float qc, qd;
+  float qa[SL], qb[SL], qc[SL], qd[SL];
+#pragma unroll
+  for (int i = 0; i < SL; ++i) {
+    qa[i] = ss_struct_load_f32(L.rq[i], min(kb + lane, last), 0, 0, 0);
+    qb[i] = ss_struct_load_f32(L.rq[i], min(kb + SS_WAVE + lane, last), 0, 0, 0);
+  }
   for (int pass = 0;;) {
-    ss_rt_pair<MODE>(L, lane, pass, npass, kb, ub, lane_u, qa, qb, qc, qd);
+    ss_rt_pair<MODE, SL>(L, lane, pass, npass, kb, ub, lane_u, qa, qb, qc, qd);
     pass += 2;
     kb += 2 * SS_WAVE;
     ub += 2 * L.step;
     if (pass >= npass) break;
-    ss_rt_pair<MODE>(L, lane, pass, npass, kb, ub, lane_u, qc, qd, qa, qb);
+    ss_rt_pair<MODE, SL>(L, lane, pass, npass, kb, ub, lane_u, qc, qd, qa, qb);
     pass += 2;
     kb += 2 * SS_WAVE;
     ub += 2 * L.step;
@@ -1095,8 +1146,11 @@ __device__ __forceinline__ void ss_rt_passes(const SSRtLine& L, int lane) {
   }
 }
 
+// slices slice0 + blockIdx.y * SL .. + SL - 1
+template <int SL>
 __global__ __launch_bounds__(RT_LINES * SS_WAVE) void k_radon_transpose_lines(SSDevPlan p, const float* __restrict__ qT,
-                                                                             int j0, int j1, float* __restrict__ outB,
+                                                                             int j0, int j1, int slice0,
+                                                                             float* __restrict__ outB,
                                                                              float* __restrict__ outA) {
   extern __shared__ float sh[];
   const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
@@ -1107,14 +1161,18 @@ __global__ __launch_bounds__(RT_LINES * SS_WAVE) void k_radon_transpose_lines(SS
   const int branch = blockIdx.x >= (unsigned)groups ? 1 : 0;
   const int s = ((int)blockIdx.x - branch * groups) * nw + wv;  // image row (B) or column (A)
   if (s >= n) return;  // waves work independently: no block barrier follows
-  const size_t slice = blockIdx.y;
-  const float* q = qT + slice * A * R;
+  const size_t slice = (size_t)slice0 + (size_t)blockIdx.y * SL;
+  const size_t sino_size = (size_t)A * R;
+  const float* q = qT + slice * sino_size;
   SSRtLine L;
   // LDS is addressed as sh + offset, so that every access compiles to an LDS instruction
-  L.line = sh + wv * ss_rt_line_stride(n);
+#pragma unroll
+  for (int i = 0; i < SL; ++i) {
+    L.line[i] = sh + (wv * SL + i) * ss_rt_line_stride(n);
+    for (int c = lane; c < ss_rt_line_stride(n); c += SS_WAVE) L.line[i][c] = 0.f;
+  }
   L.n = n;
   L.s = s;
-  for (int i = lane; i < ss_rt_line_stride(n); i += SS_WAVE) L.line[i] = 0.f;
   const ss_const_angles angles = (ss_const_angles)p.angles;
   for (int j = ss_rt_next(angles, j0, j1, branch); j < j1; j = ss_rt_next(angles, j + 1, j1, branch)) {
     const __attribute__((address_space(4))) SSAngle& a = angles[j];
@@ -1152,29 +1210,34 @@ __global__ __launch_bounds__(RT_LINES * SS_WAVE) void k_radon_transpose_lines(SS
       }
     }
     L.delta = a.delta;
-    // record k of the angle's range row (8 bytes) and sinogram row (4 bytes); every index
+    // record k of the angle's range row (8 bytes) and sinogram rows (4 bytes); every index
     // used is clamped to [0, R-1]
     L.rb = __builtin_amdgcn_make_buffer_rsrc((void*)(p.bounds + (size_t)j * R), (short)sizeof(int2),
                                              (int)((uint32_t)R * 8u), SS_RSRC_DWORD3);
-    L.rq = __builtin_amdgcn_make_buffer_rsrc((void*)(q + (size_t)j * R), (short)sizeof(float),
-                                             (int)((uint32_t)R * 4u), SS_RSRC_DWORD3);
+#pragma unroll
+    for (int i = 0; i < SL; ++i)
+      L.rq[i] = __builtin_amdgcn_make_buffer_rsrc((void*)(q + i * sino_size + (size_t)j * R), (short)sizeof(float),
+                                                  (int)((uint32_t)R * 4u), SS_RSRC_DWORD3);
     L.gamma = gamma;
     L.u0 = a.alpha0 + (int64_t)s * a.beta + (int64_t)k_lo * gamma + ((int64_t)1 << 32);
     L.step = (int64_t)SS_WAVE * gamma;
     const uint64_t ag = (uint64_t)(gamma < 0 ? -gamma : gamma);
     if (ag >= (uint64_t)SS_FIX_ONE_I64) {
-      ss_rt_passes<RT_DISTINCT>(L, lane);
+      ss_rt_passes<RT_DISTINCT, SL>(L, lane);
     } else if (ag * RT_PHASES_MAX >= (uint64_t)SS_FIX_ONE_I64) {
       // lanes `phases` apart are at least one cell apart
       L.phases = ss_rfl((int)(((uint64_t)SS_FIX_ONE_I64 + ag - 1) / ag));
       L.my_phase = lane % L.phases;
-      ss_rt_passes<RT_PHASED>(L, lane);
+      ss_rt_passes<RT_PHASED, SL>(L, lane);
     } else {
-      ss_rt_passes<RT_ATOMIC>(L, lane);
+      ss_rt_passes<RT_ATOMIC, SL>(L, lane);
     }
   }
-  float* dst = (branch ? outA : outB) + (slice * n + s) * n;
-  for (int c = lane; c < n; c += SS_WAVE) dst[c] = L.line[c + 1];
+#pragma unroll
+  for (int i = 0; i < SL; ++i) {
+    float* dst = (branch ? outA : outB) + ((slice + i) * n + s) * n;
+    for (int c = lane; c < n; c += SS_WAVE) dst[c] = L.line[i][c + 1];
+  }
 }
 
 // out[s][r][c] += t[s][c][r], 32x32 tiles
@@ -1232,12 +1295,12 @@ static size_t ss_lds_per_block() {
   return (size_t)v;
 }
 
-// lines (waves) per block for the line kernel: the most whose lines fit one block's LDS
-// (one line of the largest image, 16384 pixels, takes 64 KiB)
-static int ss_rt_lines(int n) {
+// lines (waves) per block for the line kernel holding `sl` slices per wave: the most whose
+// lines fit one block's LDS (one line of the largest image, 16384 pixels, takes 64 KiB)
+static int ss_rt_lines(int n, int sl) {
   const size_t budget = ss_lds_per_block();
   for (int nl = RT_LINES; nl >= 1; nl >>= 1)
-    if (ss_rt_line_lds(n, nl) <= budget) return nl;
+    if (ss_rt_line_lds(n, nl * sl) <= budget) return nl;
   return 0;
 }
 
@@ -1248,17 +1311,31 @@ size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices) {
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, float* scratch, hipStream_t stream) {
   const int n = p.g.n;
-  const int nl = ss_rt_lines(n);
-  if (nl == 0) return hipErrorInvalidConfiguration;  // a line does not fit the LDS
-  const size_t lds = ss_rt_line_lds(n, nl);
-  const int groups = (n + nl - 1) / nl;
-  hipLaunchKernelGGL(k_radon_transpose_lines, dim3(2 * groups, nslices), dim3(nl * SS_WAVE), lds, stream, p, sinoT,
-                     j0, j1, out, scratch);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  // two slices per wave where the image is small enough for both lines to leave room for
+  // many waves per CU (n <= 1024: 8 KiB per wave); an odd last slice goes alone
+  const int pairs = (n <= 1024) ? nslices / 2 : 0;
+  if (pairs > 0) {
+    const int nl = ss_rt_lines(n, 2);
+    if (nl == 0) return hipErrorInvalidConfiguration;
+    const int groups = (n + nl - 1) / nl;
+    hipLaunchKernelGGL(k_radon_transpose_lines<2>, dim3(2 * groups, pairs), dim3(nl * SS_WAVE),
+                       ss_rt_line_lds(n, 2 * nl), stream, p, sinoT, j0, j1, 0, out, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int rest = nslices - 2 * pairs;
+  if (rest > 0) {
+    const int nl = ss_rt_lines(n, 1);
+    if (nl == 0) return hipErrorInvalidConfiguration;  // a line does not fit the LDS
+    const int groups = (n + nl - 1) / nl;
+    hipLaunchKernelGGL(k_radon_transpose_lines<1>, dim3(2 * groups, rest), dim3(nl * SS_WAVE),
+                       ss_rt_line_lds(n, nl), stream, p, sinoT, j0, j1, 2 * pairs, out, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const int t = (n + 31) / 32;
   hipLaunchKernelGGL(k_transpose_add, dim3(t, t, nslices), dim3(256), 0, stream, scratch, out, n);
-  e = hipGetLastError();
+  const hipError_t e = hipGetLastError();
   if (e != hipSuccess || p.nirregular == 0) return e;
   hipLaunchKernelGGL(k_radon_transpose_fixup, dim3(p.nirregular, nslices), dim3(256), 0, stream, p, sinoT, j0, j1,
                      out);

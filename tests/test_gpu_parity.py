@@ -251,3 +251,19 @@ def test_transpose_wide_image(torch_mod):
     assert mism <= ADJOINT
     ref = O.radon_transpose(y.astype(np.float64), n)
     assert rel_l2(Rty, ref) <= REL_L2
+
+
+@pytest.mark.parametrize("nslices", [2, 3, 5])
+def test_transpose_slice_batches(torch_mod, nslices):
+    """Slices of a batch go two per wave (an odd last one alone): every slice equals the
+    single-slice transpose bit for bit (the same operations in the same order)."""
+    torch = torch_mod
+    n, R, A = 96, 100, 37
+    print("transpose slices", nslices, flush=True)
+    p = _plan(n, R, A)
+    y = torch.from_numpy(np.random.default_rng(nslices).random((nslices, R, A)).astype(np.float32)).cuda()
+    got = p.radon_transpose(y).cpu().numpy()
+    for i in range(nslices):
+        one = p.radon_transpose(y[i]).cpu().numpy()
+        assert np.array_equal(got[i], one), i
+    assert rel_l2(got[-1], O.radon_transpose(y[-1].cpu().numpy().astype(np.float64), n)) <= REL_L2
