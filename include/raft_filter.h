@@ -38,7 +38,9 @@ void raft_fbp(double cutoff, raft_image sino, raft_image image, raft_radon_metho
  * weight of input sample m in output sample m + d, |d| < R; taps[0] = 0), computed in FP64
  * on the host.  No reference counterpart (the reference filters through FFTW). */
 int raft_ss_filter1d_ramp_taps(int R, double cutoff, double dt, float *host_taps);
-/* Toeplitz filter of nrows contiguous length-R rows with the device taps (2R floats). */
+/* Toeplitz filter of nrows contiguous length-R rows with the device taps (2R floats);
+ * an FFT convolution up to R = 2048, a direct sum up to R = 6826 (larger R: error).
+ * Not in place. */
 int raft_ss_filter_rows_toeplitz(const float *d_in, float *d_out, const float *d_taps, int R, int nrows,
                                  void *stream);
 

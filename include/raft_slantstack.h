@@ -201,7 +201,9 @@ int raft_ss_ramp_kernel(int R, double tmin, double tmax, float *host_g);
 #define RAFT_SS_FILTER_COS_RAMP 0
 #define RAFT_SS_FILTER_TIKHONOV_RAMP 1
 int raft_ss_filter_kernel(int R, double tmin, double tmax, int kind, double param, float *host_g);
-/* Circular convolution of nrows contiguous length-R rows with the device kernel d_g. */
+/* Circular convolution of nrows contiguous length-R rows with the device kernel d_g;
+ * an FFT convolution up to R = 2048, a direct sum up to R = 6826 (larger R: error).
+ * Not in place. */
 int raft_ss_filter_rows(const float *d_in, float *d_out, const float *d_g, int R, int nrows, void *stream);
 /* filter + backprojection (pyraft/inversion/inversion.py:48-56) */
 int raft_ss_fbp(raft_ss_plan *plan, const float *d_sino, float *d_image, int nslices, void *stream);
