@@ -428,15 +428,24 @@ int ensure_ramp(raft_ss_plan* p, hipStream_t stream) {
   const int R = p->g.nrays;
   std::vector<float> gk(R);
   compute_ramp(R, p->sino_corners[3], p->sino_corners[1], gk.data());
-  float* d = nullptr;
-  HIP_TRY(hipMalloc(&d, sizeof(float) * R));
-  HIP_TRY(hipMemcpyAsync(d, gk.data(), sizeof(float) * R, hipMemcpyHostToDevice, stream));
+  struct Owned {  // freed on every early return
+    void* p = nullptr;
+    ~Owned() {
+      if (p) (void)hipFree(p);
+    }
+  } taps_mem, fft_mem;
+  HIP_TRY(hipMalloc(&taps_mem.p, sizeof(float) * R));
+  HIP_TRY(hipMemcpyAsync(taps_mem.p, gk.data(), sizeof(float) * R, hipMemcpyHostToDevice, stream));
+  SSFftTaps taps{};
   if (const size_t bytes = ss_fft_taps_bytes(R)) {
-    HIP_TRY(hipMalloc(&p->d_ramp_fft, bytes));
-    HIP_TRY(ss_fft_taps_build(d, R, false, p->d_ramp_fft, &p->ramp_fft, stream));
+    HIP_TRY(hipMalloc(&fft_mem.p, bytes));
+    HIP_TRY(ss_fft_taps_build(static_cast<const float*>(taps_mem.p), R, false, fft_mem.p, &taps, stream));
   }
   HIP_TRY(hipStreamSynchronize(stream));
-  p->d_ramp = d;
+  p->d_ramp = static_cast<float*>(taps_mem.p);
+  p->d_ramp_fft = fft_mem.p;
+  p->ramp_fft = taps;
+  taps_mem.p = fft_mem.p = nullptr;
   return 0;
 }
 
