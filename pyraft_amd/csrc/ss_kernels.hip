@@ -1731,10 +1731,14 @@ __global__ __launch_bounds__(256) void k_batch_transpose(SSBatchLaunch L, const 
   }
 }
 
+// Eight waves per SIMD (four 512-thread blocks per CU, as the per-plan k_radon gets): the
+// slice's plan values held in scalar registers would otherwise push the kernel to 87
+// VGPRs, two blocks per CU, and a second round of blocks for a 1024^2 OSEM subset.  The
+// scalar values that do not fit go to VGPR lanes (no scratch memory).
 template <int MODE>
-__global__ __launch_bounds__(512) void k_batch_radon(SSBatchLaunch L, const float2* __restrict__ gA,
-                                                     const float2* __restrict__ gB, int split, float* __restrict__ out,
-                                                     const float* __restrict__ meas, double* __restrict__ ll) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_radon(
+    SSBatchLaunch L, const float2* __restrict__ gA, const float2* __restrict__ gB, int split, float* __restrict__ out,
+    const float* __restrict__ meas, double* __restrict__ ll) {
   __shared__ SSRadonLds lds;
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
     const SSBatchEntry e = ss_entry_of(L, item);
