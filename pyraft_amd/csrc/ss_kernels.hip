@@ -775,7 +775,13 @@ __device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __re
     // a stage whose angles are all interior for this tile runs without predicates
     const bool mixed = __ballot(lane < nj && q.flag != 1) != 0ull;
     __syncthreads();  // the previous stage (or item) is done with win / par
-    if (wv == 0) lds.par[lane] = q;
+    if (wv == 0) {  // the used fields only (whole-struct copies kept the padding in scratch)
+      lds.par[lane].f0 = q.f0;
+      lds.par[lane].b = q.b;
+      lds.par[lane].c = q.c;
+      lds.par[lane].flag = q.flag;
+      lds.par[lane].base = q.base;
+    }
     // staging: wave wv fills the windows of angles wv, wv+4, ...; base is at least one
     // ray below the tile's lowest U, so every pixel has u >= 1 and (int)u < W - 1
     for (int jj = wv; jj < nj; jj += 4) {
@@ -809,7 +815,12 @@ __device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __re
       }
     } else {
       for (int jj = 0; jj < nj; ++jj) {
-        const BPAngle qa = lds.par[jj];
+        BPAngle qa;
+        qa.f0 = lds.par[jj].f0;
+        qa.b = lds.par[jj].b;
+        qa.c = lds.par[jj].c;
+        qa.flag = lds.par[jj].flag;
+        qa.base = lds.par[jj].base;
         const int flag = __builtin_amdgcn_readfirstlane(qa.flag);
         if (flag == 0) continue;
         const float2* wrow = lds.win + jj * W;
