@@ -82,6 +82,7 @@ GEOMS = [
     ((24, 24), 30, 45, (1.0, -1.0, -1.0, 1.0), (math.pi, -1.2, 0.0, 1.2)),                # flipped axes
     ((40, 40), 41, 4, DEF_IC, (0.0, 1.0, math.pi / 2, -1.0)),                            # 0, 30, 60, 90 degrees
     ((17, 29), 19, 13, DEF_IC, (0.0, 1.5, 2 * math.pi, -1.5)),                           # rays off the image
+    ((64, 20), 70, 33, (-0.5, 2.0, 0.5, -2.0), (0.1, 2.2, 3.0, -2.2)),                  # tall image
 ]
 
 
