@@ -257,8 +257,10 @@ int raft_ss_plan_tune(raft_ss_plan *plan, int reps, int flags, double *times_us,
  * slices with their own n_i, R_i, A_i and corners.  Images and sinograms are packed
  * back to back: slice i's image starts at img_off[i] floats (n_i x n_i) and its
  * sinogram at sino_off[i] floats (R_i x A_i, or A_i x R_i angle-major).  Each
- * operator runs every slice in one persistent launch per kernel (a fixed grid of
- * resident blocks walking the work items of all slices, heaviest first), and with
+ * operator runs every slice in one launch per kernel: the Radon and backprojection
+ * kernels as persistent launches (a fixed grid of resident blocks walking the work
+ * items of all slices, heaviest first), the layout kernels (gather layouts, sinogram
+ * transposes) with one block per tile of every slice; and with
  * RAFT_SS_BATCH_GRAPH the whole call is captured once into a HIP graph and
  * replayed by a single graph launch on later calls with the same arguments. */
 typedef struct raft_ss_batch raft_ss_batch;

@@ -3,8 +3,9 @@
 The reference runs one slice per call (``raft_radon_slantstack_gpu``,
 raft/raft_radon_gpu.cu:16-35; ``raft_em_gpu``, raft/raft_em_gpu.cu:20-88) and loops
 over OSEM subsets in Python (pyraft/iterative/em.py:265-320).  A ``Batch`` runs all
-slices of a batch in one persistent launch per kernel, and with ``graph=True`` the
-whole call is captured once into a HIP graph and replayed by one graph launch.
+slices of a batch in one launch per kernel (persistent, heaviest items first, for the
+Radon and backprojection), and with ``graph=True`` the whole call is captured once
+into a HIP graph and replayed by one graph launch.
 
 Data live in packed float32 device buffers: slice i's image is the n_i x n_i block
 at ``img_off[i]`` and its sinogram the R_i x A_i block (or A_i x R_i, angle-major)

@@ -47,7 +47,7 @@ static SSGeom make_geom(int n, int nrays, int nangles, const double* ic, const d
   g.n = n;
   g.nrays = nrays;
   g.nangles = nangles;
-  g.pitch = (n + 3) & ~1;  // n + 2 rounded up to even: 16-byte aligned gather rows
+  g.pitch = (n + 2 + 15) & ~15;  // n + 2 rounded up to whole 128-byte lines of records
   g.xmin = ic[0];
   g.ymin = ic[3];
   g.dx = (ic[2] - ic[0]) / (n - 1);   // raft/raft_radon_slantstack.c:150

@@ -311,7 +311,7 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
     k.e.item0 = 0;
     switch (kind) {
       case SS_KIND_GATHER: {
-        const int t = (n + 2 + 31) / 32;
+        const int t = ss_gather_tiles(n);
         k.items = t * t;
         k.cost = 1.0;
         break;

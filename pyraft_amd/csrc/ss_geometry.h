@@ -20,7 +20,7 @@ struct SSGeom {
   int n;        // image side
   int nrays;    // R
   int nangles;  // A
-  int pitch;    // row length of the Radon gather layouts: n + 2 rounded up to even
+  int pitch;    // row length of the Radon gather layouts: n + 2 rounded up to a multiple of 16
   double xmin, ymin, dx, dy;
   double tmin, dt;
   double th0, dth;

@@ -4,7 +4,7 @@
 //   image      [slice][n][n]        row-major, row 0 = ymin (bottom)
 //   sino       [slice][R][A]        the reference layout (angle contiguous)
 //   sinoT      [slice][A][R]        angle-major (detector contiguous), internal
-//   gatherA/B  [slice][n][pitch]    Radon gather layouts (pitch = n+2 rounded up to even) of float2 (lower tap, difference to the
+//   gatherA/B  [slice][n][pitch]    Radon gather layouts (pitch = n+2 rounded up to 16) of float2 (lower tap, difference to the
 //                                   upper tap) per interpolation cell; A: image columns, B: rows
 #ifndef PYRAFT_AMD_SS_KERNELS_H
 #define PYRAFT_AMD_SS_KERNELS_H
@@ -53,7 +53,7 @@ struct SSBatchSlice {
   SSDevPlan p;
   int64_t img_off;   // floats: image [n][n]
   int64_t sino_off;  // floats: sinogram of R*A values (either layout)
-  int64_t lay_off;   // float2: one gather layout [n][n+2]
+  int64_t lay_off;   // float2: one gather layout [n][pitch]
 };
 struct SSBatchEntry {
   int slice, j0, j1, item0;
@@ -73,6 +73,10 @@ hipError_t ss_launch_plan_bounds(const SSGeom& g, const SSAngle* angles, int2* b
 hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4* wbounds, hipStream_t stream);
 
 // ---- layouts
+// the gather layouts are built in square tiles of SS_GATHER_TILE records per direction;
+// records q = 0..n+1 of each row
+#define SS_GATHER_TILE 64
+static inline __host__ __device__ int ss_gather_tiles(int n) { return (n + 2 + SS_GATHER_TILE - 1) / SS_GATHER_TILE; }
 hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
                                     int nslices, hipStream_t stream);
 hipError_t ss_launch_sino_to_t(const float* sino, float* sinoT, int R, int A, int nslices, hipStream_t stream);
@@ -96,12 +100,13 @@ hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int 
                                     float* out, SSEmUpdateArgs em, hipStream_t stream);
 
 // ---- exact transpose of the Radon (reads angle-major sinogram)
-// scratch bytes the transpose needs (a transposed image per slice; 0 for the tiled form)
+// scratch bytes the transpose needs: one transposed n x n FP32 image per slice, where the
+// lines of the angles that walk image columns accumulate before k_transpose_add
 size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices);
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, float* scratch, hipStream_t stream);
 
-// ---- batched forms (items: gather = 32x32 tiles; radon = 64-ray groups x angle blocks of
+// ---- batched forms (items: gather = SS_GATHER_TILE^2 tiles; radon = 64-ray groups x angle blocks of
 //      RADON_WAVES/split angles; fixup = irregular rays; bp = pixel tiles of the variant;
 //      transpose = 32x32 tiles)
 int ss_radon_angles_per_block(int split);
@@ -121,7 +126,8 @@ hipError_t ss_launch_batch_ll(const int2* ranges, int nranges, int nslices, cons
                               hipStream_t stream);
 hipError_t ss_launch_fill_f64(double* p, double v, size_t n, hipStream_t stream);
 
-// occupancy-based resident grid of a batched kernel (blocks per CU x CUs)
+// grid of a batched kernel: the occupancy-based resident grid (blocks per CU x CUs) of the
+// Radon and backprojection kernels; one block per item for the layout kernels
 int ss_batch_grid(int kind, int variant, int mode, int total);
 enum { SS_KIND_GATHER = 0, SS_KIND_TRANSPOSE, SS_KIND_RADON, SS_KIND_FIXUP, SS_KIND_BP };
 

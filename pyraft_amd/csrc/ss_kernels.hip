@@ -109,46 +109,78 @@ hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4*
 //   gatherA[c][q] = (F(q-1, c), F(q, c) - F(q-1, c))      image columns
 // with F = the image inside [0, n)^2 and 0 outside.  Element m+1 of a row holds the
 // lower tap of interpolation cell m and the difference to the upper tap, so a Radon
-// sample is one aligned 8-byte load and a + w*d.  One tile = 32x32 values of q in
-// both directions, staged with a one-pixel halo.
+// sample is one aligned 8-byte load and a + w*d.
+//
+// One tile = SS_GATHER_TILE (64) values of q in both directions for a 256-thread block:
+// the image square with a one-pixel halo (65 x 65) goes to LDS, every halo load issued
+// before the first LDS write; then each wave writes whole 64-record rows (512
+// contiguous bytes per store) of layout B, then of layout A.  The odd LDS row length
+// puts the A phase's column reads (stride 65 words) in 64 distinct banks.
+#define SS_GT SS_GATHER_TILE
+#define SS_GT_H (SS_GATHER_TILE + 1)
 __device__ __forceinline__ void ss_gather_tile(const float* __restrict__ src, float2* __restrict__ dA,
                                                float2* __restrict__ dB, int n, int pitch, int bx, int by,
-                                               float (*T)[34]) {
-  for (int idx = threadIdx.x; idx < 33 * 33; idx += 256) {
-    const int a = idx / 33, b = idx - a * 33;
-    const int r = by - 1 + a, c = bx - 1 + b;
-    T[a][b] = (r >= 0 && r < n && c >= 0 && c < n) ? src[(size_t)r * n + c] : 0.f;
+                                               float (*T)[SS_GT_H]) {
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int LOADS = (SS_GT_H + 3) / 4;  // halo rows per wave
+  // wave wv: halo rows wv, wv + 4, ...; lane l: column bx - 1 + l, lane 0 also column bx + 63
+  const int c = bx - 1 + lane;
+  const bool c_in = c >= 0 && c < n;
+  const bool e_in = lane == 0 && bx + SS_GT - 1 < n;
+  float v[LOADS], e[LOADS];
+#pragma unroll
+  for (int i = 0; i < LOADS; ++i) {
+    const int a = wv + 4 * i;
+    const int r = by - 1 + a;
+    const bool r_in = a < SS_GT_H && r >= 0 && r < n;
+    const float* row = src + (size_t)(r_in ? r : 0) * n;
+    v[i] = (r_in && c_in) ? row[c] : 0.f;
+    e[i] = (r_in && e_in) ? row[bx + SS_GT - 1] : 0.f;
   }
-  __syncthreads();
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int i = ty; i < 32; i += 8) {
-    const int r = by + i, q = bx + tx;
-    if (r < n && q <= n + 1) {
-      const float v = T[i + 1][tx];
-      dB[(size_t)r * pitch + q] = make_float2(v, T[i + 1][tx + 1] - v);
+#pragma unroll
+  for (int i = 0; i < LOADS; ++i) {
+    const int a = wv + 4 * i;
+    if (a < SS_GT_H) {
+      T[a][lane] = v[i];
+      if (lane == 0) T[a][SS_GT] = e[i];
     }
   }
-  for (int j = ty; j < 32; j += 8) {
-    const int c = bx + j, q = by + tx;
-    if (c < n && q <= n + 1) {
-      const float v = T[tx][j + 1];
-      dA[(size_t)c * pitch + q] = make_float2(v, T[tx + 1][j + 1] - v);
+  __syncthreads();
+  // B: record (r, q) = (F(r, q-1), F(r, q) - F(r, q-1)) = T[r - by + 1][q - bx (+1)]
+  const int qb = bx + lane;
+#pragma unroll 4
+  for (int i = wv; i < SS_GT; i += 4) {
+    const int r = by + i;
+    if (r < n && qb <= n + 1) {
+      const float lo = T[i + 1][lane];
+      dB[(size_t)r * pitch + qb] = make_float2(lo, T[i + 1][lane + 1] - lo);
+    }
+  }
+  // A: record (c, q) = (F(q-1, c), F(q, c) - F(q-1, c)) = T[q - by (+1)][c - bx + 1]
+  const int qa = by + lane;
+#pragma unroll 4
+  for (int j = wv; j < SS_GT; j += 4) {
+    const int cc = bx + j;
+    if (cc < n && qa <= n + 1) {
+      const float lo = T[lane][j + 1];
+      dA[(size_t)cc * pitch + qa] = make_float2(lo, T[lane + 1][j + 1] - lo);
     }
   }
 }
 
 __global__ __launch_bounds__(256) void k_gather_pairs(const float* __restrict__ img, float2* __restrict__ gA,
                                                       float2* __restrict__ gB, int n, int pitch) {
-  __shared__ float T[33][34];  // F(by-1+a, bx-1+b)
+  __shared__ float T[SS_GT_H][SS_GT_H];  // F(by-1+a, bx-1+b)
   const size_t slice = blockIdx.z;
-  ss_gather_tile(img + slice * n * n, gA + slice * n * pitch, gB + slice * n * pitch, n, pitch, blockIdx.x * 32,
-                 blockIdx.y * 32, T);
+  ss_gather_tile(img + slice * n * n, gA + slice * n * pitch, gB + slice * n * pitch, n, pitch, blockIdx.x * SS_GT,
+                 blockIdx.y * SS_GT, T);
 }
 
 hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
                                     int nslices, hipStream_t stream) {
   const int n = p.g.n;
-  const int tiles = (n + 2 + 31) / 32;
+  const int tiles = ss_gather_tiles(n);
   dim3 grid(tiles, tiles, nslices);
   hipLaunchKernelGGL(k_gather_pairs, grid, dim3(256), 0, stream, image, gatherA, gatherB, n, p.g.pitch);
   return hipGetLastError();
@@ -1712,15 +1744,15 @@ __device__ __forceinline__ SSBatchEntry ss_entry_of(const SSBatchLaunch& L, int 
 
 __global__ __launch_bounds__(256) void k_batch_gather(SSBatchLaunch L, const float* __restrict__ images,
                                                       float2* __restrict__ gA, float2* __restrict__ gB) {
-  __shared__ float T[33][34];
+  __shared__ float T[SS_GT_H][SS_GT_H];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
     const SSBatchEntry e = ss_entry_of(L, item);
     const SSBatchSlice s = ss_uniform_copy(L.slices + e.slice);
     const int n = s.p.g.n;
-    const int tiles = (n + 2 + 31) / 32;
+    const int tiles = ss_gather_tiles(n);
     const int local = item - e.item0;
     const int ty = local / tiles, tx = local - ty * tiles;
-    ss_gather_tile(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch, tx * 32, ty * 32, T);
+    ss_gather_tile(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch, tx * SS_GT, ty * SS_GT, T);
     __syncthreads();
   }
 }
@@ -1838,14 +1870,12 @@ static int ss_prime_at_most(int g) {
 
 int ss_batch_grid(int kind, int variant, int mode, int total) {
   if (total <= 0) return 0;
+  // the layout kernels' items are single tiles: one block per item lets the dispatcher
+  // overlap their loads and stores across blocks, where a resident grid walks its items
+  // one after another (4% of a batched OSEM iteration at 256 slices)
+  if (kind == SS_KIND_GATHER || kind == SS_KIND_TRANSPOSE) return total;
   int g = 0;
   switch (kind) {
-    case SS_KIND_GATHER:
-      g = ss_resident_blocks(k_batch_gather, 256);
-      break;
-    case SS_KIND_TRANSPOSE:
-      g = ss_resident_blocks(k_batch_transpose, 256);
-      break;
     case SS_KIND_RADON:
       g = mode == SS_OUT_EM_RATIO ? ss_resident_blocks(k_batch_radon<SS_OUT_EM_RATIO>, 512)
                                   : ss_resident_blocks(k_batch_radon<SS_OUT_SINOT>, 512);
