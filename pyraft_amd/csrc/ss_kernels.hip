@@ -111,78 +111,79 @@ hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4*
 // lower tap of interpolation cell m and the difference to the upper tap, so a Radon
 // sample is one aligned 8-byte load and a + w*d.
 //
-// One tile = SS_GATHER_TILE (64) values of q in both directions for a 256-thread block:
-// the image square with a one-pixel halo (65 x 65) goes to LDS, every halo load issued
-// before the first LDS write; then each wave writes whole 64-record rows (512
-// contiguous bytes per store) of layout B, then of layout A.  The odd LDS row length
-// puts the A phase's column reads (stride 65 words) in 64 distinct banks.
-#define SS_GT SS_GATHER_TILE
-#define SS_GT_H (SS_GATHER_TILE + 1)
+// One tile = GT values of q in both directions (GT = 32 or 64) for a block of NW waves:
+// the image square with a one-pixel halo ((GT+1)^2) goes to LDS, every halo load issued
+// before the first LDS write; then the waves write rows of GT records (GT * 8
+// contiguous bytes), first of layout B, then of layout A, 64 / GT rows per store.  64 x 64
+// tiles with 8 waves serve launches of many tiles; 32 x 32 tiles with 4 waves serve small
+// ones (one 1024^2 image), where the larger tiles leave the CUs short of waves.
+template <int GT, int NW>
 __device__ __forceinline__ void ss_gather_tile(const float* __restrict__ src, float2* __restrict__ dA,
                                                float2* __restrict__ dB, int n, int pitch, int bx, int by,
-                                               float (*T)[SS_GT_H]) {
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  constexpr int LOADS = (SS_GT_H + 3) / 4;  // halo rows per wave
-  // wave wv: halo rows wv, wv + 4, ...; lane l: column bx - 1 + l, lane 0 also column bx + 63
-  const int c = bx - 1 + lane;
-  const bool c_in = c >= 0 && c < n;
-  const bool e_in = lane == 0 && bx + SS_GT - 1 < n;
-  float v[LOADS], e[LOADS];
+                                               float (*T)[GT + 1]) {
+  constexpr int H = GT + 1;                       // halo side
+  constexpr int NT = NW * SS_WAVE;                // threads
+  constexpr int LOADS = (H * H + NT - 1) / NT;    // halo values per thread
+  constexpr int SUB = SS_WAVE / GT;               // rows per wave store
+  float v[LOADS];
 #pragma unroll
   for (int i = 0; i < LOADS; ++i) {
-    const int a = wv + 4 * i;
-    const int r = by - 1 + a;
-    const bool r_in = a < SS_GT_H && r >= 0 && r < n;
-    const float* row = src + (size_t)(r_in ? r : 0) * n;
-    v[i] = (r_in && c_in) ? row[c] : 0.f;
-    e[i] = (r_in && e_in) ? row[bx + SS_GT - 1] : 0.f;
+    const int idx = (int)threadIdx.x + NT * i;
+    const int a = idx / H, b = idx - a * H;
+    const int r = by - 1 + a, c = bx - 1 + b;
+    const bool in = idx < H * H && r >= 0 && r < n && c >= 0 && c < n;
+    v[i] = in ? src[(size_t)r * n + c] : 0.f;
   }
 #pragma unroll
   for (int i = 0; i < LOADS; ++i) {
-    const int a = wv + 4 * i;
-    if (a < SS_GT_H) {
-      T[a][lane] = v[i];
-      if (lane == 0) T[a][SS_GT] = e[i];
-    }
+    const int idx = (int)threadIdx.x + NT * i;
+    if (idx < H * H) T[idx / H][idx % H] = v[i];
   }
   __syncthreads();
+  const int lane = threadIdx.x & (SS_WAVE - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / SS_WAVE);
+  const int x = lane % GT, sub = lane / GT;
   // B: record (r, q) = (F(r, q-1), F(r, q) - F(r, q-1)) = T[r - by + 1][q - bx (+1)]
-  const int qb = bx + lane;
-#pragma unroll 4
-  for (int i = wv; i < SS_GT; i += 4) {
+  const int qb = bx + x;
+  for (int i = wv * SUB + sub; i < GT; i += NW * SUB) {
     const int r = by + i;
     if (r < n && qb <= n + 1) {
-      const float lo = T[i + 1][lane];
-      dB[(size_t)r * pitch + qb] = make_float2(lo, T[i + 1][lane + 1] - lo);
+      const float lo = T[i + 1][x];
+      dB[(size_t)r * pitch + qb] = make_float2(lo, T[i + 1][x + 1] - lo);
     }
   }
   // A: record (c, q) = (F(q-1, c), F(q, c) - F(q-1, c)) = T[q - by (+1)][c - bx + 1]
-  const int qa = by + lane;
-#pragma unroll 4
-  for (int j = wv; j < SS_GT; j += 4) {
+  const int qa = by + x;
+  for (int j = wv * SUB + sub; j < GT; j += NW * SUB) {
     const int cc = bx + j;
     if (cc < n && qa <= n + 1) {
-      const float lo = T[lane][j + 1];
-      dA[(size_t)cc * pitch + qa] = make_float2(lo, T[lane + 1][j + 1] - lo);
+      const float lo = T[x][j + 1];
+      dA[(size_t)cc * pitch + qa] = make_float2(lo, T[x + 1][j + 1] - lo);
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_gather_pairs(const float* __restrict__ img, float2* __restrict__ gA,
-                                                      float2* __restrict__ gB, int n, int pitch) {
-  __shared__ float T[SS_GT_H][SS_GT_H];  // F(by-1+a, bx-1+b)
+template <int GT, int NW>
+__global__ __launch_bounds__(NW * SS_WAVE) void k_gather_pairs(const float* __restrict__ img, float2* __restrict__ gA,
+                                                               float2* __restrict__ gB, int n, int pitch) {
+  __shared__ float T[GT + 1][GT + 1];  // F(by-1+a, bx-1+b)
   const size_t slice = blockIdx.z;
-  ss_gather_tile(img + slice * n * n, gA + slice * n * pitch, gB + slice * n * pitch, n, pitch, blockIdx.x * SS_GT,
-                 blockIdx.y * SS_GT, T);
+  ss_gather_tile<GT, NW>(img + slice * n * n, gA + slice * n * pitch, gB + slice * n * pitch, n, pitch,
+                         blockIdx.x * GT, blockIdx.y * GT, T);
 }
 
 hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
                                     int nslices, hipStream_t stream) {
   const int n = p.g.n;
   const int tiles = ss_gather_tiles(n);
-  dim3 grid(tiles, tiles, nslices);
-  hipLaunchKernelGGL(k_gather_pairs, grid, dim3(256), 0, stream, image, gatherA, gatherB, n, p.g.pitch);
+  if ((long long)tiles * tiles * nslices >= 1024) {  // at least 4 large tiles per CU
+    hipLaunchKernelGGL((k_gather_pairs<SS_GATHER_TILE, 8>), dim3(tiles, tiles, nslices), dim3(8 * SS_WAVE), 0, stream,
+                       image, gatherA, gatherB, n, p.g.pitch);
+  } else {
+    const int t32 = (n + 2 + 31) / 32;
+    hipLaunchKernelGGL((k_gather_pairs<32, 4>), dim3(t32, t32, nslices), dim3(4 * SS_WAVE), 0, stream, image,
+                       gatherA, gatherB, n, p.g.pitch);
+  }
   return hipGetLastError();
 }
 
@@ -1742,9 +1743,9 @@ __device__ __forceinline__ SSBatchEntry ss_entry_of(const SSBatchLaunch& L, int 
   return e;
 }
 
-__global__ __launch_bounds__(256) void k_batch_gather(SSBatchLaunch L, const float* __restrict__ images,
-                                                      float2* __restrict__ gA, float2* __restrict__ gB) {
-  __shared__ float T[SS_GT_H][SS_GT_H];
+__global__ __launch_bounds__(8 * SS_WAVE) void k_batch_gather(SSBatchLaunch L, const float* __restrict__ images,
+                                                              float2* __restrict__ gA, float2* __restrict__ gB) {
+  __shared__ float T[SS_GATHER_TILE + 1][SS_GATHER_TILE + 1];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
     const SSBatchEntry e = ss_entry_of(L, item);
     const SSBatchSlice s = ss_uniform_copy(L.slices + e.slice);
@@ -1752,7 +1753,8 @@ __global__ __launch_bounds__(256) void k_batch_gather(SSBatchLaunch L, const flo
     const int tiles = ss_gather_tiles(n);
     const int local = item - e.item0;
     const int ty = local / tiles, tx = local - ty * tiles;
-    ss_gather_tile(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch, tx * SS_GT, ty * SS_GT, T);
+    ss_gather_tile<SS_GATHER_TILE, 8>(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch,
+                                      tx * SS_GATHER_TILE, ty * SS_GATHER_TILE, T);
     __syncthreads();
   }
 }
@@ -1903,7 +1905,7 @@ int ss_batch_grid(int kind, int variant, int mode, int total) {
 hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, const float* images, float2* gatherA, float2* gatherB,
                                   hipStream_t stream) {
   if (L.total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_gather, dim3(L.grid), dim3(256), 0, stream, L, images, gatherA, gatherB);
+  hipLaunchKernelGGL(k_batch_gather, dim3(L.grid), dim3(8 * SS_WAVE), 0, stream, L, images, gatherA, gatherB);
   return hipGetLastError();
 }
 
