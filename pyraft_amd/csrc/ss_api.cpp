@@ -491,12 +491,14 @@ extern "C" int raft_ss_radon_range(const raft_ss_plan* p, const float* d_image, 
   const SSDevPlan dp = p->dev();
   const size_t layout = (size_t)nslices * p->g.n * p->g.pitch;
   DevBuf ws;
-  HIP_TRY(ws.alloc(2 * layout * sizeof(float2), stream));
+  HIP_TRY(ws.alloc(2 * layout * sizeof(float2) + ss_radon_scratch(dp, j0, j1, nslices), stream));
   float2* gA = ws.as<float2>();
   float2* gB = gA + layout;
+  float* slabs = reinterpret_cast<float*>(gB + layout);
   HIP_TRY(ss_launch_gather_layouts(dp, d_image, gA, gB, nslices, stream));
   SSEmRatioArgs none{nullptr, nullptr};
-  HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, angle_major ? SS_OUT_SINOT : SS_OUT_SINO, d_out, none, stream));
+  HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, angle_major ? SS_OUT_SINOT : SS_OUT_SINO, d_out, none, slabs,
+                          stream));
   return 0;
 }
 
@@ -624,13 +626,15 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   const size_t layout = (size_t)nslices * n * p->g.pitch;
   // LL partials of all subsets of one iteration, reduced once per iteration
   std::vector<int> poff(nsubsets + 1, 0);
+  size_t slab_bytes = 0;  // the largest subset's slab partial sums
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
     poff[b + 1] = poff[b] + ss_radon_partials(dp, j0, j1, nslices);
+    slab_bytes = std::max(slab_bytes, ss_radon_scratch(dp, j0, j1, nslices));
   }
   const int npartials = poff[nsubsets];
   const size_t bytes = sizeof(float2) * 2 * layout + sizeof(float) * (2 * sino_sz + (size_t)nsubsets * img_sz) +
-                       sizeof(double) * ((size_t)npartials + (size_t)std::max(niter, 1) + 8);
+                       sizeof(double) * ((size_t)npartials + (size_t)std::max(niter, 1) + 8) + slab_bytes;
   DevBuf ws;
   HIP_TRY(ws.alloc(bytes, stream));
   float2* gA = ws.as<float2>();
@@ -641,6 +645,7 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   double* partials = reinterpret_cast<double*>(weights + (size_t)nsubsets * img_sz);
   partials = reinterpret_cast<double*>(((uintptr_t)partials + 7) & ~(uintptr_t)7);
   double* d_ll = partials + npartials;
+  float* slabs = reinterpret_cast<float*>(d_ll + std::max(niter, 1));
 
   HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
   // W_b = BP_b(1) with the same backprojection kernel
@@ -657,7 +662,7 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
       const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
       HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
       SSEmRatioArgs ea{measT, partials + poff[b]};
-      HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, stream));
+      HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, slabs, stream));
       SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
       HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, stream));
     }
@@ -686,7 +691,9 @@ extern "C" int raft_ss_tr_em(raft_ss_plan* p, const float* d_raw, const float* d
   const size_t img_sz = (size_t)nslices * n * n;
   const size_t layout = (size_t)nslices * n * p->g.pitch;
   DevBuf ws;
-  HIP_TRY(ws.alloc(sizeof(float2) * 2 * layout + sizeof(float) * (3 * sino_sz + 2 * img_sz), stream));
+  HIP_TRY(ws.alloc(sizeof(float2) * 2 * layout + sizeof(float) * (3 * sino_sz + 2 * img_sz) +
+                       ss_radon_scratch(dp, 0, A, nslices),
+                   stream));
   float2* gA = ws.as<float2>();
   float2* gB = gA + layout;
   float* rawT = reinterpret_cast<float*>(gB + layout);
@@ -694,6 +701,7 @@ extern "C" int raft_ss_tr_em(raft_ss_plan* p, const float* d_raw, const float* d
   float* pT = flatT + sino_sz;
   float* w = pT + sino_sz;
   float* b = w + img_sz;
+  float* slabs = b + img_sz;
   SSEmUpdateArgs none{nullptr, nullptr};
   SSEmRatioArgs no_ratio{nullptr, nullptr};
   HIP_TRY(ss_launch_sino_to_t(d_raw, rawT, R, A, nslices, stream));
@@ -701,7 +709,7 @@ extern "C" int raft_ss_tr_em(raft_ss_plan* p, const float* d_raw, const float* d
   HIP_TRY(ss_launch_backprojection(dp, rawT, 0, A, nslices, SS_BP_PLAIN, w, none, stream));
   for (int it = 0; it < niter; ++it) {
     HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
-    HIP_TRY(ss_launch_radon(dp, gA, gB, 0, A, nslices, SS_OUT_SINOT, pT, no_ratio, stream));
+    HIP_TRY(ss_launch_radon(dp, gA, gB, 0, A, nslices, SS_OUT_SINOT, pT, no_ratio, slabs, stream));
     HIP_TRY(ss_launch_tr_expected(pT, flatT, sino_sz, stream));
     HIP_TRY(ss_launch_backprojection(dp, pT, 0, A, nslices, SS_BP_PLAIN, b, none, stream));
     HIP_TRY(ss_launch_tr_update(d_x, b, w, img_sz, stream));

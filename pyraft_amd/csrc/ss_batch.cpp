@@ -157,7 +157,8 @@ struct BatchLaunch {
   std::vector<SSBatchEntry> entries;
   SSBatchEntry* d_entries = nullptr;
   int total = 0, grid = 0;
-  int split = 1;               // Radon: waves per (angle, ray group)
+  bool slabs = false;          // Radon / finish: the launch works in row slabs (partial sums)
+  size_t part_floats = 0;      // Radon / finish: partial sums of every entry
   std::vector<int> entry_of;   // slice -> entry index (-1: not in this launch)
   BatchLaunch() = default;
   BatchLaunch(const BatchLaunch&) = delete;
@@ -222,7 +223,7 @@ struct raft_ss_batch {
   Buf slices;                // SSBatchSlice[nslices]
   bool slices_dirty = true;
   std::map<std::string, std::unique_ptr<BatchLaunch>> launches;
-  Buf lay, t0, t1, weights, partials, ll, ranges;
+  Buf lay, t0, t1, weights, partials, ll, ranges, slab;
   int generation = 0;        // bumped whenever a buffer moves or the variants change
   int weights_S = 0;         // subsets the cached BP_b(1) images were computed for
   int weights_generation = -1;
@@ -280,7 +281,9 @@ void subset_range(int A, int S, int bsub, int* j0, int* j1) {
 }
 
 // Items per entry and cost per item of each launch kind; entries sorted by
-// decreasing cost per item so the cheapest items run last.
+// decreasing cost per item so the cheapest items run last.  A Radon launch (and its
+// finishing pass) of fewer than SS_RADON_SLAB_BLOCKS blocks divides every entry's
+// sample rows into slabs; the entries' partial sums are laid out in slice order.
 int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, int mode, int v) {
   struct Cand {
     SSBatchEntry e;
@@ -288,17 +291,16 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
     double cost;
   };
   std::vector<Cand> c;
-  long long waves = 0;
+  long long blocks = 0;
   for (int i = 0; i < b->nslices; ++i) {
     const raft_ss_plan& p = b->plan(i);
     int j0 = 0, j1 = p.g.nangles;
     if (S > 0) subset_range(p.g.nangles, S, bsub, &j0, &j1);
-    waves += (long long)p.nwaves * (j1 - j0);
+    blocks += ss_radon_blocks(p.nwaves, j0, j1);
   }
-  int split = 1;
-  if (kind == SS_KIND_RADON)
-    while (split < 8 && waves * split < 256LL * 32) split *= 2;
-  const int per_block = ss_radon_angles_per_block(split);
+  const bool radon = kind == SS_KIND_RADON || kind == SS_KIND_RADON_FINISH;
+  L.slabs = radon && blocks < SS_RADON_SLAB_BLOCKS;
+  L.part_floats = 0;
   for (int i = 0; i < b->nslices; ++i) {
     const raft_ss_plan& p = b->plan(i);
     const int n = p.g.n, R = p.g.nrays, A = p.g.nangles;
@@ -309,6 +311,14 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
     k.e.j0 = j0;
     k.e.j1 = j1;
     k.e.item0 = 0;
+    k.e.nslab = 1;
+    k.e.slab_rows = n;
+    k.e.part_off = 0;
+    if (L.slabs && j1 > j0) {
+      k.e.nslab = ss_radon_slab_plan(n, blocks, &k.e.slab_rows);
+      k.e.part_off = (int64_t)L.part_floats;
+      L.part_floats += (size_t)k.e.nslab * (size_t)(j1 - j0) * R;
+    }
     switch (kind) {
       case SS_KIND_GATHER: {
         const int t = ss_gather_tiles(n);
@@ -321,8 +331,12 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
         k.cost = 1.0;
         break;
       case SS_KIND_RADON:
-        k.items = p.nwaves * ((j1 - j0 + per_block - 1) / per_block);
-        k.cost = (double)n * per_block;
+        k.items = (int)ss_radon_blocks(p.nwaves, j0, j1) * k.e.nslab;
+        k.cost = (double)k.e.slab_rows * 8;
+        break;
+      case SS_KIND_RADON_FINISH:  // only in slab form: 256 rays per item
+        k.items = L.slabs ? (int)(((long long)(j1 - j0) * R + 255) / 256) : 0;
+        k.cost = (double)k.e.nslab;
         break;
       case SS_KIND_FIXUP:
         k.items = p.nirregular;
@@ -349,7 +363,6 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
   }
   if (total > 0x7fffffffLL) return ss_set_error(RAFT_SS_EINVAL, "batch too large (%lld work items)", total);
   L.total = (int)total;
-  L.split = split;
   L.grid = ss_batch_grid(kind, v, mode, L.total);
   if (!L.entries.empty()) {
     HIP_TRY(hipMalloc(&L.d_entries, sizeof(SSBatchEntry) * L.entries.size()));
@@ -533,20 +546,24 @@ extern "C" int raft_ss_batch_radon(raft_ss_batch* b, const float* d_images, floa
   std::lock_guard<std::mutex> lock(b->mu);
   hipStream_t stream = as_stream(stream_);
   const int mode = angle_major ? SS_OUT_SINOT : SS_OUT_SINO;
-  BatchLaunch *Lg, *Lr, *Lf;
+  BatchLaunch *Lg, *Lr, *Lz, *Lf;
   if ((rc = sync_slices(b)) || (rc = get_launch(b, &Lg, SS_KIND_GATHER, 0, 0, 0, 0)) ||
       (rc = get_launch(b, &Lr, SS_KIND_RADON, 0, 0, mode, 0)) ||
+      (rc = get_launch(b, &Lz, SS_KIND_RADON_FINISH, 0, 0, mode, 0)) ||
       (rc = get_launch(b, &Lf, SS_KIND_FIXUP, 0, 0, mode, 0)) ||
-      (rc = ensure_buf(b, b->lay, sizeof(float2) * 2 * (size_t)b->lay_total)))
+      (rc = ensure_buf(b, b->lay, sizeof(float2) * 2 * (size_t)b->lay_total)) ||
+      (rc = ensure_buf(b, b->slab, sizeof(float) * std::max<size_t>(Lr->part_floats, 1))))
     return rc;
   float2* gA = b->lay.as<float2>();
   float2* gB = gA + b->lay_total;
+  float* part = Lr->slabs ? b->slab.as<float>() : nullptr;
   const SSBatchSlice* ds = b->slices.as<SSBatchSlice>();
   char key[128];
   snprintf(key, sizeof(key), "radon|%p|%p|%d", (const void*)d_images, (void*)d_sinos, mode);
   return run_body(b, flags & RAFT_SS_BATCH_GRAPH, key, stream, [&](hipStream_t s) -> int {
     HIP_TRY(ss_launch_batch_gather(Lg->dev(ds), d_images, gA, gB, s));
-    HIP_TRY(ss_launch_batch_radon(Lr->dev(ds), gA, gB, Lr->split, mode, d_sinos, nullptr, nullptr, s));
+    HIP_TRY(ss_launch_batch_radon(Lr->dev(ds), gA, gB, part, mode, d_sinos, nullptr, nullptr, s));
+    if (part) HIP_TRY(ss_launch_batch_radon_finish(Lz->dev(ds), part, mode, d_sinos, nullptr, nullptr, s));
     HIP_TRY(ss_launch_batch_radon_fixup(Lf->dev(ds), gA, gB, mode, d_sinos, nullptr, nullptr, s));
     return 0;
   });
@@ -598,31 +615,37 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
   if ((rc = sync_slices(b)) || (rc = get_launch(b, &Lg, SS_KIND_GATHER, 0, 0, 0, 0)) ||
       (rc = get_launch(b, &Lt, SS_KIND_TRANSPOSE, 0, 0, 0, 0)))
     return rc;
-  std::vector<BatchLaunch*> Lr(S), Lf(S), Lbw((size_t)S * vs.size()), Lbu((size_t)S * vs.size());
+  std::vector<BatchLaunch*> Lr(S), Lz(S), Lm(S), Lf(S), Lbw((size_t)S * vs.size()), Lbu((size_t)S * vs.size());
   std::vector<long long> poff(S + 1, 0);
+  size_t part_floats = 1;
   for (int s = 0; s < S; ++s) {
     if ((rc = get_launch(b, &Lr[s], SS_KIND_RADON, S, s, SS_OUT_EM_RATIO, 0)) ||
+        (rc = get_launch(b, &Lz[s], SS_KIND_RADON_FINISH, S, s, SS_OUT_EM_RATIO, 0)) ||
         (rc = get_launch(b, &Lf[s], SS_KIND_FIXUP, S, s, SS_OUT_EM_RATIO, 0)))
       return rc;
     for (size_t q = 0; q < vs.size(); ++q)
       if ((rc = get_launch(b, &Lbw[s * vs.size() + q], SS_KIND_BP, S, s, SS_BP_PLAIN, vs[q])) ||
           (rc = get_launch(b, &Lbu[s * vs.size() + q], SS_KIND_BP, S, s, SS_BP_EM_UPDATE, vs[q])))
         return rc;
-    poff[s + 1] = poff[s] + Lr[s]->total + Lf[s]->total;
+    // the launch whose items write the LL partials: the finishing pass in slab form
+    Lm[s] = Lr[s]->slabs ? Lz[s] : Lr[s];
+    if (Lr[s]->slabs) part_floats = std::max(part_floats, Lr[s]->part_floats);
+    poff[s + 1] = poff[s] + Lm[s]->total + Lf[s]->total;
   }
-  // per-slice (start, count) partial ranges: radon then fixup items of every subset
+  // per-slice (start, count) partial ranges: radon (or finish) then fixup items of every subset
   std::vector<int2> ranges((size_t)ns * 2 * S);
   for (int i = 0; i < ns; ++i)
     for (int s = 0; s < S; ++s) {
       int2 r0{0, 0}, r1{0, 0};
-      const int er = Lr[s]->entry_of[i], ef = Lf[s]->entry_of[i];
+      const BatchLaunch& M = *Lm[s];
+      const int er = M.entry_of[i], ef = Lf[s]->entry_of[i];
       if (er >= 0) {
-        const int next = er + 1 < (int)Lr[s]->entries.size() ? Lr[s]->entries[er + 1].item0 : Lr[s]->total;
-        r0 = int2{(int)(poff[s] + Lr[s]->entries[er].item0), next - Lr[s]->entries[er].item0};
+        const int next = er + 1 < (int)M.entries.size() ? M.entries[er + 1].item0 : M.total;
+        r0 = int2{(int)(poff[s] + M.entries[er].item0), next - M.entries[er].item0};
       }
       if (ef >= 0) {
         const int next = ef + 1 < (int)Lf[s]->entries.size() ? Lf[s]->entries[ef + 1].item0 : Lf[s]->total;
-        r1 = int2{(int)(poff[s] + Lr[s]->total + Lf[s]->entries[ef].item0), next - Lf[s]->entries[ef].item0};
+        r1 = int2{(int)(poff[s] + M.total + Lf[s]->entries[ef].item0), next - Lf[s]->entries[ef].item0};
       }
       ranges[((size_t)i * S + s) * 2] = r0;
       ranges[((size_t)i * S + s) * 2 + 1] = r1;
@@ -634,7 +657,8 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
       (rc = ensure_buf(b, b->weights, sizeof(float) * (size_t)S * b->img_total)) ||
       (rc = ensure_buf(b, b->partials, sizeof(double) * (size_t)std::max<long long>(poff[S], 1))) ||
       (rc = ensure_buf(b, b->ll, sizeof(double) * (size_t)std::max(niter, 1) * ns)) ||
-      (rc = ensure_buf(b, b->ranges, sizeof(int2) * ranges.size())))
+      (rc = ensure_buf(b, b->ranges, sizeof(int2) * ranges.size())) ||
+      (rc = ensure_buf(b, b->slab, sizeof(float) * part_floats)))
     return rc;
   HIP_TRY(hipMemcpy(b->ranges.p, ranges.data(), sizeof(int2) * ranges.size(), hipMemcpyHostToDevice));
   const SSBatchSlice* ds = b->slices.as<SSBatchSlice>();
@@ -646,6 +670,7 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
   double* part = b->partials.as<double>();
   double* lld = b->ll.as<double>();
   const int2* rg = b->ranges.as<int2>();
+  float* slab = b->slab.as<float>();
 
   // ---- W_b = BP_b(1) per slice and subset, computed once per (S, variants, buffers)
   if (b->weights_S != S || b->weights_generation != b->generation) {
@@ -665,10 +690,12 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
     for (int it = 0; it < niter; ++it) {
       for (int s = 0; s < S; ++s) {
         HIP_TRY(ss_launch_batch_gather(Lg->dev(ds), d_x, gA, gB, st));
-        HIP_TRY(ss_launch_batch_radon(Lr[s]->dev(ds), gA, gB, Lr[s]->split, SS_OUT_EM_RATIO, rhoT, measT,
-                                      part + poff[s], st));
+        float* sp = Lr[s]->slabs ? slab : nullptr;
+        HIP_TRY(ss_launch_batch_radon(Lr[s]->dev(ds), gA, gB, sp, SS_OUT_EM_RATIO, rhoT, measT, part + poff[s], st));
+        if (sp)
+          HIP_TRY(ss_launch_batch_radon_finish(Lz[s]->dev(ds), sp, SS_OUT_EM_RATIO, rhoT, measT, part + poff[s], st));
         HIP_TRY(ss_launch_batch_radon_fixup(Lf[s]->dev(ds), gA, gB, SS_OUT_EM_RATIO, rhoT, measT,
-                                            part + poff[s] + Lr[s]->total, st));
+                                            part + poff[s] + Lm[s]->total, st));
         for (size_t q = 0; q < vs.size(); ++q)
           HIP_TRY(ss_launch_batch_backprojection(Lbu[s * vs.size() + q]->dev(ds), vs[q], rhoT, SS_BP_EM_UPDATE,
                                                  nullptr, d_x, W + (size_t)s * b->img_total, st));

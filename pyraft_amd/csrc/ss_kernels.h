@@ -57,6 +57,10 @@ struct SSBatchSlice {
 };
 struct SSBatchEntry {
   int slice, j0, j1, item0;
+  // Radon launches of few blocks and their finishing pass: the entry's sample rows in
+  // nslab slabs of slab_rows, its partial ray sums at part[part_off ...] ([z][j - j0][k])
+  int nslab, slab_rows;
+  int64_t part_off;
 };
 struct SSBatchLaunch {
   const SSBatchSlice* slices;  // device
@@ -87,8 +91,22 @@ struct SSEmRatioArgs {
   const float* meas;   // measured sinogram, angle-major [slice][A][R]
   double* ll_partials; // one FP64 partial per block (+ one per irregular ray)
 };
+// A Radon launch of fewer than SS_RADON_SLAB_BLOCKS blocks divides the sample rows of
+// every (angle block, ray group) into slabs taken by separate blocks, which write partial
+// ray sums; a finishing pass adds them.  ss_radon_slab_plan: slab count (1: no division)
+// and rows per slab of an n x n image in a launch of `blocks` blocks.
+#define SS_RADON_SLAB_BLOCKS 1024
+int ss_radon_slab_plan(int n, long long blocks, int* rows);
+// blocks of the Radon of angles [j0, j1) on an image with `nwaves` ray groups
+static inline long long ss_radon_blocks(int nwaves, int j0, int j1) {
+  return (long long)nwaves * ((j1 - j0 + 7) / 8);
+}
+// scratch: ss_radon_scratch(p, j0, j1, nslices) bytes (the slabs' partial sums; 0 bytes,
+// and scratch may be null, when the launch is not divided)
+size_t ss_radon_scratch(const SSDevPlan& p, int j0, int j1, int nslices);
 hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const float2* gatherB, int j0, int j1,
-                           int nslices, int mode, float* out, SSEmRatioArgs em, hipStream_t stream);
+                           int nslices, int mode, float* out, SSEmRatioArgs em, float* scratch, hipStream_t stream);
+// FP64 log-likelihood partials an EM_RATIO launch writes (ll_partials[0 .. count))
 int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices);
 
 // ---- backprojection (reads angle-major sinogram)
@@ -106,17 +124,20 @@ size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices);
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, float* scratch, hipStream_t stream);
 
-// ---- batched forms (items: gather = SS_GATHER_TILE^2 tiles; radon = 64-ray groups x angle blocks of
-//      RADON_WAVES/split angles; fixup = irregular rays; bp = pixel tiles of the variant;
-//      transpose = 32x32 tiles)
-int ss_radon_angles_per_block(int split);
+// ---- batched forms (items: gather = SS_GATHER_TILE^2 tiles; radon = 64-ray groups x blocks of
+//      8 angles x row slabs; finish = 256 rays of an entry; fixup = irregular rays; bp = pixel
+//      tiles of the variant; transpose = 32x32 tiles)
 int ss_bp_tiles(int n, int variant);
 hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, const float* images, float2* gatherA, float2* gatherB,
                                   hipStream_t stream);
 hipError_t ss_launch_batch_transpose(const SSBatchLaunch& L, const float* in, float* out, int to_angle_major,
                                      hipStream_t stream);
-hipError_t ss_launch_batch_radon(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, int split,
+// part != nullptr: the entries' partial sums go to part (launch in slab form, then
+// ss_launch_batch_radon_finish with `mode`); otherwise the Radon stores in `mode` itself
+hipError_t ss_launch_batch_radon(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, float* part,
                                  int mode, float* out, const float* meas, double* ll_partials, hipStream_t stream);
+hipError_t ss_launch_batch_radon_finish(const SSBatchLaunch& L, const float* part, int mode, float* out,
+                                        const float* meas, double* ll_partials, hipStream_t stream);
 hipError_t ss_launch_batch_radon_fixup(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, int mode,
                                        float* out, const float* meas, double* ll_partials, hipStream_t stream);
 hipError_t ss_launch_batch_backprojection(const SSBatchLaunch& L, int variant, const float* sinoT, int mode,
@@ -129,7 +150,7 @@ hipError_t ss_launch_fill_f64(double* p, double v, size_t n, hipStream_t stream)
 // grid of a batched kernel: the occupancy-based resident grid (blocks per CU x CUs) of the
 // Radon and backprojection kernels; one block per item for the layout kernels
 int ss_batch_grid(int kind, int variant, int mode, int total);
-enum { SS_KIND_GATHER = 0, SS_KIND_TRANSPOSE, SS_KIND_RADON, SS_KIND_FIXUP, SS_KIND_BP };
+enum { SS_KIND_GATHER = 0, SS_KIND_TRANSPOSE, SS_KIND_RADON, SS_KIND_FIXUP, SS_KIND_BP, SS_KIND_RADON_FINISH };
 
 // ---- FBP row filter (rows of length R, circular convolution with g)
 // FFT form of the row filter (ss_kernels.hip): the spectrum of a tap set, built on the

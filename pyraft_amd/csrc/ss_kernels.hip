@@ -392,8 +392,10 @@ __device__ __forceinline__ void ss_radon_rows(const Rd& rd, int sa, int sb, int 
 }
 
 // One block's work: rays [grp*64, grp*64+64) of the RADON_WAVES/split angles starting
-// at j0 + ablk*(RADON_WAVES/split), clipped to [j0, j1).  gA/gB/out/meas point at this
-// slice's data; in EM mode the block's FP64 log-likelihood partial goes to *ll_slot.
+// at j0 + ablk*(RADON_WAVES/split), clipped to [j0, j1), over the sample rows
+// [row_lo, row_hi).  gA/gB/out/meas point at this slice's data; in EM mode the block's
+// FP64 log-likelihood partial goes to *ll_slot.  With `part_out` the block writes its
+// partial ray sums there instead ([j - j0][k], delta not applied) for a finishing pass.
 //
 // split == 1 and all angles of the block on one branch (the usual case): the block
 // walks its sample rows in chunks of up to RADON_CHUNK rows.  For each chunk it
@@ -405,8 +407,9 @@ __device__ __forceinline__ void ss_radon_rows(const Rd& rd, int sa, int sb, int 
 template <int MODE>
 __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2* __restrict__ gA,
                                               const float2* __restrict__ gB, int j0, int j1, int split, int grp,
-                                              int ablk, float* __restrict__ out, const float* __restrict__ meas,
-                                              double* ll_slot, SSRadonLds& lds) {
+                                              int ablk, int row_lo, int row_hi, float* __restrict__ out,
+                                              const float* __restrict__ meas, double* ll_slot,
+                                              float* __restrict__ part_out, SSRadonLds& lds) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int per_block = RADON_WAVES / split;
@@ -464,8 +467,8 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
     }
     bmin = ss_rfl(bmin);
     bmax = ss_rfl(bmax);
-    s_lo = ss_rfl(s_lo);
-    s_hi = ss_rfl(s_hi);
+    s_lo = max(ss_rfl(s_lo), row_lo);
+    s_hi = min(ss_rfl(s_hi), row_hi - 1);
     staged = bmax < 0 || bmin == bmax;  // block-uniform
     if (staged && bmax >= 0) {
       const float2* lay = bmax ? gA : gB;
@@ -555,11 +558,12 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
     }
   }
   if (!staged && active && branch >= 0) {
-    // this wave's share [sa, sb) of the group's sample range [wb.x, wb.y]
-    const int len = wb.y - wb.x + 1;
+    // this wave's share [sa, sb) of the group's sample range [wb.x, wb.y] within the rows
+    const int r0 = max(wb.x, row_lo), r1 = min(wb.y + 1, row_hi);
+    const int len = max(r1 - r0, 0);
     const int chunk = (len + split - 1) / split;
-    const int sa = wb.x + part * chunk;
-    const int sb = min(wb.y + 1, sa + chunk);
+    const int sa = r0 + part * chunk;
+    const int sb = min(r1, sa + chunk);
     // u carries the trajectory plus one cell: its high word is the record index m + 1
     // of the lower tap in the gather row, its low word the interpolation weight
     int64_t u = a_k + (int64_t)sa * beta + ((int64_t)1 << 32);
@@ -587,8 +591,12 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
   }
   double ll = 0.0;
   if (active && part == 0 && k < R) {
-    const float val = fmaf(acc_d, SS_INV_FIX, acc_a) * delta;
-    ss_radon_store<MODE>(p, j, k, val, out, meas, &ll);
+    if (part_out) {
+      part_out[(size_t)(j - j0) * R + k] = fmaf(acc_d, SS_INV_FIX, acc_a);
+    } else {
+      const float val = fmaf(acc_d, SS_INV_FIX, acc_a) * delta;
+      ss_radon_store<MODE>(p, j, k, val, out, meas, &ll);
+    }
   }
   if (MODE == SS_OUT_EM_RATIO) {
     const double tot = ss_block_sum_f64(ll, lds.red);
@@ -596,18 +604,52 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
   }
 }
 
+// blockIdx.z = slice * nslab + z: slab z takes sample rows [z * slab_rows, (z + 1) * slab_rows).
+// With `part` the blocks write partial ray sums to part[slice][z][j - j0][k] for
+// k_radon_slab_finish; without it (nslab = 1) they store the output themselves.
 template <int MODE>
 __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
-                                               const float2* __restrict__ gB, int j0, int j1, int split,
-                                               float* __restrict__ out, SSEmRatioArgs em) {
+                                               const float2* __restrict__ gB, int j0, int j1, int split, int nslab,
+                                               int slab_rows, float* __restrict__ out, float* __restrict__ part,
+                                               SSEmRatioArgs em) {
   __shared__ SSRadonLds lds;
-  const size_t slice = blockIdx.z;
+  const int z = (int)(blockIdx.z % (unsigned)nslab);
+  const size_t slice = blockIdx.z / (unsigned)nslab;
   const size_t lay = slice * p.g.n * p.g.pitch;
   const size_t so = slice * p.g.nangles * p.g.nrays;
   double* slot = nullptr;
   if (MODE == SS_OUT_EM_RATIO) slot = em.ll_partials + (slice * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  ss_radon_item<MODE>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, split, blockIdx.x, blockIdx.y, out + so,
-                      MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr, slot, lds);
+  float* po = part ? part + (slice * nslab + z) * (size_t)(j1 - j0) * p.g.nrays : nullptr;
+  ss_radon_item<MODE>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, split, blockIdx.x, blockIdx.y, z * slab_rows,
+                      min(p.g.n, (z + 1) * slab_rows), out + so, MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr,
+                      slot, po, lds);
+}
+
+// Sums the slabs' partial sums of rays [j0, j1) x [0, R) of each slice and stores them
+// as k_radon does (delta applied; in EM mode the ratio, and one FP64 log-likelihood
+// partial per block at ll_partials[slice * gridDim.x + blockIdx.x]).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_radon_slab_finish(SSDevPlan p, const float* __restrict__ part, int nslab,
+                                                           int j0, int j1, float* __restrict__ out, SSEmRatioArgs em) {
+  __shared__ double red[4];
+  const size_t slice = blockIdx.y;
+  const int R = p.g.nrays;
+  const size_t per = (size_t)(j1 - j0) * R;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t so = slice * p.g.nangles * (size_t)R;
+  double ll = 0.0;
+  if (idx < per) {
+    const int jj = (int)(idx / (unsigned)R), k = (int)(idx - (size_t)jj * R);
+    const float* src = part + slice * nslab * per + idx;
+    float sum = 0.f;
+    for (int z = 0; z < nslab; ++z) sum += src[(size_t)z * per];
+    ss_radon_store<MODE>(ss_radon_view(p), j0 + jj, k, sum * p.angles[j0 + jj].delta, out + so,
+                         MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr, &ll);
+  }
+  if (MODE == SS_OUT_EM_RATIO) {
+    const double tot = ss_block_sum_f64(ll, red);
+    if (threadIdx.x == 0) em.ll_partials[slice * gridDim.x + blockIdx.x] = tot;
+  }
 }
 
 // Irregular rays (knife-edge angles): one block per ray, samples across threads,
@@ -665,41 +707,91 @@ __global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* 
                             MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr, slot, redf);
 }
 
-// waves per (angle, ray group): enough to put ~32 waves on each of the 256 CUs
-static int ss_radon_split(const SSDevPlan& p, int j0, int j1, int nslices) {
-  const long long waves = (long long)p.nwaves * (j1 - j0) * nslices;
-  int split = 1;
-  while (split < RADON_WAVES && waves * split < 256LL * 32) split *= 2;
-  return split;
+// A launch of few blocks (a few angles: one OSEM subset) divides the sample rows of every
+// (angle block, ray group) into slabs handled by separate blocks, so that every CU gets
+// blocks while each block keeps the LDS-staged path; a finishing pass then adds the
+// slabs' partial sums.  Rows per slab are a multiple of RADON_CHUNK.
+static_assert(RADON_WAVES == 8, "ss_radon_blocks counts blocks of 8 angles");
+int ss_radon_slab_plan(int n, long long blocks, int* rows) {
+  *rows = n;
+  if (blocks >= SS_RADON_SLAB_BLOCKS) return 1;
+  const long long want = (SS_RADON_SLAB_BLOCKS + std::max(blocks, 1LL) - 1) / std::max(blocks, 1LL);
+  int L = (int)((n + want - 1) / want);
+  L = (L + RADON_CHUNK - 1) / RADON_CHUNK * RADON_CHUNK;
+  const int count = (n + L - 1) / L;
+  if (count <= 1) return 1;
+  *rows = L;
+  return count;
 }
 
-int ss_radon_angles_per_block(int split) { return RADON_WAVES / split; }
+static int ss_radon_slabs(const SSDevPlan& p, int j0, int j1, int nslices, int* rows) {
+  *rows = p.g.n;
+  if (j1 <= j0) return 1;
+  return ss_radon_slab_plan(p.g.n, ss_radon_blocks(p.nwaves, j0, j1) * nslices, rows);
+}
+
+static int ss_radon_finish_blocks(int nj, int R) { return (int)(((long long)nj * R + 255) / 256); }
+
+size_t ss_radon_scratch(const SSDevPlan& p, int j0, int j1, int nslices) {
+  int rows;
+  const int nslab = ss_radon_slabs(p, j0, j1, nslices, &rows);
+  return nslab > 1 ? sizeof(float) * (size_t)nslab * nslices * (size_t)(j1 - j0) * p.g.nrays : 0;
+}
 
 int ss_radon_partials(const SSDevPlan& p, int j0, int j1, int nslices) {
-  const int per_block = RADON_WAVES / ss_radon_split(p, j0, j1, nslices);
-  const int gy = (j1 - j0 + per_block - 1) / per_block;
-  return p.nwaves * gy * nslices + p.nirregular * nslices;
+  int rows;
+  const int nslab = ss_radon_slabs(p, j0, j1, nslices, &rows);
+  const int main = nslab > 1 ? ss_radon_finish_blocks(j1 - j0, p.g.nrays)
+                             : p.nwaves * ((j1 - j0 + RADON_WAVES - 1) / RADON_WAVES);
+  return (main + p.nirregular) * nslices;
 }
 
 hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const float2* gatherB, int j0, int j1,
-                           int nslices, int mode, float* out, SSEmRatioArgs em, hipStream_t stream) {
+                           int nslices, int mode, float* out, SSEmRatioArgs em, float* scratch, hipStream_t stream) {
   if (j1 <= j0) return hipSuccess;
-  const int split = ss_radon_split(p, j0, j1, nslices);
-  const int per_block = RADON_WAVES / split;
-  dim3 grid(p.nwaves, (j1 - j0 + per_block - 1) / per_block, nslices);
+  int rows;
+  const int nslab = ss_radon_slabs(p, j0, j1, nslices, &rows);
+  if (nslab > 1 && !scratch) return hipErrorInvalidValue;
+  dim3 grid(p.nwaves, (j1 - j0 + RADON_WAVES - 1) / RADON_WAVES, nslices * nslab);
   dim3 block(RADON_WAVES * SS_WAVE);
-  const int partial_base = p.nwaves * grid.y * nslices;
-  switch (mode) {
-    case SS_OUT_SINO:
-      hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, split, out, em);
-      break;
-    case SS_OUT_SINOT:
-      hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, split, out, em);
-      break;
-    default:
-      hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, split, out,
-                         em);
-      break;
+  int partial_base;
+  if (nslab > 1) {
+    const SSEmRatioArgs none{nullptr, nullptr};
+    hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, nslab, rows,
+                       nullptr, scratch, none);
+    const int fb = ss_radon_finish_blocks(j1 - j0, p.g.nrays);
+    dim3 fin(fb, nslices);
+    switch (mode) {
+      case SS_OUT_SINO:
+        hipLaunchKernelGGL(k_radon_slab_finish<SS_OUT_SINO>, fin, dim3(256), 0, stream, p, scratch, nslab, j0, j1, out,
+                           em);
+        break;
+      case SS_OUT_SINOT:
+        hipLaunchKernelGGL(k_radon_slab_finish<SS_OUT_SINOT>, fin, dim3(256), 0, stream, p, scratch, nslab, j0, j1,
+                           out, em);
+        break;
+      default:
+        hipLaunchKernelGGL(k_radon_slab_finish<SS_OUT_EM_RATIO>, fin, dim3(256), 0, stream, p, scratch, nslab, j0, j1,
+                           out, em);
+        break;
+    }
+    partial_base = fb * nslices;
+  } else {
+    switch (mode) {
+      case SS_OUT_SINO:
+        hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, 1, p.g.n, out,
+                           nullptr, em);
+        break;
+      case SS_OUT_SINOT:
+        hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, 1, p.g.n,
+                           out, nullptr, em);
+        break;
+      default:
+        hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, 1, p.g.n,
+                           out, nullptr, em);
+        break;
+    }
+    partial_base = (int)(grid.x * grid.y) * nslices;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || p.nirregular == 0) return e;
@@ -1740,6 +1832,9 @@ __device__ __forceinline__ SSBatchEntry ss_entry_of(const SSBatchLaunch& L, int 
   e.j0 = __builtin_amdgcn_readfirstlane(e.j0);
   e.j1 = __builtin_amdgcn_readfirstlane(e.j1);
   e.item0 = __builtin_amdgcn_readfirstlane(e.item0);
+  e.nslab = __builtin_amdgcn_readfirstlane(e.nslab);
+  e.slab_rows = __builtin_amdgcn_readfirstlane(e.slab_rows);
+  e.part_off = ss_rfl64(e.part_off);
   return e;
 }
 
@@ -1780,10 +1875,13 @@ __global__ __launch_bounds__(256) void k_batch_transpose(SSBatchLaunch L, const 
 // slice's plan values held in scalar registers would otherwise push the kernel to 87
 // VGPRs, two blocks per CU, and a second round of blocks for a 1024^2 OSEM subset.  The
 // scalar values that do not fit go to VGPR lanes (no scratch memory).
+//
+// Item = (row slab, ray group, block of 8 angles) of an entry, slab fastest.  With `part`
+// the items write their partial ray sums for k_batch_radon_finish.
 template <int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_radon(
-    SSBatchLaunch L, const float2* __restrict__ gA, const float2* __restrict__ gB, int split, float* __restrict__ out,
-    const float* __restrict__ meas, double* __restrict__ ll) {
+    SSBatchLaunch L, const float2* __restrict__ gA, const float2* __restrict__ gB, float* __restrict__ part,
+    float* __restrict__ out, const float* __restrict__ meas, double* __restrict__ ll) {
   __shared__ SSRadonLds lds;
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
     const SSBatchEntry e = ss_entry_of(L, item);
@@ -1800,10 +1898,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     v.nwaves = ss_uniform_copy(&sp->p.nwaves);
     const int64_t lay_off = ss_uniform_copy(&sp->lay_off), sino_off = ss_uniform_copy(&sp->sino_off);
     const int local = item - e.item0;
-    const int ablk = local / v.nwaves, grp = local - ablk * v.nwaves;
-    ss_radon_item<MODE>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, split, grp, ablk, out + sino_off,
-                        MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr,
-                        MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, lds);
+    const int ns = max(e.nslab, 1);
+    const int z = local % ns, r = local / ns;
+    const int ablk = r / v.nwaves, grp = r - ablk * v.nwaves;
+    const int rows = ns > 1 ? e.slab_rows : v.n;
+    float* po = part ? part + e.part_off + (size_t)z * (e.j1 - e.j0) * v.R : nullptr;
+    ss_radon_item<MODE>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, 1, grp, ablk, z * rows, min(v.n, (z + 1) * rows),
+                        out + sino_off, MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr,
+                        MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, po, lds);
     __syncthreads();
   }
 }
@@ -1819,6 +1921,40 @@ __global__ __launch_bounds__(256) void k_batch_radon_fixup(SSBatchLaunch L, cons
     ss_radon_fixup_item<MODE>(s.p, gA + s.lay_off, gB + s.lay_off, e.j0, e.j1, item - e.item0, out + s.sino_off,
                               MODE == SS_OUT_EM_RATIO ? meas + s.sino_off : nullptr,
                               MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, redf);
+    __syncthreads();
+  }
+}
+
+// Item = 256 consecutive rays [j - j0][k] of an entry: the sum of its nslab partial sums,
+// stored as k_batch_radon stores (EM mode: one FP64 log-likelihood partial per item).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_batch_radon_finish(SSBatchLaunch L, const float* __restrict__ part,
+                                                            float* __restrict__ out, const float* __restrict__ meas,
+                                                            double* __restrict__ ll) {
+  __shared__ double red[4];
+  for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
+    const SSBatchEntry e = ss_entry_of(L, item);
+    const SSBatchSlice* sp = L.slices + e.slice;
+    SSRadonView v;
+    v.angles = ss_uniform_copy(&sp->p.angles);
+    v.R = ss_uniform_copy(&sp->p.g.nrays);
+    v.A = ss_uniform_copy(&sp->p.g.nangles);
+    const int64_t sino_off = ss_uniform_copy(&sp->sino_off);
+    const size_t per = (size_t)(e.j1 - e.j0) * v.R;
+    const size_t idx = (size_t)(item - e.item0) * 256 + threadIdx.x;
+    double l = 0.0;
+    if (idx < per) {
+      const int jj = (int)(idx / (unsigned)v.R), k = (int)(idx - (size_t)jj * v.R);
+      const float* src = part + e.part_off + idx;
+      float sum = 0.f;
+      for (int z = 0; z < max(e.nslab, 1); ++z) sum += src[(size_t)z * per];
+      ss_radon_store<MODE>(v, e.j0 + jj, k, sum * v.angles[e.j0 + jj].delta, out + sino_off,
+                           MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr, &l);
+    }
+    if (MODE == SS_OUT_EM_RATIO) {
+      const double tot = ss_block_sum_f64(l, red);
+      if (threadIdx.x == 0) ll[item] = tot;
+    }
     __syncthreads();
   }
 }
@@ -1875,7 +2011,7 @@ int ss_batch_grid(int kind, int variant, int mode, int total) {
   // the layout kernels' items are single tiles: one block per item lets the dispatcher
   // overlap their loads and stores across blocks, where a resident grid walks its items
   // one after another (4% of a batched OSEM iteration at 256 slices)
-  if (kind == SS_KIND_GATHER || kind == SS_KIND_TRANSPOSE) return total;
+  if (kind == SS_KIND_GATHER || kind == SS_KIND_TRANSPOSE || kind == SS_KIND_RADON_FINISH) return total;
   int g = 0;
   switch (kind) {
     case SS_KIND_RADON:
@@ -1916,22 +2052,41 @@ hipError_t ss_launch_batch_transpose(const SSBatchLaunch& L, const float* in, fl
   return hipGetLastError();
 }
 
-hipError_t ss_launch_batch_radon(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, int split,
+hipError_t ss_launch_batch_radon(const SSBatchLaunch& L, const float2* gatherA, const float2* gatherB, float* part,
                                  int mode, float* out, const float* meas, double* ll_partials, hipStream_t stream) {
   if (L.total <= 0) return hipSuccess;
   const dim3 grid(L.grid), block(RADON_WAVES * SS_WAVE);
+  if (part) mode = SS_OUT_SINOT;  // partial sums only; the output mode is the finishing pass's
   switch (mode) {
     case SS_OUT_SINO:
-      hipLaunchKernelGGL(k_batch_radon<SS_OUT_SINO>, grid, block, 0, stream, L, gatherA, gatherB, split, out, meas,
+      hipLaunchKernelGGL(k_batch_radon<SS_OUT_SINO>, grid, block, 0, stream, L, gatherA, gatherB, part, out, meas,
                          ll_partials);
       break;
     case SS_OUT_SINOT:
-      hipLaunchKernelGGL(k_batch_radon<SS_OUT_SINOT>, grid, block, 0, stream, L, gatherA, gatherB, split, out, meas,
+      hipLaunchKernelGGL(k_batch_radon<SS_OUT_SINOT>, grid, block, 0, stream, L, gatherA, gatherB, part, out, meas,
                          ll_partials);
       break;
     default:
-      hipLaunchKernelGGL(k_batch_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, L, gatherA, gatherB, split, out,
-                         meas, ll_partials);
+      hipLaunchKernelGGL(k_batch_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, L, gatherA, gatherB, part, out, meas,
+                         ll_partials);
+  }
+  return hipGetLastError();
+}
+
+hipError_t ss_launch_batch_radon_finish(const SSBatchLaunch& L, const float* part, int mode, float* out,
+                                        const float* meas, double* ll_partials, hipStream_t stream) {
+  if (L.total <= 0) return hipSuccess;
+  const dim3 grid(L.grid), block(256);
+  switch (mode) {
+    case SS_OUT_SINO:
+      hipLaunchKernelGGL(k_batch_radon_finish<SS_OUT_SINO>, grid, block, 0, stream, L, part, out, meas, ll_partials);
+      break;
+    case SS_OUT_SINOT:
+      hipLaunchKernelGGL(k_batch_radon_finish<SS_OUT_SINOT>, grid, block, 0, stream, L, part, out, meas, ll_partials);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batch_radon_finish<SS_OUT_EM_RATIO>, grid, block, 0, stream, L, part, out, meas,
+                         ll_partials);
   }
   return hipGetLastError();
 }
