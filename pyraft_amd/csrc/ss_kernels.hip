@@ -278,10 +278,9 @@ __device__ __forceinline__ void ss_radon_store(const SSRadonView& p, int j, int 
   }
 }
 
-// One (angle, 64-ray group) is handled by `split` waves, each taking a contiguous
-// share of the group's sample range; the shares are summed through LDS.  A block
-// is 8 waves = 8/split angles.  split grows when the launch is too small to fill
-// the chip (few angles, e.g. one OSEM subset).
+// A block is 8 waves: 8 consecutive angles over the same 64-ray group, one angle per wave.
+// Launches too small to fill the chip (few angles, e.g. one OSEM subset) divide the sample
+// rows into slabs taken by separate blocks instead (ss_radon_slab_plan).
 #define RADON_WAVES 8
 
 typedef float ss_f32x2 __attribute__((ext_vector_type(2)));
@@ -303,8 +302,6 @@ __device__ __forceinline__ ss_f32x2 ss_pair(__amdgpu_buffer_rsrc_t rs, uint32_t 
 #define RADON_CHUNK 32      // image rows (gather rows) per staged chunk, at most
 
 struct SSRadonLds {
-  float comb_a[RADON_WAVES][SS_WAVE];
-  float comb_d[RADON_WAVES][SS_WAVE];
   double red[RADON_WAVES];
   int wmeta[RADON_WAVES][4];  // per wave: branch (-1: no samples), first row, last row
   int wwin[RADON_WAVES][2];   // per wave: lowest and highest cell of the current chunk
@@ -391,30 +388,28 @@ __device__ __forceinline__ void ss_radon_rows(const Rd& rd, int sa, int sb, int 
   }
 }
 
-// One block's work: rays [grp*64, grp*64+64) of the RADON_WAVES/split angles starting
-// at j0 + ablk*(RADON_WAVES/split), clipped to [j0, j1), over the sample rows
+// One block's work: rays [grp*64, grp*64+64) of the RADON_WAVES angles starting at
+// j0 + ablk*RADON_WAVES, clipped to [j0, j1), over the sample rows
 // [row_lo, row_hi).  gA/gB/out/meas point at this slice's data; in EM mode the block's
 // FP64 log-likelihood partial goes to *ll_slot.  With `part_out` the block writes its
 // partial ray sums there instead ([j - j0][k], delta not applied) for a finishing pass.
 //
-// split == 1 and all angles of the block on one branch (the usual case): the block
-// walks its sample rows in chunks of up to RADON_CHUNK rows.  For each chunk it
-// bounds the cells every wave can touch (the trajectory is affine in ray and row, so
-// the extremes are at the corners), copies that window of the gather rows into LDS
-// with contiguous loads, and the waves interpolate from LDS.  A window too wide for
-// LDS falls back to direct gathers for that chunk.  Otherwise (split > 1, or a block
-// straddling the branch switch) every wave gathers directly from the layout.
+// All angles of the block on one branch (the usual case): the block walks its sample
+// rows in chunks of up to RADON_CHUNK rows.  For each chunk it bounds the cells every
+// wave can touch (the trajectory is affine in ray and row, so the extremes are at the
+// corners), copies that window of the gather rows into LDS with contiguous loads, and
+// the waves interpolate from LDS.  A window too wide for LDS falls back to direct
+// gathers for that chunk.  A block straddling the branch switch gathers directly from
+// the layouts.
 template <int MODE>
 __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2* __restrict__ gA,
-                                              const float2* __restrict__ gB, int j0, int j1, int split, int grp,
-                                              int ablk, int row_lo, int row_hi, float* __restrict__ out,
+                                              const float2* __restrict__ gB, int j0, int j1, int grp, int ablk,
+                                              int row_lo, int row_hi, float* __restrict__ out,
                                               const float* __restrict__ meas, double* ll_slot,
                                               float* __restrict__ part_out, SSRadonLds& lds) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int per_block = RADON_WAVES / split;
-  const int aw = wv / split, part = wv - aw * split;
-  const int j = j0 + ablk * per_block + aw;
+  const int j = j0 + ablk * RADON_WAVES + wv;
   const int R = p.R, n = p.n, pitch = p.pitch;
   const int k = grp * SS_WAVE + lane;
   const bool active = j < j1;
@@ -447,7 +442,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
   if (p.ray_alpha && active && k < R) a_k = p.ray_alpha[(size_t)j * R + k];
 
   bool staged = false;
-  if (split == 1) {
+  {  // staged or direct: decided per block
     if (lane == 0) {
       lds.wmeta[wv][0] = branch;
       lds.wmeta[wv][1] = wb.x;
@@ -558,12 +553,8 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
     }
   }
   if (!staged && active && branch >= 0) {
-    // this wave's share [sa, sb) of the group's sample range [wb.x, wb.y] within the rows
-    const int r0 = max(wb.x, row_lo), r1 = min(wb.y + 1, row_hi);
-    const int len = max(r1 - r0, 0);
-    const int chunk = (len + split - 1) / split;
-    const int sa = r0 + part * chunk;
-    const int sb = min(r1, sa + chunk);
+    // the group's sample range [wb.x, wb.y] within the rows
+    const int sa = max(wb.x, row_lo), sb = min(wb.y + 1, row_hi);
     // u carries the trajectory plus one cell: its high word is the record index m + 1
     // of the lower tap in the gather row, its low word the interpolation weight
     int64_t u = a_k + (int64_t)sa * beta + ((int64_t)1 << 32);
@@ -578,19 +569,8 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
         (void*)lay, (short)sizeof(float2), (int)((uint32_t)n * (uint32_t)pitch * 8u), SS_RSRC_DWORD3);
     ss_radon_rows(SSGatherReader{rs, row_bytes}, sa, sb, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
   }
-  if (split > 1) {
-    lds.comb_a[wv][lane] = acc_a;
-    lds.comb_d[wv][lane] = acc_d;
-    __syncthreads();
-    if (part == 0) {
-      for (int q = 1; q < split; ++q) {
-        acc_a += lds.comb_a[wv + q][lane];
-        acc_d += lds.comb_d[wv + q][lane];
-      }
-    }
-  }
   double ll = 0.0;
-  if (active && part == 0 && k < R) {
+  if (active && k < R) {
     if (part_out) {
       part_out[(size_t)(j - j0) * R + k] = fmaf(acc_d, SS_INV_FIX, acc_a);
     } else {
@@ -609,7 +589,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
 // k_radon_slab_finish; without it (nslab = 1) they store the output themselves.
 template <int MODE>
 __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
-                                               const float2* __restrict__ gB, int j0, int j1, int split, int nslab,
+                                               const float2* __restrict__ gB, int j0, int j1, int nslab,
                                                int slab_rows, float* __restrict__ out, float* __restrict__ part,
                                                SSEmRatioArgs em) {
   __shared__ SSRadonLds lds;
@@ -620,7 +600,7 @@ __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __rest
   double* slot = nullptr;
   if (MODE == SS_OUT_EM_RATIO) slot = em.ll_partials + (slice * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   float* po = part ? part + (slice * nslab + z) * (size_t)(j1 - j0) * p.g.nrays : nullptr;
-  ss_radon_item<MODE>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, split, blockIdx.x, blockIdx.y, z * slab_rows,
+  ss_radon_item<MODE>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, blockIdx.x, blockIdx.y, z * slab_rows,
                       min(p.g.n, (z + 1) * slab_rows), out + so, MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr,
                       slot, po, lds);
 }
@@ -757,7 +737,7 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const floa
   int partial_base;
   if (nslab > 1) {
     const SSEmRatioArgs none{nullptr, nullptr};
-    hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, nslab, rows,
+    hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, nslab, rows,
                        nullptr, scratch, none);
     const int fb = ss_radon_finish_blocks(j1 - j0, p.g.nrays);
     dim3 fin(fb, nslices);
@@ -779,15 +759,15 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const floa
   } else {
     switch (mode) {
       case SS_OUT_SINO:
-        hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, 1, p.g.n, out,
+        hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, p.g.n, out,
                            nullptr, em);
         break;
       case SS_OUT_SINOT:
-        hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, 1, p.g.n,
+        hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, p.g.n,
                            out, nullptr, em);
         break;
       default:
-        hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, 1, p.g.n,
+        hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, p.g.n,
                            out, nullptr, em);
         break;
     }
@@ -1898,12 +1878,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     v.nwaves = ss_uniform_copy(&sp->p.nwaves);
     const int64_t lay_off = ss_uniform_copy(&sp->lay_off), sino_off = ss_uniform_copy(&sp->sino_off);
     const int local = item - e.item0;
-    const int ns = max(e.nslab, 1);
+    // a launch in slab form runs in SINOT mode (partial sums only), so the EM-ratio form
+    // is never divided: its slab arithmetic is compiled out
+    constexpr bool divisible = MODE != SS_OUT_EM_RATIO;
+    const int ns = divisible ? max(e.nslab, 1) : 1;
     const int z = local % ns, r = local / ns;
     const int ablk = r / v.nwaves, grp = r - ablk * v.nwaves;
     const int rows = ns > 1 ? e.slab_rows : v.n;
-    float* po = part ? part + e.part_off + (size_t)z * (e.j1 - e.j0) * v.R : nullptr;
-    ss_radon_item<MODE>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, 1, grp, ablk, z * rows, min(v.n, (z + 1) * rows),
+    float* po = divisible && part ? part + e.part_off + (size_t)z * (e.j1 - e.j0) * v.R : nullptr;
+    ss_radon_item<MODE>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, grp, ablk, z * rows, min(v.n, (z + 1) * rows),
                         out + sino_off, MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr,
                         MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, po, lds);
     __syncthreads();
