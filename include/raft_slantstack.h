@@ -217,7 +217,10 @@ int raft_ss_tr_em(raft_ss_plan *plan, const float *d_raw, const float *d_flat, f
  * starting image on entry and the estimate on exit.  Per subset: forward projection
  * with the guarded ratio s/p and FP64 log-likelihood partials fused in its epilogue,
  * then backprojection with x <- x * BP_b(ratio) / BP_b(1) fused in its epilogue.
- * ll_host (optional) receives niter values of sum(s log p - p) over rays with p > 0. */
+ * ll_host (optional) receives niter values of sum(s log p - p) over rays with p > 0.
+ * The subsets' backprojection variant is the plan's when set by
+ * raft_ss_plan_set_bp_variant; otherwise the first call per subset size times the
+ * variants on one subset's angle range (not under stream capture) and keeps the fastest. */
 int raft_ss_em(raft_ss_plan *plan, const float *d_sino, float *d_x, int niter, int nsubsets, int nslices,
                double *ll_host, void *stream);
 
@@ -238,7 +241,8 @@ int raft_ss_phantom(const double *desc_rows, int nell, int lines, int cols, cons
  * coordinate, which is taken relative to the tile origin (~1e-7 relative). */
 #define RAFT_SS_BP_NVARIANTS 6
 int raft_ss_bp_variant_shape(int variant, int *rows, int *stage);
-/* select a variant for a plan (RAFT_SS_EINVAL if the window does not fit its stage) */
+/* select a variant for a plan (RAFT_SS_EINVAL if the window does not fit its stage); it is
+ * then also the variant of raft_ss_em's subset backprojections */
 int raft_ss_plan_set_bp_variant(raft_ss_plan *plan, int variant);
 /* time every runnable variant on this plan's full angle range (warm-up + best of
  * `reps` launches, HIP events on `stream`), keep the fastest and return its index.

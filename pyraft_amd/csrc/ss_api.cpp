@@ -647,13 +647,19 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   double* d_ll = partials + npartials;
   float* slabs = reinterpret_cast<float*>(d_ll + std::max(niter, 1));
 
+  // the subsets' backprojection variant, tuned for their angle count (launches of few
+  // angles favour smaller tiles than a full-range backprojection)
+  int bv = p->bp_variant;
+  if ((rc = ss_plan_range_variant(p, 0, (int)(A / nsubsets), stream, &bv))) return rc;
+  const SSDevPlan dpb = p->dev(bv);
+
   HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
   // W_b = BP_b(1) with the same backprojection kernel
   HIP_TRY(ss_launch_fill(rhoT, 1.0f, sino_sz, stream));
   SSEmUpdateArgs none{nullptr, nullptr};
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-    HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
+    HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
                                      stream));
   }
   if (niter > 0) HIP_TRY(hipMemsetAsync(d_ll, 0, sizeof(double) * niter, stream));
@@ -664,7 +670,7 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
       SSEmRatioArgs ea{measT, partials + poff[b]};
       HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, slabs, stream));
       SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
-      HIP_TRY(ss_launch_backprojection(dp, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, stream));
+      HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, stream));
     }
     HIP_TRY(ss_launch_sum_partials(partials, npartials, d_ll + it, stream));
   }

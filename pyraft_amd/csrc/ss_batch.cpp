@@ -46,6 +46,7 @@ extern "C" int raft_ss_plan_set_bp_variant(raft_ss_plan* p, int v) {
   rc = check_variant(p, v);
   if (rc) return rc;
   p->bp_variant = v;
+  p->bp_variant_fixed = true;
   return 0;
 }
 
@@ -149,6 +150,49 @@ extern "C" int raft_ss_plan_tune(raft_ss_plan* p, int reps, int flags, double* t
   return best;
 }
 
+int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, int* variant) {
+  *variant = p->bp_variant;
+  if (p->bp_variant_fixed || j1 <= j0) return 0;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) return 0;
+  const std::string key = plan_key(p) + "|range" + std::to_string(j1 - j0);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      *variant = it->second.variant;
+      return 0;
+    }
+  }
+  const int n = p->g.n, R = p->g.nrays, A = p->g.nangles;
+  DevBuf ws;
+  HIP_TRY(ws.alloc(sizeof(float) * ((size_t)R * A + (size_t)n * n), stream));
+  float* sinoT = ws.as<float>();
+  float* img = sinoT + (size_t)R * A;
+  HIP_TRY(ss_launch_fill(sinoT, 1.0f, (size_t)R * A, stream));
+  TuneEntry te;
+  int best = -1, rc = 0;
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v) {
+    te.times[v] = -1.0;
+    if (p->bp_angles_v[v] <= 0) continue;
+    const SSDevPlan dp = p->dev(v);
+    SSEmUpdateArgs none{nullptr, nullptr};
+    auto run = [&]() -> int {
+      HIP_TRY(ss_launch_backprojection(dp, sinoT, j0, j1, 1, SS_BP_PLAIN, img, none, stream));
+      return 0;
+    };
+    if ((rc = time_best(run, 3, stream, &te.times[v]))) return rc;
+    if (best < 0 || te.times[v] < te.times[best]) best = v;
+  }
+  if (best < 0) return ss_set_error(RAFT_SS_EINTERNAL, "no runnable backprojection variant");
+  te.variant = best;
+  *variant = best;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune[key] = te;
+  return 0;
+}
+
 // ------------------------------------------------------------------ batches
 
 namespace {
@@ -220,6 +264,7 @@ struct raft_ss_batch {
   std::vector<int64_t> img_off, sino_off, lay_off;
   int64_t img_total = 0, sino_total = 0, lay_total = 0;
   std::vector<int> variant;  // backprojection variant per slice
+  int gather_tile = SS_GATHER_TILE;  // gather-layout tile side (ss_batch_gather_tile)
   Buf slices;                // SSBatchSlice[nslices]
   bool slices_dirty = true;
   std::map<std::string, std::unique_ptr<BatchLaunch>> launches;
@@ -321,7 +366,7 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
     }
     switch (kind) {
       case SS_KIND_GATHER: {
-        const int t = ss_gather_tiles(n);
+        const int t = (n + 2 + b->gather_tile - 1) / b->gather_tile;
         k.items = t * t;
         k.cost = 1.0;
         break;
@@ -490,6 +535,7 @@ extern "C" int raft_ss_batch_create(raft_ss_batch** out, int nslices, const int*
     b->lay_off[i] = b->lay_total;
     b->lay_total = align_up(b->lay_total + (int64_t)n[i] * it->second->g.pitch, 32);
   }
+  b->gather_tile = ss_batch_gather_tile(n, nslices);
   *out = b.release();
   return 0;
 }
@@ -561,7 +607,7 @@ extern "C" int raft_ss_batch_radon(raft_ss_batch* b, const float* d_images, floa
   char key[128];
   snprintf(key, sizeof(key), "radon|%p|%p|%d", (const void*)d_images, (void*)d_sinos, mode);
   return run_body(b, flags & RAFT_SS_BATCH_GRAPH, key, stream, [&](hipStream_t s) -> int {
-    HIP_TRY(ss_launch_batch_gather(Lg->dev(ds), d_images, gA, gB, s));
+    HIP_TRY(ss_launch_batch_gather(Lg->dev(ds), b->gather_tile, d_images, gA, gB, s));
     HIP_TRY(ss_launch_batch_radon(Lr->dev(ds), gA, gB, part, mode, d_sinos, nullptr, nullptr, s));
     if (part) HIP_TRY(ss_launch_batch_radon_finish(Lz->dev(ds), part, mode, d_sinos, nullptr, nullptr, s));
     HIP_TRY(ss_launch_batch_radon_fixup(Lf->dev(ds), gA, gB, mode, d_sinos, nullptr, nullptr, s));
@@ -689,7 +735,7 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
     HIP_TRY(ss_launch_batch_transpose(Lt->dev(ds), d_sinos, measT, 1, st));
     for (int it = 0; it < niter; ++it) {
       for (int s = 0; s < S; ++s) {
-        HIP_TRY(ss_launch_batch_gather(Lg->dev(ds), d_x, gA, gB, st));
+        HIP_TRY(ss_launch_batch_gather(Lg->dev(ds), b->gather_tile, d_x, gA, gB, st));
         float* sp = Lr[s]->slabs ? slab : nullptr;
         HIP_TRY(ss_launch_batch_radon(Lr[s]->dev(ds), gA, gB, sp, SS_OUT_EM_RATIO, rhoT, measT, part + poff[s], st));
         if (sp)

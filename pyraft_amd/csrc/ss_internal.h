@@ -27,6 +27,13 @@ int ss_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2
 
 // ------------------------------------------------------------------ plan
 
+struct raft_ss_plan;
+// Backprojection variant for launches over angles [j0, j1) (OSEM subsets): the fastest
+// variant of a timed plain launch over that range, cached per process by device,
+// geometry and range length.  The plan's own variant when it was set explicitly, when
+// the range is empty, or while `stream` is being captured (no timing then).
+int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, int* variant);
+
 struct raft_ss_plan {
   int device = 0;
   SSGeom g{};
@@ -44,6 +51,7 @@ struct raft_ss_plan {
   int ndegenerate = 0;
   int nwaves = 0;
   int bp_variant = SS_BP_DEFAULT_VARIANT;
+  bool bp_variant_fixed = false;  // set by raft_ss_plan_set_bp_variant: no per-range tuning
   int bp_window_v[SS_BP_NVARIANTS] = {0};  // per backprojection variant: staged window
   int bp_angles_v[SS_BP_NVARIANTS] = {0};  //   and angles per stage (0: not runnable)
   float* d_ramp = nullptr;  // FBP convolution kernel, built on first use

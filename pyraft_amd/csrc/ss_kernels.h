@@ -128,8 +128,11 @@ hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int
 //      8 angles x row slabs; finish = 256 rays of an entry; fixup = irregular rays; bp = pixel
 //      tiles of the variant; transpose = 32x32 tiles)
 int ss_bp_tiles(int n, int variant);
-hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, const float* images, float2* gatherA, float2* gatherB,
-                                  hipStream_t stream);
+// gather tile side for a batch of images with sides n[0..count): SS_GATHER_TILE when the
+// batch has at least 1024 such tiles, else 32 (items are tiles of that side)
+int ss_batch_gather_tile(const int* n, int count);
+hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, int tile, const float* images, float2* gatherA,
+                                  float2* gatherB, hipStream_t stream);
 hipError_t ss_launch_batch_transpose(const SSBatchLaunch& L, const float* in, float* out, int to_angle_major,
                                      hipStream_t stream);
 // part != nullptr: the entries' partial sums go to part (launch in slab form, then

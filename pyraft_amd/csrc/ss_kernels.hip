@@ -690,12 +690,15 @@ __global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* 
 // A launch of few blocks (a few angles: one OSEM subset) divides the sample rows of every
 // (angle block, ray group) into slabs handled by separate blocks, so that every CU gets
 // blocks while each block keeps the LDS-staged path; a finishing pass then adds the
-// slabs' partial sums.  Rows per slab are a multiple of RADON_CHUNK.
+// slabs' partial sums.  Rows per slab are a multiple of RADON_CHUNK, and the divided
+// launch stays within SS_RADON_SLAB_BLOCKS blocks (4 per CU, one round: a second round
+// for a few blocks would double the launch time).
 static_assert(RADON_WAVES == 8, "ss_radon_blocks counts blocks of 8 angles");
 int ss_radon_slab_plan(int n, long long blocks, int* rows) {
   *rows = n;
   if (blocks >= SS_RADON_SLAB_BLOCKS) return 1;
-  const long long want = (SS_RADON_SLAB_BLOCKS + std::max(blocks, 1LL) - 1) / std::max(blocks, 1LL);
+  const long long want = SS_RADON_SLAB_BLOCKS / std::max(blocks, 1LL);
+  if (want <= 1) return 1;
   int L = (int)((n + want - 1) / want);
   L = (L + RADON_CHUNK - 1) / RADON_CHUNK * RADON_CHUNK;
   const int count = (n + L - 1) / L;
@@ -1818,18 +1821,19 @@ __device__ __forceinline__ SSBatchEntry ss_entry_of(const SSBatchLaunch& L, int 
   return e;
 }
 
-__global__ __launch_bounds__(8 * SS_WAVE) void k_batch_gather(SSBatchLaunch L, const float* __restrict__ images,
-                                                              float2* __restrict__ gA, float2* __restrict__ gB) {
-  __shared__ float T[SS_GATHER_TILE + 1][SS_GATHER_TILE + 1];
+// GT x GT tiles with NW waves (ss_batch_gather_tile picks the shape per launch)
+template <int GT, int NW>
+__global__ __launch_bounds__(NW * SS_WAVE) void k_batch_gather(SSBatchLaunch L, const float* __restrict__ images,
+                                                               float2* __restrict__ gA, float2* __restrict__ gB) {
+  __shared__ float T[GT + 1][GT + 1];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
     const SSBatchEntry e = ss_entry_of(L, item);
     const SSBatchSlice s = ss_uniform_copy(L.slices + e.slice);
     const int n = s.p.g.n;
-    const int tiles = ss_gather_tiles(n);
+    const int tiles = (n + 2 + GT - 1) / GT;
     const int local = item - e.item0;
     const int ty = local / tiles, tx = local - ty * tiles;
-    ss_gather_tile<SS_GATHER_TILE, 8>(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch,
-                                      tx * SS_GATHER_TILE, ty * SS_GATHER_TILE, T);
+    ss_gather_tile<GT, NW>(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch, tx * GT, ty * GT, T);
     __syncthreads();
   }
 }
@@ -2021,10 +2025,21 @@ int ss_batch_grid(int kind, int variant, int mode, int total) {
   return g < total ? ss_prime_at_most(g) : total;
 }
 
-hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, const float* images, float2* gatherA, float2* gatherB,
-                                  hipStream_t stream) {
+int ss_batch_gather_tile(const int* n, int count) {
+  long long big = 0;
+  for (int i = 0; i < count; ++i) big += (long long)ss_gather_tiles(n[i]) * ss_gather_tiles(n[i]);
+  return big >= 1024 ? SS_GATHER_TILE : 32;  // the per-plan rule: at least 4 large tiles per CU
+}
+
+hipError_t ss_launch_batch_gather(const SSBatchLaunch& L, int tile, const float* images, float2* gatherA,
+                                  float2* gatherB, hipStream_t stream) {
   if (L.total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_gather, dim3(L.grid), dim3(8 * SS_WAVE), 0, stream, L, images, gatherA, gatherB);
+  if (tile == SS_GATHER_TILE)
+    hipLaunchKernelGGL((k_batch_gather<SS_GATHER_TILE, 8>), dim3(L.grid), dim3(8 * SS_WAVE), 0, stream, L, images,
+                       gatherA, gatherB);
+  else
+    hipLaunchKernelGGL((k_batch_gather<32, 4>), dim3(L.grid), dim3(4 * SS_WAVE), 0, stream, L, images, gatherA,
+                       gatherB);
   return hipGetLastError();
 }
 
