@@ -1086,15 +1086,16 @@ enum { RT_DISTINCT = 0, RT_PHASED = 1, RT_ATOMIC = 2 };
 
 // A wave owns image line s of SL slices (SL = 2 shares the geometry work of a pass
 // between two slices of a batch; every slice has its own LDS line and sinogram row).
+#define RT_SL_MAX 4  // slices per wave, at most
 struct SSRtLine {
-  float* line[2];            // LDS lines: cell c + 1 holds pixel c; cells n+2, n+3 are spare
+  float* line[RT_SL_MAX];    // LDS lines: cell c + 1 holds pixel c; cells n+2, n+3 are spare
   int n, s;                  // line length, line index (the forward's sample)
   int k_lo, k_hi;            // candidate rays
   bool all_edge;             // every sample reads its ray's exact range
   int p_first, p_last;       // passes p_first .. p_last - 1 hold only valid samples
   int phases, my_phase;      // RT_PHASED
   __amdgpu_buffer_rsrc_t rb;     // the angle's range row (int2)
-  __amdgpu_buffer_rsrc_t rq[2];  // the angle's sinogram row (float) of each slice
+  __amdgpu_buffer_rsrc_t rq[RT_SL_MAX];  // the angle's sinogram row (float) of each slice
   int64_t gamma;             // trajectory step per ray
   int64_t u0, step;          // trajectory (+1 cell) of ray k_lo, and its step per pass
   float delta;
@@ -1430,25 +1431,36 @@ size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices) {
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, float* scratch, hipStream_t stream) {
   const int n = p.g.n;
-  // two slices per wave where the image is small enough for both lines to leave room for
-  // many waves per CU (n <= 1024: 8 KiB per wave); an odd last slice goes alone
-  const int pairs = (n <= 1024) ? nslices / 2 : 0;
+  // several slices per wave where the image is small enough for their lines to leave
+  // room for many waves per CU: four for n <= 512, two for n <= 1024 (8 KiB per wave);
+  // the remaining slices go in pairs, then alone
+  const int quads = (n <= 512) ? nslices / 4 : 0;
+  const int pairs = (n <= 1024) ? (nslices - 4 * quads) / 2 : 0;
+  const int singles = nslices - 4 * quads - 2 * pairs;
+  if (quads > 0) {
+    const int nl = ss_rt_lines(n, 4);
+    if (nl == 0) return hipErrorInvalidConfiguration;
+    const int groups = (n + nl - 1) / nl;
+    hipLaunchKernelGGL(k_radon_transpose_lines<4>, dim3(2 * groups, quads), dim3(nl * SS_WAVE),
+                       ss_rt_line_lds(n, 4 * nl), stream, p, sinoT, j0, j1, 0, out, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (pairs > 0) {
     const int nl = ss_rt_lines(n, 2);
     if (nl == 0) return hipErrorInvalidConfiguration;
     const int groups = (n + nl - 1) / nl;
     hipLaunchKernelGGL(k_radon_transpose_lines<2>, dim3(2 * groups, pairs), dim3(nl * SS_WAVE),
-                       ss_rt_line_lds(n, 2 * nl), stream, p, sinoT, j0, j1, 0, out, scratch);
+                       ss_rt_line_lds(n, 2 * nl), stream, p, sinoT, j0, j1, 4 * quads, out, scratch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  const int rest = nslices - 2 * pairs;
-  if (rest > 0) {
+  if (singles > 0) {
     const int nl = ss_rt_lines(n, 1);
     if (nl == 0) return hipErrorInvalidConfiguration;  // a line does not fit the LDS
     const int groups = (n + nl - 1) / nl;
-    hipLaunchKernelGGL(k_radon_transpose_lines<1>, dim3(2 * groups, rest), dim3(nl * SS_WAVE),
-                       ss_rt_line_lds(n, nl), stream, p, sinoT, j0, j1, 2 * pairs, out, scratch);
+    hipLaunchKernelGGL(k_radon_transpose_lines<1>, dim3(2 * groups, singles), dim3(nl * SS_WAVE),
+                       ss_rt_line_lds(n, nl), stream, p, sinoT, j0, j1, 4 * quads + 2 * pairs, out, scratch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
