@@ -408,7 +408,7 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
   }
   if (total > 0x7fffffffLL) return ss_set_error(RAFT_SS_EINVAL, "batch too large (%lld work items)", total);
   L.total = (int)total;
-  L.grid = ss_batch_grid(kind, v, mode, L.total);
+  L.grid = ss_batch_grid(kind, v, kind == SS_KIND_RADON && L.slabs ? SS_OUT_PARTIAL : mode, L.total);
   if (!L.entries.empty()) {
     HIP_TRY(hipMalloc(&L.d_entries, sizeof(SSBatchEntry) * L.entries.size()));
     HIP_TRY(hipMemcpy(L.d_entries, L.entries.data(), sizeof(SSBatchEntry) * L.entries.size(),

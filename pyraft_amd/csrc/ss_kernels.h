@@ -14,8 +14,9 @@
 
 #include "ss_geometry.h"
 
-// Radon output modes
-enum { SS_OUT_SINO = 0, SS_OUT_SINOT = 1, SS_OUT_EM_RATIO = 2 };
+// Radon output modes; SS_OUT_PARTIAL: a row slab's partial ray sums (launches of few
+// blocks), which a finishing pass stores in one of the other modes
+enum { SS_OUT_SINO = 0, SS_OUT_SINOT = 1, SS_OUT_EM_RATIO = 2, SS_OUT_PARTIAL = 3 };
 // Backprojection output modes
 enum { SS_BP_PLAIN = 0, SS_BP_EM_UPDATE = 1 };
 

@@ -391,8 +391,8 @@ __device__ __forceinline__ void ss_radon_rows(const Rd& rd, int sa, int sb, int 
 // One block's work: rays [grp*64, grp*64+64) of the RADON_WAVES angles starting at
 // j0 + ablk*RADON_WAVES, clipped to [j0, j1), over the sample rows
 // [row_lo, row_hi).  gA/gB/out/meas point at this slice's data; in EM mode the block's
-// FP64 log-likelihood partial goes to *ll_slot.  With `part_out` the block writes its
-// partial ray sums there instead ([j - j0][k], delta not applied) for a finishing pass.
+// FP64 log-likelihood partial goes to *ll_slot.  In SS_OUT_PARTIAL mode the block writes
+// its partial ray sums to part_out ([j - j0][k], delta not applied) for a finishing pass.
 //
 // All angles of the block on one branch (the usual case): the block walks its sample
 // rows in chunks of up to RADON_CHUNK rows.  For each chunk it bounds the cells every
@@ -571,7 +571,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
   }
   double ll = 0.0;
   if (active && k < R) {
-    if (part_out) {
+    if (MODE == SS_OUT_PARTIAL) {
       part_out[(size_t)(j - j0) * R + k] = fmaf(acc_d, SS_INV_FIX, acc_a);
     } else {
       const float val = fmaf(acc_d, SS_INV_FIX, acc_a) * delta;
@@ -585,8 +585,8 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
 }
 
 // blockIdx.z = slice * nslab + z: slab z takes sample rows [z * slab_rows, (z + 1) * slab_rows).
-// With `part` the blocks write partial ray sums to part[slice][z][j - j0][k] for
-// k_radon_slab_finish; without it (nslab = 1) they store the output themselves.
+// In SS_OUT_PARTIAL mode the blocks write partial ray sums to part[slice][z][j - j0][k] for
+// k_radon_slab_finish; in the other modes (nslab = 1) they store the output themselves.
 template <int MODE>
 __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
                                                const float2* __restrict__ gB, int j0, int j1, int nslab,
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __rest
   const size_t so = slice * p.g.nangles * p.g.nrays;
   double* slot = nullptr;
   if (MODE == SS_OUT_EM_RATIO) slot = em.ll_partials + (slice * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  float* po = part ? part + (slice * nslab + z) * (size_t)(j1 - j0) * p.g.nrays : nullptr;
+  float* po = MODE == SS_OUT_PARTIAL ? part + (slice * nslab + z) * (size_t)(j1 - j0) * p.g.nrays : nullptr;
   ss_radon_item<MODE>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, blockIdx.x, blockIdx.y, z * slab_rows,
                       min(p.g.n, (z + 1) * slab_rows), out + so, MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr,
                       slot, po, lds);
@@ -740,7 +740,7 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const floa
   int partial_base;
   if (nslab > 1) {
     const SSEmRatioArgs none{nullptr, nullptr};
-    hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, nslab, rows,
+    hipLaunchKernelGGL(k_radon<SS_OUT_PARTIAL>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, nslab, rows,
                        nullptr, scratch, none);
     const int fb = ss_radon_finish_blocks(j1 - j0, p.g.nrays);
     dim3 fin(fb, nslices);
@@ -1872,8 +1872,9 @@ __global__ __launch_bounds__(256) void k_batch_transpose(SSBatchLaunch L, const 
 // VGPRs, two blocks per CU, and a second round of blocks for a 1024^2 OSEM subset.  The
 // scalar values that do not fit go to VGPR lanes (no scratch memory).
 //
-// Item = (row slab, ray group, block of 8 angles) of an entry, slab fastest.  With `part`
-// the items write their partial ray sums for k_batch_radon_finish.
+// Item = (row slab, ray group, block of 8 angles) of an entry, slab fastest.  In
+// SS_OUT_PARTIAL mode the items write their partial ray sums for k_batch_radon_finish;
+// launches in the other modes are never divided into slabs.
 template <int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_radon(
     SSBatchLaunch L, const float2* __restrict__ gA, const float2* __restrict__ gB, float* __restrict__ part,
@@ -1894,14 +1895,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     v.nwaves = ss_uniform_copy(&sp->p.nwaves);
     const int64_t lay_off = ss_uniform_copy(&sp->lay_off), sino_off = ss_uniform_copy(&sp->sino_off);
     const int local = item - e.item0;
-    // a launch in slab form runs in SINOT mode (partial sums only), so the EM-ratio form
-    // is never divided: its slab arithmetic is compiled out
-    constexpr bool divisible = MODE != SS_OUT_EM_RATIO;
+    constexpr bool divisible = MODE == SS_OUT_PARTIAL;
     const int ns = divisible ? max(e.nslab, 1) : 1;
     const int z = local % ns, r = local / ns;
     const int ablk = r / v.nwaves, grp = r - ablk * v.nwaves;
     const int rows = ns > 1 ? e.slab_rows : v.n;
-    float* po = divisible && part ? part + e.part_off + (size_t)z * (e.j1 - e.j0) * v.R : nullptr;
+    float* po = divisible ? part + e.part_off + (size_t)z * (e.j1 - e.j0) * v.R : nullptr;
     ss_radon_item<MODE>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, grp, ablk, z * rows, min(v.n, (z + 1) * rows),
                         out + sino_off, MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr,
                         MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, po, lds);
@@ -2014,8 +2013,9 @@ int ss_batch_grid(int kind, int variant, int mode, int total) {
   int g = 0;
   switch (kind) {
     case SS_KIND_RADON:
-      g = mode == SS_OUT_EM_RATIO ? ss_resident_blocks(k_batch_radon<SS_OUT_EM_RATIO>, 512)
-                                  : ss_resident_blocks(k_batch_radon<SS_OUT_SINOT>, 512);
+      g = mode == SS_OUT_EM_RATIO  ? ss_resident_blocks(k_batch_radon<SS_OUT_EM_RATIO>, 512)
+          : mode == SS_OUT_PARTIAL ? ss_resident_blocks(k_batch_radon<SS_OUT_PARTIAL>, 512)
+                                   : ss_resident_blocks(k_batch_radon<SS_OUT_SINOT>, 512);
       break;
     case SS_KIND_FIXUP:
       g = ss_resident_blocks(k_batch_radon_fixup<SS_OUT_EM_RATIO>, 256);
@@ -2066,8 +2066,12 @@ hipError_t ss_launch_batch_radon(const SSBatchLaunch& L, const float2* gatherA, 
                                  int mode, float* out, const float* meas, double* ll_partials, hipStream_t stream) {
   if (L.total <= 0) return hipSuccess;
   const dim3 grid(L.grid), block(RADON_WAVES * SS_WAVE);
-  if (part) mode = SS_OUT_SINOT;  // partial sums only; the output mode is the finishing pass's
+  if (part) mode = SS_OUT_PARTIAL;  // the output mode is the finishing pass's
   switch (mode) {
+    case SS_OUT_PARTIAL:
+      hipLaunchKernelGGL(k_batch_radon<SS_OUT_PARTIAL>, grid, block, 0, stream, L, gatherA, gatherB, part, out, meas,
+                         ll_partials);
+      break;
     case SS_OUT_SINO:
       hipLaunchKernelGGL(k_batch_radon<SS_OUT_SINO>, grid, block, 0, stream, L, gatherA, gatherB, part, out, meas,
                          ll_partials);
