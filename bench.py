@@ -443,7 +443,9 @@ def main():
     # PCIe transfers and FP64<->FP32 conversion included), same workload
     print("bench: host-buffer drop-in path", file=sys.stderr, flush=True)
     import pyraft_amd as pr
-    host_img = img.double().cpu().numpy()
+    # a pyraft image (default corners [-1,1]^2): a bare ndarray would be read as an
+    # ellipse description, as pyraft's radon does (pyraft/sinogram/sino.py:175-212)
+    host_img = pr.image(img.double().cpu().numpy())
     pr.radon(host_img, (R, A), method=pr.radon_method.SLANTSTACK)  # plan cache, warm-up
     t0 = time.perf_counter()
     host_sino = pr.radon(host_img, (R, A), method=pr.radon_method.SLANTSTACK)

@@ -58,17 +58,22 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   // in FP32, without FP32's N*eps growth over a ray of N equal terms
   double acc = 0.0;
   int F = dda_wave_start(alive, ray.f, a.dfast);
+  // the pixel's offset in the (fast, slow) image, stepped with the ray (an image side is
+  // at most 16384, so a slice's offsets fit an int); meaningful while the lane is alive
+  int pix = alive ? ray.f * ns + ray.s : 0;
+  const int fast_step = a.dfast * ns;
   // lanes start at most nf moves apart and then take at most nf + 1 moves each
   for (int it = 0; it <= 2 * nf + 1; ++it) {
     if (__ballot(alive) == 0) break;
     if (alive && ray.f == F) {
-      acc = fma((double)src[(size_t)ray.f * ns + ray.s], ray.len, acc);
+      acc = fma((double)src[pix], ray.len, acc);
       if (ray.slow_next) {
         ray.s = dda_step(ray.s, a.dslow);
+        pix += a.dslow;
         ray.len = a.len_full - ray.len;
         ray.slow_next = 0;
         alive = dda_inside(ray.f, ray.s, nf, ns);
-        if (alive) acc = fma((double)src[(size_t)ray.f * ns + ray.s], ray.len, acc);
+        if (alive) acc = fma((double)src[pix], ray.len, acc);
       }
       if (alive) {
         if (a.fixed) {
@@ -80,6 +85,7 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
         } else {
           dda_fast_move(a, ray);
         }
+        pix += fast_step;
         alive = dda_inside(ray.f, ray.s, nf, ns);
       }
     }
