@@ -6,7 +6,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <mutex>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/raft_slantstack.h"
@@ -98,26 +101,103 @@ void ss_filter_kernel(int R, double tmin, double tmax, int kind, double param, f
 
 // ------------------------------------------------------------------ host <-> raft_image staging
 
+// Pinned staging buffers reused across calls: pinning tens of MB costs milliseconds, more
+// than the copies.  Released buffers go back to the pool, which keeps the 4 largest; the
+// pool itself is never destroyed, so no HIP call runs at process exit.  A release first
+// waits for the device, as hipHostFree does, so that no copy into or out of the buffer
+// is in flight when another call takes it (the error paths return before their own wait).
+struct SSPinnedPool {
+  std::mutex mu;
+  std::vector<std::pair<void*, size_t>> free;
+};
+inline SSPinnedPool& ss_pinned_pool() {
+  static SSPinnedPool* pool = new SSPinnedPool();
+  return *pool;
+}
+
 struct PinnedF32 {
   float* p = nullptr;
+  size_t bytes = 0;
   PinnedF32() = default;
   PinnedF32(const PinnedF32&) = delete;
   PinnedF32& operator=(const PinnedF32&) = delete;
   ~PinnedF32() {
-    if (p) (void)hipHostFree(p);
+    if (!p) return;
+    (void)hipDeviceSynchronize();
+    SSPinnedPool& pool = ss_pinned_pool();
+    std::lock_guard<std::mutex> lock(pool.mu);
+    pool.free.emplace_back(p, bytes);
+    if (pool.free.size() > 4) {
+      auto it = std::min_element(pool.free.begin(), pool.free.end(),
+                                 [](const std::pair<void*, size_t>& a, const std::pair<void*, size_t>& b) {
+                                   return a.second < b.second;
+                                 });
+      (void)hipHostFree(it->first);
+      pool.free.erase(it);
+    }
   }
-  hipError_t alloc(size_t n) { return hipHostMalloc(reinterpret_cast<void**>(&p), n * sizeof(float) + 4, 0); }
+  // n floats: the smallest pooled buffer that holds them, else a new one
+  hipError_t alloc(size_t n) {
+    const size_t need = n * sizeof(float) + 4;
+    SSPinnedPool& pool = ss_pinned_pool();
+    {
+      std::lock_guard<std::mutex> lock(pool.mu);
+      auto best = pool.free.end();
+      for (auto it = pool.free.begin(); it != pool.free.end(); ++it)
+        if (it->second >= need && (best == pool.free.end() || it->second < best->second)) best = it;
+      if (best != pool.free.end()) {
+        p = static_cast<float*>(best->first);
+        bytes = best->second;
+        pool.free.erase(best);
+        return hipSuccess;
+      }
+    }
+    void* q = nullptr;
+    const hipError_t e = hipHostMalloc(&q, need, 0);
+    if (e != hipSuccess) return e;
+    p = static_cast<float*>(q);
+    bytes = need;
+    return hipSuccess;
+  }
 };
+
+// f(i0, i1) over rows [0, lines) in contiguous parts on up to 16 threads (the host's
+// hardware threads); matrices under 2^20 elements stay on the calling thread, and parts
+// whose thread cannot start run on it too
+template <class F>
+inline void ss_for_rows(int lines, size_t cols, F f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  int t = (int)std::min<unsigned>(hw ? hw : 1u, 16u);
+  if ((size_t)lines * cols < ((size_t)1 << 20)) t = 1;
+  t = std::max(1, std::min(t, lines));
+  const int per = (lines + t - 1) / t;
+  std::vector<std::thread> th;
+  int a = per;
+  for (; a < lines; a += per) {
+    try {
+      th.emplace_back(f, a, std::min(lines, a + per));
+    } catch (...) {
+      break;
+    }
+  }
+  f(0, std::min(lines, per));
+  if (a < lines) f(a, lines);
+  for (auto& x : th) x.join();
+}
 
 // strided FP64 matrix -> dense row-major FP32
 inline void gather_f64(const raft_matrix& m, float* dst) {
-  for (int i = 0; i < m.lines; ++i)
-    for (int j = 0; j < m.columns; ++j) dst[(size_t)i * m.columns + j] = (float)RAFT_MATRIX_AT(m, i, j);
+  ss_for_rows(m.lines, (size_t)m.columns, [&m, dst](int i0, int i1) {
+    for (int i = i0; i < i1; ++i)
+      for (int j = 0; j < m.columns; ++j) dst[(size_t)i * m.columns + j] = (float)RAFT_MATRIX_AT(m, i, j);
+  });
 }
 
 inline void scatter_f64(const float* src, const raft_matrix& m) {
-  for (int i = 0; i < m.lines; ++i)
-    for (int j = 0; j < m.columns; ++j) RAFT_MATRIX_AT(m, i, j) = (double)src[(size_t)i * m.columns + j];
+  ss_for_rows(m.lines, (size_t)m.columns, [src, &m](int i0, int i1) {
+    for (int i = i0; i < i1; ++i)
+      for (int j = 0; j < m.columns; ++j) RAFT_MATRIX_AT(m, i, j) = (double)src[(size_t)i * m.columns + j];
+  });
 }
 
 inline void corners_of(const raft_image& im, double* c) {
