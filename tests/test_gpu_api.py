@@ -126,3 +126,29 @@ def test_em_osem_make_em(pr):
     xo = pr.osem(s, niter=2, nblocks=4)
     refo, _ = O.osem(_f32(s), n, 2, nsubsets=4)
     assert rel_l2(xo, refo) <= REL_L2
+
+
+def test_large_host_arrays_match_device_path(pr):
+    """Images of 2^20 elements and more convert FP64 <-> FP32 on several host threads and
+    stage through pooled pinned buffers: the host-array drop-ins must give exactly what the
+    device-resident calls give on the same FP32-rounded data (a strided view included, and
+    two calls in a row reusing the pooled buffers)."""
+    import torch
+    from pyraft_amd.device import Plan
+    n, R, A = 1100, 1100, 64
+    print("large host arrays", n, R, A, flush=True)
+    rng = np.random.default_rng(1100)
+    view = pr.image(rng.random((2 * n, 2 * n)))[::2, ::2]   # strides of 2 rows and 2 columns
+    view.top_left, view.bottom_right = (-1.0, 1.0), (1.0, -1.0)
+    assert not view.flags.c_contiguous
+    p = Plan(n, R, A)
+    ref_sino = p.radon(torch.from_numpy(np.asarray(view, dtype=np.float32)).cuda()).double().cpu().numpy()
+    for _ in range(2):
+        s = pr.radon(view, (R, A), method=pr.radon_method.SLANTSTACK)
+        assert np.array_equal(np.asarray(s), ref_sino)
+    y = np.ascontiguousarray(ref_sino)
+    ref_bp = p.backprojection(torch.from_numpy(y.astype(np.float32)).cuda()).double().cpu().numpy()
+    plan = pr.backprojection_plan(y.shape, (n, n), method=pr.radon_method.SLANTSTACK)
+    for _ in range(2):
+        b = pr.backprojection(y, plan)
+        assert np.array_equal(np.asarray(b), ref_bp)
