@@ -584,6 +584,15 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
   }
 }
 
+// The slabs' partial sums of one ray, added in slab order (the per-plan and batched
+// finishing passes give the same bits); unrolled so that the loads issue together.
+__device__ __forceinline__ float ss_slab_sum(const float* __restrict__ src, size_t per, int nslab) {
+  float sum = 0.f;
+#pragma unroll 8
+  for (int z = 0; z < nslab; ++z) sum += src[(size_t)z * per];
+  return sum;
+}
+
 // blockIdx.z = slice * nslab + z: slab z takes sample rows [z * slab_rows, (z + 1) * slab_rows).
 // In SS_OUT_PARTIAL mode the blocks write partial ray sums to part[slice][z][j - j0][k] for
 // k_radon_slab_finish; in the other modes (nslab = 1) they store the output themselves.
@@ -622,7 +631,7 @@ __global__ __launch_bounds__(256) void k_radon_slab_finish(SSDevPlan p, const fl
     const int jj = (int)(idx / (unsigned)R), k = (int)(idx - (size_t)jj * R);
     const float* src = part + slice * nslab * per + idx;
     float sum = 0.f;
-    for (int z = 0; z < nslab; ++z) sum += src[(size_t)z * per];
+    sum = ss_slab_sum(src, per, nslab);
     ss_radon_store<MODE>(ss_radon_view(p), j0 + jj, k, sum * p.angles[j0 + jj].delta, out + so,
                          MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr, &ll);
   }
@@ -1945,7 +1954,7 @@ __global__ __launch_bounds__(256) void k_batch_radon_finish(SSBatchLaunch L, con
       const int jj = (int)(idx / (unsigned)v.R), k = (int)(idx - (size_t)jj * v.R);
       const float* src = part + e.part_off + idx;
       float sum = 0.f;
-      for (int z = 0; z < max(e.nslab, 1); ++z) sum += src[(size_t)z * per];
+      sum = ss_slab_sum(src, per, max(e.nslab, 1));
       ss_radon_store<MODE>(v, e.j0 + jj, k, sum * v.angles[e.j0 + jj].delta, out + sino_off,
                            MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr, &l);
     }
