@@ -126,11 +126,11 @@ def _bp_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC, variant=0):
     """Emulate the kernel's FP32 window index for every pixel tile of variant
     `variant` (32 columns x 8*rows rows) at the tile corners; return the extremes
     and the window W the library stages."""
-    W6 = (ctypes.c_int * 6)()
-    J6 = (ctypes.c_int * 6)()
+    Wv = (ctypes.c_int * len(BP_ROWS))()   # RAFT_SS_BP_NVARIANTS entries each
+    Jv = (ctypes.c_int * len(BP_ROWS))()
     assert lib.raft_ss_plan_host_bp_staging(n, R, A, (ctypes.c_double * 4)(*ic), (ctypes.c_double * 4)(*sc),
-                                            W6, J6) == 0
-    W, J = W6[variant], J6[variant]
+                                            Wv, Jv) == 0
+    W, J = Wv[variant], Jv[variant]
     if J == 0:
         return None
     assert 1 <= J <= 64 and J * W <= BP_STAGE[variant]
@@ -163,7 +163,7 @@ def _bp_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC, variant=0):
     return lo, hi, W
 
 
-@pytest.mark.parametrize("variant", range(6))
+@pytest.mark.parametrize("variant", range(len(BP_ROWS)))
 @pytest.mark.parametrize("n,R,A", [(64, 64, 45), (65, 70, 37), (512, 512, 720), (2048, 2048, 1500), (100, 300, 50)])
 def test_backprojection_window_in_bounds(lib, n, R, A, variant):
     """The kernel reads wrow[(int)u] and the pair's difference to ray (int)u + 1: both
