@@ -74,6 +74,61 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def reference_case(dev, reps=50):
+    """The reference's own CPU-runnable case: 256x256 Shepp-Logan (10 ellipses), 180 angles
+    in [0, pi), 256 bins over [-1, 1].  Radon, backprojection and FBP on the GPU (device-
+    resident, HIP events on the launching stream, mean of `reps` after a warm-up); the
+    reference C's time for the same case is in cpu_baseline."""
+    import numpy as np
+    import torch
+    import pyraft_amd as pr
+    from pyraft_amd.device import Plan
+    n, R, A = 256, 256, 180
+    f = np.asarray(pr.shepplogan((n, n)), dtype=np.float32)
+    p = Plan(n, R, A)
+    x = torch.from_numpy(f).to(dev)
+    s = p.radon(x)
+
+    def ev(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return round(a.elapsed_time(b) / reps * 1e3, 2)
+
+    return {"workload": "256x256 Shepp-Logan, 180 angles in [0,pi), 256 bins over [-1,1] (the reference's "
+                        "CPU-runnable case); device-resident, mean of %d" % reps,
+            "radon_us": ev(lambda: p.radon(x)), "backprojection_us": ev(lambda: p.backprojection(s)),
+            "fbp_us": ev(lambda: p.fbp(s))}
+
+
+def _cpu_reference_case(O, ref, nthreads):
+    """The reference C (or the oracle) on the 256^2 Shepp-Logan case: median of 3."""
+    import numpy as np
+    import pyraft_amd as pr
+    n, R, A = 256, 256, 180
+    img = np.asarray(pr.shepplogan((n, n)), dtype=np.float32).astype(np.float64)
+    radon = (lambda: O.ref_radon(img, R, A, nthreads=nthreads)) if ref else (
+        lambda: O.radon(img, R, A, nthreads=nthreads))
+    bp = (lambda y: O.ref_backprojection(y, n, nthreads=nthreads)) if ref else (
+        lambda y: O.backprojection(y, n, nthreads=nthreads))
+    sino = radon()
+    tr, tb = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        sino = radon()
+        t1 = time.perf_counter()
+        bp(sino)
+        tb.append(time.perf_counter() - t1)
+        tr.append(t1 - t0)
+    return {"workload": "256x256 Shepp-Logan, 180 angles, 256 bins", "radon_us": round(statistics.median(tr) * 1e6, 1),
+            "backprojection_us": round(statistics.median(tb) * 1e6, 1)}
+
+
 def cpu_baseline(n, angles):
     """Reference C (compiled from the reference sources into oracle/_ref) on a
     bounded sample of the same workload: the full 2048^2 image with a uniformly
@@ -114,6 +169,7 @@ def cpu_baseline(n, angles):
         "radon_gups": round(upd / tr / 1e9, 4),
         "backprojection_gups": round(upd / tb / 1e9, 4),
         "cpu_model": cpu_model(),
+        "reference_case": _cpu_reference_case(O, kind == "reference", nthreads),
     }
 
 
@@ -481,6 +537,9 @@ def main():
 
     if not args.no_batch:
         result["batched"] = batched_leg(dev, args.batch_slices, args.batch_iters)
+
+    print("bench: reference case (256^2 Shepp-Logan, 180 angles)", file=sys.stderr, flush=True)
+    result["reference_case"] = reference_case(dev)
 
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
