@@ -143,6 +143,12 @@ static int bp_variants(raft_ss_plan* p) {
     return set_error(RAFT_SS_EINVAL, "too many detector bins per pixel (window %d)",
                      p->bp_window_v[SS_BP_DEFAULT_VARIANT]);
   p->bp_variant = p->bp_angles_v[SS_BP_DEFAULT_VARIANT] > 0 ? SS_BP_DEFAULT_VARIANT : first;
+  // an image with fewer than 4 default tiles per CU starts from the 2-rows-per-thread
+  // tiles (twice the blocks); the autotuner may still pick another variant
+  const int kSmallVariant = 1;  // 2 pixel rows per thread, 2048 staged pairs
+  if (p->bp_variant == SS_BP_DEFAULT_VARIANT && kSSBpRows[kSmallVariant] == 2 && p->bp_angles_v[kSmallVariant] > 0 &&
+      ss_bp_tiles(p->g.n, SS_BP_DEFAULT_VARIANT) < 1024)
+    p->bp_variant = kSmallVariant;
   return 0;
 }
 
