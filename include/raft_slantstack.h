@@ -220,7 +220,9 @@ int raft_ss_tr_em(raft_ss_plan *plan, const float *d_raw, const float *d_flat, f
  * ll_host (optional) receives niter values of sum(s log p - p) over rays with p > 0.
  * The subsets' backprojection variant is the plan's when set by
  * raft_ss_plan_set_bp_variant; otherwise the first call per subset size times the
- * variants on one subset's angle range (not under stream capture) and keeps the fastest. */
+ * variants on one subset's angle range (not under stream capture) and keeps the fastest.
+ * Variants differ by ~1e-7 relative in FP32 rounding, so a timed choice can change the
+ * low bits between processes; setting the variant makes them reproducible. */
 int raft_ss_em(raft_ss_plan *plan, const float *d_sino, float *d_x, int niter, int nsubsets, int nslices,
                double *ll_host, void *stream);
 
