@@ -222,7 +222,11 @@ int raft_ss_tr_em(raft_ss_plan *plan, const float *d_raw, const float *d_flat, f
  * raft_ss_plan_set_bp_variant; otherwise the first call per subset size times the
  * variants on one subset's angle range (not under stream capture) and keeps the fastest.
  * Variants differ by ~1e-7 relative in FP32 rounding, so a timed choice can change the
- * low bits between processes; setting the variant makes them reproducible. */
+ * low bits between processes; setting the variant makes them reproducible.
+ * Stream capture: with ll_host NULL the call enqueues kernels and its stream-ordered
+ * workspace allocation and release only (no memset, copy or wait), so it can be
+ * captured into a graph; with ll_host set it copies the log-likelihood back and waits
+ * for the stream, and returns RAFT_SS_EINVAL while the stream is being captured. */
 int raft_ss_em(raft_ss_plan *plan, const float *d_sino, float *d_x, int niter, int nsubsets, int nslices,
                double *ll_host, void *stream);
 

@@ -268,6 +268,35 @@ void oracle_backprojection_slantstack(raft_image sinogram, raft_image backprojec
   fan_out(bp_work, &ctx, n * n, nthreads);
 }
 
+/* The same pixels for a list of image rows only: out[i][col] = pixel (rows[i], col) of
+   oracle_backprojection_slantstack on an n x n image with the corners of `geometry`
+   (whose data is not read).  Lets a full-size backprojection be checked on a sample
+   of its rows. */
+typedef struct {
+  raft_image sino, geom;
+  int j0, j1;
+  const int *rows;
+  double *out;
+} bp_rows_ctx;
+
+static void bp_rows_work(void *vctx, long b, long e) {
+  bp_rows_ctx *ctx = (bp_rows_ctx *)vctx;
+  const int n = ctx->geom.data.lines;
+  for (long w = b; w < e; ++w) {
+    const long i = w / n;
+    const int col = (int)(w % n);
+    ctx->out[w] = angle_sum(&ctx->sino, &ctx->geom, ctx->rows[i], col, ctx->j0, ctx->j1);
+  }
+}
+
+void oracle_backprojection_rows(raft_image sinogram, raft_image geometry, int j0, int j1, const int *rows,
+                                int nrows, double *out, int nthreads) {
+  if (j0 < 0) j0 = 0;
+  if (j1 > sinogram.data.columns) j1 = sinogram.data.columns;
+  bp_rows_ctx ctx = {sinogram, geometry, j0, j1, rows, out};
+  fan_out(bp_rows_work, &ctx, (long)nrows * geometry.data.lines, nthreads);
+}
+
 /* ------------------------------------------------------------------ */
 /* exact transpose of the slant-stack Radon (scatter form)              */
 /* ------------------------------------------------------------------ */

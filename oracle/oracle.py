@@ -9,7 +9,8 @@ Contents (each restates the cited reference routine):
 * ``radon`` / ``backprojection`` / ``radon_transpose`` -- ctypes calls into
   ``liboracle.so`` (oracle.c), restating raft/raft_radon_slantstack.c:122-214 and
   raft/raft_backprojection_slantstack.c:121-181, plus the exact adjoint of the
-  forward (no reference counterpart).
+  forward (no reference counterpart).  ``backprojection_rows`` evaluates the same
+  backprojection on a list of image rows only (full-size checks on a row sample).
 * ``lowpassino_fbp`` / ``lowpassino_reg_fbp`` -- numpy restatements of
   pyraft/filters/filters.py:36-68 and 73-111.
 * ``filter1d_ramp`` -- numpy restatement of raft/raft_filter.cpp:118-213 (the reference
@@ -117,6 +118,9 @@ def lib():
         _lib.oracle_radon_sample_map.restype = None
         _lib.oracle_bp_sample_map.argtypes = [RaftImage, RaftImage, ctypes.c_void_p]
         _lib.oracle_bp_sample_map.restype = None
+        _lib.oracle_backprojection_rows.argtypes = [RaftImage, RaftImage, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+        _lib.oracle_backprojection_rows.restype = None
     return _lib
 
 
@@ -165,7 +169,8 @@ SINO_TL, SINO_BR = (0.0, 1.0), (np.pi, -1.0)
 
 
 def _threads(nthreads):
-    return int(nthreads) if nthreads else (os.cpu_count() or 1)
+    """Default: one thread per core, at most 16 (a GPU host's share per GPU)."""
+    return int(nthreads) if nthreads else min(16, os.cpu_count() or 1)
 
 
 # ---------------------------------------------------------------- projectors
@@ -190,6 +195,24 @@ def backprojection(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_
     lib().oracle_backprojection_slantstack(_image(sino, sino_tl, sino_br), _image(img, img_tl, img_br),
                                            j0, j1, _threads(nthreads))
     return img
+
+
+def backprojection_rows(sino, n, rows, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
+                       j0=0, j1=None, nthreads=None):
+    """Rows `rows` of ``backprojection(sino, n, ...)``: a (len(rows), n) FP64 array,
+    each pixel computed exactly as in the full image."""
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int32).reshape(-1)
+    if rows.size and (rows.min() < 0 or rows.max() >= n):
+        raise ValueError("rows outside [0, %d)" % n)
+    geom = np.zeros((n, 1))  # corners and the line count n; the data are not read
+    geom_img = RaftImage(RaftMatrix(geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, 1, 1, 1),
+                         float(img_tl[0]), float(img_tl[1]), float(img_br[0]), float(img_br[1]))
+    out = np.zeros((rows.size, n))
+    j1 = sino.shape[1] if j1 is None else j1
+    lib().oracle_backprojection_rows(_image(sino, sino_tl, sino_br), geom_img, j0, j1, rows.ctypes.data,
+                                     int(rows.size), out.ctypes.data, _threads(nthreads))
+    return out
 
 
 def radon_transpose(sino, n, img_tl=IMG_TL, img_br=IMG_BR, sino_tl=SINO_TL, sino_br=SINO_BR,
@@ -466,13 +489,10 @@ def lowpassino_reg_fbp(sino, L, tl=SINO_TL, br=SINO_BR):
     return np.fft.ifft(np.fft.fft(sino, axis=0) * G, axis=0).real
 
 
-def filter1d_ramp(sino, cutoff, tl=SINO_TL, br=SINO_BR):
-    """raft/raft_filter.cpp:118-213 restated with numpy's FFT (FFTW is absent here): per
-    column, zero-pad to Z = 2R - 1, real FFT, multiply bins 0..R/2 by ramp/Z (bins
-    R/2+1..R-1 untouched, as the reference's loop bound osize = R/2 + 1 leaves them),
-    unnormalised inverse (FFTW's c2r: numpy's irfft times Z), keep the first R samples."""
-    sino = np.asarray(sino, dtype=np.float64)
-    R = sino.shape[0]
+def filter1d_ramp_vector(R, cutoff, tl=SINO_TL, br=SINO_BR):
+    """The ramp of raft/raft_filter.cpp:184-206: F = R/2 + 1 values, delta * i for
+    i < int(F * min(cutoff, 1)) and 0 after, delta = pi / (F - 1) / |dt| with
+    dt = (br_y - tl_y) / (R - 1) (raft/raft_image.h:60-61)."""
     F = R // 2 + 1
     cutoff = cutoff if cutoff <= 1.0 else 1.0
     delta = 3.14159265358979323846264338327 / (F - 1)
@@ -481,6 +501,18 @@ def filter1d_ramp(sino, cutoff, tl=SINO_TL, br=SINO_BR):
     filt = np.zeros(F)
     if length > 0:
         filt[:min(length, F)] = delta * np.arange(min(length, F))
+    return filt
+
+
+def filter1d_ramp(sino, cutoff, tl=SINO_TL, br=SINO_BR):
+    """raft/raft_filter.cpp:118-213 restated with numpy's FFT (FFTW is absent here): per
+    column, zero-pad to Z = 2R - 1, real FFT, multiply bins 0..R/2 by ramp/Z (bins
+    R/2+1..R-1 untouched, as the reference's loop bound osize = R/2 + 1 leaves them),
+    unnormalised inverse (FFTW's c2r: numpy's irfft times Z), keep the first R samples."""
+    sino = np.asarray(sino, dtype=np.float64)
+    R = sino.shape[0]
+    F = R // 2 + 1
+    filt = filter1d_ramp_vector(R, cutoff, tl, br)
     Z = 2 * R - 1
     xz = np.zeros((Z, sino.shape[1]))
     xz[:R] = sino

@@ -626,6 +626,12 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
   if (nsubsets < 1 || nsubsets > A) return set_error(RAFT_SS_EINVAL, "nsubsets must be in [1, %d]", A);
   hipStream_t stream = as_stream(stream_);
+  if (ll_host) {  // the host copy needs a wait, which a stream capture cannot hold
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return set_error(RAFT_SS_EINVAL, "em: ll_host must be NULL while the stream is being captured");
+  }
   const SSDevPlan dp = p->dev();
   const size_t sino_sz = (size_t)nslices * R * A;
   const size_t img_sz = (size_t)nslices * n * n;
@@ -668,7 +674,6 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
     HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
                                      stream));
   }
-  if (niter > 0) HIP_TRY(hipMemsetAsync(d_ll, 0, sizeof(double) * niter, stream));
   for (int it = 0; it < niter; ++it) {
     for (int b = 0; b < nsubsets; ++b) {
       const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);

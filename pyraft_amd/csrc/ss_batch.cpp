@@ -153,9 +153,6 @@ extern "C" int raft_ss_plan_tune(raft_ss_plan* p, int reps, int flags, double* t
 int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, int* variant) {
   *variant = p->bp_variant;
   if (p->bp_variant_fixed || j1 <= j0) return 0;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  HIP_TRY(hipStreamIsCapturing(stream, &cs));
-  if (cs != hipStreamCaptureStatusNone) return 0;
   const std::string key = plan_key(p) + "|range" + std::to_string(j1 - j0);
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -165,6 +162,10 @@ int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, i
       return 0;
     }
   }
+  // no timing inside a stream capture: the plan's variant until an eager call tunes
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) return 0;
   const int n = p->g.n, R = p->g.nrays, A = p->g.nangles;
   DevBuf ws;
   HIP_TRY(ws.alloc(sizeof(float) * ((size_t)R * A + (size_t)n * n), stream));

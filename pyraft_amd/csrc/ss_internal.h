@@ -34,7 +34,8 @@ struct raft_ss_plan;
 // Backprojection variant for launches over angles [j0, j1) (OSEM subsets): the fastest
 // variant of a timed plain launch over that range, cached per process by device,
 // geometry and range length.  The plan's own variant when it was set explicitly, when
-// the range is empty, or while `stream` is being captured (no timing then).
+// the range is empty, or while `stream` is being captured and no earlier call has
+// timed this range (no timing under capture).
 int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, int* variant);
 
 struct raft_ss_plan {

@@ -1761,14 +1761,14 @@ hipError_t ss_launch_tr_update(float* x, const float* b, const float* w, size_t 
 // small helpers
 // =====================================================================================
 
-// dst += sum(partials[0..n)) in a fixed order (one block)
+// dst = sum(partials[0..n)) in a fixed order (one block); a store, so the caller clears nothing
 __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int n,
                                                       double* __restrict__ dst) {
   __shared__ double red[4];
   double v = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) v += partials[i];
   const double tot = ss_block_sum_f64(v, red);
-  if (threadIdx.x == 0) *dst += tot;
+  if (threadIdx.x == 0) *dst = tot;
 }
 
 hipError_t ss_launch_sum_partials(const double* partials, int n, double* dst, hipStream_t stream) {

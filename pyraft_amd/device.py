@@ -43,6 +43,22 @@ def _check_tensor(t, shape_tail, name):
     return t.contiguous()
 
 
+def _check_out(t, shape, device, name):
+    """An output tensor the kernels write through its data pointer: float32, on the
+    input's device, contiguous and of exactly `shape` (a copy would drop the results)."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch.Tensor" % name)
+    if t.dtype != torch.float32:
+        raise TypeError("%s must be float32 (got %s)" % (name, t.dtype))
+    if t.device != device:
+        raise TypeError("%s is on %s, the input on %s" % (name, t.device, device))
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError("%s has shape %s, expected %s" % (name, tuple(t.shape), tuple(shape)))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    return t
+
+
 def _nslices(t, tail):
     n = 1
     for d in t.shape[:-tail]:
@@ -169,6 +185,8 @@ class Plan:
         shape = lead + ((self.nangles, self.nrays) if angle_major else (self.nrays, self.nangles))
         if out is None:
             out = torch.zeros(shape, dtype=torch.float32, device=x.device)
+        else:
+            _check_out(out, shape, x.device, "out")
         j0, j1 = self._range(angles)
         with torch.cuda.device(self.device):
             check(libraft.raft_ss_radon_range(self._handle, x.data_ptr(), out.data_ptr(), int(angle_major), j0, j1,

@@ -27,16 +27,15 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def brute_filter1d_ramp(x, cutoff, dt):
+def brute_filter1d_ramp(x, ramp):
     """raft/raft_filter.cpp:70-92 for one column with explicit sums: Z = 2R - 1 point DFT
-    of the zero-padded column, bins k <= R/2 (and their mirror Z - k) times ramp(k)/Z,
-    the rest untouched, unnormalised inverse, first R samples."""
+    of the zero-padded column, bins k <= R/2 (and their mirror Z - k) times ramp[k]/Z,
+    the rest untouched, unnormalised inverse, first R samples.  `ramp` is given by the
+    caller (hand-written values below), so the filter's definition is not re-derived."""
     R = len(x)
     Z = 2 * R - 1
     F = R // 2 + 1
-    c = min(cutoff, 1.0) if cutoff == cutoff else 1.0
-    delta = math.pi / (F - 1) / abs(dt)
-    ramp = [delta * i if i < int(F * c) else 0.0 for i in range(F)]
+    assert len(ramp) == F
     n = np.arange(Z)
     xz = np.zeros(Z)
     xz[:R] = x
@@ -50,16 +49,63 @@ def brute_filter1d_ramp(x, cutoff, dt):
     return y.real
 
 
-@pytest.mark.parametrize("R,cutoff", [(7, 1.0), (8, 1.0), (12, 0.5), (9, 2.0), (10, 0.0)])
+PI = math.pi
+# raft/raft_filter.cpp:199-206 by hand for t in [-1, 1] (dt = 2 / (R - 1)):
+#   F = R/2 + 1 values, delta = pi / (F - 1) / dt, ramp[i] = delta * i for i < int(F * cutoff)
+HAND_RAMPS = {
+    (7, 1.0): PI * np.array([0.0, 1.0, 2.0, 3.0]),                       # F = 4, dt = 1/3: delta = pi
+    (8, 1.0): 7.0 * PI / 8.0 * np.array([0.0, 1.0, 2.0, 3.0, 4.0]),     # F = 5, dt = 2/7: delta = 7 pi / 8
+    (8, 2.0): 7.0 * PI / 8.0 * np.array([0.0, 1.0, 2.0, 3.0, 4.0]),     # cutoff clamped to 1
+    (12, 0.5): 11.0 * PI / 12.0 * np.array([0.0, 1.0, 2.0, 0, 0, 0, 0]),  # F = 7: int(3.5) = 3 values
+    (9, 0.6): PI * np.array([0.0, 1.0, 2.0, 0.0, 0.0]),                # F = 5, dt = 1/4: 5 * 0.6 == 3.0
+    (10, 0.0): np.zeros(6),                                             # cutoff 0: no bin kept
+}
+
+
+def test_filter1d_ramp_vector_by_hand():
+    tl, br = (0.0, 1.0), (PI, -1.0)
+    for (R, cutoff), want in HAND_RAMPS.items():
+        got = O.filter1d_ramp_vector(R, cutoff, tl, br)
+        np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-14, err_msg=str((R, cutoff)))
+    # a detector span of [-2, 2]: dt = 4 / (R - 1), the slope halves
+    np.testing.assert_allclose(O.filter1d_ramp_vector(7, 1.0, (0.0, 2.0), (PI, -2.0)),
+                               PI / 2.0 * np.array([0.0, 1.0, 2.0, 3.0]), rtol=1e-14)
+
+
+@pytest.mark.parametrize("R,cutoff", sorted(HAND_RAMPS))
 def test_filter1d_ramp_restatement_matches_explicit_dft(R, cutoff):
     rng = np.random.default_rng(R)
     sino = rng.random((R, 3))
-    tl, br = (0.0, 1.0), (math.pi, -1.0)
+    tl, br = (0.0, 1.0), (PI, -1.0)
     got = O.filter1d_ramp(sino, cutoff, tl, br)
-    dt = (br[1] - tl[1]) / (R - 1)
     for a in range(3):
-        ref = brute_filter1d_ramp(sino[:, a], cutoff, dt)
+        ref = brute_filter1d_ramp(sino[:, a], HAND_RAMPS[(R, cutoff)])
         assert np.abs(got[:, a] - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), a
+
+
+@pytest.mark.parametrize("R", [7, 8, 10, 13])
+def test_filter1d_ramp_cutoff_zero_closed_form(R):
+    """cutoff = 0: bins 0..R/2 of the 2R - 1 point spectrum are zeroed, bins R/2+1..R-1
+    (and their mirrors) pass unscaled, and the inverse is unnormalised (factor Z).  So
+    the output is Z times the first R samples of the zero-padded column with its
+    lowest R/2 + 1 frequencies (and mirrors) projected out.  Built from the real
+    cosine/sine basis of the Z-point grid, not from an FFT."""
+    rng = np.random.default_rng(100 + R)
+    x = rng.random(R)
+    Z = 2 * R - 1
+    xz = np.zeros(Z)
+    xz[:R] = x
+    n = np.arange(Z)
+    low = np.full(Z, xz.mean())                     # bin 0
+    for k in range(1, R // 2 + 1):                  # bins 1..R/2 with their mirrors
+        c, s = np.cos(2 * PI * k * n / Z), np.sin(2 * PI * k * n / Z)
+        low += (2.0 / Z) * (c * np.dot(xz, c) + s * np.dot(xz, s))
+    want = Z * (xz - low)[:R]
+    got = O.filter1d_ramp(x[:, None], 0.0, (0.0, 1.0), (PI, -1.0))[:, 0]
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    # the reference's misreadings this separates: filtering every bin gives 0, and a
+    # normalised inverse gives want / Z
+    assert np.abs(want).max() > 0.1
 
 
 @pytest.fixture(scope="module")

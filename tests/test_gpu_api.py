@@ -152,3 +152,61 @@ def test_large_host_arrays_match_device_path(pr):
     for _ in range(2):
         b = pr.backprojection(y, plan)
         assert np.array_equal(np.asarray(b), ref_bp)
+
+
+def test_radon_out_is_validated(pr):
+    """Plan.radon(out=...) hands out's data pointer to the kernel: a tensor of another
+    shape, dtype or layout is refused instead of written past its end."""
+    import torch
+    from pyraft_amd.device import Plan
+    print("radon out validation", flush=True)
+    n, R, A = 64, 64, 30
+    plan = Plan(n, R, A)
+    x = torch.rand((n, n), device="cuda")
+    ok = torch.empty((A, R), device="cuda")
+    plan.radon(x, angle_major=True, out=ok)
+    for bad, exc in ((torch.empty((A - 1, R), device="cuda"), ValueError),
+                     (torch.empty((R, A), device="cuda"), ValueError),
+                     (torch.empty((A, R), device="cuda", dtype=torch.float64), TypeError),
+                     (torch.empty((R, A), device="cuda").t(), ValueError),
+                     (torch.empty((A, R)), TypeError)):
+        with pytest.raises(exc):
+            plan.radon(x, angle_major=True, out=bad)
+    ref = plan.radon(x, angle_major=True)
+    assert torch.equal(ok, ref)
+
+
+def test_em_under_stream_capture(pr):
+    """raft_ss_em without ll_host enqueues kernels and stream-ordered workspace only, so
+    it can be captured into a HIP graph: every replay (with eager work and host waits
+    between replays) equals the eager call bit for bit.  With ll_host it is refused
+    while the stream is being captured."""
+    import torch
+    from pyraft_amd.device import Plan
+    from pyraft_amd.raftypes import RaftError
+    print("em under stream capture", flush=True)
+    n, A, S, niter = 96, 60, 4, 3
+    plan = Plan(n, n, A)
+    f = torch.rand((n, n), device="cuda") + 0.5
+    s = plan.radon(f) * 50.0
+    eager = plan.em(s, niter, S)          # tunes the subsets' variant (cached for the capture)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        with torch.cuda.graph(g, stream=stream):
+            xg = plan.em(s, niter, S)
+    torch.cuda.current_stream().wait_stream(stream)
+    for rep in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(xg, eager), rep
+        plan.radon(torch.rand((n, n), device="cuda"))   # other work between replays
+        torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        g2 = torch.cuda.CUDAGraph()
+        with pytest.raises(RaftError):
+            with torch.cuda.graph(g2, stream=stream):
+                plan.em(s, niter, S, return_ll=True)
+    torch.cuda.synchronize()

@@ -2,9 +2,11 @@
 
 2048 x 2048, 1500 angles, 2048 bins (the benchmark workload): Radon columns and
 backprojections over contiguous angle blocks (including theta = 0, the knife-edge
-block around pi/2 and the last angles) against the FP64 oracle, the full-size
-adjoint identity, and the FBP row filter.  1024 x 1024, 720 angles, 16 subsets:
-two OSEM iterations against the oracle's recurrence.
+block around pi/2 and the last angles) against the FP64 oracle; the full-range
+backprojection (the launch the benchmark times) with every kernel variant on a
+sample of image rows; the full-size adjoint identity, and the FBP row filter.
+1024 x 1024, 720 angles, 16 subsets: ten OSEM iterations against the oracle's
+recurrence, the log-likelihood compared at every iteration.
 """
 import math
 
@@ -62,6 +64,43 @@ def test_backprojection_2048_blocks(big):
         assert e <= REL_L2
 
 
+def test_backprojection_2048_full_range_every_variant(big):
+    """All 1500 angles in one launch, as the benchmark runs it, with each backprojection
+    variant (rows per thread, staged pairs) that fits this geometry, on 26 image rows
+    (the edges, the centre and a stride of 128) against the oracle
+    (raft/raft_backprojection_slantstack.c:43-181)."""
+    torch, plan, f, s = big
+    from pyraft_amd.device import Plan, bp_variant_shape
+    from pyraft_amd.raftypes import BP_NVARIANTS, RaftError
+    n = 2048
+    rows = sorted(set([0, 1, 2, 1023, 1024, 2045, 2046, 2047] + list(range(64, n, 128))))
+    sg = s.cpu().numpy().astype(np.float64)
+    ref = O.backprojection_rows(sg, n, rows)
+    sT = s.t().contiguous()
+    p = Plan(n, n, 1500)
+    info = p.info()
+    ran = 0
+    for v in range(BP_NVARIANTS):
+        try:
+            p.set_bp_variant(v)
+        except RaftError as e:
+            print("variant", v, "not runnable here:", e, flush=True)
+            continue
+        got = p.backprojection(sT, angle_major=True).cpu().numpy()
+        e = rel_l2(got[rows], ref)
+        worst = max(rel_l2(got[r], ref[i]) for i, r in enumerate(rows) if np.linalg.norm(ref[i]) > 0)
+        print("full-range backprojection variant", v, bp_variant_shape(v), "rel-L2 %.2e, worst row %.2e"
+              % (e, worst), flush=True)
+        assert e <= REL_L2 and worst <= 4 * REL_L2
+        ran += 1
+    assert ran >= 2, info
+    # and the pyraft (rays, angles) layout entry gives the same image as the angle-major one
+    p.set_bp_variant(3 if ran > 3 else v)
+    a = p.backprojection(s).cpu().numpy()
+    b = p.backprojection(sT, angle_major=True).cpu().numpy()
+    assert np.array_equal(a, b)
+
+
 def test_adjoint_2048(big):
     torch, plan, f, s = big
     y = torch.rand((2048, 1500), device="cuda")
@@ -91,10 +130,17 @@ def test_osem_1024_16_subsets():
     p = p * (1e3 / p.mean())
     s = torch.from_numpy(np.random.default_rng(30).poisson(p.cpu().numpy().astype(np.float64))
                          .astype(np.float32)).cuda()
-    x, ll = plan.em(s, 2, S, return_ll=True)
-    ref, llref = O.osem(s.cpu().numpy().astype(np.float64), n, 2, S)
+    niter = 10
+    x, ll = plan.em(s, niter, S, return_ll=True)
+    x2 = plan.em(s, 2, S)
+    s64 = s.cpu().numpy().astype(np.float64)
+    ref2, llref2 = O.osem(s64, n, 2, S)
+    ref, llref = O.osem(s64, n, niter - 2, S, x0=ref2)
+    llref = llref2 + llref
+    e2 = rel_l2(x2.cpu().numpy(), ref2)
     e = rel_l2(x.cpu().numpy(), ref)
-    print("osem 1024/720/16 rel-L2", e, ll, llref)
-    assert e <= REL_L2
-    for a, b in zip(ll, llref):
+    print("osem 1024/720/16 rel-L2 after 2 iterations %.2e, after %d %.2e" % (e2, niter, e), flush=True)
+    assert e2 <= REL_L2 and e <= REL_L2
+    for k, (a, b) in enumerate(zip(ll, llref)):
+        print("  iteration %d LL %.6f oracle %.6f" % (k, a, b))
         assert abs(a - b) <= 1e-6 * abs(b)

@@ -199,8 +199,74 @@ def test_em_fixed_point_and_positivity():
     assert np.array_equal(x0, np.ones((n, n))) and ll0 == []
 
 
+@pytest.mark.parametrize("S", [3, 5, 16])
+def test_osem_subsets_fixed_point(S):
+    """OSEM with S contiguous subsets (pyraft/iterative/em.py:282-318 with the
+    raft/raft_em_gpu.cu:63-69 update): with s = R x*, every subset update maps x* to
+    itself only if subset b divides by its own weight BP_b(1) and projects exactly its
+    own angle block.  Dividing by the full-range BP(1), or projecting the wrong block,
+    moves x* by O(1)."""
+    n = R = 24
+    A = 48
+    rng = np.random.default_rng(90 + S)
+    xs = 0.5 + rng.random((n, n))
+    s = O.radon(xs, R, A)
+    x, ll = O.osem(s, n, 2, nsubsets=S, x0=xs)
+    np.testing.assert_allclose(x, xs, rtol=1e-12)
+    # the log-likelihood terms of the subsets add up to the full one: sum s log s - s
+    m = s > 0
+    full = float(np.sum(s[m] * np.log(s[m])) - np.sum(s[m]))
+    assert len(ll) == 2 and all(abs(v - full) <= 1e-9 * abs(full) for v in ll), (ll, full)
+    # the subset blocks tile the angles, and their weights add up to BP(1)
+    blocks = O.subset_bounds(A, S)
+    assert blocks[0][0] == 0 and blocks[-1][1] == A
+    assert all(b[1] == c[0] for b, c in zip(blocks, blocks[1:]))
+    ones = np.ones((R, A))
+    wsum = sum(O.backprojection(ones, n, j0=b0, j1=b1) for b0, b1 in blocks)
+    np.testing.assert_allclose(wsum, O.backprojection(ones, n), rtol=1e-12, atol=1e-12)
+    # one iteration from a flat start moves toward x* and differs from plain EM
+    x1, _ = O.osem(s, n, 1, nsubsets=S)
+    e1, _ = O.em(s, n, 1)
+    assert np.linalg.norm(x1 - xs) < np.linalg.norm(e1 - xs)
+
+
+def test_osem_planted_weight_mistake_is_caught():
+    """The check above fails for a plausible mistake planted in a copy of the
+    recurrence: dividing every subset by the full-range weight."""
+    n = R = 16
+    A = 24
+    S = 4
+    rng = np.random.default_rng(7)
+    xs = 0.5 + rng.random((n, n))
+    s = O.radon(xs, R, A)
+    W = O.backprojection(np.ones((R, A)), n)
+    x = xs.copy()
+    for b0, b1 in O.subset_bounds(A, S):
+        p = O.radon(x, R, A, j0=b0, j1=b1)[:, b0:b1]
+        rho = np.zeros((R, A))
+        rho[:, b0:b1] = np.where(p > 0, s[:, b0:b1] / np.where(p > 0, p, 1.0), 0.0)
+        x = np.where(W > 0, x * O.backprojection(rho, n, j0=b0, j1=b1) / np.where(W > 0, W, 1.0), x)
+    assert np.linalg.norm(x - xs) > 0.5 * np.linalg.norm(xs)
+
+
 def test_loglik_formula():
     s = np.array([[1.0, 2.0], [0.0, 3.0]])
     p = np.array([[2.0, 0.0], [1.0, 4.0]])
     expect = 1.0 * math.log(2.0) - 2.0 + 0.0 - 1.0 + 3.0 * math.log(4.0) - 4.0
     assert abs(O.loglik(s, p) - expect) < 1e-12
+
+
+def test_backprojection_rows_equals_full_image():
+    """oracle.backprojection_rows computes each listed row exactly as the full image does,
+    including an angle sub-range and non-default corners."""
+    rng = np.random.default_rng(12)
+    n, R, A = 37, 41, 29
+    sino = rng.random((R, A))
+    geo = dict(img_tl=(-0.8, 1.1), img_br=(1.2, -0.9), sino_tl=(0.1, 1.2), sino_br=(3.0, -1.1))
+    rows = [0, 1, 17, 35, 36, 17]
+    full = O.backprojection(sino, n, **geo)
+    assert np.array_equal(O.backprojection_rows(sino, n, rows, **geo), full[rows])
+    part = O.backprojection(sino, n, j0=5, j1=20, **geo)
+    assert np.array_equal(O.backprojection_rows(sino, n, rows, j0=5, j1=20, **geo), part[rows])
+    with pytest.raises(ValueError):
+        O.backprojection_rows(sino, n, [n], **geo)
