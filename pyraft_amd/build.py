@@ -20,7 +20,7 @@ LIBDIR = os.path.join(HERE, "lib")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("PYRAFT_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["ss_kernels.hip", "dda_kernels.hip"]
+HIP_SOURCES = ["ss_kernels.hip", "dda_kernels.hip", "ss_noalloc.hip"]
 CXX_SOURCES = ["ss_api.cpp", "ss_batch.cpp", "dda_api.cpp", "filter_api.cpp", "io_api.cpp"]
 
 HIPCC_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden",

@@ -155,6 +155,8 @@ _decl("raft_ss_plan_host_bounds_mesh", [_i, _d4, _i, _d4, _i, _d4, ctypes.POINTE
                                         ctypes.POINTER(ctypes.c_longlong)], _i)
 _decl("raft_ss_radon", [_vp, _vp, _vp, _i, _vp], _i)
 _decl("raft_ss_radon_range", [_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i)
+# allocation-free Radon (include/raft_noalloc.h): workspace in a static device array
+_decl("raft_ss_radon_noalloc", [_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i)
 _decl("raft_ss_backprojection", [_vp, _vp, _vp, _i, _vp], _i)
 _decl("raft_ss_backprojection_range", [_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i)
 _decl("raft_ss_radon_transpose", [_vp, _vp, _vp, _i, _vp], _i)
@@ -349,7 +351,7 @@ EXPORTED_SYMBOLS = (
     "raft_backprojection_slantstack_gpu", "raft_em_gpu", "raft_image_phantom_fromdesc",
     "raft_radon_fromdesc", "raft_radon_transpose_slantstack", "raft_ss_legacy_status", "raft_ss_last_error", "raft_ss_version", "raft_ss_plan_create",
     "raft_ss_plan_destroy", "raft_ss_plan_info", "raft_ss_plan_bounds", "raft_ss_plan_wave_bounds", "raft_ss_plan_host_bounds", "raft_ss_plan_host_info",
-    "raft_ss_radon", "raft_ss_radon_range", "raft_ss_backprojection", "raft_ss_backprojection_range",
+    "raft_ss_radon", "raft_ss_radon_range", "raft_ss_radon_noalloc", "raft_ss_backprojection", "raft_ss_backprojection_range",
     "raft_ss_radon_transpose", "raft_ss_radon_transpose_range", "raft_ss_fbp_filter", "raft_ss_fbp",
     "raft_ss_em", "raft_ss_phantom", "raft_ss_ramp_kernel", "raft_ss_filter_rows",
     "raft_ss_plan_host_bp_staging", "raft_ss_bp_variant_shape", "raft_ss_plan_set_bp_variant", "raft_ss_plan_tune",
