@@ -26,7 +26,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Radon/backproj GUPS at 2048²×1500 angles; EM iters/s batched vs plain, FP32"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+LDS_PEAK_GBS = 78643.2         # LDS reads: 256 CUs x 128 B per clock x 2.4 GHz
+VALU_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (published; counts packed FP32)
 BYTES_PER_UPDATE = 8.0         # two FP32 interpolation taps per linear-interpolated accumulate
+FLOPS_PER_UPDATE = 6.0         # coordinate FMA (2), lerp (sub + FMA, 3), accumulate (1)
 
 
 def parse():
@@ -45,6 +48,49 @@ def parse():
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run one 64 MiB device-to-device copy first (reference for HBM byte counters)")
     return ap.parse_args()
+
+
+def roofline_of(kernel, updates, seconds, traffic, traffic_src):
+    """Roofline of the dominant kernel against the on-chip unit that binds it.
+
+    Both kernels read their interpolation taps from LDS (8 B, one (tap, difference)
+    record or staged pair, per update) and issue ~6 FP32 operations per update; HBM
+    traffic is a small fraction of the tap bytes because windows are staged once per
+    block.  The Radon is bound by the LDS read port (rays sit gamma = 1..1.41 records
+    apart, so a 16-lane group of 8-byte reads spans more than the 16 bank pairs and
+    takes two passes), the backprojection by VALU issue.  `frac` is the kernel's
+    algorithmic demand on that unit over the unit's peak, <= 1 by construction:
+      lds:  8 B x updates / time  over  LDS_PEAK_GBS
+      valu: 6 flops x updates / time  over  VALU_PEAK_TFLOPS
+    The other fractions are reported beside it: measured HBM bytes (rocprofv3
+    FETCH_SIZE + WRITE_SIZE, copy-calibrated) over the HBM peak, and the uncached-gather
+    model (8 B per update from HBM), which exceeds 1 because the taps come from LDS."""
+    lds_gbs = BYTES_PER_UPDATE * updates / seconds / 1e9
+    tflops = FLOPS_PER_UPDATE * updates / seconds / 1e12
+    if kernel.startswith("k_radon"):
+        main = {"bound": "lds", "achieved": round(lds_gbs, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                "frac": round(lds_gbs / LDS_PEAK_GBS, 4)}
+    else:
+        main = {"bound": "valu", "achieved": round(tflops, 2), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tflops / VALU_PEAK_TFLOPS, 4)}
+    main.update({
+        "kernel": kernel,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
+        "model": "updates = N^2 * A per launch; LDS: 8 B (the tap pair) per update; VALU: 6 FP32 flops per update "
+                 "(coordinate FMA, lerp, accumulate); time = mean event-timed launch of the operator",
+        "lds": {"achieved_gbs": round(lds_gbs, 1), "peak_gbs": LDS_PEAK_GBS, "frac": round(lds_gbs / LDS_PEAK_GBS, 4)},
+        "valu": {"achieved_tflops": round(tflops, 2), "peak_tflops": VALU_PEAK_TFLOPS,
+                 "frac": round(tflops / VALU_PEAK_TFLOPS, 4)},
+        "hbm_measured": None if traffic is None else {
+            "achieved_gbs": round(traffic / seconds / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
+            "frac": round(traffic / seconds / 1e9 / HBM_PEAK_GBS, 4)},
+        "uncached_gather_model": {"achieved_gbs": round(lds_gbs, 1), "peak_gbs": HBM_PEAK_GBS,
+                                  "ratio": round(lds_gbs / HBM_PEAK_GBS, 3),
+                                  "note": "8 B per update as if every tap came from HBM; exceeds 1 because the "
+                                          "taps are served from LDS"},
+    })
+    return main
 
 
 def measured_traffic(kernel):
@@ -74,11 +120,43 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def _event_ms(fn, reps, stream):
+    """Mean milliseconds per call of `reps` back-to-back calls, HIP events on `stream`."""
+    import torch
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def _graph_ms(fn, reps):
+    """The same call captured once into a HIP graph and replayed `reps` times."""
+    import torch
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        fn()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=side):
+            fn()
+        g.replay()
+        torch.cuda.synchronize()
+        ms = _event_ms(g.replay, reps, side)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    return ms
+
+
 def reference_case(dev, reps=50):
     """The reference's own CPU-runnable case: 256x256 Shepp-Logan (10 ellipses), 180 angles
-    in [0, pi), 256 bins over [-1, 1].  Radon, backprojection and FBP on the GPU (device-
-    resident, HIP events on the launching stream, mean of `reps` after a warm-up); the
-    reference C's time for the same case is in cpu_baseline."""
+    in [0, pi), 256 bins over [-1, 1].  Radon, backprojection and FBP on the GPU, device-
+    resident: `*_us` is the mean of `reps` calls enqueued from Python (host launch cost
+    included), `*_graph_us` the mean of `reps` replays of the call captured in a HIP graph
+    (device time).  The reference C's time for the same case is in cpu_baseline."""
     import numpy as np
     import torch
     import pyraft_amd as pr
@@ -88,89 +166,160 @@ def reference_case(dev, reps=50):
     p = Plan(n, R, A)
     x = torch.from_numpy(f).to(dev)
     s = p.radon(x)
-
-    def ev(fn):
+    ops = {"radon": lambda: p.radon(x), "backprojection": lambda: p.backprojection(s), "fbp": lambda: p.fbp(s)}
+    out = {"workload": "256x256 Shepp-Logan, 180 angles in [0,pi), 256 bins over [-1,1] (the reference's "
+                       "CPU-runnable case); device-resident, mean of %d calls" % reps}
+    stream = torch.cuda.current_stream()
+    for name, fn in ops.items():
         fn()
         torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(reps):
-            fn()
-        b.record()
-        b.synchronize()
-        return round(a.elapsed_time(b) / reps * 1e3, 2)
-
-    return {"workload": "256x256 Shepp-Logan, 180 angles in [0,pi), 256 bins over [-1,1] (the reference's "
-                        "CPU-runnable case); device-resident, mean of %d" % reps,
-            "radon_us": ev(lambda: p.radon(x)), "backprojection_us": ev(lambda: p.backprojection(s)),
-            "fbp_us": ev(lambda: p.fbp(s))}
+        out[name + "_us"] = round(_event_ms(fn, reps, stream) * 1e3, 2)
+    for name, fn in ops.items():
+        out[name + "_graph_us"] = round(_graph_ms(fn, reps) * 1e3, 2)
+    return out
 
 
-def _cpu_reference_case(O, ref, nthreads):
-    """The reference C (or the oracle) on the 256^2 Shepp-Logan case: median of 3."""
+def _cpu_ops(O, ref, nthreads):
+    """Radon and backprojection of the reference C (oracle/_ref) or, without it, of the
+    oracle's restatement, on an angle block [j0, j1) of an A-angle grid over [0, pi)
+    (corners of the block: theta_j0 and theta_j0 + (j1 - j0) * pi / A, so the block's
+    angles are the full grid's)."""
     import numpy as np
-    import pyraft_amd as pr
-    n, R, A = 256, 256, 180
-    img = np.asarray(pr.shepplogan((n, n)), dtype=np.float32).astype(np.float64)
-    radon = (lambda: O.ref_radon(img, R, A, nthreads=nthreads)) if ref else (
-        lambda: O.radon(img, R, A, nthreads=nthreads))
-    bp = (lambda y: O.ref_backprojection(y, n, nthreads=nthreads)) if ref else (
-        lambda y: O.backprojection(y, n, nthreads=nthreads))
-    sino = radon()
-    tr, tb = [], []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        sino = radon()
-        t1 = time.perf_counter()
-        bp(sino)
-        tb.append(time.perf_counter() - t1)
-        tr.append(t1 - t0)
-    return {"workload": "256x256 Shepp-Logan, 180 angles, 256 bins", "radon_us": round(statistics.median(tr) * 1e6, 1),
-            "backprojection_us": round(statistics.median(tb) * 1e6, 1)}
+
+    def corners(A, j0, j1):
+        dth = math.pi / A
+        return (j0 * dth, 1.0), (j0 * dth + (j1 - j0) * dth, -1.0)
+
+    if ref:
+        def radon(img, R, A, j0=0, j1=None):
+            j1 = A if j1 is None else j1
+            tl, br = corners(A, j0, j1)
+            return O.ref_radon(img, R, j1 - j0, sino_tl=tl, sino_br=br, nthreads=nthreads)
+
+        def bp(sino_block, n, A, j0=0):
+            tl, br = corners(A, j0, j0 + sino_block.shape[1])
+            return O.ref_backprojection(sino_block, n, sino_tl=tl, sino_br=br, nthreads=nthreads)
+    else:
+        def radon(img, R, A, j0=0, j1=None):
+            j1 = A if j1 is None else j1
+            return O.radon(img, R, A, j0=j0, j1=j1, nthreads=nthreads)[:, j0:j1]
+
+        def bp(sino_block, n, A, j0=0):
+            full = np.zeros((sino_block.shape[0], A))
+            full[:, j0:j0 + sino_block.shape[1]] = sino_block
+            return O.backprojection(full, n, j0=j0, j1=j0 + sino_block.shape[1], nthreads=nthreads)
+    return radon, bp
 
 
 def cpu_baseline(n, angles):
-    """Reference C (compiled from the reference sources into oracle/_ref) on a
-    bounded sample of the same workload: the full 2048^2 image with a uniformly
-    spread subset of the angles, Radon then backprojection, nthreads = cpu_count()
-    as pyraft sets it.  Falls back to the oracle restatement when _ref is absent."""
+    """The reference's CPU path on this host's cores: its own C (slant-stack Radon and
+    backprojection, FP64, compiled from the reference sources into oracle/_ref) or, where
+    that library was not built, the oracle's restatement of it (kind "port").  Threads:
+    min(16, cpu_count()), the GPU's share of the host (pyraft itself passes cpu_count()).
+
+    * 2048^2 x 1500: Radon and backprojection in full, one run each (the timed value);
+    * the reference's 256^2 Shepp-Logan case: Radon, backprojection and FBP (numpy ramp
+      filter, pyraft/filters/filters.py:36-68, then backprojection), median of 3;
+    * one OSEM iteration at 1024^2 / 720 angles / 16 subsets composed from the C operators
+      with the raft_em_gpu recurrence (raft/raft_em_gpu.cu:41-69): per subset a Radon of
+      its 45 angles, the ratio, a backprojection and the update (weights precomputed);
+    * the 256-slice uneven batch from a stratified sample: per image side the slices with
+      the fewest and the most angles, each projected and backprojected on 24 of its
+      angles; the per-update rate of each side extrapolates to the whole pass and to
+      OSEM 8 subsets x 20 iterations (each iteration one Radon and one backprojection
+      pass)."""
     import numpy as np
     from oracle import oracle as O
     from pyraft_amd.phantom import random_ellipses
-    nthreads = os.cpu_count() or 1
-    sub = 24
+    nthreads = min(16, os.cpu_count() or 1)
+    ref = O.ref_available()
+    kind = "reference" if ref else "port"
+    radon, bp = _cpu_ops(O, ref, nthreads)
+    out = {"unit": "GUPS", "cores": nthreads, "kind": kind, "cpu_model": cpu_model()}
+    if not ref:
+        out["note"] = "oracle/_ref (the reference C) is not present here: the oracle's restatement was timed"
+
+    # -- 2048^2 x 1500, in full
     img = O.phantom_fromdesc(random_ellipses(32, 2), n).astype(np.float32).astype(np.float64)
-    kind = "reference" if O.ref_available() else "port"
-    if kind == "reference":
-        radon = lambda: O.ref_radon(img, n, sub, nthreads=nthreads)
-        bp = lambda s: O.ref_backprojection(s, n, nthreads=nthreads)
-    else:
-        radon = lambda: O.radon(img, n, sub, nthreads=nthreads)
-        bp = lambda s: O.backprojection(s, n, nthreads=nthreads)
-    times = []
-    sino = radon()  # warm-up
+    radon(img[:64, :64].copy(), 64, 8)  # load, first-touch
+    t0 = time.perf_counter()
+    sino = radon(img, n, angles)
+    t1 = time.perf_counter()
+    bp(sino, n, angles)
+    t2 = time.perf_counter()
+    upd = float(n) * n * angles
+    tr, tb = t1 - t0, t2 - t1
+    out.update({"value": round(2 * upd / (tr + tb) / 1e9, 4),
+                "sample": "%dx%d image, all %d angles, %d bins: one Radon and one backprojection, FP64, "
+                          "%d threads" % (n, n, angles, n, nthreads),
+                "radon_s": round(tr, 3), "backprojection_s": round(tb, 3),
+                "radon_gups": round(upd / tr / 1e9, 4), "backprojection_gups": round(upd / tb / 1e9, 4)})
+
+    # -- the reference's 256^2 Shepp-Logan case
+    n1, R1, A1 = 256, 256, 180
+    from pyraft_amd.phantom import _shepp_logan_desc
+    f1 = O.phantom_fromdesc(_shepp_logan_desc(), n1).astype(np.float32).astype(np.float64)
+    s1 = radon(f1, R1, A1)
+    times = {"radon": [], "backprojection": [], "fbp": []}
     for _ in range(3):
         t0 = time.perf_counter()
-        sino = radon()
+        s1 = radon(f1, R1, A1)
         t1 = time.perf_counter()
-        bp(sino)
+        bp(s1, n1, A1)
         t2 = time.perf_counter()
-        times.append((t1 - t0, t2 - t1))
-    tr = statistics.median(t[0] for t in times)
-    tb = statistics.median(t[1] for t in times)
-    upd = float(n) * n * sub
-    return {
-        "value": round(2 * upd / (tr + tb) / 1e9, 4),
-        "unit": "GUPS",
-        "cores": nthreads,
-        "kind": kind,
-        "sample": "%dx%d image, %d of the %d angles (uniform over [0,pi)), %d bins; Radon+backprojection, "
-                  "median of 3 after 1 warm-up; FP64 reference C, nthreads=cpu_count()" % (n, n, sub, angles, n),
-        "radon_gups": round(upd / tr / 1e9, 4),
-        "backprojection_gups": round(upd / tb / 1e9, 4),
-        "cpu_model": cpu_model(),
-        "reference_case": _cpu_reference_case(O, kind == "reference", nthreads),
-    }
+        bp(O.lowpassino_fbp(s1), n1, A1)
+        t3 = time.perf_counter()
+        times["radon"].append(t1 - t0)
+        times["backprojection"].append(t2 - t1)
+        times["fbp"].append(t3 - t2)
+    out["reference_case"] = {"workload": "256x256 Shepp-Logan, 180 angles, 256 bins; median of 3",
+                             **{k + "_us": round(statistics.median(v) * 1e6, 1) for k, v in times.items()}}
+
+    # -- one OSEM iteration, 1024^2, 720 angles, 16 subsets
+    ne, Ae, S = 1024, 720, 16
+    fe = O.phantom_fromdesc(random_ellipses(32, 3), ne)
+    se = radon(fe, ne, Ae)
+    se = np.random.default_rng(30).poisson(se * (1e3 / se.mean())).astype(np.float64)
+    blocks = [((b * Ae) // S, ((b + 1) * Ae) // S) for b in range(S)]
+    weights = [bp(np.ones((ne, j1 - j0)), ne, Ae, j0) for j0, j1 in blocks]
+    x = np.ones((ne, ne))
+    t0 = time.perf_counter()
+    for (j0, j1), w in zip(blocks, weights):
+        p = radon(x, ne, Ae, j0, j1)
+        rho = np.where(p > 0, se[:, j0:j1] / np.where(p > 0, p, 1.0), 0.0)
+        x = np.where(w > 0, x * bp(rho, ne, Ae, j0) / np.where(w > 0, w, 1.0), x)
+    t_it = time.perf_counter() - t0
+    out["osem"] = {"workload": "OSEM 1024x1024, 720 angles, 16 subsets: one iteration (weights precomputed)",
+                   "s_per_iteration": round(t_it, 3), "iters_per_s": round(1.0 / t_it, 4),
+                   "gups": round(2.0 * ne * ne * Ae / t_it / 1e9, 4)}
+
+    # -- the 256-slice uneven batch, stratified sample, extrapolated
+    rng = np.random.default_rng(4)
+    sizes = [int(v) for v in rng.choice([256, 512, 768, 1024, 1536, 2048], 256)]
+    angs = [int(v) for v in rng.integers(180, 1801, 256)]
+    sub = 24
+    rate = {}
+    picked = []
+    for size in sorted(set(sizes)):
+        idx = [i for i, v in enumerate(sizes) if v == size]
+        for i in sorted({min(idx, key=lambda k: angs[k]), max(idx, key=lambda k: angs[k])}):
+            picked.append(i)
+            fi = O.phantom_fromdesc(random_ellipses(32, 1000 + i), size)
+            t0 = time.perf_counter()
+            si = radon(fi, size, sub)
+            bp(si, size, sub)
+            dt = time.perf_counter() - t0
+            rate.setdefault(size, []).append(2.0 * size * size * sub / dt)
+    rate = {k: statistics.mean(v) for k, v in rate.items()}
+    t_pass = sum(2.0 * s * s * a / rate[s] for s, a in zip(sizes, angs))
+    out["batched"] = {"workload": "256 uneven slices (seed 4): Radon + backprojection pass and OSEM 8 subsets x 20 "
+                                  "iterations, extrapolated",
+                      "sample": "%d slices (per image side the fewest and the most angles), %d angles each; "
+                                "per-update rate per image side" % (len(picked), sub),
+                      "radon_bp_s_extrapolated": round(t_pass, 2),
+                      "osem_iters_per_s_extrapolated": round(1.0 / t_pass, 5),
+                      "gups_by_n": {str(k): round(v / 1e9, 4) for k, v in sorted(rate.items())}}
+    return out
 
 
 def batched_leg(dev, nslices, niter, nsubsets=8, seed=4):
@@ -416,8 +565,8 @@ def main():
 
     dominant = ("k_backprojection<0, %d, %d>" % (bp_rows, bp_stage)) if t_bp >= t_radon else "k_radon<1>"
     t_dom = max(t_bp, t_radon)
-    achieved = BYTES_PER_UPDATE * upd / t_dom / 1e9
     traffic, traffic_src = measured_traffic(dominant)
+    roofline = roofline_of(dominant, upd, t_dom, traffic, traffic_src)
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -431,8 +580,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: 32 seeded random ellipses (seed 2), rasterised on the GPU",
-        "config": {"workload": "slant-stack Radon + backprojection, %dx%d image, %d angles in [0,pi), %d bins"
-                               % (n, n, A, R), "n": n, "angles": A, "rays": R},
+        "config": {"workload": "slant-stack Radon + backprojection, %dx%d image, %d angles in [0,pi), %d bins; "
+                               "device-resident, angle-major sinogram between the two (the (rays, angles) entry "
+                               "adds one %d x %d transpose)" % (n, n, A, R, R, A), "n": n, "angles": A, "rays": R},
         "radon_gups": round(upd / t_radon / 1e9, 2),
         "backprojection_gups": round(upd / t_bp / 1e9, 2),
         "radon_ms": round(t_radon * 1e3, 4),
@@ -441,11 +591,7 @@ def main():
         "exact_transpose_ms": round(transpose_ms, 4),
         "bp_variant": {"chosen": bp_variant, "rows_per_thread": bp_rows, "stage_pairs": bp_stage,
                        "tuned_us": {str(k): round(v, 1) for k, v in bp_variant_us.items()}},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_source": traffic_src,
-                     "model": "8 B (two FP32 taps) per update x N^2*A updates per launch / mean launch time; "
-                              "traffic = measured HBM bytes per launch (copy-calibrated FETCH_SIZE + WRITE_SIZE)"},
+        "roofline": roofline,
     }
 
     if not args.no_em:

@@ -183,11 +183,13 @@ class Plan:
         x = _check_tensor(image, (self.n, self.n), "image")
         lead = x.shape[:-2]
         shape = lead + ((self.nangles, self.nrays) if angle_major else (self.nrays, self.nangles))
+        j0, j1 = self._range(angles)
         if out is None:
-            out = torch.zeros(shape, dtype=torch.float32, device=x.device)
+            # the kernels write every column of the range; the rest of a partial range is zero
+            full = j0 == 0 and j1 == self.nangles
+            out = (torch.empty if full else torch.zeros)(shape, dtype=torch.float32, device=x.device)
         else:
             _check_out(out, shape, x.device, "out")
-        j0, j1 = self._range(angles)
         with torch.cuda.device(self.device):
             check(libraft.raft_ss_radon_range(self._handle, x.data_ptr(), out.data_ptr(), int(angle_major), j0, j1,
                                               _nslices(x, 2), self._stream()), "raft_ss_radon_range")
