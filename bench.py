@@ -156,7 +156,9 @@ def reference_case(dev, reps=50):
     in [0, pi), 256 bins over [-1, 1].  Radon, backprojection and FBP on the GPU, device-
     resident: `*_us` is the mean of `reps` calls enqueued from Python (host launch cost
     included), `*_graph_us` the mean of `reps` replays of the call captured in a HIP graph
-    (device time).  The reference C's time for the same case is in cpu_baseline."""
+    (a graph launch per call costs more on this runtime than the call's own kernel
+    launches, so it is reported beside the eager time, not as it).  The reference C's
+    time for the same case is in cpu_baseline."""
     import numpy as np
     import torch
     import pyraft_amd as pr

@@ -135,8 +135,11 @@ typedef struct raft_ss_plan raft_ss_plan;
 int raft_ss_plan_create(raft_ss_plan **plan, int n, int nrays, int nangles, const double *img_corners,
                         const double *sino_corners, void *stream);
 int raft_ss_plan_destroy(raft_ss_plan *plan);
-/* info[0..7] = n, nrays, nangles, #irregular rays, #degenerate angles, bp window,
- * bp angles/stage, bp kernel variant (window and stage are those of the variant) */
+/* info[0..8] = n, nrays, nangles, #irregular rays, #degenerate angles, bp window,
+ * bp angles/stage, bp kernel variant (window and stage are those of the variant), and the
+ * angle groups a one-slice full-range backprojection divides into (1: not divided; a
+ * small image's launch is divided so that it fills the GPU, and the groups' partial
+ * images are added in group order) */
 int raft_ss_plan_info(const raft_ss_plan *plan, int *info, int ninfo);
 /* copy the per-ray sample ranges [A][R] x (lo, hi) to host memory */
 int raft_ss_plan_bounds(const raft_ss_plan *plan, int *host_bounds);
@@ -253,11 +256,22 @@ int raft_ss_plan_set_bp_variant(raft_ss_plan *plan, int variant);
 /* time every runnable variant on this plan's full angle range (warm-up + best of
  * `reps` launches, HIP events on `stream`), keep the fastest and return its index.
  * times_us (optional, RAFT_SS_BP_NVARIANTS doubles) receives the times, -1 for
- * variants that do not fit.  Results are cached per process by geometry; flag
- * RAFT_SS_TUNE_FORCE measures again.  Not to be called while another thread uses
- * the plan. */
+ * variants that do not fit.  Results are cached by device architecture and geometry,
+ * per process and in a JSON file that later processes read ($PYRAFT_AMD_TUNE_CACHE, an
+ * empty value for none; else $XDG_CACHE_HOME/pyraft_amd/tune.json or
+ * $HOME/.cache/pyraft_amd/tune.json); raft_ss_em's subset variants are cached the same
+ * way.  Flag RAFT_SS_TUNE_FORCE measures again.  Not to be called while another thread
+ * uses the plan. */
 #define RAFT_SS_TUNE_FORCE 1
 int raft_ss_plan_tune(raft_ss_plan *plan, int reps, int flags, double *times_us, void *stream);
+/* Radon launches of few blocks (few angles, small images) divide each ray's samples into
+ * slabs of rows added by a finishing pass.  By default the first divided launch per
+ * (angle count, slice count) times the default slab size, twice and half of it and no
+ * division, keeps the fastest and caches it with the backprojection choices.  rows > 0
+ * pins the slab size for every Radon of the plan (rows >= n: no division); rows = 0
+ * returns to the tuned choice.  Slab sizes change the FP32 summation order (~1e-7
+ * relative); pinning one makes the bits reproducible. */
+int raft_ss_plan_set_radon_slab_rows(raft_ss_plan *plan, int rows);
 
 /* ================================================================ batches of uneven slices
  *

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <list>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -40,6 +41,47 @@ int ss_set_error(int code, const char* fmt, ...) {
 #define set_error ss_set_error
 
 extern "C" const char* raft_ss_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ per-stream scratch
+
+namespace {
+struct StreamWorkspace {
+  std::mutex mu;
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+// (device, stream) -> workspace; never destroyed, so no HIP call runs at process exit
+std::map<std::pair<int, hipStream_t>, StreamWorkspace*>* g_ws = new std::map<std::pair<int, hipStream_t>, StreamWorkspace*>();
+}  // namespace
+
+hipError_t ss_stream_scratch(hipStream_t stream, size_t bytes, void** p, std::unique_lock<std::mutex>* lock) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  StreamWorkspace* w;
+  {
+    std::lock_guard<std::mutex> g(g_ws_mu);
+    StreamWorkspace*& slot = (*g_ws)[std::make_pair(dev, stream)];
+    if (!slot) slot = new StreamWorkspace();
+    w = slot;
+  }
+  *lock = std::unique_lock<std::mutex>(w->mu);
+  if (bytes > w->bytes) {
+    if (w->p) {
+      e = hipFreeAsync(w->p, stream);
+      w->p = nullptr;
+      w->bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    e = hipMallocAsync(&w->p, want, stream);
+    if (e != hipSuccess) return e;
+    w->bytes = want;
+  }
+  *p = w->p;
+  return hipSuccess;
+}
 
 // Geometry in FP64 exactly as the reference derives it from the corners.
 static SSGeom make_geom(int n, int nrays, int nangles, const double* ic, const double* sc) {
@@ -250,7 +292,8 @@ extern "C" int raft_ss_plan_info(const raft_ss_plan* p, int* info, int ninfo) {
                       p->ndegenerate,
                       p->bp_window_v[p->bp_variant],
                       p->bp_angles_v[p->bp_variant],
-                      p->bp_variant};
+                      p->bp_variant,
+                      p->mesh ? 1 : ss_bp_groups(p->dev(), 0, p->g.nangles, 1)};
   for (int i = 0; i < ninfo && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
   return 0;
 }
@@ -494,9 +537,10 @@ extern "C" int raft_ss_radon_range(const raft_ss_plan* p, const float* d_image, 
   if ((rc = check_range(p, j0, j1))) return rc;
   if (!d_image || !d_out) return set_error(RAFT_SS_EINVAL, "null device pointer");
   hipStream_t stream = as_stream(stream_);
-  const SSDevPlan dp = p->dev();
+  SSDevPlan dp = p->dev();
+  if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &dp.radon_slab_rows))) return rc;
   const size_t layout = (size_t)nslices * p->g.n * p->g.pitch;
-  DevBuf ws;
+  Scratch ws;
   HIP_TRY(ws.alloc(2 * layout * sizeof(float2) + ss_radon_scratch(dp, j0, j1, nslices), stream));
   float2* gA = ws.as<float2>();
   float2* gB = gA + layout;
@@ -523,14 +567,16 @@ extern "C" int raft_ss_backprojection_range(const raft_ss_plan* p, const float* 
   hipStream_t stream = as_stream(stream_);
   const SSDevPlan dp = p->dev();
   const float* sT = d_sino;
-  DevBuf ws;
+  const size_t sino_floats = angle_major ? 0 : (size_t)nslices * p->g.nrays * p->g.nangles;
+  Scratch ws;
+  HIP_TRY(ws.alloc(sino_floats * sizeof(float) + ss_bp_scratch(dp, j0, j1, nslices), stream));
   if (!angle_major) {
-    HIP_TRY(ws.alloc((size_t)nslices * p->g.nrays * p->g.nangles * sizeof(float), stream));
     HIP_TRY(ss_launch_sino_to_t(d_sino, ws.as<float>(), p->g.nrays, p->g.nangles, nslices, stream));
     sT = ws.as<float>();
   }
   SSEmUpdateArgs none{nullptr, nullptr};
-  HIP_TRY(ss_launch_backprojection(dp, sT, j0, j1, nslices, SS_BP_PLAIN, d_image, none, stream));
+  HIP_TRY(ss_launch_backprojection(dp, sT, j0, j1, nslices, SS_BP_PLAIN, d_image, none, ws.as<float>() + sino_floats,
+                                   stream));
   return 0;
 }
 
@@ -550,15 +596,16 @@ extern "C" int raft_ss_radon_transpose_range(const raft_ss_plan* p, const float*
   hipStream_t stream = as_stream(stream_);
   const SSDevPlan dp = p->dev();
   const float* sT = d_sino;
-  DevBuf ws, tr;
+  const size_t sino_floats = angle_major ? 0 : (size_t)nslices * p->g.nrays * p->g.nangles;
+  const size_t scratch = ss_radon_transpose_scratch(dp, nslices);
+  Scratch ws;
+  HIP_TRY(ws.alloc(sino_floats * sizeof(float) + scratch, stream));
   if (!angle_major) {
-    HIP_TRY(ws.alloc((size_t)nslices * p->g.nrays * p->g.nangles * sizeof(float), stream));
     HIP_TRY(ss_launch_sino_to_t(d_sino, ws.as<float>(), p->g.nrays, p->g.nangles, nslices, stream));
     sT = ws.as<float>();
   }
-  const size_t scratch = ss_radon_transpose_scratch(dp, nslices);
-  if (scratch) HIP_TRY(tr.alloc(scratch, stream));
-  HIP_TRY(ss_launch_radon_transpose(dp, sT, j0, j1, nslices, d_image, tr.as<float>(), stream));
+  HIP_TRY(ss_launch_radon_transpose(dp, sT, j0, j1, nslices, d_image, scratch ? ws.as<float>() + sino_floats : nullptr,
+                                    stream));
   return 0;
 }
 
@@ -582,7 +629,7 @@ extern "C" int raft_ss_fbp_filter(raft_ss_plan* p, const float* d_sino, float* d
     HIP_TRY(plan_filter(p, d_sino, d_out, nslices * A, stream));
     return 0;
   }
-  DevBuf ws;
+  Scratch ws;
   HIP_TRY(ws.alloc(2 * total * sizeof(float), stream));
   float* t0 = ws.as<float>();
   float* t1 = t0 + total;
@@ -601,14 +648,15 @@ extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image,
   if ((rc = ensure_ramp(p, stream))) return rc;
   const int R = p->g.nrays, A = p->g.nangles;
   const size_t total = (size_t)nslices * R * A;
-  DevBuf ws;
-  HIP_TRY(ws.alloc(2 * total * sizeof(float), stream));
+  const SSDevPlan dp = p->dev();
+  Scratch ws;
+  HIP_TRY(ws.alloc(2 * total * sizeof(float) + ss_bp_scratch(dp, 0, A, nslices), stream));
   float* t0 = ws.as<float>();
   float* t1 = t0 + total;
   HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
   HIP_TRY(plan_filter(p, t0, t1, nslices * A, stream));
   SSEmUpdateArgs none{nullptr, nullptr};
-  HIP_TRY(ss_launch_backprojection(p->dev(), t1, 0, A, nslices, SS_BP_PLAIN, d_image, none, stream));
+  HIP_TRY(ss_launch_backprojection(dp, t1, 0, A, nslices, SS_BP_PLAIN, d_image, none, t1 + total, stream));
   return 0;
 }
 
@@ -636,18 +684,32 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   const size_t sino_sz = (size_t)nslices * R * A;
   const size_t img_sz = (size_t)nslices * n * n;
   const size_t layout = (size_t)nslices * n * p->g.pitch;
-  // LL partials of all subsets of one iteration, reduced once per iteration
+  // LL partials of all subsets of one iteration, reduced once per iteration; each subset's
+  // Radon with the plan's slab choice for its angle count
   std::vector<int> poff(nsubsets + 1, 0);
+  std::vector<SSDevPlan> dps(nsubsets, dp);
   size_t slab_bytes = 0;  // the largest subset's slab partial sums
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-    poff[b + 1] = poff[b] + ss_radon_partials(dp, j0, j1, nslices);
-    slab_bytes = std::max(slab_bytes, ss_radon_scratch(dp, j0, j1, nslices));
+    if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &dps[b].radon_slab_rows))) return rc;
+    poff[b + 1] = poff[b] + ss_radon_partials(dps[b], j0, j1, nslices);
+    slab_bytes = std::max(slab_bytes, ss_radon_scratch(dps[b], j0, j1, nslices));
   }
   const int npartials = poff[nsubsets];
+  // the subsets' backprojection variant, tuned for their angle count (launches of few
+  // angles favour smaller tiles than a full-range backprojection)
+  int bv = p->bp_variant;
+  if ((rc = ss_plan_range_variant(p, 0, (int)(A / nsubsets), stream, &bv))) return rc;
+  const SSDevPlan dpb = p->dev(bv);
+  size_t bp_bytes = 0;  // the largest subset's backprojection partial images
+  for (int b = 0; b < nsubsets; ++b) {
+    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
+    bp_bytes = std::max(bp_bytes, ss_bp_scratch(dpb, j0, j1, nslices));
+  }
   const size_t bytes = sizeof(float2) * 2 * layout + sizeof(float) * (2 * sino_sz + (size_t)nsubsets * img_sz) +
-                       sizeof(double) * ((size_t)npartials + (size_t)std::max(niter, 1) + 8) + slab_bytes;
-  DevBuf ws;
+                       sizeof(double) * ((size_t)npartials + (size_t)std::max(niter, 1) + 8) + slab_bytes +
+                       bp_bytes;
+  Scratch ws;
   HIP_TRY(ws.alloc(bytes, stream));
   float2* gA = ws.as<float2>();
   float2* gB = gA + layout;
@@ -658,12 +720,7 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   partials = reinterpret_cast<double*>(((uintptr_t)partials + 7) & ~(uintptr_t)7);
   double* d_ll = partials + npartials;
   float* slabs = reinterpret_cast<float*>(d_ll + std::max(niter, 1));
-
-  // the subsets' backprojection variant, tuned for their angle count (launches of few
-  // angles favour smaller tiles than a full-range backprojection)
-  int bv = p->bp_variant;
-  if ((rc = ss_plan_range_variant(p, 0, (int)(A / nsubsets), stream, &bv))) return rc;
-  const SSDevPlan dpb = p->dev(bv);
+  float* bp_part = slabs + slab_bytes / sizeof(float);
 
   HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
   // W_b = BP_b(1) with the same backprojection kernel
@@ -672,16 +729,16 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
     HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
-                                     stream));
+                                     bp_part, stream));
   }
   for (int it = 0; it < niter; ++it) {
     for (int b = 0; b < nsubsets; ++b) {
       const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
       HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
       SSEmRatioArgs ea{measT, partials + poff[b]};
-      HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, slabs, stream));
+      HIP_TRY(ss_launch_radon(dps[b], gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, slabs, stream));
       SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
-      HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, stream));
+      HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, bp_part, stream));
     }
     HIP_TRY(ss_launch_sum_partials(partials, npartials, d_ll + it, stream));
   }
@@ -703,13 +760,15 @@ extern "C" int raft_ss_tr_em(raft_ss_plan* p, const float* d_raw, const float* d
   if (niter < 0) return set_error(RAFT_SS_EINVAL, "niter must be >= 0");
   const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
   hipStream_t stream = as_stream(stream_);
-  const SSDevPlan dp = p->dev();
+  SSDevPlan dp = p->dev();
+  if ((rc = ss_plan_radon_rows(p, 0, A, nslices, stream, true, &dp.radon_slab_rows))) return rc;
   const size_t sino_sz = (size_t)nslices * R * A;
   const size_t img_sz = (size_t)nslices * n * n;
   const size_t layout = (size_t)nslices * n * p->g.pitch;
-  DevBuf ws;
-  HIP_TRY(ws.alloc(sizeof(float2) * 2 * layout + sizeof(float) * (3 * sino_sz + 2 * img_sz) +
-                       ss_radon_scratch(dp, 0, A, nslices),
+  Scratch ws;
+  const size_t slab_bytes = ss_radon_scratch(dp, 0, A, nslices);
+  HIP_TRY(ws.alloc(sizeof(float2) * 2 * layout + sizeof(float) * (3 * sino_sz + 2 * img_sz) + slab_bytes +
+                       ss_bp_scratch(dp, 0, A, nslices),
                    stream));
   float2* gA = ws.as<float2>();
   float2* gB = gA + layout;
@@ -719,16 +778,17 @@ extern "C" int raft_ss_tr_em(raft_ss_plan* p, const float* d_raw, const float* d
   float* w = pT + sino_sz;
   float* b = w + img_sz;
   float* slabs = b + img_sz;
+  float* bp_part = slabs + slab_bytes / sizeof(float);
   SSEmUpdateArgs none{nullptr, nullptr};
   SSEmRatioArgs no_ratio{nullptr, nullptr};
   HIP_TRY(ss_launch_sino_to_t(d_raw, rawT, R, A, nslices, stream));
   HIP_TRY(ss_launch_sino_to_t(d_flat, flatT, R, A, nslices, stream));
-  HIP_TRY(ss_launch_backprojection(dp, rawT, 0, A, nslices, SS_BP_PLAIN, w, none, stream));
+  HIP_TRY(ss_launch_backprojection(dp, rawT, 0, A, nslices, SS_BP_PLAIN, w, none, bp_part, stream));
   for (int it = 0; it < niter; ++it) {
     HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
     HIP_TRY(ss_launch_radon(dp, gA, gB, 0, A, nslices, SS_OUT_SINOT, pT, no_ratio, slabs, stream));
     HIP_TRY(ss_launch_tr_expected(pT, flatT, sino_sz, stream));
-    HIP_TRY(ss_launch_backprojection(dp, pT, 0, A, nslices, SS_BP_PLAIN, b, none, stream));
+    HIP_TRY(ss_launch_backprojection(dp, pT, 0, A, nslices, SS_BP_PLAIN, b, none, bp_part, stream));
     HIP_TRY(ss_launch_tr_update(d_x, b, w, img_sz, stream));
   }
   return 0;

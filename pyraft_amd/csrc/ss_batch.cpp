@@ -9,7 +9,10 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <map>
@@ -47,6 +50,13 @@ extern "C" int raft_ss_plan_set_bp_variant(raft_ss_plan* p, int v) {
   if (rc) return rc;
   p->bp_variant = v;
   p->bp_variant_fixed = true;
+  return 0;
+}
+
+extern "C" int raft_ss_plan_set_radon_slab_rows(raft_ss_plan* p, int rows) {
+  if (!p) return ss_set_error(RAFT_SS_EINVAL, "null plan");
+  if (rows < 0) return ss_set_error(RAFT_SS_EINVAL, "slab rows must be >= 0 (got %d)", rows);
+  p->radon_rows_fixed = rows;
   return 0;
 }
 
@@ -96,6 +106,111 @@ struct TuneEntry {
 std::mutex g_tune_mu;
 std::map<std::string, TuneEntry> g_tune;
 
+// ---- persisted choices
+//
+// The tuners' choices are kept per process in g_tune and in a JSON file, so that later
+// processes on the same device architecture skip the timing: $PYRAFT_AMD_TUNE_CACHE (an
+// empty value: no file), else $XDG_CACHE_HOME/pyraft_amd/tune.json, else
+// $HOME/.cache/pyraft_amd/tune.json.  Keys name the architecture (gcnArchName) and the
+// geometry (and the angle-range length of OSEM subsets).  One entry per line:
+//   {"format": "<kTuneFormat>",
+//    "entries": {
+//     "<key>": {"variant": 3, "times_us": [t0, t1, t2, t3, t4, t5]},
+//     ...
+//    }}
+// A file of another format is ignored and a line that does not parse is skipped (that
+// geometry is timed again).  After each new choice the file is written whole to a
+// temporary name and renamed over the old one.  Callers hold g_tune_mu.
+const char* const kTuneFormat = "pyraft_amd tuning cache 2: backprojection variants x6, angle groups";
+bool g_tune_loaded = false;
+
+std::string tune_cache_path() {
+  const char* env = getenv("PYRAFT_AMD_TUNE_CACHE");
+  if (env) return env;  // "" disables the file
+  const char* xdg = getenv("XDG_CACHE_HOME");
+  if (xdg && *xdg) return std::string(xdg) + "/pyraft_amd/tune.json";
+  const char* home = getenv("HOME");
+  if (home && *home) return std::string(home) + "/.cache/pyraft_amd/tune.json";
+  return "";
+}
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const std::string path = tune_cache_path();
+  if (path.empty()) return;
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) return;
+  char line[1024];
+  bool format_ok = false;
+  while (fgets(line, sizeof(line), f)) {
+    if (!format_ok) {
+      const char* q = strstr(line, "\"format\": \"");
+      if (q) {
+        q += strlen("\"format\": \"");
+        const size_t len = strlen(kTuneFormat);
+        format_ok = strncmp(q, kTuneFormat, len) == 0 && q[len] == '"';
+        if (!format_ok) break;
+      }
+      continue;
+    }
+    const char* a = strchr(line, '"');
+    const char* b = a ? strchr(a + 1, '"') : nullptr;
+    if (!a || !b || b == a + 1) continue;
+    TuneEntry te;
+    double* t = te.times;
+    if (sscanf(b + 1, ": {\"variant\": %d, \"times_us\": [%lf, %lf, %lf, %lf, %lf, %lf]}", &te.variant, &t[0], &t[1],
+               &t[2], &t[3], &t[4], &t[5]) != 1 + SS_BP_NVARIANTS)
+      continue;
+    if (te.variant < 0 || te.variant >= SS_BP_NVARIANTS) continue;
+    g_tune[std::string(a + 1, b)] = te;
+  }
+  fclose(f);
+}
+
+// mkdir -p of the directory part of `path`
+void make_parent_dirs(const std::string& path) {
+  for (size_t i = 1; i < path.size(); ++i)
+    if (path[i] == '/') (void)mkdir(path.substr(0, i).c_str(), 0755);
+}
+
+void tune_save_locked() {
+  const std::string path = tune_cache_path();
+  if (path.empty()) return;
+  make_parent_dirs(path);
+  const std::string tmp = path + ".tmp." + std::to_string((long)getpid());
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return;  // not writable: the choices stay per process
+  fprintf(f, "{\"format\": \"%s\",\n \"entries\": {\n", kTuneFormat);
+  size_t i = 0;
+  for (const auto& kv : g_tune) {
+    const double* t = kv.second.times;
+    fprintf(f, "  \"%s\": {\"variant\": %d, \"times_us\": [%.3f, %.3f, %.3f, %.3f, %.3f, %.3f]}%s\n",
+            kv.first.c_str(), kv.second.variant, t[0], t[1], t[2], t[3], t[4], t[5],
+            ++i < g_tune.size() ? "," : "");
+  }
+  fprintf(f, " }}\n");
+  const bool ok = fclose(f) == 0;
+  if (!ok || rename(tmp.c_str(), path.c_str()) != 0) (void)remove(tmp.c_str());
+}
+
+// the cached choice for `key` (loading the file on first use)
+bool tune_find(const std::string& key, TuneEntry* te) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_load_locked();
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) return false;
+  *te = it->second;
+  return true;
+}
+
+void tune_store(const std::string& key, const TuneEntry& te) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_load_locked();
+  g_tune[key] = te;
+  tune_save_locked();
+}
+
 std::string plan_key(const raft_ss_plan* p) {
   char buf[512];
   snprintf(buf, sizeof(buf), "%s|%d|%d|%d|%a|%a|%a|%a|%a|%a|%a|%a", device_arch().c_str(), p->g.n, p->g.nrays,
@@ -112,20 +227,21 @@ extern "C" int raft_ss_plan_tune(raft_ss_plan* p, int reps, int flags, double* t
   if ((rc = check_uniform(p, "plan_tune"))) return rc;
   hipStream_t stream = as_stream(stream_);
   const std::string key = plan_key(p);
-  if (!(flags & RAFT_SS_TUNE_FORCE)) {
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      p->bp_variant = it->second.variant;
-      if (times_us) std::copy(it->second.times, it->second.times + SS_BP_NVARIANTS, times_us);
-      return p->bp_variant;
-    }
+  TuneEntry cached;
+  if (!(flags & RAFT_SS_TUNE_FORCE) && tune_find(key, &cached) && p->bp_angles_v[cached.variant] > 0) {
+    p->bp_variant = cached.variant;
+    if (times_us) std::copy(cached.times, cached.times + SS_BP_NVARIANTS, times_us);
+    return p->bp_variant;
   }
   const int n = p->g.n, R = p->g.nrays, A = p->g.nangles;
+  size_t part_bytes = 0;  // angle-group partial images of the variants that divide their launch
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v)
+    if (p->bp_angles_v[v] > 0) part_bytes = std::max(part_bytes, ss_bp_scratch(p->dev(v), 0, A, 1));
   DevBuf ws;
-  HIP_TRY(ws.alloc(sizeof(float) * ((size_t)R * A + (size_t)n * n), stream));
+  HIP_TRY(ws.alloc(sizeof(float) * ((size_t)R * A + (size_t)n * n) + part_bytes, stream));
   float* sinoT = ws.as<float>();
   float* img = sinoT + (size_t)R * A;
+  float* part = img + (size_t)n * n;
   HIP_TRY(ss_launch_fill(sinoT, 1.0f, (size_t)R * A, stream));
   TuneEntry te;
   int best = -1;
@@ -135,7 +251,7 @@ extern "C" int raft_ss_plan_tune(raft_ss_plan* p, int reps, int flags, double* t
     const SSDevPlan dp = p->dev(v);
     SSEmUpdateArgs none{nullptr, nullptr};
     auto run = [&]() -> int {
-      HIP_TRY(ss_launch_backprojection(dp, sinoT, 0, A, 1, SS_BP_PLAIN, img, none, stream));
+      HIP_TRY(ss_launch_backprojection(dp, sinoT, 0, A, 1, SS_BP_PLAIN, img, none, part, stream));
       return 0;
     };
     if ((rc = time_best(run, reps, stream, &te.times[v]))) return rc;
@@ -145,8 +261,7 @@ extern "C" int raft_ss_plan_tune(raft_ss_plan* p, int reps, int flags, double* t
   te.variant = best;
   p->bp_variant = best;
   if (times_us) std::copy(te.times, te.times + SS_BP_NVARIANTS, times_us);
-  std::lock_guard<std::mutex> lock(g_tune_mu);
-  g_tune[key] = te;
+  tune_store(key, te);
   return best;
 }
 
@@ -154,23 +269,24 @@ int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, i
   *variant = p->bp_variant;
   if (p->bp_variant_fixed || j1 <= j0) return 0;
   const std::string key = plan_key(p) + "|range" + std::to_string(j1 - j0);
-  {
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      *variant = it->second.variant;
-      return 0;
-    }
+  TuneEntry cached;
+  if (tune_find(key, &cached) && p->bp_angles_v[cached.variant] > 0) {
+    *variant = cached.variant;
+    return 0;
   }
   // no timing inside a stream capture: the plan's variant until an eager call tunes
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   HIP_TRY(hipStreamIsCapturing(stream, &cs));
   if (cs != hipStreamCaptureStatusNone) return 0;
   const int n = p->g.n, R = p->g.nrays, A = p->g.nangles;
+  size_t part_bytes = 0;
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v)
+    if (p->bp_angles_v[v] > 0) part_bytes = std::max(part_bytes, ss_bp_scratch(p->dev(v), j0, j1, 1));
   DevBuf ws;
-  HIP_TRY(ws.alloc(sizeof(float) * ((size_t)R * A + (size_t)n * n), stream));
+  HIP_TRY(ws.alloc(sizeof(float) * ((size_t)R * A + (size_t)n * n) + part_bytes, stream));
   float* sinoT = ws.as<float>();
   float* img = sinoT + (size_t)R * A;
+  float* part = img + (size_t)n * n;
   HIP_TRY(ss_launch_fill(sinoT, 1.0f, (size_t)R * A, stream));
   TuneEntry te;
   int best = -1, rc = 0;
@@ -180,7 +296,7 @@ int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, i
     const SSDevPlan dp = p->dev(v);
     SSEmUpdateArgs none{nullptr, nullptr};
     auto run = [&]() -> int {
-      HIP_TRY(ss_launch_backprojection(dp, sinoT, j0, j1, 1, SS_BP_PLAIN, img, none, stream));
+      HIP_TRY(ss_launch_backprojection(dp, sinoT, j0, j1, 1, SS_BP_PLAIN, img, none, part, stream));
       return 0;
     };
     if ((rc = time_best(run, 3, stream, &te.times[v]))) return rc;
@@ -189,8 +305,81 @@ int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, i
   if (best < 0) return ss_set_error(RAFT_SS_EINTERNAL, "no runnable backprojection variant");
   te.variant = best;
   *variant = best;
-  std::lock_guard<std::mutex> lock(g_tune_mu);
-  g_tune[key] = te;
+  tune_store(key, te);
+  return 0;
+}
+
+int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipStream_t stream, bool may_time,
+                       int* rows) {
+  *rows = 0;
+  const int n = p->g.n;
+  if (p->radon_rows_fixed > 0) {  // raft_ss_plan_set_radon_slab_rows
+    *rows = p->radon_rows_fixed;
+    return 0;
+  }
+  if (j1 <= j0 || p->mesh) return 0;  // mesh plans keep the rule (their key would not name the tables)
+  SSDevPlan dp = p->dev();
+  int r0 = n;
+  if (ss_radon_slabs(dp, j0, j1, nslices, &r0) <= 1) return 0;  // the rule does not divide
+  const long long pk = ((long long)(j1 - j0) << 20) | (long long)std::min(nslices, (1 << 20) - 1);
+  {
+    std::lock_guard<std::mutex> lock(p->radon_mutex);
+    auto it = p->radon_rows.find(pk);
+    if (it != p->radon_rows.end()) {
+      *rows = it->second;
+      return 0;
+    }
+  }
+  if (!may_time) return 0;  // the rule (no lookups that allocate)
+  // candidates: the rule's slab rows, twice and half of them (multiples of 32), no division
+  std::vector<int> cand{r0};
+  if (2 * r0 < n) cand.push_back(2 * r0);
+  if (r0 / 2 >= 32 && (r0 / 2) % 32 == 0) cand.push_back(r0 / 2);
+  cand.push_back(n);
+  const std::string key = plan_key(p) + "|radon|range" + std::to_string(j1 - j0) + "|s" + std::to_string(nslices);
+  TuneEntry cached;
+  int chosen = -1;
+  if (tune_find(key, &cached) && cached.variant >= 0 && cached.variant < (int)cand.size()) chosen = cand[cached.variant];
+  if (chosen < 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return 0;  // the rule, untimed
+    const size_t layout = (size_t)nslices * n * p->g.pitch;
+    size_t slab_bytes = 0;
+    for (int r : cand) {
+      dp.radon_slab_rows = r;
+      slab_bytes = std::max(slab_bytes, ss_radon_scratch(dp, j0, j1, nslices));
+    }
+    const size_t img = (size_t)nslices * n * n, sino = (size_t)nslices * p->g.nrays * p->g.nangles;
+    DevBuf ws;
+    HIP_TRY(ws.alloc(2 * layout * sizeof(float2) + (img + sino) * sizeof(float) + slab_bytes, stream));
+    float2* gA = ws.as<float2>();
+    float2* gB = gA + layout;
+    float* x = reinterpret_cast<float*>(gB + layout);
+    float* out = x + img;
+    float* slabs = out + sino;
+    HIP_TRY(ss_launch_fill(x, 1.0f, img, stream));
+    HIP_TRY(ss_launch_gather_layouts(dp, x, gA, gB, nslices, stream));
+    TuneEntry te;
+    for (int v = 0; v < SS_BP_NVARIANTS; ++v) te.times[v] = -1.0;
+    int best = 0, rc = 0;
+    for (int c = 0; c < (int)cand.size() && c < SS_BP_NVARIANTS; ++c) {
+      dp.radon_slab_rows = cand[c];
+      SSEmRatioArgs none{nullptr, nullptr};
+      auto run = [&]() -> int {
+        HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_SINOT, out, none, slabs, stream));
+        return 0;
+      };
+      if ((rc = time_best(run, 3, stream, &te.times[c]))) return rc;
+      if (te.times[c] < te.times[best]) best = c;
+    }
+    te.variant = best;
+    tune_store(key, te);
+    chosen = cand[best];
+  }
+  std::lock_guard<std::mutex> lock(p->radon_mutex);
+  p->radon_rows[pk] = chosen;
+  *rows = chosen;
   return 0;
 }
 

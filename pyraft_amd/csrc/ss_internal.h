@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <utility>
@@ -37,6 +38,15 @@ struct raft_ss_plan;
 // the range is empty, or while `stream` is being captured and no earlier call has
 // timed this range (no timing under capture).
 int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, int* variant);
+// Sample rows per slab of the plan's Radon over angle ranges of j1 - j0 angles on nslices
+// slices (SSDevPlan::radon_slab_rows).  Launches the default rule does not divide keep it
+// (0).  Divided launches take the fastest of the rule's slab size, twice and half of it and
+// no division, timed once per process (gather layouts built first, not timed) and cached
+// with the backprojection choices; no timing while `stream` is captured, and with
+// `may_time` false only the plan's earlier choice or the rule (no host allocation).  A
+// value pinned by raft_ss_plan_set_radon_slab_rows applies to every launch.
+int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipStream_t stream, bool may_time,
+                       int* rows);
 
 struct raft_ss_plan {
   int device = 0;
@@ -62,6 +72,9 @@ struct raft_ss_plan {
   void* d_ramp_fft = nullptr;  // its FFT-form spectrum and twiddles (if R fits the FFT form)
   SSFftTaps ramp_fft{};
   std::mutex ramp_mutex;
+  int radon_rows_fixed = 0;  // raft_ss_plan_set_radon_slab_rows (0: tuned / rule)
+  mutable std::mutex radon_mutex;
+  mutable std::map<long long, int> radon_rows;  // (range length, slices) -> tuned slab rows
   SSDevPlan dev() const { return dev(bp_variant); }
   // the plan as the kernels see it, with backprojection variant v
   SSDevPlan dev(int v) const {
@@ -78,6 +91,7 @@ struct raft_ss_plan {
     p.bp_variant = v;
     p.bp_window = bp_window_v[v];
     p.bp_angles = bp_angles_v[v];
+    p.radon_slab_rows = 0;
     return p;
   }
   ~raft_ss_plan() {
@@ -223,6 +237,36 @@ struct DevBuf {
   hipError_t alloc(size_t bytes, hipStream_t stream) {
     s = stream;
     return hipMallocAsync(&p, bytes ? bytes : 4, stream);
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+// Scratch of one call of a device-pointer operator.  An eager call takes the cached
+// workspace of its device and stream: grow-only, reused by every later call on that
+// stream, since calls on one stream run in stream order and never overlap; a larger
+// request releases the old buffer on the stream (after the work that used it).  The call
+// holds the workspace's lock while it enqueues, so host threads sharing a stream take
+// turns.  A call made while its stream is being captured allocates stream-ordered memory
+// of its own instead (released at the end of the call, the graph holding the allocation),
+// so that no graph refers to a cached buffer that a later call may release.
+hipError_t ss_stream_scratch(hipStream_t stream, size_t bytes, void** p, std::unique_lock<std::mutex>* lock);
+struct Scratch {
+  DevBuf own;
+  void* p = nullptr;
+  std::unique_lock<std::mutex> lock;
+  hipError_t alloc(size_t bytes, hipStream_t stream) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(stream, &cs);
+    if (e != hipSuccess) return e;
+    if (cs != hipStreamCaptureStatusNone) {
+      e = own.alloc(bytes, stream);
+      p = own.p;
+      return e;
+    }
+    return ss_stream_scratch(stream, bytes, &p, &lock);
   }
   template <class T>
   T* as() const {

@@ -18,7 +18,7 @@
 // blocks), which a finishing pass stores in one of the other modes
 enum { SS_OUT_SINO = 0, SS_OUT_SINOT = 1, SS_OUT_EM_RATIO = 2, SS_OUT_PARTIAL = 3 };
 // Backprojection output modes
-enum { SS_BP_PLAIN = 0, SS_BP_EM_UPDATE = 1 };
+enum { SS_BP_PLAIN = 0, SS_BP_EM_UPDATE = 1, SS_BP_PARTIAL = 2 };
 
 // Backprojection kernel variants (chosen per plan, by the autotuner or by hand).
 // A block is 256 threads over a tile of 32 columns x 8*rows pixel rows; each thread
@@ -43,6 +43,7 @@ struct SSDevPlan {
   int bp_variant;          // index into kSSBpRows / kSSBpStage
   int bp_window;           // rays staged per angle per pixel tile (for bp_variant)
   int bp_angles;           // angles per LDS stage (for bp_variant)
+  int radon_slab_rows;     // Radon: sample rows per slab (n: no division; 0: the default rule)
 };
 
 // ---- batched execution over slices of different geometry
@@ -102,6 +103,10 @@ int ss_radon_slab_plan(int n, long long blocks, int* rows);
 static inline long long ss_radon_blocks(int nwaves, int j0, int j1) {
   return (long long)nwaves * ((j1 - j0 + 7) / 8);
 }
+// Slabs of a Radon launch over angles [j0, j1) of nslices slices: p.radon_slab_rows when
+// set (the plan's tuned choice), else the default rule; returns the slab count (1: not
+// divided) and sets *rows.
+int ss_radon_slabs(const SSDevPlan& p, int j0, int j1, int nslices, int* rows);
 // scratch: ss_radon_scratch(p, j0, j1, nslices) bytes (the slabs' partial sums; 0 bytes,
 // and scratch may be null, when the launch is not divided)
 size_t ss_radon_scratch(const SSDevPlan& p, int j0, int j1, int nslices);
@@ -115,8 +120,20 @@ struct SSEmUpdateArgs {
   float* x;            // image updated in place
   const float* weight; // BP_b(1)
 };
+// A launch of fewer than SS_BP_SPLIT_TILES pixel tiles (a small image) divides its angles
+// into groups taken by separate blocks, at least SS_BP_GROUP_ANGLES angles each, so that
+// the launch has about SS_BP_SPLIT_BLOCKS blocks; each group writes a partial image
+// (unscaled sums) to scratch and a finishing pass adds the groups in group order, scales by
+// dtheta and stores or applies the EM update.  ss_bp_groups: the group count (1: no
+// division); ss_bp_scratch: the bytes of the partial images (0 when not divided).
+#define SS_BP_SPLIT_TILES 512
+#define SS_BP_SPLIT_BLOCKS 1024
+#define SS_BP_GROUP_ANGLES 8
+int ss_bp_groups(const SSDevPlan& p, int j0, int j1, int nslices);
+size_t ss_bp_scratch(const SSDevPlan& p, int j0, int j1, int nslices);
+// scratch: ss_bp_scratch(p, j0, j1, nslices) bytes, or nullptr (then the launch is not divided)
 hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices, int mode,
-                                    float* out, SSEmUpdateArgs em, hipStream_t stream);
+                                    float* out, SSEmUpdateArgs em, float* scratch, hipStream_t stream);
 
 // ---- exact transpose of the Radon (reads angle-major sinogram)
 // scratch bytes the transpose needs: one transposed n x n FP32 image per slice, where the

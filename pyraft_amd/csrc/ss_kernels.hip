@@ -716,10 +716,18 @@ int ss_radon_slab_plan(int n, long long blocks, int* rows) {
   return count;
 }
 
-static int ss_radon_slabs(const SSDevPlan& p, int j0, int j1, int nslices, int* rows) {
-  *rows = p.g.n;
+int ss_radon_slabs(const SSDevPlan& p, int j0, int j1, int nslices, int* rows) {
+  const int n = p.g.n;
+  *rows = n;
   if (j1 <= j0) return 1;
-  return ss_radon_slab_plan(p.g.n, ss_radon_blocks(p.nwaves, j0, j1) * nslices, rows);
+  if (p.radon_slab_rows > 0) {
+    const int L = (std::min(p.radon_slab_rows, n) + RADON_CHUNK - 1) / RADON_CHUNK * RADON_CHUNK;
+    const int count = (n + L - 1) / L;
+    if (count <= 1) return 1;
+    *rows = L;
+    return count;
+  }
+  return ss_radon_slab_plan(n, ss_radon_blocks(p.nwaves, j0, j1) * nslices, rows);
 }
 
 static int ss_radon_finish_blocks(int nj, int R) { return (int)(((long long)nj * R + 255) / 256); }
@@ -982,6 +990,8 @@ __device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __re
     const float bp = acc[i] * dth;
     if (MODE == SS_BP_PLAIN) {
       out[idx] = bp;
+    } else if (MODE == SS_BP_PARTIAL) {
+      out[idx] = acc[i];
     } else {
       const float wgt = ws[idx];
       const float x = xs[idx];
@@ -990,16 +1000,51 @@ __device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __re
   }
 }
 
+// blockIdx.z = slice * ngroups + g.  SS_BP_PARTIAL: group g takes angles
+// [j0 + g*nj/ngroups, j0 + (g+1)*nj/ngroups) and writes its unscaled sums to
+// out[slice][g][n][n]; the other modes have ngroups = 1.
 template <int MODE, int ROWS, int STAGE>
 __global__ __launch_bounds__(256) void k_backprojection(SSDevPlan p, const float* __restrict__ sinoT, int j0, int j1,
-                                                        float* __restrict__ out, SSEmUpdateArgs em) {
+                                                        int ngroups, float* __restrict__ out, SSEmUpdateArgs em) {
   __shared__ SSBpLds<STAGE> lds;
-  const size_t slice = blockIdx.z;
+  const size_t slice = blockIdx.z / (unsigned)ngroups;
+  const int g = (int)(blockIdx.z % (unsigned)ngroups);
   const size_t so = slice * p.g.nangles * p.g.nrays;
-  const size_t io = slice * p.g.n * p.g.n;
-  ss_bp_item<MODE, ROWS, STAGE>(p, sinoT + so, j0, j1, blockIdx.x * BP_TILE_W, blockIdx.y * 8 * ROWS,
-                                MODE == SS_BP_PLAIN ? out + io : nullptr, MODE == SS_BP_PLAIN ? nullptr : em.x + io,
-                                MODE == SS_BP_PLAIN ? nullptr : em.weight + io, lds);
+  const size_t per = (size_t)p.g.n * p.g.n;
+  const size_t io = slice * per;
+  int ja = j0, jb = j1;
+  if (MODE == SS_BP_PARTIAL) {
+    const long long nj = j1 - j0;
+    ja = j0 + (int)(g * nj / ngroups);
+    jb = j0 + (int)((g + 1) * nj / ngroups);
+  }
+  float* dst = MODE == SS_BP_PLAIN ? out + io : (MODE == SS_BP_PARTIAL ? out + (slice * ngroups + g) * per : nullptr);
+  ss_bp_item<MODE, ROWS, STAGE>(p, sinoT + so, ja, jb, blockIdx.x * BP_TILE_W, blockIdx.y * 8 * ROWS, dst,
+                                MODE == SS_BP_EM_UPDATE ? em.x + io : nullptr,
+                                MODE == SS_BP_EM_UPDATE ? em.weight + io : nullptr, lds);
+}
+
+// the groups' partial sums of each pixel added in group order, times dtheta, stored
+// (SS_BP_PLAIN) or applied as x <- x * bp / w (SS_BP_EM_UPDATE), as ss_bp_item stores them
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bp_groups_finish(const float* __restrict__ part, int ngroups, int n, float dth,
+                                                          float* __restrict__ out, SSEmUpdateArgs em) {
+  const size_t per = (size_t)n * n;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= per) return;
+  const size_t slice = blockIdx.y;
+  const float* src = part + slice * ngroups * per + idx;
+  float sum = 0.f;
+  for (int g = 0; g < ngroups; ++g) sum += src[(size_t)g * per];
+  const float bp = sum * dth;
+  const size_t o = slice * per + idx;
+  if (MODE == SS_BP_PLAIN) {
+    out[o] = bp;
+  } else {
+    const float wgt = em.weight[o];
+    const float x = em.x[o];
+    em.x[o] = wgt > 0.f ? x * bp / wgt : x;
+  }
 }
 
 int ss_bp_tiles(int n, int variant) {
@@ -1007,23 +1052,42 @@ int ss_bp_tiles(int n, int variant) {
   return ((n + BP_TILE_W - 1) / BP_TILE_W) * ((n + th - 1) / th);
 }
 
+int ss_bp_groups(const SSDevPlan& p, int j0, int j1, int nslices) {
+  const long long tiles = (long long)ss_bp_tiles(p.g.n, p.bp_variant) * std::max(nslices, 1);
+  const int nj = j1 - j0;
+  if (tiles >= SS_BP_SPLIT_TILES || nj < 2 * SS_BP_GROUP_ANGLES) return 1;
+  const long long want = (SS_BP_SPLIT_BLOCKS + tiles - 1) / tiles;
+  return (int)std::max(1LL, std::min(want, (long long)(nj / SS_BP_GROUP_ANGLES)));
+}
+
+size_t ss_bp_scratch(const SSDevPlan& p, int j0, int j1, int nslices) {
+  const int G = ss_bp_groups(p, j0, j1, nslices);
+  return G > 1 ? sizeof(float) * (size_t)G * nslices * p.g.n * p.g.n : 0;
+}
+
 // variant id -> (ROWS, STAGE); must match kSSBpRows / kSSBpStage
 #define SS_BP_FOR_VARIANTS(X) X(0, 4, 4096) X(1, 2, 2048) X(2, 8, 4096) X(3, 4, 2048) X(4, 2, 4096) X(5, 8, 2048)
 
 hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices, int mode,
-                                    float* out, SSEmUpdateArgs em, hipStream_t stream) {
+                                    float* out, SSEmUpdateArgs em, float* scratch, hipStream_t stream) {
   if (j1 <= j0 && mode == SS_BP_EM_UPDATE) return hipSuccess;
   const int n = p.g.n;
   const int th = 8 * kSSBpRows[p.bp_variant];
-  dim3 grid((n + BP_TILE_W - 1) / BP_TILE_W, (n + th - 1) / th, nslices);
-#define SS_BP_LAUNCH(V, ROWS, STAGE)                                                                              \
-  case V:                                                                                                         \
-    if (mode == SS_BP_PLAIN)                                                                                      \
-      hipLaunchKernelGGL((k_backprojection<SS_BP_PLAIN, ROWS, STAGE>), grid, dim3(256), 0, stream, p, sinoT, j0, \
-                         j1, out, em);                                                                            \
-    else                                                                                                          \
-      hipLaunchKernelGGL((k_backprojection<SS_BP_EM_UPDATE, ROWS, STAGE>), grid, dim3(256), 0, stream, p, sinoT, \
-                         j0, j1, out, em);                                                                        \
+  const int G = scratch ? ss_bp_groups(p, j0, j1, nslices) : 1;
+  const int kmode = G > 1 ? SS_BP_PARTIAL : mode;
+  float* kout = G > 1 ? scratch : out;
+  dim3 grid((n + BP_TILE_W - 1) / BP_TILE_W, (n + th - 1) / th, nslices * G);
+#define SS_BP_LAUNCH(V, ROWS, STAGE)                                                                               \
+  case V:                                                                                                          \
+    if (kmode == SS_BP_PLAIN)                                                                                      \
+      hipLaunchKernelGGL((k_backprojection<SS_BP_PLAIN, ROWS, STAGE>), grid, dim3(256), 0, stream, p, sinoT, j0,  \
+                         j1, 1, kout, em);                                                                         \
+    else if (kmode == SS_BP_PARTIAL)                                                                               \
+      hipLaunchKernelGGL((k_backprojection<SS_BP_PARTIAL, ROWS, STAGE>), grid, dim3(256), 0, stream, p, sinoT, j0, \
+                         j1, G, kout, em);                                                                         \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_backprojection<SS_BP_EM_UPDATE, ROWS, STAGE>), grid, dim3(256), 0, stream, p, sinoT,  \
+                         j0, j1, 1, kout, em);                                                                     \
     break;
   switch (p.bp_variant) {
     SS_BP_FOR_VARIANTS(SS_BP_LAUNCH)
@@ -1031,6 +1095,16 @@ hipError_t ss_launch_backprojection(const SSDevPlan& p, const float* sinoT, int 
       return hipErrorInvalidValue;
   }
 #undef SS_BP_LAUNCH
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || G == 1) return e;
+  const size_t per = (size_t)n * n;
+  dim3 fin((unsigned)((per + 255) / 256), nslices);
+  if (mode == SS_BP_PLAIN)
+    hipLaunchKernelGGL(k_bp_groups_finish<SS_BP_PLAIN>, fin, dim3(256), 0, stream, scratch, G, n, (float)p.g.dth, out,
+                       em);
+  else
+    hipLaunchKernelGGL(k_bp_groups_finish<SS_BP_EM_UPDATE>, fin, dim3(256), 0, stream, scratch, G, n,
+                       (float)p.g.dth, out, em);
   return hipGetLastError();
 }
 

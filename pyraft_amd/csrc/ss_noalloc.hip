@@ -45,7 +45,10 @@ extern "C" int raft_ss_radon_noalloc(const raft_ss_plan* p, const float* d_image
   if (rc) return rc;
   if ((rc = check_range(p, j0, j1))) return rc;
   if (!d_image || !d_out) return ss_set_error(RAFT_SS_EINVAL, "null device pointer");
-  const SSDevPlan dp = p->dev();
+  SSDevPlan dp = p->dev();
+  // the plan's slab choice when an earlier call made one (this entry never times)
+  hipStream_t stream = as_stream(stream_);
+  if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, false, &dp.radon_slab_rows))) return rc;
   const size_t layout = (size_t)nslices * p->g.n * p->g.pitch;
   const size_t layout_bytes = 2 * layout * sizeof(float2);
   const size_t need = layout_bytes + ss_radon_scratch(dp, j0, j1, nslices);
@@ -56,7 +59,6 @@ extern "C" int raft_ss_radon_noalloc(const raft_ss_plan* p, const float* d_image
                         nslices, p->g.n, p->g.n, need, (size_t)RAFT_SS_NOALLOC_SCRATCH_BYTES);
   unsigned char* base = nullptr;
   if ((rc = scratch_address(p->device, &base))) return rc;
-  hipStream_t stream = as_stream(stream_);
   float2* gA = reinterpret_cast<float2*>(base);
   float2* gB = gA + layout;
   float* slabs = reinterpret_cast<float*>(base + layout_bytes);

@@ -21,8 +21,35 @@ DEFAULT_IMG_CORNERS = (-1.0, 1.0, 1.0, -1.0)
 DEFAULT_SINO_CORNERS = (0.0, 1.0, math.pi, -1.0)
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def current_stream_handle(device=None):
+    """hipStream_t of torch's current stream on `device` (default: the current device)."""
+    if _raw_stream is not None:
+        idx = None
+        if device is not None:
+            idx = torch.device(device).index if not isinstance(device, int) else device
+        return ctypes.c_void_p(_raw_stream(torch.cuda.current_device() if idx is None else idx))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class _on_device:
+    """torch.cuda.device(dev) that skips the switch when dev is already current."""
+    __slots__ = ("index", "prev")
+
+    def __init__(self, dev):
+        self.index = dev.index
+
+    def __enter__(self):
+        self.prev = torch.cuda.current_device()
+        if self.prev != self.index:
+            torch.cuda.set_device(self.index)
+
+    def __exit__(self, *exc):
+        if self.prev != self.index:
+            torch.cuda.set_device(self.prev)
+        return False
 
 
 def _require_gpu():
@@ -84,7 +111,7 @@ class Plan:
         self._handle = ctypes.c_void_p()
         ic = (ctypes.c_double * 4)(*self.img_corners)
         sc = (ctypes.c_double * 4)(*self.sino_corners)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_plan_create(ctypes.byref(self._handle), self.n, self.nrays, self.nangles,
                                               ic, sc, current_stream_handle()), "raft_ss_plan_create")
 
@@ -107,7 +134,7 @@ class Plan:
         self._handle = ctypes.c_void_p()
         ic = (ctypes.c_double * 4)(*self.img_corners)
         dp = ctypes.POINTER(ctypes.c_double)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_plan_create_mesh(ctypes.byref(self._handle), self.n, ic, self.nrays,
                                                    ts.ctypes.data_as(dp), self.nangles, ths.ctypes.data_as(dp),
                                                    current_stream_handle()), "raft_ss_plan_create_mesh")
@@ -132,21 +159,26 @@ class Plan:
         return self._handle
 
     def info(self):
-        out = (ctypes.c_int * 8)()
-        check(libraft.raft_ss_plan_info(self._handle, out, 8), "raft_ss_plan_info")
+        out = (ctypes.c_int * 9)()
+        check(libraft.raft_ss_plan_info(self._handle, out, 9), "raft_ss_plan_info")
         keys = ("n", "nrays", "nangles", "irregular_rays", "degenerate_angles", "bp_window", "bp_angles",
-                "bp_variant")
+                "bp_variant", "bp_groups")
         return dict(zip(keys, list(out)))
 
     def set_bp_variant(self, variant):
         """Use backprojection kernel variant ``variant`` (see ``bp_variant_shape``)."""
         check(libraft.raft_ss_plan_set_bp_variant(self._handle, int(variant)), "raft_ss_plan_set_bp_variant")
 
+    def set_radon_slab_rows(self, rows):
+        """Pin the sample rows per slab of this plan's divided Radon launches (rows >= n:
+        no division; 0: back to the tuned choice)."""
+        check(libraft.raft_ss_plan_set_radon_slab_rows(self._handle, int(rows)), "raft_ss_plan_set_radon_slab_rows")
+
     def tune(self, reps=3, force=False):
         """Time every backprojection variant on this geometry, keep the fastest.
         Returns (variant, {variant: microseconds}) (variants that do not fit are left out)."""
         times = (ctypes.c_double * BP_NVARIANTS)()
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             v = libraft.raft_ss_plan_tune(self._handle, int(reps), TUNE_FORCE if force else 0, times, self._stream())
         check(v if v < 0 else 0, "raft_ss_plan_tune")
         return v, {i: times[i] for i in range(BP_NVARIANTS) if times[i] >= 0}
@@ -190,7 +222,7 @@ class Plan:
             out = (torch.empty if full else torch.zeros)(shape, dtype=torch.float32, device=x.device)
         else:
             _check_out(out, shape, x.device, "out")
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_radon_range(self._handle, x.data_ptr(), out.data_ptr(), int(angle_major), j0, j1,
                                               _nslices(x, 2), self._stream()), "raft_ss_radon_range")
         return out
@@ -201,7 +233,7 @@ class Plan:
         s = _check_tensor(sino, tail, "sinogram")
         out = torch.empty(s.shape[:-2] + (self.n, self.n), dtype=torch.float32, device=s.device)
         j0, j1 = self._range(angles)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_backprojection_range(self._handle, s.data_ptr(), out.data_ptr(), int(angle_major),
                                                        j0, j1, _nslices(s, 2), self._stream()),
                   "raft_ss_backprojection_range")
@@ -213,7 +245,7 @@ class Plan:
         s = _check_tensor(sino, tail, "sinogram")
         out = torch.empty(s.shape[:-2] + (self.n, self.n), dtype=torch.float32, device=s.device)
         j0, j1 = self._range(angles)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_radon_transpose_range(self._handle, s.data_ptr(), out.data_ptr(), int(angle_major),
                                                         j0, j1, _nslices(s, 2), self._stream()),
                   "raft_ss_radon_transpose_range")
@@ -223,7 +255,7 @@ class Plan:
         tail = (self.nangles, self.nrays) if angle_major else (self.nrays, self.nangles)
         s = _check_tensor(sino, tail, "sinogram")
         out = torch.empty_like(s)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_fbp_filter(self._handle, s.data_ptr(), out.data_ptr(), int(angle_major),
                                              _nslices(s, 2), self._stream()), "raft_ss_fbp_filter")
         return out
@@ -231,7 +263,7 @@ class Plan:
     def fbp(self, sino):
         s = _check_tensor(sino, (self.nrays, self.nangles), "sinogram")
         out = torch.empty(s.shape[:-2] + (self.n, self.n), dtype=torch.float32, device=s.device)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_fbp(self._handle, s.data_ptr(), out.data_ptr(), _nslices(s, 2), self._stream()),
                   "raft_ss_fbp")
         return out
@@ -245,7 +277,7 @@ class Plan:
         else:
             x = _check_tensor(x0, (self.n, self.n), "x0").clone()
         ll = (ctypes.c_double * max(1, int(niter)))()
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_ss_em(self._handle, s.data_ptr(), x.data_ptr(), int(niter), int(nsubsets),
                                      _nslices(s, 2), ll if return_ll else None, self._stream()), "raft_ss_em")
         if return_ll:
@@ -320,7 +352,7 @@ class DdaPlan:
         self._handle = ctypes.c_void_p()
         ic = (ctypes.c_double * 4)(*self.img_corners)
         sc = (ctypes.c_double * 4)(*self.sino_corners)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_dda_plan_create(ctypes.byref(self._handle), self.shape[0], self.shape[1], self.nrays,
                                                self.nangles, ic, sc, current_stream_handle()), "raft_dda_plan_create")
 
@@ -345,7 +377,7 @@ class DdaPlan:
         """(..., lines, cols) -> (..., R, A)."""
         x = _check_tensor(image, self.shape, "image")
         out = torch.empty(x.shape[:-2] + (self.nrays, self.nangles), dtype=torch.float32, device=x.device)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_dda_radon(self._handle, x.data_ptr(), out.data_ptr(), _nslices(x, 2), self._stream()),
                   "raft_dda_radon")
         return out
@@ -354,7 +386,7 @@ class DdaPlan:
         """(..., R, A) -> (..., lines, cols), the transpose of ``radon``."""
         s = _check_tensor(sino, (self.nrays, self.nangles), "sinogram")
         out = torch.empty(s.shape[:-2] + self.shape, dtype=torch.float32, device=s.device)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_dda_radon_transpose(self._handle, s.data_ptr(), out.data_ptr(), _nslices(s, 2),
                                                    self._stream()), "raft_dda_radon_transpose")
         return out
@@ -363,7 +395,7 @@ class DdaPlan:
         """(..., R, A) -> (..., lines, cols): R^T sino / R^T 1 (NaN where no ray passes)."""
         s = _check_tensor(sino, (self.nrays, self.nangles), "sinogram")
         out = torch.empty(s.shape[:-2] + self.shape, dtype=torch.float32, device=s.device)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_dda_backprojection(self._handle, s.data_ptr(), out.data_ptr(), _nslices(s, 2),
                                                   self._stream()), "raft_dda_backprojection")
         return out
@@ -371,6 +403,6 @@ class DdaPlan:
     def weights(self):
         """R^T 1, the backprojection's normalisation, (lines, cols)."""
         out = torch.empty(self.shape, dtype=torch.float32, device=self.device)
-        with torch.cuda.device(self.device):
+        with _on_device(self.device):
             check(libraft.raft_dda_weights(self._handle, out.data_ptr(), self._stream()), "raft_dda_weights")
         return out

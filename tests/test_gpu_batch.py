@@ -4,7 +4,8 @@
   autotuner keeps a variant that fits;
 * a batch of slices with different n, R, A and corners gives each slice the result
   of its own per-plan call: bit-identical backprojection (same variant, same
-  device function), Radon within 1e-6 (the batched launch may split sample ranges
+  device function; within 1e-6 where the one-slice call divides its angles into
+  groups to fill the GPU), Radon within 1e-6 (the batched launch may split sample ranges
   differently) and the oracle within 1e-5;
 * batched OSEM matches per-slice OSEM, log-likelihood included;
 * the graph-captured form is bit-identical to the direct form on every replay, with
@@ -131,7 +132,10 @@ def test_batch_backprojection_bit_identical(torch_mod):
         p = Plan(n, R, A, ic, sc)
         p.set_bp_variant(b.info(i)["bp_variant"])
         own = p.backprojection(torch.from_numpy(sinos[i]).cuda()).cpu().numpy()
-        assert np.array_equal(got, own), i
+        if p.info()["bp_groups"] == 1:
+            assert np.array_equal(got, own), i
+        else:  # the one-slice launch divides its angles into groups: other FP32 summation order
+            assert rel_l2(got, own) <= 1e-6, i
         ref = O.backprojection(sinos[i].astype(np.float64), n, **_geo(ic, sc))
         assert rel_l2(got, ref) <= REL_L2, i
 

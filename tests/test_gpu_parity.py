@@ -267,3 +267,35 @@ def test_transpose_slice_batches(torch_mod, nslices):
         one = p.radon_transpose(y[i]).cpu().numpy()
         assert np.array_equal(got[i], one), i
     assert rel_l2(got[-1], O.radon_transpose(y[-1].cpu().numpy().astype(np.float64), n)) <= REL_L2
+
+
+@pytest.mark.parametrize("n,R,A,ns", [(256, 256, 180, 1), (100, 103, 61, 3), (64, 64, 200, 2)])
+def test_small_image_angle_groups(torch_mod, n, R, A, ns):
+    """Small images divide a backprojection's angles into groups (partial images added
+    in group order by a finishing pass) so that the launch fills the GPU: plain
+    backprojection, FBP and EM (weights and multiplicative update) against the oracle,
+    several slices per launch, every slice as its own one-slice call."""
+    torch = torch_mod
+    p = _plan(n, R, A)
+    print("angle groups n=%d R=%d A=%d slices %d: %d groups" % (n, R, A, ns, p.info()["bp_groups"]), flush=True)
+    assert p.info()["bp_groups"] > 1
+    rng = np.random.default_rng(n + A)
+    s = rng.random((ns, R, A)).astype(np.float32)
+    st = torch.from_numpy(s).cuda()
+    got = p.backprojection(st).cpu().numpy()
+    fbp = p.fbp(st).cpu().numpy()
+    for i in range(ns):
+        s64 = s[i].astype(np.float64)
+        assert rel_l2(got[i], O.backprojection(s64, n)) <= REL_L2, i
+        assert rel_l2(fbp[i], O.fbp(s64, n)) <= REL_L2, i
+        one = p.backprojection(st[i]).cpu().numpy()
+        assert rel_l2(one, got[i]) <= 1e-6, i
+    x = (0.5 + rng.random((n, n))).astype(np.float32)
+    meas = p.radon(torch.from_numpy(x).cuda()) * 20.0
+    em, ll = p.em(meas, 2, 4, return_ll=True)
+    ref, llref = O.osem(meas.cpu().numpy().astype(np.float64), n, 2, 4)
+    e = rel_l2(em.cpu().numpy(), ref)
+    print("  em rel-L2 %.2e" % e, flush=True)
+    assert e <= REL_L2
+    for a, b in zip(ll, llref):
+        assert abs(a - b) <= 1e-6 * abs(b)

@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 
 REL_L2 = 1e-5
 
-# n, R, A, slices: 12 blocks of (8 angles x 64 rays) per launch, so every launch is divided
-# (96 rows: 3 slabs of 32; 200 rows: 7 slabs, the last of 8 rows)
+# n, R, A, slices: launches of few blocks of (8 angles x 64 rays), with slabs pinned at 32
+# rows (96 rows: 3 slabs; 200 rows: 7 slabs, the last of 8 rows)
 CASES = [(96, 100, 9, 3), (200, 150, 16, 2), (130, 70, 5, 4)]
 
 
@@ -42,6 +42,7 @@ def test_radon_row_slabs_several_slices(torch_mod, case):
     rng = np.random.default_rng(n + R + A)
     x = rng.random((ns, n, n)).astype(np.float32)
     p = Plan(n, R, A)
+    p.set_radon_slab_rows(32)  # the divided form whatever the tuner would pick
     xd = torch.from_numpy(x).cuda()
     got = p.radon(xd).cpu().numpy()
     got_t = p.radon(xd, angle_major=True).cpu().numpy()
@@ -65,6 +66,7 @@ def test_osem_row_slabs_several_slices(torch_mod, case):
     x = rng.random((ns, n, n))
     s = np.stack([O.radon(x[i], R, A) for i in range(ns)]) + 0.5
     p = Plan(n, R, A)
+    p.set_radon_slab_rows(32)
     got, ll = p.em(torch.from_numpy(s.astype(np.float32)).cuda(), 3, nsubsets=3, return_ll=True)
     got = got.cpu().numpy()
     ll_ref = np.zeros(3)

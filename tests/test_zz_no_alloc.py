@@ -42,8 +42,10 @@ def test_radon_noalloc_matches_allocating_entry():
         shape = (ns, A, R) if am else (ns, R, A)
         got = torch.full(shape, -1.0, device="cuda")
         ref = torch.full(shape, -1.0, device="cuda")
-        assert _call(noalloc, plan, x, got, am, j0, j1, ns) == 0, libraft.raft_ss_last_error()
+        # the allocating entry first: it makes the plan's launch-shape choice, which the
+        # allocation-free entry then reuses
         assert _call(alloc, plan, x, ref, am, j0, j1, ns) == 0, libraft.raft_ss_last_error()
+        assert _call(noalloc, plan, x, got, am, j0, j1, ns) == 0, libraft.raft_ss_last_error()
         torch.cuda.synchronize()
         assert torch.equal(got, ref)
         cols = got[:, j0:j1] if am else got[..., j0:j1]
