@@ -505,6 +505,7 @@ def main():
     print("bench: plan n=%d R=%d A=%d" % (n, R, A), file=sys.stderr, flush=True)
     plan = Plan(n, R, A)
     bp_variant, bp_variant_us = plan.tune(reps=3)   # autotuned backprojection tile / LDS stage
+    rt_variant, rt_variant_us = plan.tune_transpose(nslices=1, reps=3)  # and the exact transpose's
     from pyraft_amd.device import bp_variant_shape
     bp_rows, bp_stage = bp_variant_shape(bp_variant)
     img = phantom(random_ellipses(32, 2), n)
@@ -593,6 +594,7 @@ def main():
         "exact_transpose_ms": round(transpose_ms, 4),
         "bp_variant": {"chosen": bp_variant, "rows_per_thread": bp_rows, "stage_pairs": bp_stage,
                        "tuned_us": {str(k): round(v, 1) for k, v in bp_variant_us.items()}},
+        "transpose_variant": {"chosen": rt_variant, "tuned_us": {str(k): round(v, 1) for k, v in rt_variant_us.items()}},
         "roofline": roofline,
     }
 
@@ -665,6 +667,7 @@ def main():
     print("bench: autograd 64 x 512^2 x 720", file=sys.stderr, flush=True)
     from pyraft_amd.autograd import radon as ad_radon
     aplan = Plan(512, 512, 720)
+    aplan.tune_transpose(nslices=64, reps=2)   # the backward's tile shape for 64-slice launches
     xa = torch.rand((64, 512, 512), device=dev, requires_grad=True)
     ya = torch.rand((64, 512, 720), device=dev)
     for _ in range(2):

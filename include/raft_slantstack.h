@@ -171,6 +171,12 @@ int raft_ss_plan_host_info(int n, int nrays, int nangles, const double *img_corn
  * window does not fit variant v); arrays of RAFT_SS_BP_NVARIANTS ints */
 int raft_ss_plan_host_bp_staging(int n, int nrays, int nangles, const double *img_corners,
                                  const double *sino_corners, int *windows, int *per_stage);
+/* pixel-driven exact transpose staging computed on the host: windows[v], per_stage[v] as
+ * above (0 angles when some angle's rays are closer than one cell: the line form runs),
+ * and per angle (table, nangles x 5 doubles) its transpose constants rt_ig, rt_a0, rt_b,
+ * branch (1: samples walk image columns) and ray spacing |gamma| in cells */
+int raft_ss_plan_host_rt_staging(int n, int nrays, int nangles, const double *img_corners,
+                                 const double *sino_corners, int *windows, int *per_stage, double *table);
 
 /* ================================================================ device-pointer operators */
 
@@ -272,6 +278,14 @@ int raft_ss_plan_tune(raft_ss_plan *plan, int reps, int flags, double *times_us,
  * returns to the tuned choice.  Slab sizes change the FP32 summation order (~1e-7
  * relative); pinning one makes the bits reproducible. */
 int raft_ss_plan_set_radon_slab_rows(raft_ss_plan *plan, int rows);
+/* The exact transpose runs pixel tiles of the backprojection's shapes (variant v: 32
+ * columns x 8*rows[v] rows, stage[v] staged pairs) when every angle's rays are at least
+ * one cell apart, else an image-line form.  This times the runnable tile variants on
+ * `nslices` slices over the full angle range, keeps the fastest for the plan and returns
+ * it (-1 and no timing when the line form applies); times_us as raft_ss_plan_tune.
+ * Cached with the other choices. */
+int raft_ss_plan_tune_transpose(raft_ss_plan *plan, int nslices, int reps, int flags, double *times_us,
+                                void *stream);
 
 /* ================================================================ batches of uneven slices
  *

@@ -174,8 +174,34 @@ static void bp_staging(const std::vector<SSAngle>& angles, int v, int* window, i
   *per_stage = W <= kSSBpStage[v] ? std::min(64, kSSBpStage[v] / W) : 0;
 }
 
+// Pixel-driven transpose staging for tile variant v: rays a tile can reach at one angle
+// (kappa per column and per row from the angle's transpose constants, + margins for the
+// floor of the tile minimum, the second candidate and FP32 rounding) and angles per stage;
+// 0 angles when some angle's rays are closer than one cell (the form needs >= 1).
+static void rt_staging(const std::vector<SSAngle>& angles, int v, int* window, int* per_stage) {
+  const double th = 8.0 * kSSBpRows[v] - 1.0;
+  double span = 0.0;
+  for (const SSAngle& a : angles) {
+    if (!(fabs((double)a.gamma) >= SS_FIX_ONE)) {
+      *window = 0;
+      *per_stage = 0;
+      return;
+    }
+    const double kb = a.branch_a ? a.rt_b : a.rt_ig, kc = a.branch_a ? a.rt_ig : a.rt_b;
+    span = std::max(span, 31.0 * fabs(kb) + th * fabs(kc));
+  }
+  int W = (int)ceil(span) + 5;
+  W = (W + 3) & ~3;
+  *window = W;
+  *per_stage = W <= kSSBpStage[v] ? std::min(64, kSSBpStage[v] / W) : 0;
+}
+
 // all variants' staging; the default variant, or the first runnable one
 static int bp_variants(raft_ss_plan* p) {
+  p->rt_variant = -1;
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v) rt_staging(p->h_angles, v, &p->rt_window_v[v], &p->rt_angles_v[v]);
+  for (int v : {SS_RT_DEFAULT_VARIANT, 0, 1, 2, 4, 5})
+    if (p->rt_variant < 0 && p->rt_angles_v[v] > 0) p->rt_variant = v;
   int first = -1;
   for (int v = 0; v < SS_BP_NVARIANTS; ++v) {
     bp_staging(p->h_angles, v, &p->bp_window_v[v], &p->bp_angles_v[v]);
@@ -432,6 +458,31 @@ extern "C" int raft_ss_plan_host_bp_staging(int n, int nrays, int nangles, const
   int rc = validate_geom(g, angles);
   if (rc) return rc;
   for (int v = 0; v < SS_BP_NVARIANTS; ++v) bp_staging(angles, v, &windows[v], &per_stage[v]);
+  return 0;
+}
+
+extern "C" int raft_ss_plan_host_rt_staging(int n, int nrays, int nangles, const double* img_corners,
+                                            const double* sino_corners, int* windows, int* per_stage,
+                                            double* table) {
+  if (!img_corners || !sino_corners || !windows || !per_stage) return set_error(RAFT_SS_EINVAL, "null argument");
+  SSGeom g = make_geom(n, nrays, nangles, img_corners, sino_corners);
+  std::vector<SSAngle> angles;
+  int ndeg = 0;
+  if (n < 2 || nrays < 2 || nangles < 1) return validate_geom(g, angles);
+  build_angles(g, angles, &ndeg);
+  int rc = validate_geom(g, angles);
+  if (rc) return rc;
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v) rt_staging(angles, v, &windows[v], &per_stage[v]);
+  if (table)
+    for (int j = 0; j < nangles; ++j) {
+      const SSAngle& a = angles[j];
+      double* t = table + 5 * (size_t)j;
+      t[0] = a.rt_ig;
+      t[1] = a.rt_a0;
+      t[2] = a.rt_b;
+      t[3] = a.branch_a;
+      t[4] = fabs((double)a.gamma) / SS_FIX_ONE;
+    }
   return 0;
 }
 

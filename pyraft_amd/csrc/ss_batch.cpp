@@ -265,6 +265,54 @@ extern "C" int raft_ss_plan_tune(raft_ss_plan* p, int reps, int flags, double* t
   return best;
 }
 
+extern "C" int raft_ss_plan_tune_transpose(raft_ss_plan* p, int nslices, int reps, int flags, double* times_us,
+                                           void* stream_) {
+  int rc = check_plan(p, std::max(nslices, 1));
+  if (rc) return rc;
+  if ((rc = check_uniform(p, "plan_tune_transpose"))) return rc;
+  if (nslices < 1) return ss_set_error(RAFT_SS_EINVAL, "nslices must be >= 1");
+  if (times_us)
+    for (int v = 0; v < SS_BP_NVARIANTS; ++v) times_us[v] = -1.0;
+  if (p->rt_variant < 0) return -1;  // line form: nothing to tune
+  hipStream_t stream = as_stream(stream_);
+  const std::string key = plan_key(p) + "|transpose|s" + std::to_string(nslices);
+  TuneEntry cached;
+  if (!(flags & RAFT_SS_TUNE_FORCE) && tune_find(key, &cached) && p->rt_angles_v[cached.variant] > 0) {
+    p->rt_variant = cached.variant;
+    if (times_us) std::copy(cached.times, cached.times + SS_BP_NVARIANTS, times_us);
+    return p->rt_variant;
+  }
+  const int n = p->g.n, R = p->g.nrays, A = p->g.nangles;
+  DevBuf ws;
+  HIP_TRY(ws.alloc(sizeof(float) * (size_t)nslices * ((size_t)R * A + (size_t)n * n), stream));
+  float* sinoT = ws.as<float>();
+  float* img = sinoT + (size_t)nslices * R * A;
+  HIP_TRY(ss_launch_fill(sinoT, 1.0f, (size_t)nslices * R * A, stream));
+  TuneEntry te;
+  int best = -1;
+  const int keep = p->rt_variant;
+  for (int v = 0; v < SS_BP_NVARIANTS; ++v) {
+    te.times[v] = -1.0;
+    if (p->rt_angles_v[v] <= 0) continue;
+    p->rt_variant = v;
+    const SSDevPlan dp = p->dev();
+    auto run = [&]() -> int {
+      HIP_TRY(ss_launch_radon_transpose(dp, sinoT, 0, A, nslices, img, nullptr, stream));
+      return 0;
+    };
+    rc = time_best(run, reps, stream, &te.times[v]);
+    p->rt_variant = keep;
+    if (rc) return rc;
+    if (best < 0 || te.times[v] < te.times[best]) best = v;
+  }
+  if (best < 0) return ss_set_error(RAFT_SS_EINTERNAL, "no runnable transpose variant");
+  te.variant = best;
+  p->rt_variant = best;
+  if (times_us) std::copy(te.times, te.times + SS_BP_NVARIANTS, times_us);
+  tune_store(key, te);
+  return best;
+}
+
 int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, int* variant) {
   *variant = p->bp_variant;
   if (p->bp_variant_fixed || j1 <= j0) return 0;

@@ -68,6 +68,9 @@ struct raft_ss_plan {
   bool bp_variant_fixed = false;  // set by raft_ss_plan_set_bp_variant: no per-range tuning
   int bp_window_v[SS_BP_NVARIANTS] = {0};  // per backprojection variant: staged window
   int bp_angles_v[SS_BP_NVARIANTS] = {0};  //   and angles per stage (0: not runnable)
+  int rt_variant = -1;                     // pixel-driven transpose: tile variant (-1: line form)
+  int rt_window_v[SS_BP_NVARIANTS] = {0};  //   its staged window and angles per stage per variant
+  int rt_angles_v[SS_BP_NVARIANTS] = {0};
   float* d_ramp = nullptr;  // FBP convolution kernel, built on first use
   void* d_ramp_fft = nullptr;  // its FFT-form spectrum and twiddles (if R fits the FFT form)
   SSFftTaps ramp_fft{};
@@ -92,6 +95,9 @@ struct raft_ss_plan {
     p.bp_window = bp_window_v[v];
     p.bp_angles = bp_angles_v[v];
     p.radon_slab_rows = 0;
+    p.rt_variant = rt_variant >= 0 ? rt_variant : 0;
+    p.rt_window = rt_variant >= 0 ? rt_window_v[rt_variant] : 0;
+    p.rt_angles = rt_variant >= 0 ? rt_angles_v[rt_variant] : 0;
     return p;
   }
   ~raft_ss_plan() {

@@ -27,6 +27,7 @@ enum { SS_BP_PLAIN = 0, SS_BP_EM_UPDATE = 1, SS_BP_PARTIAL = 2 };
 // min(64, stage / window) angles.
 #define SS_BP_NVARIANTS 6
 #define SS_BP_DEFAULT_VARIANT 0
+#define SS_RT_DEFAULT_VARIANT 3  // pixel-driven transpose tile (4 rows per thread, 2048 staged pairs)
 static const int kSSBpRows[SS_BP_NVARIANTS] = {4, 2, 8, 4, 2, 8};
 static const int kSSBpStage[SS_BP_NVARIANTS] = {4096, 2048, 4096, 2048, 4096, 2048};
 
@@ -44,6 +45,10 @@ struct SSDevPlan {
   int bp_window;           // rays staged per angle per pixel tile (for bp_variant)
   int bp_angles;           // angles per LDS stage (for bp_variant)
   int radon_slab_rows;     // Radon: sample rows per slab (n: no division; 0: the default rule)
+  // exact transpose, pixel-driven form: tile variant (the backprojection's shapes), rays
+  // staged per angle and angles per stage; rt_angles = 0: not available (rays closer than
+  // one cell on some angle), the line form runs instead
+  int rt_variant, rt_window, rt_angles;
 };
 
 // ---- batched execution over slices of different geometry

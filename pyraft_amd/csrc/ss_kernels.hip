@@ -6,6 +6,8 @@
 // backprojection and in the EM log-likelihood sums.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 
 #include "ss_kernels.h"
@@ -1488,6 +1490,257 @@ __global__ __launch_bounds__(256) void k_radon_transpose_fixup(SSDevPlan p, cons
   }
 }
 
+// =====================================================================================
+// exact transpose, pixel-driven (rays at least one cell apart on every line)
+//
+// The forward adds (1-w) F[m] + w F[m+1] for a valid sample whose Q32.32 position on its
+// line is u = m + w, so pixel p of line s receives Delta q_k max(0, 1 - |u_k(s) - p|)
+// from ray k: a hat one cell wide.  On line s the rays sit at u_k = (alpha0 + k gamma +
+// s beta) 2^-32, so the pixel lies at the real ray index
+//   kappa = (p 2^32 - alpha0 - s beta) / gamma = p rt_ig - rt_a0 - s rt_b
+// and ray k contributes Delta q_k max(0, 1 - g |k - kappa|), g = |gamma| 2^-32 the ray
+// spacing in cells.  With g >= 1 only k0 = floor(kappa) and k0 + 1 can contribute:
+//   w0 = clamp(1 - g f),  w1 = clamp(1 - g + g f),  f = kappa - k0.
+// kappa is affine in the pixel's column and row (branch B: p = column, s = row; branch A:
+// p = row, s = column), so a pixel tile evaluates it as the backprojection evaluates its
+// detector coordinate: per angle of an LDS stage the tile origin's kappa in FP64 relative
+// to a staged base ray, FP32 offsets per pixel, and staged (Delta q_k, Delta q_{k+1})
+// pairs.  Both branches add into the same pixels in one pass.
+//
+// Validity.  A sample is valid when its FP64 predicate coordinate x lies in [0, n-1); on
+// a regular angle x is within ~(k + s) 2^-33 cells of the Q32.32 trajectory, so every
+// sample whose cell lies in [1, n-3] is valid and pixels p in [2, n-3] need no test.  In
+// tiles holding pixels p < 2 or p > n-3 a regular angle tests, at those pixels, each
+// candidate's position on its line, u = p - f gs or p + (1 - f) gs (gs the signed
+// spacing), which the FP32
+// evaluation gives within ~2e-5 cells: u in [eps, n-1-eps) is valid, u < -eps or
+// u >= n-1+eps is not, and only u within eps = 1e-3 of 0 or n-1 reads the candidate's
+// exact range (s in [lo_k, hi_k]).  An angle that is not regular (degenerate, or with
+// irregular rays) reads the exact range of every candidate; irregular rays have empty
+// ranges here and the fixup kernel adds them.  The weights are FP32 evaluations of the
+// forward's (~1e-5 cells).
+// =====================================================================================
+
+#ifndef RTP_UNROLL
+#define RTP_UNROLL 2  // angles per unrolled step of the pixel-driven transpose
+#endif
+
+struct RTPAngle {
+  float f0, b, c, g;
+  int flag;  // 0: no candidate ray on the detector, 1: no validity test, 2: test by position
+             // (regular angle, tile at a line end), 3: exact ranges (angle not regular)
+  int base;
+  int branch_a;
+  float gs;  // signed ray spacing in cells (gamma 2^-32)
+};
+
+// SL = 2: the block takes two slices of a launch and shares each pixel's geometry (kappa,
+// floor, fraction, weights) between them; a staged record then holds both slices' pairs.
+template <int STAGE, int SL>
+struct SSRtpLds {
+  typedef typename std::conditional<SL == 2, float4, float2>::type Rec;
+  Rec win[SL == 2 ? STAGE / 2 : STAGE];  // per ray: (Delta q_k, Delta q_{k+1}) of each slice
+  RTPAngle par[64];
+};
+
+template <int ROWS, int STAGE, int SL>
+__device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __restrict__ sl0,
+                                            const float* __restrict__ sl1, int j0, int j1, int c0, int r0,
+                                            float* __restrict__ out0, float* __restrict__ out1,
+                                            SSRtpLds<STAGE, SL>& lds) {
+  typedef typename SSRtpLds<STAGE, SL>::Rec Rec;
+  const int n = p.g.n, R = p.g.nrays;
+  const int W = p.rt_window;
+  const int J = SL == 2 ? p.rt_angles / 2 : p.rt_angles;  // the stage holds half as many records
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tx = tid & 31, ty = tid >> 5;
+  const int col = c0 + tx;
+  constexpr int TH = 8 * ROWS;
+  // two accumulators per pixel and slice (the two candidates' terms) keep the FMA chains short
+  float acc0[SL][ROWS], acc1[SL][ROWS];
+  float fty[ROWS];
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    fty[i] = (float)(ty + 8 * i);
+#pragma unroll
+    for (int z = 0; z < SL; ++z) acc0[z][i] = acc1[z][i] = 0.f;
+  }
+  const float ftx = (float)tx;
+  // tile pixels along each branch's cell axis: columns (branch B), rows (branch A)
+  const bool inner_b = c0 >= 2 && c0 + BP_TILE_W - 1 <= n - 3;
+  const bool inner_a = r0 >= 2 && r0 + TH - 1 <= n - 3;
+
+  for (int jb = j0; jb < j1; jb += J) {
+    const int nj = min(J, j1 - jb);
+    RTPAngle q{0.f, 0.f, 0.f, 0.f, 0, 0, 0, 0.f};
+    float delta = 0.f;
+    if (lane < nj) {
+      const SSAngle& a = p.angles[jb + lane];
+      const double kb = a.branch_a ? -a.rt_b : a.rt_ig;  // kappa per column
+      const double kc = a.branch_a ? a.rt_ig : -a.rt_b;  // kappa per row
+      const double K = -a.rt_a0 + (double)c0 * kb + (double)r0 * kc;
+      const double eb = (double)(BP_TILE_W - 1) * kb, ec = (double)(TH - 1) * kc;
+      const double kmin = K + fmin(0.0, eb) + fmin(0.0, ec);
+      const double kmax = K + fmax(0.0, eb) + fmax(0.0, ec);
+      const double base = floor(kmin) - 1.0;
+      q.f0 = (float)(K - base);
+      q.b = (float)kb;
+      q.c = (float)kc;
+      q.g = (float)(fabs((double)a.gamma) * (1.0 / SS_FIX_ONE));
+      q.gs = (float)((double)a.gamma * (1.0 / SS_FIX_ONE));
+      q.base = (int)base;
+      q.branch_a = a.branch_a;
+      delta = a.delta;
+      if (kmax < -1.0 || kmin >= (double)R)
+        q.flag = 0;
+      else if (!a.regular)
+        q.flag = 3;
+      else if (a.branch_a ? inner_a : inner_b)
+        q.flag = 1;
+      else
+        q.flag = 2;
+    }
+    const bool checked = __ballot(lane < nj && q.flag >= 2) != 0ull;
+    __syncthreads();  // the previous stage is done with win / par
+    if (wv == 0) {
+      lds.par[lane].f0 = q.f0;
+      lds.par[lane].b = q.b;
+      lds.par[lane].c = q.c;
+      lds.par[lane].g = q.g;
+      lds.par[lane].flag = q.flag;
+      lds.par[lane].base = q.base;
+      lds.par[lane].branch_a = q.branch_a;
+      lds.par[lane].gs = q.gs;
+    }
+    // staging: wave wv fills the windows of angles wv, wv+4, ...; kappa - base >= 1 for every
+    // pixel, so the truncated offset is the floor and stays below W - 1
+    for (int jj = wv; jj < nj; jj += 4) {
+      const int fl = __builtin_amdgcn_readlane(q.flag, jj);
+      const int bs = __builtin_amdgcn_readlane(q.base, jj);
+      const float dl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), jj));
+      const float* row0 = sl0 + (size_t)(jb + jj) * R;
+      const float* row1 = SL == 2 ? sl1 + (size_t)(jb + jj) * R : nullptr;
+      for (int w = lane; w < W; w += 64) {
+        const int ray = bs + w;
+        const bool in0 = fl != 0 && (unsigned)ray < (unsigned)R, in1 = fl != 0 && (unsigned)(ray + 1) < (unsigned)R;
+        const float a0 = in0 ? row0[ray] * dl : 0.f, a1 = in1 ? row0[ray + 1] * dl : 0.f;
+        if constexpr (SL == 2) {
+          const float b0 = in0 ? row1[ray] * dl : 0.f, b1 = in1 ? row1[ray + 1] * dl : 0.f;
+          reinterpret_cast<float4*>(lds.win)[jj * W + w] = make_float4(a0, a1, b0, b1);
+        } else {
+          reinterpret_cast<float2*>(lds.win)[jj * W + w] = make_float2(a0, a1);
+        }
+      }
+    }
+    __syncthreads();
+    if (!checked) {
+#pragma unroll RTP_UNROLL
+      for (int jj = 0; jj < nj; ++jj) {
+        const float f0 = lds.par[jj].f0, pb = lds.par[jj].b, pc = lds.par[jj].c, g = lds.par[jj].g;
+        const float g1 = 1.f - g;
+        const Rec* wrow = lds.win + jj * W;
+        const float t0 = fmaf(ftx, pb, f0);
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+          const float kap = fmaf(fty[i], pc, t0);
+          const Rec t = wrow[(int)kap];
+          const float f = __builtin_amdgcn_fractf(kap);
+          const float w0 = __builtin_amdgcn_fmed3f(fmaf(-g, f, 1.f), 0.f, 1.f);
+          const float w1 = __builtin_amdgcn_fmed3f(fmaf(g, f, g1), 0.f, 1.f);
+          acc0[0][i] = fmaf(w0, t.x, acc0[0][i]);
+          acc1[0][i] = fmaf(w1, t.y, acc1[0][i]);
+          if constexpr (SL == 2) {
+            acc0[1][i] = fmaf(w0, t.z, acc0[1][i]);
+            acc1[1][i] = fmaf(w1, t.w, acc1[1][i]);
+          }
+        }
+      }
+    } else {
+      for (int jj = 0; jj < nj; ++jj) {
+        const int flag = __builtin_amdgcn_readfirstlane(lds.par[jj].flag);
+        if (flag == 0) continue;
+        const float f0 = lds.par[jj].f0, pb = lds.par[jj].b, pc = lds.par[jj].c, g = lds.par[jj].g;
+        const float g1 = 1.f - g;
+        const Rec* wrow = lds.win + jj * W;
+        const float t0 = fmaf(ftx, pb, f0);
+        const int base = lds.par[jj].base;
+        const int branch_a = __builtin_amdgcn_readfirstlane(lds.par[jj].branch_a);
+        const float gs = lds.par[jj].gs;
+        const int2* rng = p.bounds + (size_t)(jb + jj) * R;
+        const float top = (float)(n - 1);
+        constexpr float eps = 1e-3f;
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+          const float kap = fmaf(fty[i], pc, t0);
+          const int kr = (int)kap;
+          const Rec t = wrow[kr];
+          const float f = __builtin_amdgcn_fractf(kap);
+          float w0 = __builtin_amdgcn_fmed3f(fmaf(-g, f, 1.f), 0.f, 1.f);
+          float w1 = __builtin_amdgcn_fmed3f(fmaf(g, f, g1), 0.f, 1.f);
+          const int row = r0 + ty + 8 * i;
+          const int pix = branch_a ? row : col;  // the pixel's cell on its sample line
+          if (flag == 3 || (flag == 2 && (pix <= 1 || pix >= n - 2))) {
+            const int sline = branch_a ? col : row;  // the candidates' sample line
+            const float pcell = (float)pix;
+            const int k0 = base + kr;
+            int keep;
+            if (flag == 2) {
+              // position on the line decides, except within eps of either end
+              const float u0 = fmaf(-f, gs, pcell), u1 = u0 + gs;
+              keep = ((u0 >= eps && u0 < top - eps) ? 1 : 0) | ((u1 >= eps && u1 < top - eps) ? 2 : 0);
+              const bool amb0 = fabsf(u0) < eps || fabsf(u0 - top) < eps;
+              const bool amb1 = fabsf(u1) < eps || fabsf(u1 - top) < eps;
+              if (amb0 && (unsigned)k0 < (unsigned)R) {
+                const int2 b0 = rng[k0];
+                keep |= (sline >= b0.x && sline <= b0.y) ? 1 : 0;
+              }
+              if (amb1 && (unsigned)(k0 + 1) < (unsigned)R) {
+                const int2 b1 = rng[k0 + 1];
+                keep |= (sline >= b1.x && sline <= b1.y) ? 2 : 0;
+              }
+            } else {
+              // the candidates' exact ranges
+              int2 b0 = make_int2(0, -1), b1 = make_int2(0, -1);
+              if ((unsigned)k0 < (unsigned)R) b0 = rng[k0];
+              if ((unsigned)(k0 + 1) < (unsigned)R) b1 = rng[k0 + 1];
+              keep = ((sline >= b0.x && sline <= b0.y) ? 1 : 0) | ((sline >= b1.x && sline <= b1.y) ? 2 : 0);
+            }
+            w0 = (keep & 1) ? w0 : 0.f;
+            w1 = (keep & 2) ? w1 : 0.f;
+          }
+          acc0[0][i] = fmaf(w0, t.x, acc0[0][i]);
+          acc1[0][i] = fmaf(w1, t.y, acc1[0][i]);
+          if constexpr (SL == 2) {
+            acc0[1][i] = fmaf(w0, t.z, acc0[1][i]);
+            acc1[1][i] = fmaf(w1, t.w, acc1[1][i]);
+          }
+        }
+      }
+    }
+  }
+  if (col >= n) return;
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    const int row = r0 + ty + 8 * i;
+    if (row >= n) continue;
+    out0[(size_t)row * n + col] = acc0[0][i] + acc1[0][i];
+    if constexpr (SL == 2) out1[(size_t)row * n + col] = acc0[1][i] + acc1[1][i];
+  }
+}
+
+// slices slice0 + blockIdx.z * SL .. + SL - 1
+template <int ROWS, int STAGE, int SL>
+__global__ __launch_bounds__(256) void k_radon_transpose_px(SSDevPlan p, const float* __restrict__ qT, int j0, int j1,
+                                                            int slice0, float* __restrict__ out) {
+  __shared__ SSRtpLds<STAGE, SL> lds;
+  const size_t slice = (size_t)slice0 + (size_t)blockIdx.z * SL;
+  const size_t sino = (size_t)p.g.nangles * p.g.nrays, img = (size_t)p.g.n * p.g.n;
+  ss_rtp_item<ROWS, STAGE, SL>(p, qT + slice * sino, qT + (slice + SL - 1) * sino, j0, j1, blockIdx.x * BP_TILE_W,
+                               blockIdx.y * 8 * ROWS, out + slice * img, out + (slice + SL - 1) * img, lds);
+}
+
 // LDS a block may use on the current device (gfx950: 160 KiB, also assumed if the query
 // fails)
 static size_t ss_lds_per_block() {
@@ -1508,12 +1761,47 @@ static int ss_rt_lines(int n, int sl) {
 }
 
 size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices) {
-  return sizeof(float) * (size_t)nslices * p.g.n * p.g.n;
+  return p.rt_angles > 0 ? 0 : sizeof(float) * (size_t)nslices * p.g.n * p.g.n;
 }
 
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, float* scratch, hipStream_t stream) {
   const int n = p.g.n;
+  if (p.rt_angles > 0) {  // the pixel-driven form (every angle's rays at least one cell apart)
+    const int th = 8 * kSSBpRows[p.rt_variant];
+    // slices in pairs (a pair shares each pixel's geometry) when a pair's stage holds at
+    // least 8 angles, the last odd slice alone
+    const int pairs = (p.rt_angles / 2 >= 8) ? nslices / 2 : 0;
+    const int singles = nslices - 2 * pairs;
+    for (int part = 0; part < 2; ++part) {
+      const int count = part == 0 ? pairs : singles;
+      if (count == 0) continue;
+      dim3 grid((n + BP_TILE_W - 1) / BP_TILE_W, (n + th - 1) / th, count);
+      const int slice0 = part == 0 ? 0 : 2 * pairs;
+#define SS_RTP_LAUNCH(V, ROWS, STAGE)                                                                            \
+  case V:                                                                                                        \
+    if (part == 0)                                                                                               \
+      hipLaunchKernelGGL((k_radon_transpose_px<ROWS, STAGE, 2>), grid, dim3(256), 0, stream, p, sinoT, j0, j1, \
+                         slice0, out);                                                                           \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_radon_transpose_px<ROWS, STAGE, 1>), grid, dim3(256), 0, stream, p, sinoT, j0, j1, \
+                         slice0, out);                                                                           \
+    break;
+      switch (p.rt_variant) {
+        SS_BP_FOR_VARIANTS(SS_RTP_LAUNCH)
+        default:
+          return hipErrorInvalidValue;
+      }
+#undef SS_RTP_LAUNCH
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || p.nirregular == 0) return e;
+    hipLaunchKernelGGL(k_radon_transpose_fixup, dim3(p.nirregular, nslices), dim3(256), 0, stream, p, sinoT, j0, j1,
+                       out);
+    return hipGetLastError();
+  }
   // several slices per wave where the image is small enough for their lines to leave
   // room for many waves per CU: four for n <= 512, two for n <= 1024 (8 KiB per wave);
   // the remaining slices go in pairs, then alone

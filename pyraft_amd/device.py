@@ -183,6 +183,17 @@ class Plan:
         check(v if v < 0 else 0, "raft_ss_plan_tune")
         return v, {i: times[i] for i in range(BP_NVARIANTS) if times[i] >= 0}
 
+    def tune_transpose(self, nslices=1, reps=3, force=False):
+        """Time the exact transpose's tile variants on `nslices` slices, keep the fastest.
+        Returns (variant, {variant: microseconds}); variant -1 when the line form applies."""
+        times = (ctypes.c_double * BP_NVARIANTS)()
+        with _on_device(self.device):
+            v = libraft.raft_ss_plan_tune_transpose(self._handle, int(nslices), int(reps),
+                                                    TUNE_FORCE if force else 0, times, self._stream())
+        if v < -1:
+            check(v, "raft_ss_plan_tune_transpose")
+        return v, {i: times[i] for i in range(BP_NVARIANTS) if times[i] >= 0}
+
     def bounds(self):
         import numpy
         b = numpy.zeros((self.nangles, self.nrays, 2), numpy.int32)

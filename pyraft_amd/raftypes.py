@@ -172,9 +172,11 @@ _decl("raft_ss_filter_kernel", [_i, ctypes.c_double, ctypes.c_double, _i, ctypes
 _decl("raft_ss_filter1d_ramp_taps", [_i, ctypes.c_double, ctypes.c_double, _c_float_p], _i)
 _decl("raft_ss_filter_rows_toeplitz", [_vp, _vp, _vp, _i, _i, _vp], _i)
 _decl("raft_ss_plan_host_bp_staging", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i)
+_decl("raft_ss_plan_host_rt_staging", [_i, _i, _i, _d4, _d4, ctypes.POINTER(_i), ctypes.POINTER(_i), _vp], _i)
 _decl("raft_ss_bp_variant_shape", [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i)
 _decl("raft_ss_plan_set_bp_variant", [_vp, _i], _i)
 _decl("raft_ss_plan_set_radon_slab_rows", [_vp, _i], _i)
+_decl("raft_ss_plan_tune_transpose", [_vp, _i, _i, _i, _d4, _vp], _i)
 _decl("raft_ss_plan_tune", [_vp, _i, _i, _d4, _vp], _i)
 _decl("raft_ss_batch_create", [ctypes.POINTER(_vp), _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
                                _d4, _d4, _vp], _i)

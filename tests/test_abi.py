@@ -175,6 +175,66 @@ def test_backprojection_window_in_bounds(lib, n, R, A, variant):
     assert lo >= 0 and hi <= W - 1, (lo, hi, W)
 
 
+def _rt_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC, variant=3):
+    """Emulate the pixel-driven transpose's FP32 window index floor(kappa - base) for the
+    corner pixels of every tile of `variant` and every angle; return the extremes and the
+    window W the library stages (None when the form does not apply)."""
+    Wv = (ctypes.c_int * len(BP_ROWS))()
+    Jv = (ctypes.c_int * len(BP_ROWS))()
+    table = np.zeros((A, 5))
+    assert lib.raft_ss_plan_host_rt_staging(n, R, A, (ctypes.c_double * 4)(*ic), (ctypes.c_double * 4)(*sc),
+                                            Wv, Jv, table.ctypes.data) == 0
+    W, J = Wv[variant], Jv[variant]
+    if J == 0:
+        return None
+    assert 1 <= J <= 64 and J * W <= BP_STAGE[variant]
+    assert np.all(table[:, 4] >= 1.0)
+    tile_h = 8 * BP_ROWS[variant]
+    ig, a0, rb, br = table[:, 0], table[:, 1], table[:, 2], table[:, 3] != 0
+    kb = np.where(br, -rb, ig)  # kappa per column (the kernel's expressions)
+    kc = np.where(br, ig, -rb)  # kappa per row
+    lo, hi = np.inf, -np.inf
+    f32 = np.float32
+    for c0 in range(0, n, 32):
+        for r0 in range(0, n, tile_h):
+            K = -a0 + c0 * kb + r0 * kc
+            kmin = K + np.minimum(0, 31 * kb) + np.minimum(0, (tile_h - 1) * kc)
+            base = np.floor(kmin) - 1
+            f0 = (K - base).astype(f32)
+            for tx in (0, 31):
+                t0 = (np.float64(tx) * kb.astype(f32).astype(np.float64) + f0).astype(f32)
+                for ty in (0, tile_h - 1):
+                    k = (np.float64(ty) * kc.astype(f32).astype(np.float64) + t0).astype(f32)
+                    ii = np.floor(k)
+                    lo, hi = min(lo, ii.min()), max(hi, ii.max())
+    return lo, hi, W
+
+
+@pytest.mark.parametrize("variant", [3, 0, 1])
+@pytest.mark.parametrize("n,R,A,ic,sc", [(64, 64, 45, DEF_IC, DEF_SC), (65, 61, 37, DEF_IC, DEF_SC),
+                                         (512, 512, 720, DEF_IC, DEF_SC), (2048, 2048, 1500, DEF_IC, DEF_SC),
+                                         (100, 40, 50, DEF_IC, DEF_SC),
+                                         (48, 40, 91, (-0.7, 1.3, 1.2, -0.9), (-math.pi / 2, 1.1, math.pi / 2, -0.8))])
+def test_transpose_window_in_bounds(lib, n, R, A, ic, sc, variant):
+    """The pixel-driven transpose reads the staged pair at floor(kappa - base): inside the
+    window, and kappa - base >= 0 so truncation equals the floor."""
+    r = _rt_window_extremes(lib, n, R, A, ic, sc, variant=variant)
+    if r is None:
+        pytest.skip("rays closer than a cell, or the window does not fit this variant")
+    lo, hi, W = r
+    assert lo >= 0 and hi <= W - 1, (lo, hi, W)
+
+
+def test_transpose_form_needs_rays_a_cell_apart(lib):
+    """Detectors with bins narrower than the pixels (rays closer than one cell) keep the
+    line form: no pixel-driven staging."""
+    Wv = (ctypes.c_int * len(BP_ROWS))()
+    Jv = (ctypes.c_int * len(BP_ROWS))()
+    assert lib.raft_ss_plan_host_rt_staging(64, 150, 30, (ctypes.c_double * 4)(*DEF_IC),
+                                            (ctypes.c_double * 4)(*DEF_SC), Wv, Jv, None) == 0
+    assert all(Jv[v] == 0 for v in range(len(BP_ROWS)))
+
+
 @pytest.mark.parametrize("R", [64, 65, 256, 367])
 def test_ramp_kernel_matches_numpy(lib, R):
     g = np.zeros(R, np.float32)

@@ -254,12 +254,15 @@ def test_transpose_wide_image(torch_mod):
 
 
 @pytest.mark.parametrize("nslices", [2, 3, 5])
-def test_transpose_slice_batches(torch_mod, nslices):
-    """Slices of a batch go two per wave (an odd last one alone): every slice equals the
-    single-slice transpose bit for bit (the same operations in the same order)."""
+@pytest.mark.parametrize("R", [100, 90])
+def test_transpose_slice_batches(torch_mod, nslices, R):
+    """Slices of a batch go two at a time (an odd last one alone): every slice equals the
+    single-slice transpose bit for bit (the same operations in the same order).  R = 100 >
+    n puts rays closer than a pixel (the line form, two slices per wave); R = 90 keeps
+    them at least a pixel apart (the pixel form, two slices per block)."""
     torch = torch_mod
-    n, R, A = 96, 100, 37
-    print("transpose slices", nslices, flush=True)
+    n, A = 96, 37
+    print("transpose slices", nslices, "R", R, flush=True)
     p = _plan(n, R, A)
     y = torch.from_numpy(np.random.default_rng(nslices).random((nslices, R, A)).astype(np.float32)).cuda()
     got = p.radon_transpose(y).cpu().numpy()
