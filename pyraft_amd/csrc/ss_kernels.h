@@ -102,11 +102,14 @@ struct SSEmRatioArgs {
 // every (angle block, ray group) into slabs taken by separate blocks, which write partial
 // ray sums; a finishing pass adds them.  ss_radon_slab_plan: slab count (1: no division)
 // and rows per slab of an n x n image in a launch of `blocks` blocks.
-#define SS_RADON_SLAB_BLOCKS 1024
+#ifndef RADON_WAVES
+#define RADON_WAVES 8  // angles (one per wave) of a Radon block
+#endif
+#define SS_RADON_SLAB_BLOCKS (8192 / RADON_WAVES)
 int ss_radon_slab_plan(int n, long long blocks, int* rows);
 // blocks of the Radon of angles [j0, j1) on an image with `nwaves` ray groups
 static inline long long ss_radon_blocks(int nwaves, int j0, int j1) {
-  return (long long)nwaves * ((j1 - j0 + 7) / 8);
+  return (long long)nwaves * ((j1 - j0 + RADON_WAVES - 1) / RADON_WAVES);
 }
 // Slabs of a Radon launch over angles [j0, j1) of nslices slices: p.radon_slab_rows when
 // set (the plan's tuned choice), else the default rule; returns the slab count (1: not

@@ -283,7 +283,6 @@ __device__ __forceinline__ void ss_radon_store(const SSRadonView& p, int j, int 
 // A block is 8 waves: 8 consecutive angles over the same 64-ray group, one angle per wave.
 // Launches too small to fill the chip (few angles, e.g. one OSEM subset) divide the sample
 // rows into slabs taken by separate blocks instead (ss_radon_slab_plan).
-#define RADON_WAVES 8
 
 typedef float ss_f32x2 __attribute__((ext_vector_type(2)));
 #define SS_RSRC_DWORD3 0x00020000  // gfx9 buffer word 3: 32-bit data format, no swizzle
@@ -300,13 +299,24 @@ __device__ __forceinline__ ss_f32x2 ss_pair(__amdgpu_buffer_rsrc_t rs, uint32_t 
   return ss_struct_load_f32x2(rs, rec, 0, (int)row_off, 0);
 }
 
+#ifndef RADON_LDS_CAP
 #define RADON_LDS_CAP 4096  // staged (tap, difference) records per block (32 KiB)
-#define RADON_CHUNK 32      // image rows (gather rows) per staged chunk, at most
+#endif
+#ifndef RADON_CHUNK
+#define RADON_CHUNK 32  // image rows (gather rows) per staged chunk, at most
+#endif
+#ifndef RADON_ROW_ROUND
+#define RADON_ROW_ROUND 8  // chunk rows rounded down to a multiple of this where the window allows
+#endif
+
+#define RADON_TABLE SS_WAVE  // chunks per window table: one (chunk, wave) pair per thread
 
 struct SSRadonLds {
   double red[RADON_WAVES];
-  int wmeta[RADON_WAVES][4];  // per wave: branch (-1: no samples), first row, last row
-  int wwin[RADON_WAVES][2];   // per wave: lowest and highest cell of the current chunk
+  int64_t wpar[RADON_WAVES][3];  // per wave: lowest and highest trajectory offset of its rays, slope
+  int wmeta[RADON_WAVES][4];     // per wave: branch (-1: no samples), first row, last row
+  int wred[2][RADON_WAVES];      // per wave: widest window of a table pass (two passes in turn)
+  int2 table[RADON_TABLE];       // per chunk of the super-chunk: first staged record, records per row
   __attribute__((aligned(16))) float2 stage[RADON_LDS_CAP + 2];  // chunk window; record CAP stays (0, 0)
 };
 
@@ -322,8 +332,8 @@ __device__ __forceinline__ int64_t ss_rfl64(int64_t v) {
 }
 __device__ __forceinline__ float ss_rflf(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
-// Sample readers: (row s, trajectory u) -> (lower tap, difference).
-// Global: u is the Q32.32 trajectory plus one cell, its high word the record index.
+// Direct sample reader: (row s, trajectory u) -> (lower tap, difference), where u is
+// the Q32.32 trajectory plus one cell, its high word the record index.
 struct SSGatherReader {
   __amdgpu_buffer_rsrc_t rs;
   uint32_t row_bytes;
@@ -332,11 +342,54 @@ struct SSGatherReader {
     return make_float2(t.x, t.y);
   }
 };
-// LDS: u carries the row offset of the staged window in its high word already.
-struct SSStageReader {
-  const float2* st;
-  __device__ __forceinline__ float2 operator()(int, int64_t u) const { return st[(int)(u >> 32)]; }
-};
+
+// G consecutive samples s, s+1, ... of one lane's ray from the staged window, all G
+// reads issued before the first is used.  With CHECK, a sample at or beyond `end` or
+// outside the lane's own range [lo, lo + span] reads record RADON_LDS_CAP, which holds
+// (0, 0), instead of branching: it adds exactly zero to both sums.  u advances G steps.
+template <int G, bool CHECK>
+__device__ __forceinline__ void ss_radon_group(const float2* __restrict__ st, int s, int end, int64_t& u,
+                                               int64_t step, int lo, unsigned span, float& acc_a, float& acc_d) {
+  float2 t[G];
+  uint32_t w[G];
+  int idx[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    idx[g] = (int)(u >> 32);
+    if (CHECK) idx[g] = (s + g < end && (unsigned)(s + g - lo) <= span) ? idx[g] : RADON_LDS_CAP;
+    w[g] = (uint32_t)u;
+    u += step;
+    __asm__ volatile("" : "+v"(u));  // one 64-bit add per sample, not offsets re-derived per sample
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) t[g] = st[idx[g]];
+  float sa = t[0].x;
+  if (G == 4) sa = (t[0].x + t[1].x) + (t[2].x + t[3].x);
+  if (G == 8) sa = ((t[0].x + t[1].x) + (t[2].x + t[3].x)) + ((t[4].x + t[5].x) + (t[6].x + t[7].x));
+  acc_a += sa;
+#pragma unroll
+  for (int g = 0; g < G; ++g) acc_d = fmaf((float)w[g], t[g].y, acc_d);
+}
+
+// Samples [sa, sb) of one lane's ray from the staged window (u carries the row offset
+// in its high word): groups of 4 with the range test before zlo (outside the group's
+// common range), unchecked groups of 8 in [zlo, zhi1), and the rest (the last group
+// of the common range and the samples from zhi1 on) again in checked groups of 4.
+__device__ __forceinline__ void ss_radon_rows_staged(const float2* __restrict__ st, int sa, int sb, int zlo,
+                                                     int zhi1, int64_t u, int64_t step, int lo, unsigned span,
+                                                     float& acc_a, float& acc_d) {
+  int s = sa;
+  const int h_end = min(sb, zlo);
+  if (s < h_end) {
+    const int64_t u0 = u;
+    for (; s < h_end; s += 4) ss_radon_group<4, true>(st, s, h_end, u, step, lo, span, acc_a, acc_d);
+    s = h_end;
+    u = u0 + (int64_t)(h_end - sa) * step;
+  }
+  const int m_end = min(sb, zhi1);
+  for (; s + 8 <= m_end; s += 8) ss_radon_group<8, false>(st, s, 0, u, step, lo, span, acc_a, acc_d);
+  for (; s < sb; s += 4) ss_radon_group<4, true>(st, s, sb, u, step, lo, span, acc_a, acc_d);
+}
 
 // Samples [sa, sb) of one lane's ray: predicated before zlo and from zhi1 on (outside
 // the group's common range), unpredicated in between with 4 samples in flight, each
@@ -388,6 +441,51 @@ __device__ __forceinline__ void ss_radon_rows(const Rd& rd, int sa, int sb, int 
     }
     u += beta;
   }
+}
+
+// Window table of `nch` chunks of `rows` sample rows from row sc (the last chunk ends
+// at s_hi at the latest): for each chunk, the gather records [rec0, rec0 + Wc) of each
+// row that the block's waves read in its rows (Wc = 0: none).  Each thread takes one
+// (chunk, wave) pair, bounds the cells of that wave's rays over the chunk's rows at the
+// corners (the trajectory is affine in ray and row; valid samples have cells in
+// [-1, n-1]), and the RADON_WAVES lanes of a chunk take the union.  Returns the widest
+// Wc of the table (block-uniform); `pass` selects the reduction slots, so that a second
+// pass right after the first needs no barrier in between.
+__device__ __forceinline__ int ss_radon_windows(SSRadonLds& lds, int pass, int sc, int rows, int nch, int s_hi,
+                                                int n) {
+  const int t = threadIdx.x;
+  const int c = t / RADON_WAVES, w = t % RADON_WAVES;
+  int cmin = 0x7fffffff, cmax = -0x7fffffff;
+  if (c < nch) {
+    const int s0 = sc + c * rows, s1 = min(s0 + rows, s_hi + 1);
+    const int ra = max(s0, lds.wmeta[w][1]), rb = min(s1 - 1, lds.wmeta[w][2]);
+    if (lds.wmeta[w][0] >= 0 && ra <= rb) {
+      const int64_t beta = lds.wpar[w][2];
+      const int64_t c00 = lds.wpar[w][0] + (int64_t)ra * beta;
+      const int64_t c10 = lds.wpar[w][1] + (int64_t)ra * beta;
+      const int64_t ds = (int64_t)(rb - ra) * beta;
+      const int m0 = (int)(c00 >> 32), m1 = (int)(c10 >> 32);
+      const int m2 = (int)((c00 + ds) >> 32), m3 = (int)((c10 + ds) >> 32);
+      cmin = max(min(min(m0, m1), min(m2, m3)), -1);
+      cmax = min(max(max(m0, m1), max(m2, m3)), n - 1);
+    }
+  }
+  for (int off = 1; off < RADON_WAVES; off <<= 1) {
+    cmin = min(cmin, __shfl_xor(cmin, off));
+    cmax = max(cmax, __shfl_xor(cmax, off));
+  }
+  // cells c_lo..c_hi need records c_lo+1..c_hi+1; rec0 is even so that rows of the
+  // window start 16-byte aligned in the layout (even pitch) and in LDS (even Wc)
+  const int rec0 = (cmin + 1) & ~1;
+  int wc = cmax >= cmin ? ((cmax + 2 - rec0) + 1) & ~1 : 0;
+  if (c >= nch) wc = 0;
+  if (c < nch && w == 0) lds.table[c] = make_int2(rec0, wc);
+  for (int off = 32; off > 0; off >>= 1) wc = max(wc, __shfl_xor(wc, off));
+  if ((t & (SS_WAVE - 1)) == 0) lds.wred[pass][t / SS_WAVE] = wc;
+  __syncthreads();
+  int wmax = 0;
+  for (int i = 0; i < RADON_WAVES; ++i) wmax = max(wmax, lds.wred[pass][i]);
+  return ss_rfl(wmax);
 }
 
 // One block's work: rays [grp*64, grp*64+64) of the RADON_WAVES angles starting at
@@ -445,10 +543,19 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
 
   bool staged = false;
   {  // staged or direct: decided per block
+    // the wave's extreme trajectory offsets (over its rays on the detector)
+    int64_t amin = (k < R) ? a_k : INT64_MAX, amax = (k < R) ? a_k : INT64_MIN;
+    for (int off = 32; off > 0; off >>= 1) {
+      amin = min(amin, (int64_t)__shfl_xor((long long)amin, off));
+      amax = max(amax, (int64_t)__shfl_xor((long long)amax, off));
+    }
     if (lane == 0) {
       lds.wmeta[wv][0] = branch;
       lds.wmeta[wv][1] = wb.x;
       lds.wmeta[wv][2] = wb.y;
+      lds.wpar[wv][0] = amin;
+      lds.wpar[wv][1] = amax;
+      lds.wpar[wv][2] = beta;
     }
     if (threadIdx.x == 0) lds.stage[RADON_LDS_CAP] = make_float2(0.f, 0.f);
     __syncthreads();
@@ -472,85 +579,71 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
           (void*)lay, (short)sizeof(float2), (int)((uint32_t)n * (uint32_t)pitch * 8u), SS_RSRC_DWORD3);
       const SSGatherReader grd{rs, row_bytes};
-      const SSStageReader srd{lds.stage};
-      // the wave's extreme trajectory offsets (over its rays on the detector)
-      int64_t amin = (k < R) ? a_k : INT64_MAX, amax = (k < R) ? a_k : INT64_MIN;
-      for (int off = 32; off > 0; off >>= 1) {
-        amin = min(amin, (int64_t)__shfl_xor((long long)amin, off));
-        amax = max(amax, (int64_t)__shfl_xor((long long)amax, off));
-      }
-      amin = ss_rfl64(amin);
-      amax = ss_rfl64(amax);
-      for (int s0 = s_lo; s0 <= s_hi;) {
-        int s1 = min(s0 + RADON_CHUNK, s_hi + 1);
-        // this wave's cells over its rows of the chunk (valid samples have m in [-1, n-1])
-        const int ra = max(s0, wb.x), rb = min(s1 - 1, wb.y);
-        if (lane == 0) {
-          int cmin = 0x7fffffff, cmax = -0x7fffffff;
-          if (branch >= 0 && ra <= rb) {
-            const int64_t c00 = amin + (int64_t)ra * beta;
-            const int64_t c10 = amax + (int64_t)ra * beta;
-            const int64_t ds = (int64_t)(rb - ra) * beta;
-            const int m0 = (int)(c00 >> 32), m1 = (int)(c10 >> 32);
-            const int m2 = (int)((c00 + ds) >> 32), m3 = (int)((c10 + ds) >> 32);
-            cmin = max(min(min(m0, m1), min(m2, m3)), -1);
-            cmax = min(max(max(m0, m1), max(m2, m3)), n - 1);
-          }
-          lds.wwin[wv][0] = cmin;
-          lds.wwin[wv][1] = cmax;
+      // the pure trajectory of this lane's ray at the first row of the current chunk
+      int64_t t0 = a_k + (int64_t)s_lo * beta;
+      // rows in super-chunks of up to RADON_TABLE chunks: one pass over the table gives
+      // every chunk's window and the widest; chunks of RADON_CHUNK rows unless that
+      // window would not fit, then of the rows it allows (a second pass)
+      for (int sc = s_lo; sc <= s_hi;) {
+        int rows = RADON_CHUNK;
+        int nch = min(RADON_TABLE, (s_hi - sc + rows) / rows);
+        const int wmax = ss_radon_windows(lds, 0, sc, rows, nch, s_hi, n);
+        if (wmax * rows > RADON_LDS_CAP) {
+          int r2 = RADON_LDS_CAP / wmax;
+          if (r2 >= RADON_ROW_ROUND) r2 = r2 / RADON_ROW_ROUND * RADON_ROW_ROUND;
+          rows = max(r2, 1);
+          nch = min(RADON_TABLE, (s_hi - sc + rows) / rows);
+          (void)ss_radon_windows(lds, 1, sc, rows, nch, s_hi, n);
         }
-        __syncthreads();  // windows published; every wave is done with the previous chunk
-        int c_lo = 0x7fffffff, c_hi = -0x7fffffff;
-        for (int w = 0; w < RADON_WAVES; ++w) {
-          c_lo = min(c_lo, lds.wwin[w][0]);
-          c_hi = max(c_hi, lds.wwin[w][1]);
-        }
-        c_lo = ss_rfl(c_lo);
-        c_hi = ss_rfl(c_hi);
-        // the window holds gather records [rec0, rec0 + Wc) of each row: cells
-        // c_lo..c_hi need records c_lo+1..c_hi+1; rec0 is even so that rows of the
-        // window start 16-byte aligned in the layout (even pitch) and in LDS (even Wc)
-        const int rec0 = (c_lo + 1) & ~1;
-        const int Wc = c_hi >= c_lo ? ((c_hi + 2 - rec0) + 1) & ~1 : 0;
-        const int rows = Wc > 0 ? min(s1 - s0, RADON_LDS_CAP / Wc) : 0;
-        if (Wc > 0 && rows > 0) {
-          s1 = s0 + rows;
-          // direct global -> LDS copies, 16 bytes (two records) per lane: wave wv copies
-          // rows wv, wv+8, ...; the last record read is rec0 + Wc - 1 <= c_hi + 2 <= n + 1
-          for (int r = wv; r < rows; r += RADON_WAVES) {
-            const float2* src = lay + (size_t)(s0 + r) * pitch + rec0;
-            float2* dst = lds.stage + r * Wc;
+        const int64_t chunk_step = (int64_t)rows * beta;
+        for (int c = 0; c < nch; ++c) {
+          const int s0 = sc + c * rows, s1 = min(s0 + rows, s_hi + 1);
+          const int2 tw = lds.table[c];
+          const int rec0 = ss_rfl(tw.x), Wc = ss_rfl(tw.y);
+          const bool fits = Wc > 0 && (s1 - s0) * Wc <= RADON_LDS_CAP;  // block-uniform
+          if (c > 0) __syncthreads();  // every wave is done with the previous chunk
+          if (fits) {
+            // direct global -> LDS copies, 16 bytes (two records) per lane: wave wv copies
+            // rows wv, wv+8, ...; the last record read is rec0 + Wc - 1 <= c_hi + 2 <= n + 1
+            const float2* src0 = lay + (size_t)(s0 + wv) * pitch + rec0 + 2 * lane;
             for (int seg = 0; seg < Wc; seg += 2 * SS_WAVE) {
-              if (seg + 2 * lane < Wc)
-                __builtin_amdgcn_global_load_lds((ss_global_cptr)(src + seg + 2 * lane), (ss_lds_ptr)(dst + seg), 16,
-                                                 0, 0);
+              if (seg + 2 * lane < Wc) {
+                const float2* src = src0 + seg;
+                float2* dst = lds.stage + wv * Wc + seg;
+                for (int r = wv; r < s1 - s0; r += RADON_WAVES) {
+                  __builtin_amdgcn_global_load_lds((ss_global_cptr)src, (ss_lds_ptr)dst, 16, 0, 0);
+                  src += (size_t)RADON_WAVES * pitch;
+                  dst += RADON_WAVES * Wc;
+                }
+              }
             }
           }
-        }
-        __syncthreads();  // window staged
-        const int ra2 = max(s0, wb.x), rb2 = min(s1 - 1, wb.y);
-        if (branch >= 0 && ra2 <= rb2) {
-          if (Wc > 0 && rows > 0) {
-            // high word: staged record index (cell + 1 - rec0) + (row - s0) * Wc
-            const int64_t rowstep = (int64_t)Wc << 32;
-            int64_t u = a_k + (int64_t)ra2 * beta + ((int64_t)(1 - rec0) << 32) + (int64_t)(ra2 - s0) * rowstep;
-            int64_t step = beta + rowstep;
-            if (empty) {
-              u = (int64_t)RADON_LDS_CAP << 32;
-              step = 0;
+          __syncthreads();  // window staged
+          const int ra2 = max(s0, wb.x), rb2 = min(s1 - 1, wb.y);
+          if (branch >= 0 && ra2 <= rb2 && Wc > 0) {
+            const int64_t t = t0 + (int64_t)(ra2 - s0) * beta;
+            if (fits) {
+              // high word: staged record index (cell + 1 - rec0) + (row - s0) * Wc
+              int64_t u = t + ((int64_t)(1 - rec0 + (ra2 - s0) * Wc) << 32);
+              int64_t step = beta + ((int64_t)Wc << 32);
+              if (empty) {
+                u = (int64_t)RADON_LDS_CAP << 32;
+                step = 0;
+              }
+              ss_radon_rows_staged(lds.stage, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
+            } else {
+              int64_t u = t + ((int64_t)1 << 32);
+              int64_t step = beta;
+              if (empty) {
+                u = 0;
+                step = 0;
+              }
+              ss_radon_rows(grd, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
             }
-            ss_radon_rows(srd, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
-          } else if (Wc > 0) {
-            int64_t u = a_k + (int64_t)ra2 * beta + ((int64_t)1 << 32);
-            int64_t step = beta;
-            if (empty) {
-              u = 0;
-              step = 0;
-            }
-            ss_radon_rows(grd, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
           }
+          t0 += chunk_step;
         }
-        s0 = s1;
+        sc += nch * rows;
       }
     }
   }
@@ -599,7 +692,7 @@ __device__ __forceinline__ float ss_slab_sum(const float* __restrict__ src, size
 // In SS_OUT_PARTIAL mode the blocks write partial ray sums to part[slice][z][j - j0][k] for
 // k_radon_slab_finish; in the other modes (nslab = 1) they store the output themselves.
 template <int MODE>
-__global__ __launch_bounds__(512) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
+__global__ __launch_bounds__(RADON_WAVES * SS_WAVE) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
                                                const float2* __restrict__ gB, int j0, int j1, int nslab,
                                                int slab_rows, float* __restrict__ out, float* __restrict__ part,
                                                SSEmRatioArgs em) {
@@ -704,7 +797,6 @@ __global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* 
 // slabs' partial sums.  Rows per slab are a multiple of RADON_CHUNK, and the divided
 // launch stays within SS_RADON_SLAB_BLOCKS blocks (4 per CU, one round: a second round
 // for a few blocks would double the launch time).
-static_assert(RADON_WAVES == 8, "ss_radon_blocks counts blocks of 8 angles");
 int ss_radon_slab_plan(int n, long long blocks, int* rows) {
   *rows = n;
   if (blocks >= SS_RADON_SLAB_BLOCKS) return 1;
@@ -2247,7 +2339,7 @@ __global__ __launch_bounds__(256) void k_batch_transpose(SSBatchLaunch L, const 
 // SS_OUT_PARTIAL mode the items write their partial ray sums for k_batch_radon_finish;
 // launches in the other modes are never divided into slabs.
 template <int MODE>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_radon(
+__global__ __launch_bounds__(RADON_WAVES * SS_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_radon(
     SSBatchLaunch L, const float2* __restrict__ gA, const float2* __restrict__ gB, float* __restrict__ part,
     float* __restrict__ out, const float* __restrict__ meas, double* __restrict__ ll) {
   __shared__ SSRadonLds lds;
@@ -2384,9 +2476,9 @@ int ss_batch_grid(int kind, int variant, int mode, int total) {
   int g = 0;
   switch (kind) {
     case SS_KIND_RADON:
-      g = mode == SS_OUT_EM_RATIO  ? ss_resident_blocks(k_batch_radon<SS_OUT_EM_RATIO>, 512)
-          : mode == SS_OUT_PARTIAL ? ss_resident_blocks(k_batch_radon<SS_OUT_PARTIAL>, 512)
-                                   : ss_resident_blocks(k_batch_radon<SS_OUT_SINOT>, 512);
+      g = mode == SS_OUT_EM_RATIO  ? ss_resident_blocks(k_batch_radon<SS_OUT_EM_RATIO>, RADON_WAVES * SS_WAVE)
+          : mode == SS_OUT_PARTIAL ? ss_resident_blocks(k_batch_radon<SS_OUT_PARTIAL>, RADON_WAVES * SS_WAVE)
+                                   : ss_resident_blocks(k_batch_radon<SS_OUT_SINOT>, RADON_WAVES * SS_WAVE);
       break;
     case SS_KIND_FIXUP:
       g = ss_resident_blocks(k_batch_radon_fixup<SS_OUT_EM_RATIO>, 256);
