@@ -711,6 +711,100 @@ extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image,
   return 0;
 }
 
+// Launch geometry of an OSEM run: each subset's Radon with the plan's slab choice for its
+// angle count, the LL partial offsets of the subsets, the subsets' backprojection variant
+// (tuned for their angle count: launches of few angles favour smaller tiles than a
+// full-range backprojection) and the workspace it needs.  Timing choices are made here,
+// so a later call of ss_em_launch (possibly under stream capture) only looks them up.
+int ss_em_prepare(const raft_ss_plan* p, int nsubsets, int nslices, int niter, hipStream_t stream, SSEmLaunch* L) {
+  const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
+  int rc = 0;
+  L->S = nsubsets;
+  L->dp = p->dev();
+  L->poff.assign(nsubsets + 1, 0);
+  L->dps.assign(nsubsets, L->dp);
+  size_t slab_bytes = 0;  // the largest subset's slab partial sums
+  for (int b = 0; b < nsubsets; ++b) {
+    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
+    if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &L->dps[b].radon_slab_rows))) return rc;
+    L->poff[b + 1] = L->poff[b] + ss_radon_partials(L->dps[b], j0, j1, nslices);
+    slab_bytes = std::max(slab_bytes, ss_radon_scratch(L->dps[b], j0, j1, nslices));
+  }
+  int bv = p->bp_variant;
+  if ((rc = ss_plan_range_variant(const_cast<raft_ss_plan*>(p), 0, (int)(A / nsubsets), stream, &bv))) return rc;
+  L->dpb = p->dev(bv);
+  size_t bp_bytes = 0;  // the largest subset's backprojection partial images
+  for (int b = 0; b < nsubsets; ++b) {
+    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
+    bp_bytes = std::max(bp_bytes, ss_bp_scratch(L->dpb, j0, j1, nslices));
+  }
+  // workspace: two gather layouts, the measured and ratio sinograms (detector-major),
+  // the S weight images, the LL partials, the LL trace, slab sums, BP partial images
+  const size_t sino_sz = (size_t)nslices * R * A, img_sz = (size_t)nslices * n * n;
+  const size_t layout = (size_t)nslices * n * p->g.pitch;
+  size_t off = sizeof(float2) * 2 * layout;
+  L->meas_off = off;
+  off += sizeof(float) * sino_sz;
+  L->rho_off = off;
+  off += sizeof(float) * sino_sz;
+  L->weights_off = off;
+  off += sizeof(float) * (size_t)nsubsets * img_sz;
+  off = (off + 7) & ~(size_t)7;
+  L->partials_off = off;
+  off += sizeof(double) * (size_t)L->poff[nsubsets];
+  L->ll_off = off;
+  off += sizeof(double) * (size_t)std::max(niter, 1);
+  L->slabs_off = off;
+  off += slab_bytes;
+  L->bp_off = off;
+  L->bytes = off + bp_bytes;
+  return 0;
+}
+
+// The OSEM kernels of a prepared run on workspace `ws` (L.bytes): W_b = BP_b(1), then
+// per iteration and subset the gather, the Radon with the ratio and LL partials in its
+// epilogue and the backprojection with the update in its epilogue; the LL of iteration
+// it goes to d_ll[it] (device).  Kernel launches only: no allocation, copy or wait.
+int ss_em_launch(const raft_ss_plan* p, const SSEmLaunch& L, const float* d_sino, float* d_x, int niter,
+                 int nslices, void* ws, double* d_ll, hipStream_t stream) {
+  const int R = p->g.nrays, A = p->g.nangles, n = p->g.n, S = L.S;
+  const size_t sino_sz = (size_t)nslices * R * A, img_sz = (size_t)nslices * n * n;
+  const size_t layout = (size_t)nslices * n * p->g.pitch;
+  const int npartials = L.poff[S];
+  char* base = static_cast<char*>(ws);
+  float2* gA = reinterpret_cast<float2*>(base);
+  float2* gB = gA + layout;
+  float* measT = reinterpret_cast<float*>(base + L.meas_off);
+  float* rhoT = reinterpret_cast<float*>(base + L.rho_off);
+  float* weights = reinterpret_cast<float*>(base + L.weights_off);
+  double* partials = reinterpret_cast<double*>(base + L.partials_off);
+  float* slabs = reinterpret_cast<float*>(base + L.slabs_off);
+  float* bp_part = reinterpret_cast<float*>(base + L.bp_off);
+  if (!d_ll) d_ll = reinterpret_cast<double*>(base + L.ll_off);
+
+  HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
+  // W_b = BP_b(1) with the same backprojection kernel
+  HIP_TRY(ss_launch_fill(rhoT, 1.0f, sino_sz, stream));
+  SSEmUpdateArgs none{nullptr, nullptr};
+  for (int b = 0; b < S; ++b) {
+    const int j0 = (int)(((long long)b * A) / S), j1 = (int)(((long long)(b + 1) * A) / S);
+    HIP_TRY(ss_launch_backprojection(L.dpb, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
+                                     bp_part, stream));
+  }
+  for (int it = 0; it < niter; ++it) {
+    for (int b = 0; b < S; ++b) {
+      const int j0 = (int)(((long long)b * A) / S), j1 = (int)(((long long)(b + 1) * A) / S);
+      HIP_TRY(ss_launch_gather_layouts(L.dp, d_x, gA, gB, nslices, stream));
+      SSEmRatioArgs ea{measT, partials + L.poff[b]};
+      HIP_TRY(ss_launch_radon(L.dps[b], gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, slabs, stream));
+      SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
+      HIP_TRY(ss_launch_backprojection(L.dpb, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, bp_part, stream));
+    }
+    HIP_TRY(ss_launch_sum_partials(partials, npartials, d_ll + it, stream));
+  }
+  return 0;
+}
+
 // Ordered-subsets EM.  d_x holds the starting image on entry (the reference starts
 // from ones, raft/raft_em_gpu.cu:41-46) and the result on exit.  Subset b covers
 // angle columns [b*A/S, (b+1)*A/S); S = 1 is the plain EM of raft_em_gpu.
@@ -722,7 +816,7 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
   if ((rc = check_uniform(p, "em"))) return rc;
   if (!d_sino || !d_x) return set_error(RAFT_SS_EINVAL, "null device pointer");
   if (niter < 0) return set_error(RAFT_SS_EINVAL, "niter must be >= 0");
-  const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
+  const int A = p->g.nangles;
   if (nsubsets < 1 || nsubsets > A) return set_error(RAFT_SS_EINVAL, "nsubsets must be in [1, %d]", A);
   hipStream_t stream = as_stream(stream_);
   if (ll_host) {  // the host copy needs a wait, which a stream capture cannot hold
@@ -731,70 +825,14 @@ extern "C" int raft_ss_em(raft_ss_plan* p, const float* d_sino, float* d_x, int 
     if (cs != hipStreamCaptureStatusNone)
       return set_error(RAFT_SS_EINVAL, "em: ll_host must be NULL while the stream is being captured");
   }
-  const SSDevPlan dp = p->dev();
-  const size_t sino_sz = (size_t)nslices * R * A;
-  const size_t img_sz = (size_t)nslices * n * n;
-  const size_t layout = (size_t)nslices * n * p->g.pitch;
-  // LL partials of all subsets of one iteration, reduced once per iteration; each subset's
-  // Radon with the plan's slab choice for its angle count
-  std::vector<int> poff(nsubsets + 1, 0);
-  std::vector<SSDevPlan> dps(nsubsets, dp);
-  size_t slab_bytes = 0;  // the largest subset's slab partial sums
-  for (int b = 0; b < nsubsets; ++b) {
-    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-    if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &dps[b].radon_slab_rows))) return rc;
-    poff[b + 1] = poff[b] + ss_radon_partials(dps[b], j0, j1, nslices);
-    slab_bytes = std::max(slab_bytes, ss_radon_scratch(dps[b], j0, j1, nslices));
-  }
-  const int npartials = poff[nsubsets];
-  // the subsets' backprojection variant, tuned for their angle count (launches of few
-  // angles favour smaller tiles than a full-range backprojection)
-  int bv = p->bp_variant;
-  if ((rc = ss_plan_range_variant(p, 0, (int)(A / nsubsets), stream, &bv))) return rc;
-  const SSDevPlan dpb = p->dev(bv);
-  size_t bp_bytes = 0;  // the largest subset's backprojection partial images
-  for (int b = 0; b < nsubsets; ++b) {
-    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-    bp_bytes = std::max(bp_bytes, ss_bp_scratch(dpb, j0, j1, nslices));
-  }
-  const size_t bytes = sizeof(float2) * 2 * layout + sizeof(float) * (2 * sino_sz + (size_t)nsubsets * img_sz) +
-                       sizeof(double) * ((size_t)npartials + (size_t)std::max(niter, 1) + 8) + slab_bytes +
-                       bp_bytes;
+  SSEmLaunch L;
+  if ((rc = ss_em_prepare(p, nsubsets, nslices, niter, stream, &L))) return rc;
   Scratch ws;
-  HIP_TRY(ws.alloc(bytes, stream));
-  float2* gA = ws.as<float2>();
-  float2* gB = gA + layout;
-  float* measT = reinterpret_cast<float*>(gB + layout);
-  float* rhoT = measT + sino_sz;
-  float* weights = rhoT + sino_sz;
-  double* partials = reinterpret_cast<double*>(weights + (size_t)nsubsets * img_sz);
-  partials = reinterpret_cast<double*>(((uintptr_t)partials + 7) & ~(uintptr_t)7);
-  double* d_ll = partials + npartials;
-  float* slabs = reinterpret_cast<float*>(d_ll + std::max(niter, 1));
-  float* bp_part = slabs + slab_bytes / sizeof(float);
-
-  HIP_TRY(ss_launch_sino_to_t(d_sino, measT, R, A, nslices, stream));
-  // W_b = BP_b(1) with the same backprojection kernel
-  HIP_TRY(ss_launch_fill(rhoT, 1.0f, sino_sz, stream));
-  SSEmUpdateArgs none{nullptr, nullptr};
-  for (int b = 0; b < nsubsets; ++b) {
-    const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-    HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_PLAIN, weights + (size_t)b * img_sz, none,
-                                     bp_part, stream));
-  }
-  for (int it = 0; it < niter; ++it) {
-    for (int b = 0; b < nsubsets; ++b) {
-      const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-      HIP_TRY(ss_launch_gather_layouts(dp, d_x, gA, gB, nslices, stream));
-      SSEmRatioArgs ea{measT, partials + poff[b]};
-      HIP_TRY(ss_launch_radon(dps[b], gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, slabs, stream));
-      SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};
-      HIP_TRY(ss_launch_backprojection(dpb, rhoT, j0, j1, nslices, SS_BP_EM_UPDATE, nullptr, ua, bp_part, stream));
-    }
-    HIP_TRY(ss_launch_sum_partials(partials, npartials, d_ll + it, stream));
-  }
+  HIP_TRY(ws.alloc(L.bytes, stream));
+  if ((rc = ss_em_launch(p, L, d_sino, d_x, niter, nslices, ws.p, nullptr, stream))) return rc;
   if (ll_host && niter > 0) {
-    HIP_TRY(hipMemcpyAsync(ll_host, d_ll, sizeof(double) * niter, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(ll_host, static_cast<char*>(ws.p) + L.ll_off, sizeof(double) * niter,
+                           hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
   }
   return 0;

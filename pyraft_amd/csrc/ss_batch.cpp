@@ -507,6 +507,7 @@ struct raft_ss_batch {
   bool slices_dirty = true;
   std::map<std::string, std::unique_ptr<BatchLaunch>> launches;
   Buf lay, t0, t1, weights, partials, ll, ranges, slab;
+  Buf em_ws;  // a one-slice batch's OSEM workspace (the slice's own kernels)
   int generation = 0;        // bumped whenever a buffer moves or the variants change
   int weights_S = 0;         // subsets the cached BP_b(1) images were computed for
   int weights_generation = -1;
@@ -892,6 +893,30 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
   std::lock_guard<std::mutex> lock(b->mu);
   hipStream_t stream = as_stream(stream_);
   const int S = nsubsets, ns = b->nslices;
+  if (ns == 1) {
+    // One slice leaves nothing to pack into shared launches: the slice's own OSEM
+    // kernels (raft_ss_em's, sized and tuned for its subsets) run instead, captured
+    // into the batch's graph like the batched kernels, with the LL in the batch buffer.
+    const raft_ss_plan* p0 = &b->plan(0);
+    SSEmLaunch L;
+    if ((rc = ss_em_prepare(p0, S, 1, niter, stream, &L)) ||
+        (rc = ensure_buf(b, b->em_ws, std::max<size_t>(L.bytes, 256))) ||
+        (rc = ensure_buf(b, b->ll, sizeof(double) * (size_t)std::max(niter, 1))))
+      return rc;
+    char key1[160];
+    snprintf(key1, sizeof(key1), "em1|%p|%p|%d|%d", (const void*)d_sinos, (void*)d_x, niter, S);
+    double* lld = b->ll.as<double>();
+    void* ws = b->em_ws.p;
+    rc = run_body(b, flags & RAFT_SS_BATCH_GRAPH, key1, stream, [&](hipStream_t st) -> int {
+      return ss_em_launch(p0, L, d_sinos, d_x, niter, 1, ws, lld, st);
+    });
+    if (rc) return rc;
+    if (ll_host && niter > 0) {
+      HIP_TRY(hipMemcpyAsync(ll_host, lld, sizeof(double) * (size_t)niter, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
+    return 0;
+  }
   const std::vector<int> vs = variants_used(b);
 
   // ---- everything the captured part needs, before any capture

@@ -38,6 +38,22 @@ struct raft_ss_plan;
 // the range is empty, or while `stream` is being captured and no earlier call has
 // timed this range (no timing under capture).
 int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, int* variant);
+// An OSEM run split into its choices (ss_em_prepare: slab rows per subset, the subsets'
+// backprojection variant, workspace bytes; may time launches) and its kernels
+// (ss_em_launch: launches only, capturable; LL of iteration it to d_ll[it], or to the
+// workspace when d_ll is null).
+struct SSEmLaunch {
+  int S = 1;
+  SSDevPlan dp{}, dpb{};
+  std::vector<SSDevPlan> dps;  // per subset: the Radon with its slab choice
+  std::vector<int> poff;       // LL partial offsets of the subsets (S + 1)
+  // byte offsets in the workspace (the gather layouts start it) and its size
+  size_t meas_off = 0, rho_off = 0, weights_off = 0, partials_off = 0, ll_off = 0, slabs_off = 0, bp_off = 0;
+  size_t bytes = 0;
+};
+int ss_em_prepare(const raft_ss_plan* p, int nsubsets, int nslices, int niter, hipStream_t stream, SSEmLaunch* L);
+int ss_em_launch(const raft_ss_plan* p, const SSEmLaunch& L, const float* d_sino, float* d_x, int niter,
+                 int nslices, void* ws, double* d_ll, hipStream_t stream);
 // Sample rows per slab of the plan's Radon over angle ranges of j1 - j0 angles on nslices
 // slices (SSDevPlan::radon_slab_rows).  Launches the default rule does not divide keep it
 // (0).  Divided launches take the fastest of the rule's slab size, twice and half of it and

@@ -218,3 +218,31 @@ def test_batch_graph_replays_bit_identical(torch_mod):
         # unrelated eager work between replays
         junk = torch.randn(1 << 20, device="cuda")
         (junk * 2).sum().item()
+
+
+def test_one_slice_batch_em_is_the_plan_em(torch_mod):
+    """A one-slice batch runs the slice's own OSEM kernels: the result and the LL trace
+    equal the plan's OSEM bit for bit, eagerly and in graph replays (with eager work
+    between the replays)."""
+    torch = torch_mod
+    from pyraft_amd.device import Plan
+    n, R, A = 160, 170, 96
+    b = _batch(torch, [(n, R, A, DEF_IC, DEF_SC)])
+    p = Plan(n, R, A)
+    x0 = torch.from_numpy(_images([(n, R, A, DEF_IC, DEF_SC)], 31)[0] + 0.2).cuda()
+    sino = p.radon(x0)
+    packed = b.pack_sinos([sino])
+    own, own_ll = p.em(sino, 3, 4, return_ll=True)
+    print("eager one-slice batch em", flush=True)
+    x, ll = b.em(packed, 3, 4, return_ll=True)
+    assert torch.equal(b.image(x, 0), own)
+    assert [l[0] for l in ll] == own_ll
+    xg = torch.ones_like(x)
+    for rep in range(3):
+        print("graph replay", rep, flush=True)
+        xg.fill_(1.0)
+        gx, gll = b.em(packed, 3, 4, x=xg, return_ll=True, graph=True)
+        assert torch.equal(b.image(gx, 0), own)
+        assert [l[0] for l in gll] == own_ll
+        junk = torch.randn(1 << 18, device="cuda")
+        (junk * 2).sum().item()
