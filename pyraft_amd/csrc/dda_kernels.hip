@@ -39,6 +39,74 @@ __device__ __forceinline__ int dda_wave_start(bool alive, int f, int dfast) {
   return dfast > 0 ? key : -key;
 }
 
+// Exact-geometry angles (Q32 drift): DDA_RADON_U moves per round.  A round first steps
+// every lane's integer state (pixel offsets, liveness, lengths) through its U moves,
+// then issues all 2U loads (the pixel of each move and, where the move ends with a slow
+// step, the neighbour), then adds the round's taps in FP32 (at most 2U terms) and the
+// round's sum to the ray's FP64 total, so a ray of N equal terms does not accumulate
+// FP32's N*eps growth.  Inactive taps read pixel 0 and are dropped by a select.
+#ifndef DDA_RADON_U
+#define DDA_RADON_U 8
+#endif
+
+__device__ __forceinline__ double dda_radon_fixed(const SSDdaAngle& a, const float* __restrict__ src, SSDdaRay ray,
+                                                  bool alive, unsigned frac, int pix, int F, int nf, int ns) {
+  double acc = 0.0;
+  const int fast_step = a.dfast * ns;
+  const float len_full = (float)a.len_full, len_step = (float)a.len_step;
+  float len = (float)ray.len;
+  for (int it = 0; it <= 2 * nf + 1; it += DDA_RADON_U) {
+    if (__ballot(alive) == 0) break;
+    int p0[DDA_RADON_U], p1[DDA_RADON_U];
+    float l0[DDA_RADON_U];
+    bool u0[DDA_RADON_U], u1[DDA_RADON_U];
+#pragma unroll
+    for (int i = 0; i < DDA_RADON_U; ++i) {
+      const bool act = alive && ray.f == F;
+      u0[i] = act;
+      p0[i] = act ? pix : 0;
+      l0[i] = len;
+      // the slow step after this pixel, if the previous move's drift carried
+      const bool sm = act && ray.slow_next;
+      const int s1 = dda_step(ray.s, a.dslow);
+      const bool in1 = dda_inside(ray.f, s1, nf, ns);
+      u1[i] = sm && in1;
+      p1[i] = u1[i] ? pix + a.dslow : 0;
+      if (sm) {
+        ray.s = s1;
+        pix += a.dslow;
+      }
+      const bool live = act && (!sm || in1);
+      if (act && !live) alive = false;
+      if (live) {
+        // fast move: a carry out of the fraction is the next slow step
+        ray.f = dda_step(ray.f, a.dfast);
+        const unsigned prev = frac;
+        frac += a.step_fix;
+        ray.slow_next = frac < prev;
+        len = ray.slow_next ? (float)(0u - prev) * len_step : len_full;
+        pix += fast_step;
+        alive = dda_inside(ray.f, ray.s, nf, ns);
+      }
+      F += a.dfast;
+    }
+    float v0[DDA_RADON_U], v1[DDA_RADON_U];
+#pragma unroll
+    for (int i = 0; i < DDA_RADON_U; ++i) {
+      v0[i] = src[p0[i]];
+      v1[i] = src[p1[i]];
+    }
+    float blk = 0.f;
+#pragma unroll
+    for (int i = 0; i < DDA_RADON_U; ++i) {
+      blk = u0[i] ? fmaf(v0[i], l0[i], blk) : blk;
+      blk = u1[i] ? fmaf(v1[i], len_full - l0[i], blk) : blk;
+    }
+    acc += (double)blk;
+  }
+  return acc;
+}
+
 __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle* __restrict__ angles,
                                                    const float* __restrict__ img, const float* __restrict__ imgT,
                                                    float* __restrict__ sino) {
@@ -52,8 +120,6 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   const float* __restrict__ src = (a.fast_i ? img : imgT) + slice * (size_t)g.lines * g.cols;
   SSDdaRay ray = dda_ray_start(g, a, dda_ray_t(g, lane_ok ? r : 0));
   bool alive = lane_ok && dda_inside(ray.f, ray.s, nf, ns);
-  // Q32 drift (exact-geometry angles): the fraction's carry is the slow move
-  unsigned frac = a.fixed ? (unsigned)fmin(fmax(ray.e * a.inv_cap, 0.0), 4294967295.0) : 0u;
   // FP64 sum: the lengths are FP64 already, so a tap costs one convert and one FMA as
   // in FP32, without FP32's N*eps growth over a ray of N equal terms
   double acc = 0.0;
@@ -61,35 +127,33 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   // the pixel's offset in the (fast, slow) image, stepped with the ray (an image side is
   // at most 16384, so a slice's offsets fit an int); meaningful while the lane is alive
   int pix = alive ? ray.f * ns + ray.s : 0;
-  const int fast_step = a.dfast * ns;
-  // lanes start at most nf moves apart and then take at most nf + 1 moves each
-  for (int it = 0; it <= 2 * nf + 1; ++it) {
-    if (__ballot(alive) == 0) break;
-    if (alive && ray.f == F) {
-      acc = fma((double)src[pix], ray.len, acc);
-      if (ray.slow_next) {
-        ray.s = dda_step(ray.s, a.dslow);
-        pix += a.dslow;
-        ray.len = a.len_full - ray.len;
-        ray.slow_next = 0;
-        alive = dda_inside(ray.f, ray.s, nf, ns);
-        if (alive) acc = fma((double)src[pix], ray.len, acc);
-      }
-      if (alive) {
-        if (a.fixed) {
-          ray.f = dda_step(ray.f, a.dfast);
-          const unsigned prev = frac;
-          frac += a.step_fix;
-          ray.slow_next = frac < prev;  // carry
-          ray.len = ray.slow_next ? (double)(0u - prev) * a.len_step : a.len_full;
-        } else {
-          dda_fast_move(a, ray);
+  if (a.fixed) {
+    // Q32 drift (exact-geometry angles): the fraction's carry is the slow move
+    const unsigned frac = (unsigned)fmin(fmax(ray.e * a.inv_cap, 0.0), 4294967295.0);
+    acc = dda_radon_fixed(a, src, ray, alive, frac, pix, F, nf, ns);
+  } else {
+    const int fast_step = a.dfast * ns;
+    // lanes start at most nf moves apart and then take at most nf + 1 moves each
+    for (int it = 0; it <= 2 * nf + 1; ++it) {
+      if (__ballot(alive) == 0) break;
+      if (alive && ray.f == F) {
+        acc = fma((double)src[pix], ray.len, acc);
+        if (ray.slow_next) {
+          ray.s = dda_step(ray.s, a.dslow);
+          pix += a.dslow;
+          ray.len = a.len_full - ray.len;
+          ray.slow_next = 0;
+          alive = dda_inside(ray.f, ray.s, nf, ns);
+          if (alive) acc = fma((double)src[pix], ray.len, acc);
         }
-        pix += fast_step;
-        alive = dda_inside(ray.f, ray.s, nf, ns);
+        if (alive) {
+          dda_fast_move(a, ray);
+          pix += fast_step;
+          alive = dda_inside(ray.f, ray.s, nf, ns);
+        }
       }
+      F += a.dfast;
     }
-    F += a.dfast;
   }
   if (lane_ok) sino[(slice * g.nrays + r) * g.nangles + ai] = (float)acc;
 }
