@@ -589,7 +589,7 @@ extern "C" int raft_ss_radon_range(const raft_ss_plan* p, const float* d_image, 
   if (!d_image || !d_out) return set_error(RAFT_SS_EINVAL, "null device pointer");
   hipStream_t stream = as_stream(stream_);
   SSDevPlan dp = p->dev();
-  if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &dp.radon_slab_rows))) return rc;
+  if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &dp))) return rc;
   const size_t layout = (size_t)nslices * p->g.n * p->g.pitch;
   Scratch ws;
   HIP_TRY(ws.alloc(2 * layout * sizeof(float2) + ss_radon_scratch(dp, j0, j1, nslices), stream));
@@ -726,7 +726,7 @@ int ss_em_prepare(const raft_ss_plan* p, int nsubsets, int nslices, int niter, h
   size_t slab_bytes = 0;  // the largest subset's slab partial sums
   for (int b = 0; b < nsubsets; ++b) {
     const int j0 = (int)(((long long)b * A) / nsubsets), j1 = (int)(((long long)(b + 1) * A) / nsubsets);
-    if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &L->dps[b].radon_slab_rows))) return rc;
+    if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, true, &L->dps[b]))) return rc;
     L->poff[b + 1] = L->poff[b] + ss_radon_partials(L->dps[b], j0, j1, nslices);
     slab_bytes = std::max(slab_bytes, ss_radon_scratch(L->dps[b], j0, j1, nslices));
   }
@@ -850,7 +850,7 @@ extern "C" int raft_ss_tr_em(raft_ss_plan* p, const float* d_raw, const float* d
   const int R = p->g.nrays, A = p->g.nangles, n = p->g.n;
   hipStream_t stream = as_stream(stream_);
   SSDevPlan dp = p->dev();
-  if ((rc = ss_plan_radon_rows(p, 0, A, nslices, stream, true, &dp.radon_slab_rows))) return rc;
+  if ((rc = ss_plan_radon_rows(p, 0, A, nslices, stream, true, &dp))) return rc;
   const size_t sino_sz = (size_t)nslices * R * A;
   const size_t img_sz = (size_t)nslices * n * n;
   const size_t layout = (size_t)nslices * n * p->g.pitch;

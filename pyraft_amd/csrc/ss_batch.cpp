@@ -121,7 +121,7 @@ std::map<std::string, TuneEntry> g_tune;
 // A file of another format is ignored and a line that does not parse is skipped (that
 // geometry is timed again).  After each new choice the file is written whole to a
 // temporary name and renamed over the old one.  Callers hold g_tune_mu.
-const char* const kTuneFormat = "pyraft_amd tuning cache 2: backprojection variants x6, angle groups";
+const char* const kTuneFormat = "pyraft_amd tuning cache 3: backprojection variants x6, angle groups, radon shapes";
 bool g_tune_loaded = false;
 
 std::string tune_cache_path() {
@@ -358,44 +358,52 @@ int ss_plan_range_variant(raft_ss_plan* p, int j0, int j1, hipStream_t stream, i
 }
 
 int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipStream_t stream, bool may_time,
-                       int* rows) {
-  *rows = 0;
+                       SSDevPlan* out) {
+  out->radon_slab_rows = 0;
+  out->radon_shape = 0;
   const int n = p->g.n;
   if (p->radon_rows_fixed > 0) {  // raft_ss_plan_set_radon_slab_rows
-    *rows = p->radon_rows_fixed;
+    out->radon_slab_rows = p->radon_rows_fixed;
     return 0;
   }
   if (j1 <= j0 || p->mesh) return 0;  // mesh plans keep the rule (their key would not name the tables)
   SSDevPlan dp = p->dev();
   int r0 = n;
-  if (ss_radon_slabs(dp, j0, j1, nslices, &r0) <= 1) return 0;  // the rule does not divide
+  const bool divides = ss_radon_slabs(dp, j0, j1, nslices, &r0) > 1;
   const long long pk = ((long long)(j1 - j0) << 20) | (long long)std::min(nslices, (1 << 20) - 1);
   {
     std::lock_guard<std::mutex> lock(p->radon_mutex);
     auto it = p->radon_rows.find(pk);
     if (it != p->radon_rows.end()) {
-      *rows = it->second;
+      out->radon_slab_rows = it->second & 0xffffff;
+      out->radon_shape = it->second >> 24;
       return 0;
     }
   }
-  if (!may_time) return 0;  // the rule (no lookups that allocate)
-  // candidates: the rule's slab rows, twice and half of them (multiples of 32), no division
-  std::vector<int> cand{r0};
-  if (2 * r0 < n) cand.push_back(2 * r0);
-  if (r0 / 2 >= 32 && (r0 / 2) % 32 == 0) cand.push_back(r0 / 2);
-  cand.push_back(n);
+  if (!may_time) return 0;  // the rule, shape 0 (no lookups that allocate)
+  // candidates (slab rows, shape); rows 0 is the default rule, which does not divide here
+  std::vector<std::pair<int, int>> cand;
+  if (divides) {
+    cand = {{r0, 0}, {r0, 1}};
+    if (2 * r0 < n) cand.push_back({2 * r0, 0});
+    if (r0 / 2 >= RADON_SLAB_ALIGN && (r0 / 2) % RADON_SLAB_ALIGN == 0) cand.push_back({r0 / 2, 0});
+    cand.push_back({n, 0});
+    cand.push_back({n, 1});
+  } else {
+    cand = {{0, 0}, {0, 1}};
+  }
   const std::string key = plan_key(p) + "|radon|range" + std::to_string(j1 - j0) + "|s" + std::to_string(nslices);
   TuneEntry cached;
   int chosen = -1;
-  if (tune_find(key, &cached) && cached.variant >= 0 && cached.variant < (int)cand.size()) chosen = cand[cached.variant];
+  if (tune_find(key, &cached) && cached.variant >= 0 && cached.variant < (int)cand.size()) chosen = cached.variant;
   if (chosen < 0) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIP_TRY(hipStreamIsCapturing(stream, &cs));
     if (cs != hipStreamCaptureStatusNone) return 0;  // the rule, untimed
     const size_t layout = (size_t)nslices * n * p->g.pitch;
     size_t slab_bytes = 0;
-    for (int r : cand) {
-      dp.radon_slab_rows = r;
+    for (const auto& c : cand) {
+      dp.radon_slab_rows = c.first;
       slab_bytes = std::max(slab_bytes, ss_radon_scratch(dp, j0, j1, nslices));
     }
     const size_t img = (size_t)nslices * n * n, sino = (size_t)nslices * p->g.nrays * p->g.nangles;
@@ -404,18 +412,19 @@ int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipSt
     float2* gA = ws.as<float2>();
     float2* gB = gA + layout;
     float* x = reinterpret_cast<float*>(gB + layout);
-    float* out = x + img;
-    float* slabs = out + sino;
+    float* sout = x + img;
+    float* slabs = sout + sino;
     HIP_TRY(ss_launch_fill(x, 1.0f, img, stream));
     HIP_TRY(ss_launch_gather_layouts(dp, x, gA, gB, nslices, stream));
     TuneEntry te;
     for (int v = 0; v < SS_BP_NVARIANTS; ++v) te.times[v] = -1.0;
     int best = 0, rc = 0;
     for (int c = 0; c < (int)cand.size() && c < SS_BP_NVARIANTS; ++c) {
-      dp.radon_slab_rows = cand[c];
+      dp.radon_slab_rows = cand[c].first;
+      dp.radon_shape = cand[c].second;
       SSEmRatioArgs none{nullptr, nullptr};
       auto run = [&]() -> int {
-        HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_SINOT, out, none, slabs, stream));
+        HIP_TRY(ss_launch_radon(dp, gA, gB, j0, j1, nslices, SS_OUT_SINOT, sout, none, slabs, stream));
         return 0;
       };
       if ((rc = time_best(run, 3, stream, &te.times[c]))) return rc;
@@ -423,11 +432,12 @@ int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipSt
     }
     te.variant = best;
     tune_store(key, te);
-    chosen = cand[best];
+    chosen = best;
   }
   std::lock_guard<std::mutex> lock(p->radon_mutex);
-  p->radon_rows[pk] = chosen;
-  *rows = chosen;
+  p->radon_rows[pk] = cand[chosen].first | (cand[chosen].second << 24);
+  out->radon_slab_rows = cand[chosen].first;
+  out->radon_shape = cand[chosen].second;
   return 0;
 }
 

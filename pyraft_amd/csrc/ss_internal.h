@@ -54,15 +54,17 @@ struct SSEmLaunch {
 int ss_em_prepare(const raft_ss_plan* p, int nsubsets, int nslices, int niter, hipStream_t stream, SSEmLaunch* L);
 int ss_em_launch(const raft_ss_plan* p, const SSEmLaunch& L, const float* d_sino, float* d_x, int niter,
                  int nslices, void* ws, double* d_ll, hipStream_t stream);
-// Sample rows per slab of the plan's Radon over angle ranges of j1 - j0 angles on nslices
-// slices (SSDevPlan::radon_slab_rows).  Launches the default rule does not divide keep it
-// (0).  Divided launches take the fastest of the rule's slab size, twice and half of it and
-// no division, timed once per process (gather layouts built first, not timed) and cached
-// with the backprojection choices; no timing while `stream` is captured, and with
-// `may_time` false only the plan's earlier choice or the rule (no host allocation).  A
-// value pinned by raft_ss_plan_set_radon_slab_rows applies to every launch.
+// Launch choice of the plan's Radon over angle ranges of j1 - j0 angles on nslices slices:
+// the block shape and the sample rows per slab (SSDevPlan::radon_shape, radon_slab_rows),
+// set in *dp.  Candidates: both shapes without row slabs and, where the default rule
+// divides the launch, both shapes at the rule's slab rows and shape 0 at twice and half of
+// them.  The fastest is timed once per process (gather layouts built first, not timed)
+// and cached with the backprojection choices; no timing while `stream` is captured, and
+// with `may_time` false only the plan's earlier choice or the rule with shape 0 (no host
+// allocation).  Rows pinned by raft_ss_plan_set_radon_slab_rows apply to every launch,
+// with shape 0.
 int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipStream_t stream, bool may_time,
-                       int* rows);
+                       SSDevPlan* dp);
 
 struct raft_ss_plan {
   int device = 0;
@@ -111,6 +113,7 @@ struct raft_ss_plan {
     p.bp_window = bp_window_v[v];
     p.bp_angles = bp_angles_v[v];
     p.radon_slab_rows = 0;
+    p.radon_shape = 0;
     p.rt_variant = rt_variant >= 0 ? rt_variant : 0;
     p.rt_window = rt_variant >= 0 ? rt_window_v[rt_variant] : 0;
     p.rt_angles = rt_variant >= 0 ? rt_angles_v[rt_variant] : 0;

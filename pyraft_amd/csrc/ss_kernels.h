@@ -45,6 +45,7 @@ struct SSDevPlan {
   int bp_window;           // rays staged per angle per pixel tile (for bp_variant)
   int bp_angles;           // angles per LDS stage (for bp_variant)
   int radon_slab_rows;     // Radon: sample rows per slab (n: no division; 0: the default rule)
+  int radon_shape;         // Radon: block shape (0: 4096 staged records, 32-row chunks; 1: 4864, 40)
   // exact transpose, pixel-driven form: tile variant (the backprojection's shapes), rays
   // staged per angle and angles per stage; rt_angles = 0: not available (rays closer than
   // one cell on some angle), the line form runs instead
@@ -106,6 +107,7 @@ struct SSEmRatioArgs {
 #define RADON_WAVES 8  // angles (one per wave) of a Radon block
 #endif
 #define SS_RADON_SLAB_BLOCKS (8192 / RADON_WAVES)
+#define RADON_SLAB_ALIGN 32  // rows per row slab: a multiple of this
 int ss_radon_slab_plan(int n, long long blocks, int* rows);
 // blocks of the Radon of angles [j0, j1) on an image with `nwaves` ray groups
 static inline long long ss_radon_blocks(int nwaves, int j0, int j1) {

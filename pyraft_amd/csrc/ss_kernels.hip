@@ -299,25 +299,29 @@ __device__ __forceinline__ ss_f32x2 ss_pair(__amdgpu_buffer_rsrc_t rs, uint32_t 
   return ss_struct_load_f32x2(rs, rec, 0, (int)row_off, 0);
 }
 
-#ifndef RADON_LDS_CAP
-#define RADON_LDS_CAP 4096  // staged (tap, difference) records per block (32 KiB)
-#endif
-#ifndef RADON_CHUNK
-#define RADON_CHUNK 32  // image rows (gather rows) per staged chunk, at most
-#endif
+// Radon block shapes, picked per launch by the autotuner: staged (tap, difference)
+// records per block and sample rows per chunk.  Shape 0: 4096 records (32 KiB), chunks
+// of up to 32 rows; shape 1: 4864 records (38 KiB; four blocks still fit a CU's
+// 160 KiB), chunks of up to 40 rows.
+template <int SH>
+struct SSRadonShape {
+  static constexpr int CAP = SH == 0 ? 4096 : 4864;
+  static constexpr int CHUNK = SH == 0 ? 32 : 40;
+};
 #ifndef RADON_ROW_ROUND
 #define RADON_ROW_ROUND 8  // chunk rows rounded down to a multiple of this where the window allows
 #endif
 
 #define RADON_TABLE SS_WAVE  // chunks per window table: one (chunk, wave) pair per thread
 
+template <int CAP>
 struct SSRadonLds {
   double red[RADON_WAVES];
   int64_t wpar[RADON_WAVES][3];  // per wave: lowest and highest trajectory offset of its rays, slope
   int wmeta[RADON_WAVES][4];     // per wave: branch (-1: no samples), first row, last row
   int wred[2][RADON_WAVES];      // per wave: widest window of a table pass (two passes in turn)
   int2 table[RADON_TABLE];       // per chunk of the super-chunk: first staged record, records per row
-  __attribute__((aligned(16))) float2 stage[RADON_LDS_CAP + 2];  // chunk window; record CAP stays (0, 0)
+  __attribute__((aligned(16))) float2 stage[CAP + 2];  // chunk window; record CAP stays (0, 0)
 };
 
 typedef __attribute__((address_space(1))) const void* ss_global_cptr;
@@ -345,9 +349,9 @@ struct SSGatherReader {
 
 // G consecutive samples s, s+1, ... of one lane's ray from the staged window, all G
 // reads issued before the first is used.  With CHECK, a sample at or beyond `end` or
-// outside the lane's own range [lo, lo + span] reads record RADON_LDS_CAP, which holds
+// outside the lane's own range [lo, lo + span] reads record CAP (past the window), which holds
 // (0, 0), instead of branching: it adds exactly zero to both sums.  u advances G steps.
-template <int G, bool CHECK>
+template <int G, bool CHECK, int CAP>
 __device__ __forceinline__ void ss_radon_group(const float2* __restrict__ st, int s, int end, int64_t& u,
                                                int64_t step, int lo, unsigned span, float& acc_a, float& acc_d) {
   float2 t[G];
@@ -356,7 +360,7 @@ __device__ __forceinline__ void ss_radon_group(const float2* __restrict__ st, in
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     idx[g] = (int)(u >> 32);
-    if (CHECK) idx[g] = (s + g < end && (unsigned)(s + g - lo) <= span) ? idx[g] : RADON_LDS_CAP;
+    if (CHECK) idx[g] = (s + g < end && (unsigned)(s + g - lo) <= span) ? idx[g] : CAP;
     w[g] = (uint32_t)u;
     u += step;
     __asm__ volatile("" : "+v"(u));  // one 64-bit add per sample, not offsets re-derived per sample
@@ -375,6 +379,7 @@ __device__ __forceinline__ void ss_radon_group(const float2* __restrict__ st, in
 // in its high word): groups of 4 with the range test before zlo (outside the group's
 // common range), unchecked groups of 8 in [zlo, zhi1), and the rest (the last group
 // of the common range and the samples from zhi1 on) again in checked groups of 4.
+template <int CAP>
 __device__ __forceinline__ void ss_radon_rows_staged(const float2* __restrict__ st, int sa, int sb, int zlo,
                                                      int zhi1, int64_t u, int64_t step, int lo, unsigned span,
                                                      float& acc_a, float& acc_d) {
@@ -382,13 +387,13 @@ __device__ __forceinline__ void ss_radon_rows_staged(const float2* __restrict__ 
   const int h_end = min(sb, zlo);
   if (s < h_end) {
     const int64_t u0 = u;
-    for (; s < h_end; s += 4) ss_radon_group<4, true>(st, s, h_end, u, step, lo, span, acc_a, acc_d);
+    for (; s < h_end; s += 4) ss_radon_group<4, true, CAP>(st, s, h_end, u, step, lo, span, acc_a, acc_d);
     s = h_end;
     u = u0 + (int64_t)(h_end - sa) * step;
   }
   const int m_end = min(sb, zhi1);
-  for (; s + 8 <= m_end; s += 8) ss_radon_group<8, false>(st, s, 0, u, step, lo, span, acc_a, acc_d);
-  for (; s < sb; s += 4) ss_radon_group<4, true>(st, s, sb, u, step, lo, span, acc_a, acc_d);
+  for (; s + 8 <= m_end; s += 8) ss_radon_group<8, false, CAP>(st, s, 0, u, step, lo, span, acc_a, acc_d);
+  for (; s < sb; s += 4) ss_radon_group<4, true, CAP>(st, s, sb, u, step, lo, span, acc_a, acc_d);
 }
 
 // Samples [sa, sb) of one lane's ray: predicated before zlo and from zhi1 on (outside
@@ -451,7 +456,8 @@ __device__ __forceinline__ void ss_radon_rows(const Rd& rd, int sa, int sb, int 
 // [-1, n-1]), and the RADON_WAVES lanes of a chunk take the union.  Returns the widest
 // Wc of the table (block-uniform); `pass` selects the reduction slots, so that a second
 // pass right after the first needs no barrier in between.
-__device__ __forceinline__ int ss_radon_windows(SSRadonLds& lds, int pass, int sc, int rows, int nch, int s_hi,
+template <int CAP>
+__device__ __forceinline__ int ss_radon_windows(SSRadonLds<CAP>& lds, int pass, int sc, int rows, int nch, int s_hi,
                                                 int n) {
   const int t = threadIdx.x;
   const int c = t / RADON_WAVES, w = t % RADON_WAVES;
@@ -495,18 +501,20 @@ __device__ __forceinline__ int ss_radon_windows(SSRadonLds& lds, int pass, int s
 // its partial ray sums to part_out ([j - j0][k], delta not applied) for a finishing pass.
 //
 // All angles of the block on one branch (the usual case): the block walks its sample
-// rows in chunks of up to RADON_CHUNK rows.  For each chunk it bounds the cells every
+// rows in chunks of up to CHUNK rows.  For each chunk it bounds the cells every
 // wave can touch (the trajectory is affine in ray and row, so the extremes are at the
 // corners), copies that window of the gather rows into LDS with contiguous loads, and
 // the waves interpolate from LDS.  A window too wide for LDS falls back to direct
 // gathers for that chunk.  A block straddling the branch switch gathers directly from
 // the layouts.
-template <int MODE>
+template <int MODE, int SH>
 __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2* __restrict__ gA,
                                               const float2* __restrict__ gB, int j0, int j1, int grp, int ablk,
                                               int row_lo, int row_hi, float* __restrict__ out,
                                               const float* __restrict__ meas, double* ll_slot,
-                                              float* __restrict__ part_out, SSRadonLds& lds) {
+                                              float* __restrict__ part_out,
+                                              SSRadonLds<SSRadonShape<SH>::CAP>& lds) {
+  constexpr int CAP = SSRadonShape<SH>::CAP, CHUNK = SSRadonShape<SH>::CHUNK;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = j0 + ablk * RADON_WAVES + wv;
@@ -557,7 +565,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
       lds.wpar[wv][1] = amax;
       lds.wpar[wv][2] = beta;
     }
-    if (threadIdx.x == 0) lds.stage[RADON_LDS_CAP] = make_float2(0.f, 0.f);
+    if (threadIdx.x == 0) lds.stage[CAP] = make_float2(0.f, 0.f);
     __syncthreads();
     int bmin = 2, bmax = -1, s_lo = 0x7fffffff, s_hi = -1;
     for (int w = 0; w < RADON_WAVES; ++w) {
@@ -582,14 +590,14 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
       // the pure trajectory of this lane's ray at the first row of the current chunk
       int64_t t0 = a_k + (int64_t)s_lo * beta;
       // rows in super-chunks of up to RADON_TABLE chunks: one pass over the table gives
-      // every chunk's window and the widest; chunks of RADON_CHUNK rows unless that
+      // every chunk's window and the widest; chunks of CHUNK rows unless that
       // window would not fit, then of the rows it allows (a second pass)
       for (int sc = s_lo; sc <= s_hi;) {
-        int rows = RADON_CHUNK;
+        int rows = CHUNK;
         int nch = min(RADON_TABLE, (s_hi - sc + rows) / rows);
         const int wmax = ss_radon_windows(lds, 0, sc, rows, nch, s_hi, n);
-        if (wmax * rows > RADON_LDS_CAP) {
-          int r2 = RADON_LDS_CAP / wmax;
+        if (wmax * rows > CAP) {
+          int r2 = CAP / wmax;
           if (r2 >= RADON_ROW_ROUND) r2 = r2 / RADON_ROW_ROUND * RADON_ROW_ROUND;
           rows = max(r2, 1);
           nch = min(RADON_TABLE, (s_hi - sc + rows) / rows);
@@ -600,7 +608,7 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
           const int s0 = sc + c * rows, s1 = min(s0 + rows, s_hi + 1);
           const int2 tw = lds.table[c];
           const int rec0 = ss_rfl(tw.x), Wc = ss_rfl(tw.y);
-          const bool fits = Wc > 0 && (s1 - s0) * Wc <= RADON_LDS_CAP;  // block-uniform
+          const bool fits = Wc > 0 && (s1 - s0) * Wc <= CAP;  // block-uniform
           if (c > 0) __syncthreads();  // every wave is done with the previous chunk
           if (fits) {
             // direct global -> LDS copies, 16 bytes (two records) per lane: wave wv copies
@@ -627,10 +635,10 @@ __device__ __forceinline__ void ss_radon_item(const SSRadonView& p, const float2
               int64_t u = t + ((int64_t)(1 - rec0 + (ra2 - s0) * Wc) << 32);
               int64_t step = beta + ((int64_t)Wc << 32);
               if (empty) {
-                u = (int64_t)RADON_LDS_CAP << 32;
+                u = (int64_t)CAP << 32;
                 step = 0;
               }
-              ss_radon_rows_staged(lds.stage, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
+              ss_radon_rows_staged<CAP>(lds.stage, ra2, rb2 + 1, wb.z, wb.w + 1, u, step, lo, span, acc_a, acc_d);
             } else {
               int64_t u = t + ((int64_t)1 << 32);
               int64_t step = beta;
@@ -691,12 +699,12 @@ __device__ __forceinline__ float ss_slab_sum(const float* __restrict__ src, size
 // blockIdx.z = slice * nslab + z: slab z takes sample rows [z * slab_rows, (z + 1) * slab_rows).
 // In SS_OUT_PARTIAL mode the blocks write partial ray sums to part[slice][z][j - j0][k] for
 // k_radon_slab_finish; in the other modes (nslab = 1) they store the output themselves.
-template <int MODE>
+template <int MODE, int SH>
 __global__ __launch_bounds__(RADON_WAVES * SS_WAVE) void k_radon(SSDevPlan p, const float2* __restrict__ gA,
                                                const float2* __restrict__ gB, int j0, int j1, int nslab,
                                                int slab_rows, float* __restrict__ out, float* __restrict__ part,
                                                SSEmRatioArgs em) {
-  __shared__ SSRadonLds lds;
+  __shared__ SSRadonLds<SSRadonShape<SH>::CAP> lds;
   const int z = (int)(blockIdx.z % (unsigned)nslab);
   const size_t slice = blockIdx.z / (unsigned)nslab;
   const size_t lay = slice * p.g.n * p.g.pitch;
@@ -704,7 +712,7 @@ __global__ __launch_bounds__(RADON_WAVES * SS_WAVE) void k_radon(SSDevPlan p, co
   double* slot = nullptr;
   if (MODE == SS_OUT_EM_RATIO) slot = em.ll_partials + (slice * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   float* po = MODE == SS_OUT_PARTIAL ? part + (slice * nslab + z) * (size_t)(j1 - j0) * p.g.nrays : nullptr;
-  ss_radon_item<MODE>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, blockIdx.x, blockIdx.y, z * slab_rows,
+  ss_radon_item<MODE, SH>(ss_radon_view(p), gA + lay, gB + lay, j0, j1, blockIdx.x, blockIdx.y, z * slab_rows,
                       min(p.g.n, (z + 1) * slab_rows), out + so, MODE == SS_OUT_EM_RATIO ? em.meas + so : nullptr,
                       slot, po, lds);
 }
@@ -794,7 +802,7 @@ __global__ __launch_bounds__(256) void k_radon_fixup(SSDevPlan p, const float2* 
 // A launch of few blocks (a few angles: one OSEM subset) divides the sample rows of every
 // (angle block, ray group) into slabs handled by separate blocks, so that every CU gets
 // blocks while each block keeps the LDS-staged path; a finishing pass then adds the
-// slabs' partial sums.  Rows per slab are a multiple of RADON_CHUNK, and the divided
+// slabs' partial sums.  Rows per slab are a multiple of RADON_SLAB_ALIGN, and the divided
 // launch stays within SS_RADON_SLAB_BLOCKS blocks (4 per CU, one round: a second round
 // for a few blocks would double the launch time).
 int ss_radon_slab_plan(int n, long long blocks, int* rows) {
@@ -803,7 +811,7 @@ int ss_radon_slab_plan(int n, long long blocks, int* rows) {
   const long long want = SS_RADON_SLAB_BLOCKS / std::max(blocks, 1LL);
   if (want <= 1) return 1;
   int L = (int)((n + want - 1) / want);
-  L = (L + RADON_CHUNK - 1) / RADON_CHUNK * RADON_CHUNK;
+  L = (L + RADON_SLAB_ALIGN - 1) / RADON_SLAB_ALIGN * RADON_SLAB_ALIGN;
   const int count = (n + L - 1) / L;
   if (count <= 1) return 1;
   *rows = L;
@@ -815,7 +823,7 @@ int ss_radon_slabs(const SSDevPlan& p, int j0, int j1, int nslices, int* rows) {
   *rows = n;
   if (j1 <= j0) return 1;
   if (p.radon_slab_rows > 0) {
-    const int L = (std::min(p.radon_slab_rows, n) + RADON_CHUNK - 1) / RADON_CHUNK * RADON_CHUNK;
+    const int L = (std::min(p.radon_slab_rows, n) + RADON_SLAB_ALIGN - 1) / RADON_SLAB_ALIGN * RADON_SLAB_ALIGN;
     const int count = (n + L - 1) / L;
     if (count <= 1) return 1;
     *rows = L;
@@ -849,10 +857,19 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const floa
   dim3 grid(p.nwaves, (j1 - j0 + RADON_WAVES - 1) / RADON_WAVES, nslices * nslab);
   dim3 block(RADON_WAVES * SS_WAVE);
   int partial_base;
+  // the main pass in the plan's block shape (0 or 1)
+#define SS_RADON_LAUNCH(MODE, OUTP, PARTP, NSLAB, ROWS, EM)                                                        \
+  do {                                                                                                            \
+    if (p.radon_shape == 1)                                                                                       \
+      hipLaunchKernelGGL((k_radon<MODE, 1>), grid, block, 0, stream, p, gatherA, gatherB, j0, j1, NSLAB, ROWS,    \
+                         OUTP, PARTP, EM);                                                                        \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_radon<MODE, 0>), grid, block, 0, stream, p, gatherA, gatherB, j0, j1, NSLAB, ROWS,    \
+                         OUTP, PARTP, EM);                                                                        \
+  } while (0)
   if (nslab > 1) {
     const SSEmRatioArgs none{nullptr, nullptr};
-    hipLaunchKernelGGL(k_radon<SS_OUT_PARTIAL>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, nslab, rows,
-                       nullptr, scratch, none);
+    SS_RADON_LAUNCH(SS_OUT_PARTIAL, nullptr, scratch, nslab, rows, none);
     const int fb = ss_radon_finish_blocks(j1 - j0, p.g.nrays);
     dim3 fin(fb, nslices);
     switch (mode) {
@@ -873,18 +890,16 @@ hipError_t ss_launch_radon(const SSDevPlan& p, const float2* gatherA, const floa
   } else {
     switch (mode) {
       case SS_OUT_SINO:
-        hipLaunchKernelGGL(k_radon<SS_OUT_SINO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, p.g.n, out,
-                           nullptr, em);
+        SS_RADON_LAUNCH(SS_OUT_SINO, out, nullptr, 1, p.g.n, em);
         break;
       case SS_OUT_SINOT:
-        hipLaunchKernelGGL(k_radon<SS_OUT_SINOT>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, p.g.n,
-                           out, nullptr, em);
+        SS_RADON_LAUNCH(SS_OUT_SINOT, out, nullptr, 1, p.g.n, em);
         break;
       default:
-        hipLaunchKernelGGL(k_radon<SS_OUT_EM_RATIO>, grid, block, 0, stream, p, gatherA, gatherB, j0, j1, 1, p.g.n,
-                           out, nullptr, em);
+        SS_RADON_LAUNCH(SS_OUT_EM_RATIO, out, nullptr, 1, p.g.n, em);
         break;
     }
+#undef SS_RADON_LAUNCH
     partial_base = (int)(grid.x * grid.y) * nslices;
   }
   hipError_t e = hipGetLastError();
@@ -1822,9 +1837,17 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
   }
 }
 
+// Blocks per CU the compiler must fit (registers): eight 256-thread blocks (8 waves per
+// SIMD, 64 VGPRs) for single-slice tiles of up to 4 rows per thread, whose register need
+// is near 64 (2048^2 x 1500: 2.66 -> 2.57 ms); slice pairs would spill under that bound.
+#ifndef RTP_BLOCKS_SMALL
+#define RTP_BLOCKS_SMALL 8
+#endif
+#define RTP_MIN_BLOCKS(ROWS, SL) ((ROWS) <= 4 && (SL) == 1 ? RTP_BLOCKS_SMALL : 1)
+
 // slices slice0 + blockIdx.z * SL .. + SL - 1
 template <int ROWS, int STAGE, int SL>
-__global__ __launch_bounds__(256) void k_radon_transpose_px(SSDevPlan p, const float* __restrict__ qT, int j0, int j1,
+__global__ __launch_bounds__(256, RTP_MIN_BLOCKS(ROWS, SL)) void k_radon_transpose_px(SSDevPlan p, const float* __restrict__ qT, int j0, int j1,
                                                             int slice0, float* __restrict__ out) {
   __shared__ SSRtpLds<STAGE, SL> lds;
   const size_t slice = (size_t)slice0 + (size_t)blockIdx.z * SL;
@@ -2342,7 +2365,7 @@ template <int MODE>
 __global__ __launch_bounds__(RADON_WAVES * SS_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_radon(
     SSBatchLaunch L, const float2* __restrict__ gA, const float2* __restrict__ gB, float* __restrict__ part,
     float* __restrict__ out, const float* __restrict__ meas, double* __restrict__ ll) {
-  __shared__ SSRadonLds lds;
+  __shared__ SSRadonLds<SSRadonShape<0>::CAP> lds;
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
     const SSBatchEntry e = ss_entry_of(L, item);
     const SSBatchSlice* sp = L.slices + e.slice;
@@ -2364,7 +2387,7 @@ __global__ __launch_bounds__(RADON_WAVES * SS_WAVE) __attribute__((amdgpu_waves_
     const int ablk = r / v.nwaves, grp = r - ablk * v.nwaves;
     const int rows = ns > 1 ? e.slab_rows : v.n;
     float* po = divisible ? part + e.part_off + (size_t)z * (e.j1 - e.j0) * v.R : nullptr;
-    ss_radon_item<MODE>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, grp, ablk, z * rows, min(v.n, (z + 1) * rows),
+    ss_radon_item<MODE, 0>(v, gA + lay_off, gB + lay_off, e.j0, e.j1, grp, ablk, z * rows, min(v.n, (z + 1) * rows),
                         out + sino_off, MODE == SS_OUT_EM_RATIO ? meas + sino_off : nullptr,
                         MODE == SS_OUT_EM_RATIO ? ll + item : nullptr, po, lds);
     __syncthreads();

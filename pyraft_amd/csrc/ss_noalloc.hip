@@ -48,7 +48,7 @@ extern "C" int raft_ss_radon_noalloc(const raft_ss_plan* p, const float* d_image
   SSDevPlan dp = p->dev();
   // the plan's slab choice when an earlier call made one (this entry never times)
   hipStream_t stream = as_stream(stream_);
-  if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, false, &dp.radon_slab_rows))) return rc;
+  if ((rc = ss_plan_radon_rows(p, j0, j1, nslices, stream, false, &dp))) return rc;
   const size_t layout = (size_t)nslices * p->g.n * p->g.pitch;
   const size_t layout_bytes = 2 * layout * sizeof(float2);
   const size_t need = layout_bytes + ss_radon_scratch(dp, j0, j1, nslices);
