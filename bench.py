@@ -28,6 +28,7 @@ METRIC = "Radon/backproj GUPS at 2048²×1500 angles; EM iters/s batched vs plai
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 LDS_PEAK_GBS = 78643.2         # LDS reads: 256 CUs x 128 B per clock x 2.4 GHz
 VALU_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (published; counts packed FP32)
+CUS, CLOCK_HZ = 256, 2.4e9     # MI355X compute units and peak engine clock
 BYTES_PER_UPDATE = 8.0         # two FP32 interpolation taps per linear-interpolated accumulate
 FLOPS_PER_UPDATE = 6.0         # coordinate FMA (2), lerp (sub + FMA, 3), accumulate (1)
 
@@ -50,21 +51,23 @@ def parse():
     return ap.parse_args()
 
 
-def roofline_of(kernel, updates, seconds, traffic, traffic_src):
-    """Roofline of the dominant kernel against the on-chip unit that binds it.
+def roofline_of(kernel, updates, seconds, traffic, traffic_src, counters=None):
+    """Roofline of the dominant kernel against the on-chip unit its algorithm loads.
 
     Both kernels read their interpolation taps from LDS (8 B, one (tap, difference)
-    record or staged pair, per update) and issue ~6 FP32 operations per update; HBM
-    traffic is a small fraction of the tap bytes because windows are staged once per
-    block.  The Radon is bound by the LDS read port (rays sit gamma = 1..1.41 records
-    apart, so a 16-lane group of 8-byte reads spans more than the 16 bank pairs and
-    takes two passes), the backprojection by VALU issue.  `frac` is the kernel's
-    algorithmic demand on that unit over the unit's peak, <= 1 by construction:
-      lds:  8 B x updates / time  over  LDS_PEAK_GBS
-      valu: 6 flops x updates / time  over  VALU_PEAK_TFLOPS
-    The other fractions are reported beside it: measured HBM bytes (rocprofv3
-    FETCH_SIZE + WRITE_SIZE, copy-calibrated) over the HBM peak, and the uncached-gather
-    model (8 B per update from HBM), which exceeds 1 because the taps come from LDS."""
+    record or staged pair, per update) and do ~6 FP32 operations per update; HBM traffic
+    is a small fraction of the tap bytes because windows are staged once per block.
+    `frac` is the algorithmic demand on the unit over the unit's peak, <= 1 by
+    construction:
+      lds (the Radon):          8 B x updates / time  over  LDS_PEAK_GBS
+      valu (the backprojection): 6 flops x updates / time  over  VALU_PEAK_TFLOPS
+    VALU_PEAK_TFLOPS is the published FP32 figure, which counts packed FP32; these kernels
+    are built without packed FP32, so `valu.frac_nonpacked` (over half of it) is the
+    ceiling they can reach.  Beside it: the counters of the committed rocprofv3 summary
+    for this kernel (VALU / SALU / LDS wave-instructions per update and per CU-clock,
+    LDS bank-conflict cycles over LDS-busy cycles), the measured HBM bytes (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated) over the HBM peak, and the uncached-gather model (8 B
+    per update from HBM), which exceeds 1 because the taps come from LDS."""
     lds_gbs = BYTES_PER_UPDATE * updates / seconds / 1e9
     tflops = FLOPS_PER_UPDATE * updates / seconds / 1e12
     if kernel.startswith("k_radon"):
@@ -81,7 +84,8 @@ def roofline_of(kernel, updates, seconds, traffic, traffic_src):
                  "(coordinate FMA, lerp, accumulate); time = mean event-timed launch of the operator",
         "lds": {"achieved_gbs": round(lds_gbs, 1), "peak_gbs": LDS_PEAK_GBS, "frac": round(lds_gbs / LDS_PEAK_GBS, 4)},
         "valu": {"achieved_tflops": round(tflops, 2), "peak_tflops": VALU_PEAK_TFLOPS,
-                 "frac": round(tflops / VALU_PEAK_TFLOPS, 4)},
+                 "frac": round(tflops / VALU_PEAK_TFLOPS, 4),
+                 "frac_nonpacked": round(tflops / (VALU_PEAK_TFLOPS / 2), 4)},
         "hbm_measured": None if traffic is None else {
             "achieved_gbs": round(traffic / seconds / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
             "frac": round(traffic / seconds / 1e9 / HBM_PEAK_GBS, 4)},
@@ -90,7 +94,35 @@ def roofline_of(kernel, updates, seconds, traffic, traffic_src):
                                   "note": "8 B per update as if every tap came from HBM; exceeds 1 because the "
                                           "taps are served from LDS"},
     })
+    if counters:
+        cu_clk = seconds * CUS * CLOCK_HZ
+        c = {"source": counters["source"]}
+        for name, key in (("valu", "SQ_INSTS_VALU"), ("salu", "SQ_INSTS_SALU"), ("lds", "SQ_INSTS_LDS")):
+            if key in counters:
+                c[name + "_lane_ops_per_update"] = round(counters[key] * 64 / updates, 3)
+                c[name + "_insts_per_cu_clock"] = round(counters[key] / cu_clk, 3)
+        if "SQ_LDS_BANK_CONFLICT" in counters and counters.get("SQ_LDS_IDX_ACTIVE"):
+            c["lds_bank_conflict_share"] = round(counters["SQ_LDS_BANK_CONFLICT"] / counters["SQ_LDS_IDX_ACTIVE"], 4)
+        main["counters"] = c
     return main
+
+
+def measured_counters(kernel):
+    """SQ counters per launch of `kernel` at this workload from the committed rocprofv3
+    summary (profiles/counters_*2048x1500*.md), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "counters_*2048x1500*.md"))):
+        m = re.search(r"```json\n(.*?)\n```", open(path).read(), flags=re.S)
+        if not m:
+            continue
+        table = json.loads(m.group(1)).get("per_launch", {})
+        for key, val in table.items():
+            if key.split("/")[0] == kernel:
+                out = dict(val)
+                out["source"] = os.path.relpath(path, ROOT)
+                return out
+    return None
 
 
 def measured_traffic(kernel):
@@ -569,7 +601,7 @@ def main():
     dominant = ("k_backprojection<0, %d, %d>" % (bp_rows, bp_stage)) if t_bp >= t_radon else "k_radon<1>"
     t_dom = max(t_bp, t_radon)
     traffic, traffic_src = measured_traffic(dominant)
-    roofline = roofline_of(dominant, upd, t_dom, traffic, traffic_src)
+    roofline = roofline_of(dominant, upd, t_dom, traffic, traffic_src, measured_counters(dominant))
     result = {
         "metric": METRIC,
         "value": round(value, 2),
