@@ -270,13 +270,16 @@ int raft_ss_plan_set_bp_variant(raft_ss_plan *plan, int variant);
  * uses the plan. */
 #define RAFT_SS_TUNE_FORCE 1
 int raft_ss_plan_tune(raft_ss_plan *plan, int reps, int flags, double *times_us, void *stream);
-/* Radon launches of few blocks (few angles, small images) divide each ray's samples into
- * slabs of rows added by a finishing pass.  By default the first divided launch per
- * (angle count, slice count) times the default slab size, twice and half of it and no
- * division, keeps the fastest and caches it with the backprojection choices.  rows > 0
- * pins the slab size for every Radon of the plan (rows >= n: no division); rows = 0
- * returns to the tuned choice.  Slab sizes change the FP32 summation order (~1e-7
- * relative); pinning one makes the bits reproducible. */
+/* Each Radon launch runs in one of two block shapes (staged window of 4096 records in
+ * chunks of up to 32 rows, or 4864 records in chunks of up to 40), and launches of few
+ * blocks (few angles, small images) divide each ray's samples into slabs of rows added by
+ * a finishing pass.  By default the first launch per (angle count, slice count) times
+ * both shapes undivided and, where the default rule divides the launch, both shapes at
+ * the default slab size and shape 0 at twice and half of it, keeps the fastest and caches
+ * it with the backprojection choices.  rows > 0 pins the slab size (and shape 0) for
+ * every Radon of the plan (rows >= n: no division); rows = 0 returns to the tuned
+ * choice.  Shapes and slab sizes change the FP32 summation order (~1e-7 relative);
+ * pinning makes the bits reproducible. */
 int raft_ss_plan_set_radon_slab_rows(raft_ss_plan *plan, int rows);
 /* The exact transpose runs pixel tiles of the backprojection's shapes (variant v: 32
  * columns x 8*rows[v] rows, stage[v] staged pairs) when every angle's rays are at least
