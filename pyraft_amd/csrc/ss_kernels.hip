@@ -954,7 +954,7 @@ __device__ __forceinline__ int ss_bp_exact_valid(const SSDevPlan& p, int j, int 
 
 template <int STAGE>
 struct SSBpLds {
-  float2 win[STAGE];  // (value, difference to the next ray)
+  float2 win[STAGE];  // per staged ray w: (value - w * difference, difference to the next ray)
   BPAngle par[64];
 };
 
@@ -1029,7 +1029,10 @@ __device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __re
           if ((unsigned)ray < (unsigned)R) v0 = row[ray];
           if ((unsigned)(ray + 1) < (unsigned)R) v1 = row[ray + 1];
         }
-        lds.win[jj * W + w] = make_float2(v0, v1 - v0);
+        // (v0 - w d, d): the value at u in [w, w + 1) is then fma(u, d, first), so the
+        // interpolation needs no fraction of u
+        const float d = v1 - v0;
+        lds.win[jj * W + w] = make_float2(fmaf(-(float)w, d, v0), d);
       }
     }
     __syncthreads();
@@ -1044,7 +1047,7 @@ __device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __re
         for (int i = 0; i < ROWS; ++i) {
           const float u = fmaf(fty[i], pc, t0);
           const float2 t = wrow[(int)u];
-          acc[i] += fmaf(__builtin_amdgcn_fractf(u), t.y, t.x);
+          acc[i] += fmaf(u, t.y, t.x);
         }
       }
     } else {
@@ -1064,7 +1067,7 @@ __device__ __forceinline__ void ss_bp_item(const SSDevPlan& p, const float* __re
         for (int i = 0; i < ROWS; ++i) {
           us[i] = fmaf(fty[i], qa.c, t0);
           const float2 t = wrow[(int)us[i]];
-          vals[i] = fmaf(__builtin_amdgcn_fractf(us[i]), t.y, t.x);
+          vals[i] = fmaf(us[i], t.y, t.x);
         }
         if (flag == 2) {
           // boundary tile: per-pixel 0 <= m <= R-2 test; pixels within 1e-3 bins of an
