@@ -131,6 +131,34 @@ static inline DDA_HD void dda_fast_move(const SSDdaAngle& a, SSDdaRay& r) {
   }
 }
 
+// Closed form of an exact-geometry (Q32 drift) ray after its first pixel (f0, s0), with
+// `frac` its drift fraction and `sn0` its first pixel's slow step.  Move k >= 1 advances
+// the fraction by step_fix, and its carry is that move's slow step, so the slow steps of
+// moves 0..k number sn0 + floor((frac + k step_fix) / 2^32).  The ray leaves by the fast
+// axis after `mf` moves, or at the first slow step that would leave the slow axis (the
+// jout-th): the move kslow, which keeps its pixel and drops the slow neighbour.  So the
+// ray has min(mf, kslow + 1) moves (main pixels), and moves k < kslow take their slow
+// step.
+struct SSDdaFixedRay {
+  int moves;
+  int kslow;
+};
+static inline DDA_HD SSDdaFixedRay dda_fixed_ray(const SSDdaAngle& a, int f0, int s0, int sn0, unsigned frac, int nf,
+                                                 int ns) {
+  const long long mf = a.dfast > 0 ? (long long)nf - f0 : (long long)f0 + 1;
+  const long long jout = a.dslow > 0 ? (long long)ns - s0 : (long long)s0 + 1;
+  const long long need = jout - sn0;  // carries of moves 1.. that reach the exiting slow step
+  long long k = 0;
+  if (need > 0) {
+    const unsigned long long num = ((unsigned long long)need << 32) - frac;
+    k = (long long)((num + a.step_fix - 1) / a.step_fix);
+  }
+  SSDdaFixedRay r;
+  r.kslow = (int)(k < 0x7fffffffLL ? k : 0x7fffffffLL);
+  r.moves = (int)(mf < k + 1 ? mf : k + 1);
+  return r;
+}
+
 // entry pixel and first intersection length of the ray at t
 // (raft/raft_ddaiterator.c:108-194).  Axis-parallel rays start with len_full: the
 // reference leaves that first length as the previous ray left it (uninitialised for a

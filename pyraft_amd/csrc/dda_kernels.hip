@@ -39,72 +39,79 @@ __device__ __forceinline__ int dda_wave_start(bool alive, int f, int dfast) {
   return dfast > 0 ? key : -key;
 }
 
-// Exact-geometry angles (Q32 drift): DDA_RADON_U moves per round.  A round first steps
-// every lane's integer state (pixel offsets, liveness, lengths) through its U moves,
-// then issues all 2U loads (the pixel of each move and, where the move ends with a slow
-// step, the neighbour), then adds the round's taps in FP32 (at most 2U terms) and the
-// round's sum to the ray's FP64 total, so a ray of N equal terms does not accumulate
-// FP32's N*eps growth.  Inactive taps read pixel 0 and are dropped by a select.
+// Exact-geometry angles (Q32 drift) in closed form (dda_fixed_ray): after the ray's first
+// pixel, move k >= 1 adds step_fix to the drift fraction; its carry is the move's slow
+// step, the main pixel's length is len_full - frac len_step after a carry (else
+// len_full), the slow neighbour's frac len_step, and the ray's move count and last slow
+// step are known up front.  Each lane therefore walks its moves with a fraction add, a
+// compare and two selects per move, no per-move bounds tests:
+//   sum = len_full * sum(main) + len_step * sum(carry ? frac * ((nb or 0) - main) : 0)
+// The wave's lanes still step one line of pixels at a time (the iteration index is the
+// fast coordinate); a lane outside its moves runs the same arithmetic on a virtual state
+// set so that it reaches its move 1 exactly, and its loads (possibly out of the image)
+// go through a buffer resource that returns 0 out of range.  Moves are summed in FP32 in
+// rounds of DDA_RADON_U (two sums of at most U terms), each round added to FP64 totals.
 #ifndef DDA_RADON_U
 #define DDA_RADON_U 8
 #endif
+#define DDA_RSRC_DWORD3 0x00020000  // gfx9 buffer word 3: 32-bit data format, no swizzle
+
+__device__ float dda_struct_load_f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                     int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
 
 __device__ __forceinline__ double dda_radon_fixed(const SSDdaAngle& a, const float* __restrict__ src, SSDdaRay ray,
-                                                  bool alive, unsigned frac, int pix, int F, int nf, int ns) {
-  double acc = 0.0;
+                                                  bool alive, unsigned frac0, int F, int nf, int ns) {
+  const int npix = nf * ns;
   const int fast_step = a.dfast * ns;
-  const float len_full = (float)a.len_full, len_step = (float)a.len_step;
-  float len = (float)ray.len;
-  for (int it = 0; it <= 2 * nf + 1; it += DDA_RADON_U) {
-    if (__ballot(alive) == 0) break;
-    int p0[DDA_RADON_U], p1[DDA_RADON_U];
-    float l0[DDA_RADON_U];
-    bool u0[DDA_RADON_U], u1[DDA_RADON_U];
-#pragma unroll
-    for (int i = 0; i < DDA_RADON_U; ++i) {
-      const bool act = alive && ray.f == F;
-      u0[i] = act;
-      p0[i] = act ? pix : 0;
-      l0[i] = len;
-      // the slow step after this pixel, if the previous move's drift carried
-      const bool sm = act && ray.slow_next;
-      const int s1 = dda_step(ray.s, a.dslow);
-      const bool in1 = dda_inside(ray.f, s1, nf, ns);
-      u1[i] = sm && in1;
-      p1[i] = u1[i] ? pix + a.dslow : 0;
-      if (sm) {
-        ray.s = s1;
-        pix += a.dslow;
-      }
-      const bool live = act && (!sm || in1);
-      if (act && !live) alive = false;
-      if (live) {
-        // fast move: a carry out of the fraction is the next slow step
-        ray.f = dda_step(ray.f, a.dfast);
-        const unsigned prev = frac;
-        frac += a.step_fix;
-        ray.slow_next = frac < prev;
-        len = ray.slow_next ? (float)(0u - prev) * len_step : len_full;
-        pix += fast_step;
-        alive = dda_inside(ray.f, ray.s, nf, ns);
-      }
-      F += a.dfast;
-    }
-    float v0[DDA_RADON_U], v1[DDA_RADON_U];
-#pragma unroll
-    for (int i = 0; i < DDA_RADON_U; ++i) {
-      v0[i] = src[p0[i]];
-      v1[i] = src[p1[i]];
-    }
-    float blk = 0.f;
-#pragma unroll
-    for (int i = 0; i < DDA_RADON_U; ++i) {
-      blk = u0[i] ? fmaf(v0[i], l0[i], blk) : blk;
-      blk = u1[i] ? fmaf(v1[i], len_full - l0[i], blk) : blk;
-    }
-    acc += (double)blk;
+  double acc = 0.0;
+  int moves = 0, kslow = 0, k_start = 0;
+  if (alive) {
+    const SSDdaFixedRay fr = dda_fixed_ray(a, ray.f, ray.s, ray.slow_next, frac0, nf, ns);
+    moves = fr.moves;
+    kslow = fr.kslow;
+    k_start = (ray.f - F) * a.dfast;  // iterations before this lane's first pixel
+    // move 0: the entry pixel with its own length, and its slow neighbour
+    const int pix0 = ray.f * ns + ray.s;
+    acc = fma((double)src[pix0], ray.len, acc);
+    if (ray.slow_next && kslow > 0) acc = fma((double)src[pix0 + a.dslow], a.len_full - ray.len, acc);
   }
-  return acc;
+  // virtual state at iteration 1: after k_start iterations it is move 1's (fraction frac0,
+  // pixel pix0 + fast_step + dslow * sn0)
+  const unsigned long long ksf = (unsigned long long)(unsigned)k_start * a.step_fix;
+  unsigned frac = frac0 - (unsigned)ksf;
+  const long long wraps = (long long)(((unsigned long long)frac + ksf) >> 32);
+  int pix = alive ? (int)((long long)(ray.f * ns + ray.s) + fast_step + (ray.slow_next ? a.dslow : 0) -
+                          (long long)k_start * fast_step - wraps * a.dslow)
+                  : 0;
+  int kk = 1 - k_start;  // this lane's move index at iteration 1
+  const int mlim = moves > 1 ? moves - 1 : 0;
+  // iterations the wave needs: the last move of its longest-running lane
+  int last = alive && moves > 1 ? k_start + moves - 1 : 0;
+  for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)src, (short)sizeof(float), npix, DDA_RSRC_DWORD3);
+  double tot_a = 0.0, tot_d = 0.0;
+  for (int it = 1; it <= last; it += DDA_RADON_U) {
+    float blk_a = 0.f, blk_d = 0.f;
+#pragma unroll
+    for (int i = 0; i < DDA_RADON_U; ++i) {
+      const unsigned prev = frac;
+      frac += a.step_fix;
+      const bool carry = frac < prev;
+      const bool active = (unsigned)(kk - 1) < (unsigned)mlim;
+      const bool nbv = active && carry && kk < kslow;
+      const float main = dda_struct_load_f32(rs, pix, 0, 0, 0);
+      const float nb = dda_struct_load_f32(rs, pix + a.dslow, 0, 0, 0);
+      blk_a += active ? main : 0.f;
+      const float wd = (active && carry) ? (float)frac : 0.f;
+      blk_d = fmaf(wd, (nbv ? nb : 0.f) - main, blk_d);
+      pix += fast_step + (carry ? a.dslow : 0);
+      ++kk;
+    }
+    tot_a += (double)blk_a;
+    tot_d += (double)blk_d;
+  }
+  return acc + (tot_a * a.len_full + tot_d * a.len_step);
 }
 
 __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle* __restrict__ angles,
@@ -127,10 +134,11 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   // the pixel's offset in the (fast, slow) image, stepped with the ray (an image side is
   // at most 16384, so a slice's offsets fit an int); meaningful while the lane is alive
   int pix = alive ? ray.f * ns + ray.s : 0;
-  if (a.fixed) {
-    // Q32 drift (exact-geometry angles): the fraction's carry is the slow move
+  if (a.fixed && (long long)nf * ns < (1LL << 30)) {
+    // Q32 drift (exact-geometry angles): the fraction's carry is the slow move (the closed
+    // form's virtual pixel offsets stay within an int below 2^30 pixels)
     const unsigned frac = (unsigned)fmin(fmax(ray.e * a.inv_cap, 0.0), 4294967295.0);
-    acc = dda_radon_fixed(a, src, ray, alive, frac, pix, F, nf, ns);
+    acc = dda_radon_fixed(a, src, ray, alive, frac, F, nf, ns);
   } else {
     const int fast_step = a.dfast * ns;
     // lanes start at most nf moves apart and then take at most nf + 1 moves each
