@@ -81,6 +81,12 @@ int raft_dda_weights(raft_dda_plan *plan, float *out, void *stream);
  * Returns the number of pixels (at most cap are written) or a RAFT_SS_E* code. */
 int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, const double *img_corners,
                       const double *sino_corners, int angle, int ray, int *ij, double *len, int cap);
+/* The same, through the closed form the device Radon walks on the Q32-drift angles: the
+ * ray's move count and the move of its last slow step from its entry state, then per move
+ * the drift fraction's carry (other angles: as raft_dda_host_ray).  For checking the
+ * closed form against the move-by-move walk. */
+int raft_dda_host_ray_closed(int lines, int cols, int nrays, int nangles, const double *img_corners,
+                             const double *sino_corners, int angle, int ray, int *ij, double *len, int cap);
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

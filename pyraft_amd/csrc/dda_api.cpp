@@ -364,6 +364,53 @@ extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, co
   return count;
 }
 
+// The same pixels through the device Radon's closed form on the Q32-drift angles
+// (dda_fixed_ray: the ray's move count and last slow step up front, then per move the
+// fraction's carry), evaluated on the host; other angles as raft_dda_host_ray.
+extern "C" int raft_dda_host_ray_closed(int lines, int cols, int nrays, int nangles, const double* img_corners,
+                                        const double* sino_corners, int angle, int ray, int* ij, double* len,
+                                        int cap) {
+  if (!img_corners || !sino_corners || (cap > 0 && (!ij || !len))) return ss_set_error(RAFT_SS_EINVAL, "null argument");
+  if (lines < 2 || cols < 2 || nrays < 2 || nangles < 2 || angle < 0 || angle >= nangles || ray < 0 || ray >= nrays)
+    return ss_set_error(RAFT_SS_EINVAL, "bad geometry or ray index");
+  const SSDdaGeom g = make_geom(lines, cols, nrays, nangles, img_corners, sino_corners);
+  const double dth = (sino_corners[2] - sino_corners[0]) / (nangles - 1);
+  SSDdaAngle a = make_angle(g, sino_corners[0] + (angle * dth));
+  SSDdaGatherAngle ga;
+  if (pixel_driven(g, a, &ga, angle)) set_fixed_drift(a);
+  if (!a.fixed)
+    return raft_dda_host_ray(lines, cols, nrays, nangles, img_corners, sino_corners, angle, ray, ij, len, cap);
+  const int nf = a.fast_i ? lines : cols, ns = a.fast_i ? cols : lines;
+  const SSDdaRay r = dda_ray_start(g, a, g.t0 + (ray * g.dt));
+  if (!((unsigned)r.f < (unsigned)nf && (unsigned)r.s < (unsigned)ns)) return 0;
+  const unsigned frac0 = (unsigned)fmin(fmax(r.e * a.inv_cap, 0.0), 4294967295.0);
+  const SSDdaFixedRay fr = dda_fixed_ray(a, r.f, r.s, r.slow_next, frac0, nf, ns);
+  int count = 0;
+  auto visit = [&](int f, int s, double l) {
+    if (count < cap) {
+      ij[2 * count] = a.fast_i ? f : s;
+      ij[2 * count + 1] = a.fast_i ? s : f;
+      len[count] = l;
+    }
+    ++count;
+  };
+  int f = r.f, s = r.s;
+  visit(f, s, r.len);
+  if (r.slow_next && fr.kslow > 0) visit(f, s + a.dslow, a.len_full - r.len);
+  if (r.slow_next) s += a.dslow;
+  unsigned frac = frac0;
+  for (int k = 1; k < fr.moves; ++k) {
+    f += a.dfast;
+    const unsigned prev = frac;
+    frac += a.step_fix;
+    const bool carry = frac < prev;
+    visit(f, s, carry ? a.len_full - (double)frac * a.len_step : a.len_full);
+    if (carry && k < fr.kslow) visit(f, s + a.dslow, (double)frac * a.len_step);
+    if (carry) s += a.dslow;
+  }
+  return count;
+}
+
 // ------------------------------------------------------------------ drop-in symbols
 
 namespace {
