@@ -129,6 +129,15 @@ bool pixel_driven(const SSDdaGeom& g, const SSDdaAngle& a, SSDdaGatherAngle* out
   ga.inv_pq = 1.0 / (ga.p - q);
   ga.lmax = fmin(w / as, h / ac);
   ga.half = ga.p / fabs(g.dt);
+  ga.k0h = (g.ox * a.c + g.oy * a.s - g.t0) / g.dt - ga.half;
+  ga.kx = g.hx * a.c / g.dt;
+  ga.ky = g.hy * a.s / g.dt;
+  const double kz = fabs(g.dt) * ga.inv_pq, pz = ga.p * ga.inv_pq;
+  ga.c1 = (float)(1.0 - ga.half);
+  ga.kz = (float)kz;
+  ga.pz = (float)pz;
+  ga.pz1 = (float)(pz - kz * (2.0 - ga.half));
+  ga.kyf = (float)ga.ky;
   ga.a = index;
   // rays with a nonzero length have |t - t_c| < p: the integers of the open interval
   // (rc - half, rc + half), at most ceil(2 half) of them, from floor(rc - half) + 1

@@ -81,6 +81,13 @@ struct SSDdaGatherAngle {
   double hys;               // hy * sin: t_c step per image line
   double p, inv_pq, lmax;
   double half;              // p / |dt|: candidate rays either side of t_c
+  // Two-candidate form (k <= 2) in ray units: rc - half = k0h + j kx + i ky for pixel
+  // (i, j), where rc = (t_c - t0) / dt; with fr the fraction of rc - half, the
+  // candidates r0 = floor(rc - half) + 1 and r0 + 1 lie at u0 = c1 - fr and
+  // u1 = 2 - half - fr > 0 from rc, so z0 = clamp(pz - kz |u0|), z1 = clamp(pz1 + kz fr).
+  double k0h, kx, ky;
+  float c1, kz, pz, pz1;    // 1 - half, |dt| / (p - q), p / (p - q), pz - kz (2 - half)
+  float kyf;                // ky
   int a;                    // sinogram column
   int k;                    // candidate rays per pixel: ceil(2 half) + 1
 };

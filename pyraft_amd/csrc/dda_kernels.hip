@@ -271,6 +271,87 @@ __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGath
     if (i0 + rr < g.lines) out[(size_t)(i0 + rr) * g.cols + j] = T(sum[rr]);
 }
 
+// The two-candidate gather of a sinogram (k <= 2 for every angle) in FP32 ray units.
+// Per (thread, angle), FP64 gives rc - half at the thread's first line and splits it
+// into an integer base and an FP32 fraction g0 in [0, 1); each of the thread's G lines
+// then takes fl = floor(g), fr = fract(g) of g = g0 + rr ky, reads q[r0], q[r0 + 1]
+// (r0 = base + fl + 1) with one 8-byte buffer load and adds q0 z0 + q1 z1 with the hat
+// weights of the angle's constants.  The load's resource spans the padded row
+// [R + 4], so candidates outside [-2, R + 1] read 0 without a test (rc - half is clamped
+// to [-64, R + 64]: beyond it the G lines' candidates stay outside the row, |ky| <= 2).  Terms are
+// summed in FP32 over DDA_GATHER_FLUSH angles, those sums in FP64.
+#ifndef DDA_GATHER_FLUSH
+#define DDA_GATHER_FLUSH 16
+#endif
+#ifndef DDA_GROWS2
+#define DDA_GROWS2 8
+#endif
+typedef float dda_f2v __attribute__((ext_vector_type(2)));
+__device__ dda_f2v dda_raw_load_f2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+
+__device__ __forceinline__ int dda_floor_i32(float v) {
+  int r;
+  __asm__("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_dda_gather2(SSDdaGeom g, const SSDdaGatherAngle* __restrict__ ga, int nga,
+                                                     const float* __restrict__ rows, T* __restrict__ acc) {
+  const int j = blockIdx.x * DDA_GX + (threadIdx.x & (DDA_GX - 1));
+  const int i0 = blockIdx.y * (4 * G) + (threadIdx.x / DDA_GX) * G;
+  const size_t slice = blockIdx.z;
+  if (j >= g.cols || i0 >= g.lines) return;
+  const double jd = j, id = i0;
+  const int pitch = g.nrays + 4;
+  const double rmax = g.nrays + 64.0;
+  const float* __restrict__ srows = rows + slice * (size_t)g.nangles * pitch;
+  double tot[G];
+#pragma unroll
+  for (int rr = 0; rr < G; ++rr) tot[rr] = 0.0;
+  for (int k0 = 0; k0 < nga; k0 += DDA_GATHER_FLUSH) {
+    const int k1 = min(nga, k0 + DDA_GATHER_FLUSH);
+    float sum[G];
+#pragma unroll
+    for (int rr = 0; rr < G; ++rr) sum[rr] = 0.f;
+    for (int k = k0; k < k1; ++k) {
+      const SSDdaGatherAngle& A = ga[k];
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(srows + (size_t)A.a * pitch), (short)0, pitch * (int)sizeof(float), DDA_RSRC_DWORD3);
+      const double v = fmin(fmax(fma(jd, A.kx, fma(id, A.ky, A.k0h)), -64.0), rmax);
+      const double b = floor(v);
+      const float g0 = (float)(v - b);
+      // byte offset of the padded element of ray base + 1, kept opaque so that no
+      // constant part moves into the load's instruction offset (the range check then
+      // sees the whole offset, and one below 0 reads as out of range)
+      int off = ((int)b + 3) * (int)sizeof(float);
+      __asm__ volatile("" : "+v"(off));
+      float fr[G];
+      dda_f2v q[G];
+#pragma unroll
+      for (int rr = 0; rr < G; ++rr) {
+        const float gr = rr == 0 ? g0 : fmaf((float)rr, A.kyf, g0);
+        fr[rr] = __builtin_amdgcn_fractf(gr);
+        q[rr] = dda_raw_load_f2(rs, off + dda_floor_i32(gr) * (int)sizeof(float), 0, 0);
+      }
+#pragma unroll
+      for (int rr = 0; rr < G; ++rr) {
+        const float z0 = __builtin_amdgcn_fmed3f(fmaf(-fabsf(A.c1 - fr[rr]), A.kz, A.pz), 0.f, 1.f);
+        const float z1 = __builtin_amdgcn_fmed3f(fmaf(fr[rr], A.kz, A.pz1), 0.f, 1.f);
+        sum[rr] = fmaf(q[rr][0], z0, sum[rr]);
+        sum[rr] = fmaf(q[rr][1], z1, sum[rr]);
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < G; ++rr) tot[rr] += (double)sum[rr];
+  }
+  T* __restrict__ out = acc + slice * (size_t)g.lines * g.cols;
+#pragma unroll
+  for (int rr = 0; rr < G; ++rr)
+    if (i0 + rr < g.lines) out[(size_t)(i0 + rr) * g.cols + j] = T(tot[rr]);
+}
+
 // rows[slice][a][2 + r] = sino[slice][r][a] * scale[a], two zeros either side of each row
 __global__ __launch_bounds__(256) void k_dda_sino_rows(int R, int A, const float* __restrict__ sino,
                                                        const float* __restrict__ scale, float* __restrict__ rows) {
@@ -372,7 +453,11 @@ hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp
       hipLaunchKernelGGL(k_dda_sino_rows, dda_tiles(g.nrays, g.nangles, nslices), dim3(DDA_TILE, 8), 0, stream,
                          g.nrays, g.nangles, sino, tp.row_scale, sinoT);
     const dim3 grid((g.cols + DDA_GX - 1) / DDA_GX, (g.lines + DDA_GY - 1) / DDA_GY, nslices);
-    if (tp.kmax <= 2)
+    const int gy2 = 4 * DDA_GROWS2;
+    if (tp.kmax <= 2 && !ones)
+      hipLaunchKernelGGL((k_dda_gather2<T, DDA_GROWS2>), dim3(grid.x, (g.lines + gy2 - 1) / gy2, nslices), dim3(256),
+                         0, stream, g, tp.gather, tp.ngather, (const float*)sinoT, acc);
+    else if (tp.kmax <= 2)
       hipLaunchKernelGGL((k_dda_gather<T, 2>), grid, dim3(256), 0, stream, g, tp.gather, tp.ngather,
                          (const float*)sinoT, ones ? 1 : 0, acc);
     else

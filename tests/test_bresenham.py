@@ -331,6 +331,10 @@ def test_gpu_golden(torch_mod, golden):
     ((128, 128), 150, 121, DEF_IC, DEF_SC),
     ((100, 77), 90, 60, (-0.7, 1.3, 1.2, -0.9), (-math.pi / 2, 1.1, math.pi / 2, -0.8)),
     ((256, 256), 256, 181, DEF_IC, (0.0, 1.0, math.pi, -1.0)),
+    # rays narrower than the image: pixels beyond both edges of the ray range, and in
+    # the second case pixels more than 64 rays beyond them
+    ((96, 96), 60, 47, DEF_IC, (0.0, 0.5, math.pi, -0.5)),
+    ((160, 160), 41, 37, DEF_IC, (0.0, 0.2, math.pi, -0.2)),
 ])
 def test_gpu_random_vs_oracle(torch_mod, shape, R, A, ic, sc):
     torch = torch_mod
