@@ -32,10 +32,11 @@ struct raft_dda_plan {
   int* d_walk = nullptr;
   float* d_row_scale = nullptr;  // [A] lmax of the gather angles
   int kmax = 0;
+  int gather_ws = 0;  // segment length of the two-candidate gather, 0 when it does not apply
   double* d_weights = nullptr;  // R^T 1 in FP64, built on first use
   std::mutex weights_mutex;
   SSDdaTransposePlan tplan() const {
-    return SSDdaTransposePlan{d_angles, d_gather, (int)h_gather.size(), kmax, d_row_scale, d_walk,
+    return SSDdaTransposePlan{d_angles, d_gather, (int)h_gather.size(), kmax, gather_ws, d_row_scale, d_walk,
                               (int)h_walk.size()};
   }
   ~raft_dda_plan() {
@@ -248,10 +249,13 @@ extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, in
     }
   }
   std::vector<float> row_scale(nangles, 0.f);
+  int ws = 8;
   for (const SSDdaGatherAngle& ga : p->h_gather) {
     row_scale[ga.a] = (float)ga.lmax;
     p->kmax = std::max(p->kmax, ga.k);
+    ws = std::max(ws, dda_gather2_ws(ga));
   }
+  p->gather_ws = p->kmax <= 2 && ws <= DDA_G2_WSMAX ? ws : 0;
   HIP_TRY(hipMalloc(&p->d_row_scale, sizeof(float) * nangles));
   HIP_TRY(hipMemcpyAsync(p->d_row_scale, row_scale.data(), sizeof(float) * nangles, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMalloc(&p->d_angles, sizeof(SSDdaAngle) * nangles));

@@ -234,11 +234,27 @@ hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const 
                             int nslices, hipStream_t stream);
 // The transpose's angles split into the pixel-driven set (gather table, ngather) and the
 // ray-traced set (walk list of sinogram columns, nwalk).
+// Two-candidate gather (k_dda_gather2): block of DDA_G2_COLS columns x 4 DDA_G2_LINES
+// lines, angles staged DDA_G2_ANGLES at a time, each in a segment of at most
+// DDA_G2_WSMAX rays
+#define DDA_G2_COLS 64
+#define DDA_G2_LINES 8
+#define DDA_G2_ANGLES 16
+#define DDA_G2_WSMAX 160
+
+// segment length of the two-candidate gather for an angle: the block's span of
+// rc - half, (cols - 1)|kx| + (lines - 1)|ky|, plus 8
+static inline int dda_gather2_ws(const SSDdaGatherAngle& ga) {
+  const double span = (DDA_G2_COLS - 1) * fabs(ga.kx) + (4 * DDA_G2_LINES - 1) * fabs(ga.ky);
+  return span < DDA_G2_WSMAX ? (int)ceil(span) + 8 : DDA_G2_WSMAX + 1;
+}
+
 struct SSDdaTransposePlan {
   const SSDdaAngle* angles;
   const SSDdaGatherAngle* gather;
   int ngather;
   int kmax;                 // largest candidate count of the gather table
+  int gather_ws;            // segment length of k_dda_gather2 (0: not usable)
   const float* row_scale;   // [A]: lmax of the gather angles (0 for traced ones)
   const int* walk;
   int nwalk;

@@ -272,23 +272,23 @@ __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGath
 }
 
 // The two-candidate gather of a sinogram (k <= 2 for every angle) in FP32 ray units.
-// Per (thread, angle), FP64 gives rc - half at the thread's first line and splits it
-// into an integer base and an FP32 fraction g0 in [0, 1); each of the thread's G lines
-// then takes fl = floor(g), fr = fract(g) of g = g0 + rr ky, reads q[r0], q[r0 + 1]
-// (r0 = base + fl + 1) with one 8-byte buffer load and adds q0 z0 + q1 z1 with the hat
-// weights of the angle's constants.  The load's resource spans the padded row
-// [R + 4], so candidates outside [-2, R + 1] read 0 without a test (rc - half is clamped
-// to [-64, R + 64]: beyond it the G lines' candidates stay outside the row, |ky| <= 2).  Terms are
-// summed in FP32 over DDA_GATHER_FLUSH angles, those sums in FP64.
-#ifndef DDA_GATHER_FLUSH
-#define DDA_GATHER_FLUSH 16
-#endif
-#ifndef DDA_GROWS2
-#define DDA_GROWS2 8
-#endif
+// A block covers DDA_G2_COLS columns x 4 DDA_G2_LINES lines of pixels and takes the
+// angles in groups of DDA_G2_ANGLES: each wave stages, for a quarter of the group's
+// angles, the segment of the angle's (lmax-scaled, zero-padded) row that the block's
+// candidate rays span into LDS (rays rb .. rb + ws - 1 with rb = floor(min over the block
+// of rc - half) - 2; the resource spans the padded row, so rays off the row read 0),
+// or marks the angle as missing the block.  Then per (thread, angle) FP64 gives
+// rc - half - rb at the thread's first line and splits it into an integer base and an
+// FP32 fraction g0 in [0, 1); each of the thread's DDA_G2_LINES lines takes
+// fl = floor(g), fr = fract(g) of g = g0 + rr ky, reads q[r0], q[r0 + 1]
+// (r0 = rb + base + fl + 1) from the segment with one two-word LDS read and adds
+// q0 z0 + q1 z1 with the angle's hat weights.  The segment index base + fl + 1 lies in
+// [1, 6 + span] for span = (cols - 1)|kx| + (lines - 1)|ky| of the block (the FP32
+// fraction is within 1e-5 ray units of the FP64 value, so floor is off by at most one),
+// and ws >= span + 8.  Terms are summed in FP32 over a group, those sums in FP64.
 typedef float dda_f2v __attribute__((ext_vector_type(2)));
-__device__ dda_f2v dda_raw_load_f2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
-                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+__device__ float dda_raw_load_f32(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                  int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f32");
 
 __device__ __forceinline__ int dda_floor_i32(float v) {
   int r;
@@ -296,44 +296,75 @@ __device__ __forceinline__ int dda_floor_i32(float v) {
   return r;
 }
 
-template <typename T, int G>
+#define DDA_G2_MISS (-0x7fffffff - 1)
+typedef __attribute__((address_space(3))) float dda_lds_float;
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_dda_gather2(SSDdaGeom g, const SSDdaGatherAngle* __restrict__ ga, int nga,
-                                                     const float* __restrict__ rows, T* __restrict__ acc) {
-  const int j = blockIdx.x * DDA_GX + (threadIdx.x & (DDA_GX - 1));
-  const int i0 = blockIdx.y * (4 * G) + (threadIdx.x / DDA_GX) * G;
+                                                     const float* __restrict__ rows, int ws, T* __restrict__ acc) {
+  constexpr int G = DDA_G2_LINES;
+  constexpr int NA = DDA_G2_ANGLES;
+  __shared__ float seg[NA * DDA_G2_WSMAX];
+  __shared__ int seg_rb[NA];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int jb = blockIdx.x * DDA_G2_COLS, ib = blockIdx.y * (4 * G);
+  const int j = jb + lane;
+  const int i0 = ib + wave * G;
   const size_t slice = blockIdx.z;
-  if (j >= g.cols || i0 >= g.lines) return;
   const double jd = j, id = i0;
   const int pitch = g.nrays + 4;
-  const double rmax = g.nrays + 64.0;
   const float* __restrict__ srows = rows + slice * (size_t)g.nangles * pitch;
   double tot[G];
 #pragma unroll
   for (int rr = 0; rr < G; ++rr) tot[rr] = 0.0;
-  for (int k0 = 0; k0 < nga; k0 += DDA_GATHER_FLUSH) {
-    const int k1 = min(nga, k0 + DDA_GATHER_FLUSH);
+  for (int k0 = 0; k0 < nga; k0 += NA) {
+    const int na = min(NA, nga - k0);
+    __syncthreads();  // the previous group's segments are read
+    for (int t = wave; t < na; t += 4) {
+      const SSDdaGatherAngle& A = ga[k0 + t];
+      const double lo = fma((double)jb, A.kx, fma((double)ib, A.ky, A.k0h)) + fmin(0.0, (DDA_G2_COLS - 1) * A.kx) +
+                        fmin(0.0, (4 * G - 1) * A.ky);
+      const double hi = lo + (DDA_G2_COLS - 1) * fabs(A.kx) + (4 * G - 1) * fabs(A.ky);
+      // candidates floor(lo) + 1 .. floor(hi) + 2 meet the rays 0 .. R - 1
+      const bool meets = hi >= -4.0 && lo <= g.nrays + 4.0;
+      const int rb = meets ? (int)floor(lo) - 2 : DDA_G2_MISS;
+      if (lane == 0) seg_rb[t] = rb;
+      if (meets) {
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(srows + (size_t)A.a * pitch), (short)0, pitch * (int)sizeof(float), DDA_RSRC_DWORD3);
+        // byte offset of padded element rb + 2 (ray rb), opaque so that no constant part
+        // moves into the load's instruction offset: the range check sees the whole
+        // offset, and one below 0 reads as out of range
+        int off = (rb + 2) * (int)sizeof(float);
+        __asm__ volatile("" : "+v"(off));
+        for (int m = lane; m < ws; m += 64)
+          seg[t * DDA_G2_WSMAX + m] = dda_raw_load_f32(rs, off + m * (int)sizeof(float), 0, 0);
+      }
+    }
+    __syncthreads();
     float sum[G];
 #pragma unroll
     for (int rr = 0; rr < G; ++rr) sum[rr] = 0.f;
-    for (int k = k0; k < k1; ++k) {
-      const SSDdaGatherAngle& A = ga[k];
-      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)(srows + (size_t)A.a * pitch), (short)0, pitch * (int)sizeof(float), DDA_RSRC_DWORD3);
-      const double v = fmin(fmax(fma(jd, A.kx, fma(id, A.ky, A.k0h)), -64.0), rmax);
+    for (int t = 0; t < na; ++t) {
+      const int rb = __builtin_amdgcn_readfirstlane(seg_rb[t]);
+      if (rb == DDA_G2_MISS) continue;
+      const SSDdaGatherAngle& A = ga[k0 + t];
+      const double v = fma(jd, A.kx, fma(id, A.ky, A.k0h)) - (double)rb;
       const double b = floor(v);
       const float g0 = (float)(v - b);
-      // byte offset of the padded element of ray base + 1, kept opaque so that no
-      // constant part moves into the load's instruction offset (the range check then
-      // sees the whole offset, and one below 0 reads as out of range)
-      int off = ((int)b + 3) * (int)sizeof(float);
-      __asm__ volatile("" : "+v"(off));
+      // the angle's segment from the pixel's base, opaque so that each line's address is
+      // one shift-add of its floor
+      dda_lds_float* sg = (dda_lds_float*)(seg + t * DDA_G2_WSMAX + ((int)b + 1));
+      __asm__ volatile("" : "+v"(sg));
       float fr[G];
       dda_f2v q[G];
 #pragma unroll
       for (int rr = 0; rr < G; ++rr) {
         const float gr = rr == 0 ? g0 : fmaf((float)rr, A.kyf, g0);
+        const int fl = dda_floor_i32(gr);
         fr[rr] = __builtin_amdgcn_fractf(gr);
-        q[rr] = dda_raw_load_f2(rs, off + dda_floor_i32(gr) * (int)sizeof(float), 0, 0);
+        q[rr][0] = sg[fl];
+        q[rr][1] = sg[fl + 1];
       }
 #pragma unroll
       for (int rr = 0; rr < G; ++rr) {
@@ -346,6 +377,7 @@ __global__ __launch_bounds__(256) void k_dda_gather2(SSDdaGeom g, const SSDdaGat
 #pragma unroll
     for (int rr = 0; rr < G; ++rr) tot[rr] += (double)sum[rr];
   }
+  if (j >= g.cols) return;
   T* __restrict__ out = acc + slice * (size_t)g.lines * g.cols;
 #pragma unroll
   for (int rr = 0; rr < G; ++rr)
@@ -453,10 +485,10 @@ hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp
       hipLaunchKernelGGL(k_dda_sino_rows, dda_tiles(g.nrays, g.nangles, nslices), dim3(DDA_TILE, 8), 0, stream,
                          g.nrays, g.nangles, sino, tp.row_scale, sinoT);
     const dim3 grid((g.cols + DDA_GX - 1) / DDA_GX, (g.lines + DDA_GY - 1) / DDA_GY, nslices);
-    const int gy2 = 4 * DDA_GROWS2;
-    if (tp.kmax <= 2 && !ones)
-      hipLaunchKernelGGL((k_dda_gather2<T, DDA_GROWS2>), dim3(grid.x, (g.lines + gy2 - 1) / gy2, nslices), dim3(256),
-                         0, stream, g, tp.gather, tp.ngather, (const float*)sinoT, acc);
+    if (tp.kmax <= 2 && !ones && tp.gather_ws > 0)
+      hipLaunchKernelGGL((k_dda_gather2<T>), dim3((g.cols + DDA_G2_COLS - 1) / DDA_G2_COLS,
+                                                  (g.lines + 4 * DDA_G2_LINES - 1) / (4 * DDA_G2_LINES), nslices),
+                         dim3(256), 0, stream, g, tp.gather, tp.ngather, (const float*)sinoT, tp.gather_ws, acc);
     else if (tp.kmax <= 2)
       hipLaunchKernelGGL((k_dda_gather<T, 2>), grid, dim3(256), 0, stream, g, tp.gather, tp.ngather,
                          (const float*)sinoT, ones ? 1 : 0, acc);
