@@ -59,8 +59,10 @@ typedef struct raft_dda_plan raft_dda_plan;
 int raft_dda_plan_create(raft_dda_plan **plan, int lines, int cols, int nrays, int nangles,
                          const double *img_corners, const double *sino_corners, void *stream);
 int raft_dda_plan_destroy(raft_dda_plan *plan);
-/* info[0..5] = lines, cols, nrays, nangles, #angles whose fast axis is the line,
- * #angles the transpose traces ray by ray (the others it evaluates pixel by pixel) */
+/* info[0..6] = lines, cols, nrays, nangles, #angles whose fast axis is the line,
+ * #angles the transpose traces ray by ray, #angles whose rays all run straight along an
+ * axis, which the transpose sums per column or line (the others it evaluates pixel by
+ * pixel) */
 int raft_dda_plan_info(const raft_dda_plan *plan, int *info, int ninfo);
 
 /* sino[s] = R img[s] (overwritten), device FP32 buffers */

@@ -28,6 +28,10 @@ struct raft_dda_plan {
   SSDdaAngle* d_angles = nullptr;
   std::vector<SSDdaGatherAngle> h_gather;  // angles of the pixel-driven transpose
   std::vector<int> h_walk;                 // angles traced ray by ray in the transpose
+  std::vector<SSDdaStraight> h_straight;   // straight angles of the transpose (then no traced ones)
+  std::vector<int> h_ranges;               // their rays per slow index
+  SSDdaStraight* d_straight = nullptr;
+  int* d_ranges = nullptr;
   SSDdaGatherAngle* d_gather = nullptr;
   int* d_walk = nullptr;
   float* d_row_scale = nullptr;  // [A] lmax of the gather angles
@@ -36,14 +40,16 @@ struct raft_dda_plan {
   double* d_weights = nullptr;  // R^T 1 in FP64, built on first use
   std::mutex weights_mutex;
   SSDdaTransposePlan tplan() const {
-    return SSDdaTransposePlan{d_angles, d_gather, (int)h_gather.size(), kmax, gather_ws, d_row_scale, d_walk,
-                              (int)h_walk.size()};
+    return SSDdaTransposePlan{d_angles, d_gather, (int)h_gather.size(), kmax, gather_ws, d_straight,
+                              (int)h_straight.size(), d_ranges, d_row_scale, d_walk, (int)h_walk.size()};
   }
   ~raft_dda_plan() {
     if (d_angles) (void)hipFree(d_angles);
     if (d_row_scale) (void)hipFree(d_row_scale);
     if (d_gather) (void)hipFree(d_gather);
     if (d_walk) (void)hipFree(d_walk);
+    if (d_straight) (void)hipFree(d_straight);
+    if (d_ranges) (void)hipFree(d_ranges);
     if (d_weights) (void)hipFree(d_weights);
   }
 };
@@ -148,6 +154,49 @@ bool pixel_driven(const SSDdaGeom& g, const SSDdaAngle& a, SSDdaGatherAngle* out
   return true;
 }
 
+// Whether every ray of the (traced) angle runs straight: walked as the transpose's
+// scatter walks it, each ray inside the image visits the nf fast indices from its entry
+// edge in order at one slow index, each with length len_full, and the rays of each slow
+// index are consecutive.  Then appends the angle and its ray range per slow index.
+bool straight_angle(const SSDdaGeom& g, const SSDdaAngle& a, int index, std::vector<SSDdaStraight>& out,
+                    std::vector<int>& ranges) {
+  const int nf = a.fast_i ? g.lines : g.cols;
+  const int ns = a.fast_i ? g.cols : g.lines;
+  const int f_edge = a.dfast > 0 ? 0 : nf - 1;
+  std::vector<int> lo(ns, 0), hi(ns, 0);
+  int last_s = -1;
+  for (int r = 0; r < g.nrays; ++r) {
+    SSDdaRay ray = dda_ray_start(g, a, g.t0 + (r * g.dt));
+    if (!((unsigned)ray.f < (unsigned)nf && (unsigned)ray.s < (unsigned)ns)) continue;
+    const int s0 = ray.s;
+    int count = 0;
+    for (int it = 0; it <= nf; ++it) {
+      if (!((unsigned)ray.f < (unsigned)nf && (unsigned)ray.s < (unsigned)ns)) break;
+      if (ray.s != s0 || ray.len != a.len_full || ray.f != f_edge + count * a.dfast || ray.slow_next) return false;
+      ++count;
+      dda_fast_move(a, ray);
+    }
+    if (count != nf) return false;
+    if (s0 != last_s) {
+      if (hi[s0] != 0) return false;  // this slow index had rays before another one
+      lo[s0] = r;
+      last_s = s0;
+    }
+    hi[s0] = r + 1;
+  }
+  SSDdaStraight st;
+  st.a = index;
+  st.slow_col = a.fast_i;
+  st.off = (int)ranges.size();
+  st.len = a.len_full;
+  for (int s = 0; s < ns; ++s) {
+    ranges.push_back(lo[s]);
+    ranges.push_back(hi[s]);
+  }
+  out.push_back(st);
+  return true;
+}
+
 // Q32 drift of an angle whose traversal is exact geometry (e_step <= e_cap)
 void set_fixed_drift(SSDdaAngle& a) {
   const double frac = a.e_step / a.e_cap * 4294967296.0;
@@ -248,6 +297,27 @@ extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, in
       p->h_walk.push_back(a);
     }
   }
+  // traced angles that all run straight (the axis angles of square pixels) become
+  // vectors over the slow index; the most tilted are tried first, as they fail soonest
+  if (!p->h_walk.empty()) {
+    std::vector<int> order = p->h_walk;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      const SSDdaAngle &ax = p->h_angles[x], &ay = p->h_angles[y];
+      return fmin(fabs(ax.c), fabs(ax.s)) > fmin(fabs(ay.c), fabs(ay.s));
+    });
+    bool all = true;
+    for (int a : order)
+      if (!straight_angle(g, p->h_angles[a], a, p->h_straight, p->h_ranges)) {
+        all = false;
+        break;
+      }
+    if (all) {
+      p->h_walk.clear();
+    } else {
+      p->h_straight.clear();
+      p->h_ranges.clear();
+    }
+  }
   std::vector<float> row_scale(nangles, 0.f);
   int ws = 8;
   for (const SSDdaGatherAngle& ga : p->h_gather) {
@@ -269,6 +339,14 @@ extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, in
   if (!p->h_walk.empty())
     HIP_TRY(hipMemcpyAsync(p->d_walk, p->h_walk.data(), sizeof(int) * p->h_walk.size(), hipMemcpyHostToDevice,
                            stream));
+  if (!p->h_straight.empty()) {
+    HIP_TRY(hipMalloc(&p->d_straight, sizeof(SSDdaStraight) * p->h_straight.size()));
+    HIP_TRY(hipMalloc(&p->d_ranges, sizeof(int) * p->h_ranges.size()));
+    HIP_TRY(hipMemcpyAsync(p->d_straight, p->h_straight.data(), sizeof(SSDdaStraight) * p->h_straight.size(),
+                           hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(p->d_ranges, p->h_ranges.data(), sizeof(int) * p->h_ranges.size(), hipMemcpyHostToDevice,
+                           stream));
+  }
   HIP_TRY(hipStreamSynchronize(stream));
   *out = p.release();
   return 0;
@@ -283,7 +361,8 @@ extern "C" int raft_dda_plan_info(const raft_dda_plan* p, int* info, int ninfo) 
   if (!p || !info) return ss_set_error(RAFT_SS_EINVAL, "null argument");
   int nfast = 0;
   for (const SSDdaAngle& a : p->h_angles) nfast += a.fast_i;
-  const int vals[] = {p->g.lines, p->g.cols, p->g.nrays, p->g.nangles, nfast, (int)p->h_walk.size()};
+  const int vals[] = {p->g.lines, p->g.cols, p->g.nrays, p->g.nangles, nfast, (int)p->h_walk.size(),
+                      (int)p->h_straight.size()};
   for (int i = 0; i < ninfo && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
   return 0;
 }

@@ -249,12 +249,27 @@ static inline int dda_gather2_ws(const SSDdaGatherAngle& ga) {
   return span < DDA_G2_WSMAX ? (int)ceil(span) + 8 : DDA_G2_WSMAX + 1;
 }
 
+// An angle of the transpose whose every ray runs straight along the fast axis: one slow
+// index, all nf fast indices, each with length len (the axis angles).  Its R^T is a
+// vector over the slow index, q summed over the rays of each slow index s (a contiguous
+// range, ranges[off + 2 s] .. ranges[off + 2 s + 1] - 1) times len, added to every pixel
+// of that column (slow_col: slow index = column) or line.
+struct SSDdaStraight {
+  int a;
+  int slow_col;
+  int off;
+  double len;
+};
+
 struct SSDdaTransposePlan {
   const SSDdaAngle* angles;
   const SSDdaGatherAngle* gather;
   int ngather;
   int kmax;                 // largest candidate count of the gather table
   int gather_ws;            // segment length of k_dda_gather2 (0: not usable)
+  const SSDdaStraight* straight;  // straight angles (only when no angle is traced)
+  int nstraight;
+  const int* ranges;
   const float* row_scale;   // [A]: lmax of the gather angles (0 for traced ones)
   const int* walk;
   int nwalk;

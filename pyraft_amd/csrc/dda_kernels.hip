@@ -415,12 +415,44 @@ __global__ __launch_bounds__(256) void k_dda_fill(T* __restrict__ p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = T(0);
 }
 
-// out[lines][cols] = acc[lines][cols] + accT[cols][lines]^T (accT may be null), divided by
-// w when given.
+// Straight angles: vcol[slice][j] (vline[slice][i]) = sum over the straight angles whose
+// slow index is the column (line) of len * (sum of q over the rays of that index), q = 1
+// for `ones`; FP64 sums.  One thread per column, then per line.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dda_straight(SSDdaGeom g, const SSDdaStraight* __restrict__ st, int nst,
+                                                      const int* __restrict__ ranges, const float* __restrict__ sino,
+                                                      int ones, T* __restrict__ vcol, T* __restrict__ vline) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t slice = blockIdx.y;
+  if (idx >= g.cols + g.lines) return;
+  const int col = idx < g.cols;
+  const int s = col ? idx : idx - g.cols;
+  const float* __restrict__ q = sino + slice * (size_t)g.nrays * g.nangles;
+  double sum = 0.0;
+  for (int k = 0; k < nst; ++k) {
+    const SSDdaStraight S = st[k];
+    if (S.slow_col != col) continue;
+    const int lo = ranges[S.off + 2 * s], hi = ranges[S.off + 2 * s + 1];
+    double qs = 0.0;
+    if (ones)
+      qs = hi - lo;
+    else
+      for (int r = lo; r < hi; ++r) qs += (double)q[(size_t)r * g.nangles + S.a];
+    sum = fma(qs, S.len, sum);
+  }
+  if (col)
+    vcol[slice * g.cols + s] = T(sum);
+  else
+    vline[slice * g.lines + s] = T(sum);
+}
+
+// out[lines][cols] = acc[lines][cols] + accT[cols][lines]^T + vcol[cols] + vline[lines]
+// (accT, vcol, vline may be null), divided by w when given.
 // Block (32, 8) handles a 32 x 32 tile; accT is read along its lines through LDS.
 template <typename T, typename Out>
 __global__ __launch_bounds__(256) void k_dda_fold(int lines, int cols, const T* __restrict__ acc,
-                                                  const T* __restrict__ accT, const double* __restrict__ w,
+                                                  const T* __restrict__ accT, const T* __restrict__ vcol,
+                                                  const T* __restrict__ vline, const double* __restrict__ w,
                                                   Out* __restrict__ out) {
   __shared__ T tile[DDA_TILE][DDA_TILE + 1];
   const size_t slice = blockIdx.z;
@@ -436,7 +468,8 @@ __global__ __launch_bounds__(256) void k_dda_fold(int lines, int cols, const T* 
     const int i = i0 + ii, j = j0 + threadIdx.x;
     if (i < lines && j < cols) {
       const size_t p = (size_t)i * cols + j;
-      const T v = acc[slice * npix + p] + tile[threadIdx.x][ii];
+      T v = acc[slice * npix + p] + tile[threadIdx.x][ii];
+      if (vcol) v += vcol[slice * cols + j] + vline[slice * lines + i];
       out[slice * npix + p] = w ? Out((double)v / w[p]) : Out(v);
     }
   }
@@ -503,8 +536,19 @@ hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp
     hipLaunchKernelGGL(k_dda_scatter<T>, dim3((g.nrays + 255) / 256, tp.nwalk, nslices), dim3(256), 0, stream, g,
                        tp.angles, tp.walk, sino, ones ? 1 : 0, acc, accT);
   }
+  // straight angles (present only without traced ones): column and line vectors in accT,
+  // [nslices][cols] then [nslices][lines] (lines * cols >= lines + cols for sides >= 2)
+  T* vcol = nullptr;
+  T* vline = nullptr;
+  if (tp.nstraight > 0 && tp.nwalk == 0) {
+    vcol = accT;
+    vline = accT + (size_t)nslices * g.cols;
+    hipLaunchKernelGGL(k_dda_straight<T>, dim3((g.cols + g.lines + 255) / 256, nslices), dim3(256), 0, stream, g,
+                       tp.straight, tp.nstraight, tp.ranges, sino, ones ? 1 : 0, vcol, vline);
+  }
   hipLaunchKernelGGL((k_dda_fold<T, Out>), dda_tiles(g.lines, g.cols, nslices), dim3(DDA_TILE, 8), 0, stream, g.lines,
-                     g.cols, (const T*)acc, (const T*)(tp.nwalk > 0 ? accT : nullptr), w, out);
+                     g.cols, (const T*)acc, (const T*)(tp.nwalk > 0 ? accT : nullptr), (const T*)vcol,
+                     (const T*)vline, w, out);
   return hipGetLastError();
 }
 

@@ -377,9 +377,10 @@ class DdaPlan:
             self._handle = None
 
     def info(self):
-        out = (ctypes.c_int * 6)()
-        check(libraft.raft_dda_plan_info(self._handle, out, 6), "raft_dda_plan_info")
-        return dict(zip(("lines", "cols", "nrays", "nangles", "line_fast_angles", "traced_angles"), list(out)))
+        out = (ctypes.c_int * 7)()
+        check(libraft.raft_dda_plan_info(self._handle, out, 7), "raft_dda_plan_info")
+        return dict(zip(("lines", "cols", "nrays", "nangles", "line_fast_angles", "traced_angles", "straight_angles"),
+                        list(out)))
 
     def _stream(self):
         return current_stream_handle(self.device)

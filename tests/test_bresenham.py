@@ -360,6 +360,19 @@ def test_gpu_random_vs_oracle(torch_mod, shape, R, A, ic, sc):
 
 
 @pytest.mark.gpu
+def test_gpu_straight_angles(torch_mod):
+    """Square pixels, 181 angles over [0, pi]: the axis angles 0, pi/2 and pi run straight
+    (one column or line per ray, full length), so the transpose sums them per column or
+    line instead of tracing them; with non-square pixels the traced angles bend and stay
+    traced."""
+    from pyraft_amd.device import DdaPlan
+    info = DdaPlan((256, 256), 256, 181).info()
+    assert info["straight_angles"] == 3 and info["traced_angles"] == 0, info
+    info = DdaPlan((100, 77), 90, 60, (-0.7, 1.3, 1.2, -0.9), (-math.pi / 2, 1.1, math.pi / 2, -0.8)).info()
+    assert info["straight_angles"] == 0 and info["traced_angles"] > 0, info
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("shape,R,A", [((256, 256), 256, 181), ((2048, 2048), 2048, 1500)])
 def test_gpu_adjoint(torch_mod, shape, R, A):
     torch = torch_mod
