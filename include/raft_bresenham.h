@@ -76,19 +76,14 @@ int raft_dda_backprojection(raft_dda_plan *plan, const float *sino, float *img, 
  * plan and kept) */
 int raft_dda_weights(raft_dda_plan *plan, float *out, void *stream);
 
-/* The pixels ray `ray` of angle `angle` visits, in order, as the device Radon traces
- * them (FP64 drift where the transpose traces rays too, Q32 drift on the other angles),
- * evaluated on the host (no device needed; for checking index logic):
+/* The pixels the device Radon adds for ray `ray` of angle `angle`, in order, evaluated
+ * on the host (no device needed; for checking index logic): on angles it traces move
+ * by move (FP64 drift) the visited pixels; on exact-geometry angles the trapezoid form's
+ * nonzero candidates per fast index (Q32 stepping as on the device, weights in FP64).
  * ij[2k], ij[2k+1] = line, column and len[k] = intersection length of the k-th pixel.
  * Returns the number of pixels (at most cap are written) or a RAFT_SS_E* code. */
 int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, const double *img_corners,
                       const double *sino_corners, int angle, int ray, int *ij, double *len, int cap);
-/* The same, through the closed form the device Radon walks on the Q32-drift angles: the
- * ray's move count and the move of its last slow step from its entry state, then per move
- * the drift fraction's carry (other angles: as raft_dda_host_ray).  For checking the
- * closed form against the move-by-move walk. */
-int raft_dda_host_ray_closed(int lines, int cols, int nrays, int nangles, const double *img_corners,
-                             const double *sino_corners, int angle, int ray, int *ij, double *len, int cap);
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

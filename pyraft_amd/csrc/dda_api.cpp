@@ -112,9 +112,10 @@ SSDdaAngle make_angle(const SSDdaGeom& g, double theta) {
   a.axis = a.s == 0.0 ? 1 : (a.c == 0.0 ? 2 : 0);
   a.k_len = a.e_step > 0.0 ? a.len_full / a.e_step : 0.0;
   a.fixed = 0;
-  a.step_fix = 0;
-  a.inv_cap = 0.0;
-  a.len_step = 0.0;
+  a.tr_inv = a.tr_off = a.tr_d = a.tr_lmax = 0.0;
+  a.tr_dhi = 0;
+  a.tr_dlo = 0u;
+  a.tr_c1 = a.tr_kz = a.tr_pz = a.tr_pz1 = 0.f;
   return a;
 }
 
@@ -197,14 +198,39 @@ bool straight_angle(const SSDdaGeom& g, const SSDdaAngle& a, int index, std::vec
   return true;
 }
 
-// Q32 drift of an angle whose traversal is exact geometry (e_step <= e_cap)
-void set_fixed_drift(SSDdaAngle& a) {
-  const double frac = a.e_step / a.e_cap * 4294967296.0;
+// Trapezoid terms of an exact-geometry angle (ga: its pixel-driven record): the slow
+// pitch times the trig factor (sigma per unit t is 1 / step), sigma per fast index, the
+// half-width of the trapezoid's support in slow pixels, and z = clamp(pz - kz |u|).
+struct TrapTerms {
+  double step, d, half_s, kz, pz;
+};
+TrapTerms trap_terms(const SSDdaGeom& g, const SSDdaGatherAngle& ga, const SSDdaAngle& a) {
+  TrapTerms r;
+  r.step = a.fast_i ? g.hx * a.c : g.hy * a.s;
+  r.d = -(a.fast_i ? g.hy * a.s : g.hx * a.c) / r.step;
+  r.half_s = ga.p / fabs(r.step);
+  r.kz = fabs(r.step) * ga.inv_pq;
+  r.pz = ga.p * ga.inv_pq;
+  return r;
+}
+
+// Marks an exact-geometry angle for the Radon's trapezoid form and sets its terms
+void set_trapezoid(const SSDdaGeom& g, const SSDdaGatherAngle& ga, SSDdaAngle& a) {
   a.fixed = 0;
-  if (!(frac >= 1.0) || !(frac < 4294967295.0)) return;
-  a.step_fix = (unsigned)llround(frac);
-  a.inv_cap = 4294967296.0 / a.e_cap;
-  a.len_step = a.len_full / (double)a.step_fix;
+  const TrapTerms tt = trap_terms(g, ga, a);
+  // exact geometry gives |d| <= 1 and half_s <= 1 (two candidates, u1 >= 0) up to rounding
+  if (!isfinite(tt.d) || !(fabs(tt.d) <= 2.0) || !(tt.half_s <= 1.0 + 1e-9) || !isfinite(tt.kz)) return;
+  const long long dfix = llround(tt.d * 4294967296.0);
+  a.tr_inv = 1.0 / tt.step;
+  a.tr_off = -(g.ox * a.c + g.oy * a.s) / tt.step - tt.half_s + 3.0;
+  a.tr_d = tt.d;
+  a.tr_lmax = ga.lmax;
+  a.tr_dhi = (int)(dfix >> 32);
+  a.tr_dlo = (unsigned)(dfix & 0xffffffffLL);
+  a.tr_c1 = (float)((1.0 - tt.half_s) * 4294967296.0);
+  a.tr_kz = (float)(tt.kz / 4294967296.0);
+  a.tr_pz = (float)tt.pz;
+  a.tr_pz1 = (float)(tt.pz - tt.kz * (2.0 - tt.half_s));
   a.fixed = 1;
 }
 
@@ -292,7 +318,7 @@ extern "C" int raft_dda_plan_create(raft_dda_plan** out, int lines, int cols, in
     SSDdaAngle& an = p->h_angles[a];
     if (pixel_driven(g, an, &ga, a)) {
       p->h_gather.push_back(ga);
-      set_fixed_drift(an);
+      set_trapezoid(g, ga, an);
     } else {
       p->h_walk.push_back(a);
     }
@@ -372,9 +398,9 @@ extern "C" int raft_dda_radon(const raft_dda_plan* p, const float* img, float* s
   if (rc) return rc;
   if (!img || !sino) return ss_set_error(RAFT_SS_EINVAL, "null buffer");
   hipStream_t stream = as_stream(stream_);
-  DevBuf imgT;
-  HIP_TRY(imgT.alloc(sizeof(float) * (size_t)p->g.lines * p->g.cols * nslices, stream));
-  HIP_TRY(dda_launch_radon(p->g, p->d_angles, img, imgT.as<float>(), sino, nslices, stream));
+  DevBuf pad;
+  HIP_TRY(pad.alloc(sizeof(float) * dda_radon_scratch_floats(p->g, nslices), stream));
+  HIP_TRY(dda_launch_radon(p->g, p->d_angles, img, pad.as<float>(), sino, nslices, stream));
   return 0;
 }
 
@@ -405,10 +431,13 @@ extern "C" int raft_dda_weights(raft_dda_plan* p, float* out, void* stream_) {
   return 0;
 }
 
-// The traversal of one ray exactly as k_dda_radon runs it (Q32 drift on the angles where
-// the traversal is exact geometry, FP64 drift on the others), evaluated on the host (for
-// checking index logic without a device): (line, column) pairs and lengths of the
-// visited pixels in order.  Returns the pixel count (at most cap are written).
+// One ray of the Radon exactly as k_dda_radon sums it, evaluated on the host (for
+// checking index logic without a device): the (line, column) pairs and lengths of the
+// pixels it adds, in order.  Angles traced move by move: the visited pixels with their
+// lengths.  Exact-geometry angles (trapezoid form): per fast index of the ray's range,
+// the candidates inside the image with a nonzero weight, the Q32 stepping as on the
+// device, the weights evaluated in FP64 from the Q32 fraction (the device evaluates
+// them in FP32).  Returns the pixel count (at most cap are written).
 extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, const double* img_corners,
                                  const double* sino_corners, int angle, int ray, int* ij, double* len, int cap) {
   if (!img_corners || !sino_corners || (cap > 0 && (!ij || !len))) return ss_set_error(RAFT_SS_EINVAL, "null argument");
@@ -418,67 +447,11 @@ extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, co
   const double dth = (sino_corners[2] - sino_corners[0]) / (nangles - 1);
   SSDdaAngle a = make_angle(g, sino_corners[0] + (angle * dth));
   SSDdaGatherAngle ga;
-  if (pixel_driven(g, a, &ga, angle)) set_fixed_drift(a);
+  if (pixel_driven(g, a, &ga, angle)) set_trapezoid(g, ga, a);
   const int nf = a.fast_i ? lines : cols, ns = a.fast_i ? cols : lines;
   const double t = g.t0 + (ray * g.dt);
-  SSDdaRay r = dda_ray_start(g, a, t);
-  unsigned frac = a.fixed ? (unsigned)fmin(fmax(r.e * a.inv_cap, 0.0), 4294967295.0) : 0u;
   int count = 0;
-  auto visit = [&](void) {
-    if (count < cap) {
-      ij[2 * count] = a.fast_i ? r.f : r.s;
-      ij[2 * count + 1] = a.fast_i ? r.s : r.f;
-      len[count] = r.len;
-    }
-    ++count;
-  };
-  auto inside = [&](void) { return (unsigned)r.f < (unsigned)nf && (unsigned)r.s < (unsigned)ns; };
-  for (int it = 0; it <= nf; ++it) {
-    if (!inside()) break;
-    visit();
-    if (r.slow_next) {
-      r.s = dda_step(r.s, a.dslow);
-      r.len = a.len_full - r.len;
-      r.slow_next = 0;
-      if (!inside()) break;
-      visit();
-    }
-    if (a.fixed) {
-      r.f = dda_step(r.f, a.dfast);
-      const unsigned prev = frac;
-      frac += a.step_fix;
-      r.slow_next = frac < prev;
-      r.len = r.slow_next ? (double)(0u - prev) * a.len_step : a.len_full;
-    } else {
-      dda_fast_move(a, r);
-    }
-  }
-  return count;
-}
-
-// The same pixels through the device Radon's closed form on the Q32-drift angles
-// (dda_fixed_ray: the ray's move count and last slow step up front, then per move the
-// fraction's carry), evaluated on the host; other angles as raft_dda_host_ray.
-extern "C" int raft_dda_host_ray_closed(int lines, int cols, int nrays, int nangles, const double* img_corners,
-                                        const double* sino_corners, int angle, int ray, int* ij, double* len,
-                                        int cap) {
-  if (!img_corners || !sino_corners || (cap > 0 && (!ij || !len))) return ss_set_error(RAFT_SS_EINVAL, "null argument");
-  if (lines < 2 || cols < 2 || nrays < 2 || nangles < 2 || angle < 0 || angle >= nangles || ray < 0 || ray >= nrays)
-    return ss_set_error(RAFT_SS_EINVAL, "bad geometry or ray index");
-  const SSDdaGeom g = make_geom(lines, cols, nrays, nangles, img_corners, sino_corners);
-  const double dth = (sino_corners[2] - sino_corners[0]) / (nangles - 1);
-  SSDdaAngle a = make_angle(g, sino_corners[0] + (angle * dth));
-  SSDdaGatherAngle ga;
-  if (pixel_driven(g, a, &ga, angle)) set_fixed_drift(a);
-  if (!a.fixed)
-    return raft_dda_host_ray(lines, cols, nrays, nangles, img_corners, sino_corners, angle, ray, ij, len, cap);
-  const int nf = a.fast_i ? lines : cols, ns = a.fast_i ? cols : lines;
-  const SSDdaRay r = dda_ray_start(g, a, g.t0 + (ray * g.dt));
-  if (!((unsigned)r.f < (unsigned)nf && (unsigned)r.s < (unsigned)ns)) return 0;
-  const unsigned frac0 = (unsigned)fmin(fmax(r.e * a.inv_cap, 0.0), 4294967295.0);
-  const SSDdaFixedRay fr = dda_fixed_ray(a, r.f, r.s, r.slow_next, frac0, nf, ns);
-  int count = 0;
-  auto visit = [&](int f, int s, double l) {
+  auto emit = [&](int f, int s, double l) {
     if (count < cap) {
       ij[2 * count] = a.fast_i ? f : s;
       ij[2 * count + 1] = a.fast_i ? s : f;
@@ -486,19 +459,43 @@ extern "C" int raft_dda_host_ray_closed(int lines, int cols, int nrays, int nang
     }
     ++count;
   };
-  int f = r.f, s = r.s;
-  visit(f, s, r.len);
-  if (r.slow_next && fr.kslow > 0) visit(f, s + a.dslow, a.len_full - r.len);
-  if (r.slow_next) s += a.dslow;
-  unsigned frac = frac0;
-  for (int k = 1; k < fr.moves; ++k) {
-    f += a.dfast;
-    const unsigned prev = frac;
-    frac += a.step_fix;
-    const bool carry = frac < prev;
-    visit(f, s, carry ? a.len_full - (double)frac * a.len_step : a.len_full);
-    if (carry && k < fr.kslow) visit(f, s + a.dslow, (double)frac * a.len_step);
-    if (carry) s += a.dslow;
+  if (a.fixed) {
+    const TrapTerms tt = trap_terms(g, ga, a);
+    const double tau0 = fma(t, a.tr_inv, a.tr_off);
+    const double ka = (1.0 - tau0) / a.tr_d, kb = (ns + 2.0 - tau0) / a.tr_d;
+    const double kl = fmax(fmin(ka, kb), -1.0), kh = fmin(fmax(ka, kb), (double)nf);
+    if (!(kl <= kh)) return 0;
+    const int klo = (int)fmax(0.0, floor(kl) - 1.0), khi = (int)fmin(nf - 1.0, ceil(kh) + 1.0);
+    const double v = fmin(fmax(fma((double)klo, a.tr_d, tau0), -1073741824.0), 1073741824.0);
+    const double fv = floor(v);
+    int hi = (int)fv;
+    unsigned lo = (unsigned)((v - fv) * 4294967296.0);
+    for (int k = klo; k <= khi; ++k) {
+      const int idx = (int)std::min((unsigned)hi, (unsigned)(ns + 2));  // padded slow index
+      const double fr = lo / 4294967296.0;
+      const double u0 = fabs(1.0 - tt.half_s - fr), u1 = 2.0 - tt.half_s - fr;
+      const double z0 = fmin(1.0, fmax(0.0, tt.pz - tt.kz * u0)), z1 = fmin(1.0, fmax(0.0, tt.pz - tt.kz * u1));
+      if ((unsigned)(idx - 2) < (unsigned)ns && z0 > 0.0) emit(k, idx - 2, a.tr_lmax * z0);
+      if ((unsigned)(idx - 1) < (unsigned)ns && z1 > 0.0) emit(k, idx - 1, a.tr_lmax * z1);
+      const unsigned nlo = lo + a.tr_dlo;
+      hi += a.tr_dhi + (nlo < lo ? 1 : 0);
+      lo = nlo;
+    }
+    return count;
+  }
+  SSDdaRay r = dda_ray_start(g, a, t);
+  auto inside = [&](void) { return (unsigned)r.f < (unsigned)nf && (unsigned)r.s < (unsigned)ns; };
+  for (int it = 0; it <= nf; ++it) {
+    if (!inside()) break;
+    emit(r.f, r.s, r.len);
+    if (r.slow_next) {
+      r.s = dda_step(r.s, a.dslow);
+      r.len = a.len_full - r.len;
+      r.slow_next = 0;
+      if (!inside()) break;
+      emit(r.f, r.s, r.len);
+    }
+    dda_fast_move(a, r);
   }
   return count;
 }

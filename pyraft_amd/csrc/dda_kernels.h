@@ -55,15 +55,18 @@ struct SSDdaAngle {
   int fast_i;               // 1: the line index is the fast axis
   int dfast, dslow;
   int axis;                 // 1: sin == 0, 2: cos == 0, 0: oblique
-  // Q32 drift for the Radon on angles where the traversal is exact pixel geometry (see
-  // SSDdaGatherAngle): the drift e / e_cap as a 32-bit fraction, advanced by step_fix per
-  // fast move; a carry out of the fraction is the slow move, and the crossing length is
-  // (2^32 - fraction before the move) * len_step.  Decisions agree with the FP64 drift
-  // except within ~N 2^-32 of a pixel edge, where the lengths moved are ~N 2^-32 len_full.
-  int fixed;
-  unsigned step_fix;        // round(e_step / e_cap * 2^32), <= 2^32 - 1
-  double inv_cap;           // 2^32 / e_cap
-  double len_step;          // len_full / step_fix
+  int fixed;                // exact-geometry angle: the Radon takes the trapezoid form
+  // Trapezoid form of the Radon on these angles: along fast index k the ray crosses
+  // the slow axis at sigma = t tr_inv + (tr_off - 3) + half_s + k tr_d (slow-pixel units),
+  // and its length in the pixel at slow index j is lmax z(|sigma - j|) with
+  // z(u) = clamp((p - u |step|) / (p - q)) (p, q, lmax of SSDdaGatherAngle; |step| the
+  // slow pitch times the trig factor).  With tau = sigma - half_s + 3 the candidates
+  // are the padded slow indices floor(tau), floor(tau) + 1 (two zero columns either side),
+  // at u0 = 1 - half_s - fr and u1 = 2 - half_s - fr > 0 for fr the fraction of tau.
+  double tr_inv, tr_off, tr_d, tr_lmax;
+  int tr_dhi;               // tr_d in Q32: tr_dhi + tr_dlo 2^-32
+  unsigned tr_dlo;
+  float tr_c1, tr_kz, tr_pz, tr_pz1;  // (1 - half_s) 2^32, kz 2^-32, pz, pz - kz (2 - half_s)
 };
 
 // Pixel-driven form of the transpose for one angle.  Where the traversal is exact
@@ -138,34 +141,6 @@ static inline DDA_HD void dda_fast_move(const SSDdaAngle& a, SSDdaRay& r) {
   }
 }
 
-// Closed form of an exact-geometry (Q32 drift) ray after its first pixel (f0, s0), with
-// `frac` its drift fraction and `sn0` its first pixel's slow step.  Move k >= 1 advances
-// the fraction by step_fix, and its carry is that move's slow step, so the slow steps of
-// moves 0..k number sn0 + floor((frac + k step_fix) / 2^32).  The ray leaves by the fast
-// axis after `mf` moves, or at the first slow step that would leave the slow axis (the
-// jout-th): the move kslow, which keeps its pixel and drops the slow neighbour.  So the
-// ray has min(mf, kslow + 1) moves (main pixels), and moves k < kslow take their slow
-// step.
-struct SSDdaFixedRay {
-  int moves;
-  int kslow;
-};
-static inline DDA_HD SSDdaFixedRay dda_fixed_ray(const SSDdaAngle& a, int f0, int s0, int sn0, unsigned frac, int nf,
-                                                 int ns) {
-  const long long mf = a.dfast > 0 ? (long long)nf - f0 : (long long)f0 + 1;
-  const long long jout = a.dslow > 0 ? (long long)ns - s0 : (long long)s0 + 1;
-  const long long need = jout - sn0;  // carries of moves 1.. that reach the exiting slow step
-  long long k = 0;
-  if (need > 0) {
-    const unsigned long long num = ((unsigned long long)need << 32) - frac;
-    k = (long long)((num + a.step_fix - 1) / a.step_fix);
-  }
-  SSDdaFixedRay r;
-  r.kslow = (int)(k < 0x7fffffffLL ? k : 0x7fffffffLL);
-  r.moves = (int)(mf < k + 1 ? mf : k + 1);
-  return r;
-}
-
 // entry pixel and first intersection length of the ray at t
 // (raft/raft_ddaiterator.c:108-194).  Axis-parallel rays start with len_full: the
 // reference leaves that first length as the previous ray left it (uninitialised for a
@@ -228,9 +203,13 @@ static inline DDA_HD SSDdaRay dda_ray_start(const SSDdaGeom& g, const SSDdaAngle
 
 // ---- launchers (dda_kernels.hip)
 
-// Radon: sino[slice][R][A] (overwritten) of img[slice][lines][cols]; imgT[slice][cols][lines]
-// is scratch that the launcher fills with the transposed image.
-hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* imgT, float* sino,
+// Radon: sino[slice][R][A] (overwritten) of img[slice][lines][cols]; pad
+// (dda_radon_scratch_floats) is scratch that the launcher fills with the image's padded
+// (line, column + 4) and (column, line + 4) layouts.
+static inline size_t dda_radon_scratch_floats(const SSDdaGeom& g, int nslices) {
+  return (size_t)nslices * ((size_t)g.lines * (g.cols + 4) + (size_t)g.cols * (g.lines + 4));
+}
+hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* pad, float* sino,
                             int nslices, hipStream_t stream);
 // The transpose's angles split into the pixel-driven set (gather table, ngather) and the
 // ray-traced set (walk list of sinogram columns, nwalk).

@@ -39,79 +39,83 @@ __device__ __forceinline__ int dda_wave_start(bool alive, int f, int dfast) {
   return dfast > 0 ? key : -key;
 }
 
-// Exact-geometry angles (Q32 drift) in closed form (dda_fixed_ray): after the ray's first
-// pixel, move k >= 1 adds step_fix to the drift fraction; its carry is the move's slow
-// step, the main pixel's length is len_full - frac len_step after a carry (else
-// len_full), the slow neighbour's frac len_step, and the ray's move count and last slow
-// step are known up front.  Each lane therefore walks its moves with a fraction add, a
-// compare and two selects per move, no per-move bounds tests:
-//   sum = len_full * sum(main) + len_step * sum(carry ? frac * ((nb or 0) - main) : 0)
-// The wave's lanes still step one line of pixels at a time (the iteration index is the
-// fast coordinate); a lane outside its moves runs the same arithmetic on a virtual state
-// set so that it reaches its move 1 exactly, and its loads (possibly out of the image)
-// go through a buffer resource that returns 0 out of range.  Moves are summed in FP32 in
-// rounds of DDA_RADON_U (two sums of at most U terms), each round added to FP64 totals.
 #ifndef DDA_RADON_U
 #define DDA_RADON_U 8
 #endif
 #define DDA_RSRC_DWORD3 0x00020000  // gfx9 buffer word 3: 32-bit data format, no swizzle
 
-__device__ float dda_struct_load_f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
-                                     int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
+// Exact-geometry angles in trapezoid form (SSDdaAngle::tr_*): the ray's length in the
+// pixel at fast index k, slow index j is lmax z(|sigma_k - j|), and at most two slow
+// indices per k have z > 0.  Each lane carries tau = sigma - half_s + 3 in Q32 (integer
+// part, 32-bit fraction), stepped by tr_d per fast index with an add and a carry; per k
+// it reads the two candidates (padded slow indices floor(tau), floor(tau) + 1, clamped
+// to the padded line, whose two zero columns either side make the clamp exact) with one
+// 8-byte load and adds q0 z0 + q1 z1, z from the FP32 fraction.  The wave runs the
+// union of its lanes' fast ranges where a candidate can be inside the image; rows are
+// summed in FP32 in blocks of DDA_RADON_U, the blocks in FP64.  src is the slice's
+// padded (fast, slow + 4) image.
+typedef float dda_f2v __attribute__((ext_vector_type(2)));
+__device__ dda_f2v dda_raw_load_f2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
 
-__device__ __forceinline__ double dda_radon_fixed(const SSDdaAngle& a, const float* __restrict__ src, SSDdaRay ray,
-                                                  bool alive, unsigned frac0, int F, int nf, int ns) {
-  const int npix = nf * ns;
-  const int fast_step = a.dfast * ns;
-  double acc = 0.0;
-  int moves = 0, kslow = 0, k_start = 0;
-  if (alive) {
-    const SSDdaFixedRay fr = dda_fixed_ray(a, ray.f, ray.s, ray.slow_next, frac0, nf, ns);
-    moves = fr.moves;
-    kslow = fr.kslow;
-    k_start = (ray.f - F) * a.dfast;  // iterations before this lane's first pixel
-    // move 0: the entry pixel with its own length, and its slow neighbour
-    const int pix0 = ray.f * ns + ray.s;
-    acc = fma((double)src[pix0], ray.len, acc);
-    if (ray.slow_next && kslow > 0) acc = fma((double)src[pix0 + a.dslow], a.len_full - ray.len, acc);
-  }
-  // virtual state at iteration 1: after k_start iterations it is move 1's (fraction frac0,
-  // pixel pix0 + fast_step + dslow * sn0)
-  const unsigned long long ksf = (unsigned long long)(unsigned)k_start * a.step_fix;
-  unsigned frac = frac0 - (unsigned)ksf;
-  const long long wraps = (long long)(((unsigned long long)frac + ksf) >> 32);
-  int pix = alive ? (int)((long long)(ray.f * ns + ray.s) + fast_step + (ray.slow_next ? a.dslow : 0) -
-                          (long long)k_start * fast_step - wraps * a.dslow)
-                  : 0;
-  int kk = 1 - k_start;  // this lane's move index at iteration 1
-  const int mlim = moves > 1 ? moves - 1 : 0;
-  // iterations the wave needs: the last move of its longest-running lane
-  int last = alive && moves > 1 ? k_start + moves - 1 : 0;
-  for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off));
-  const __amdgpu_buffer_rsrc_t rs =
-      __builtin_amdgcn_make_buffer_rsrc((void*)src, (short)sizeof(float), npix, DDA_RSRC_DWORD3);
-  double tot_a = 0.0, tot_d = 0.0;
-  for (int it = 1; it <= last; it += DDA_RADON_U) {
-    float blk_a = 0.f, blk_d = 0.f;
+struct SSDdaTrapState {
+  int hi;
+  unsigned lo;
+};
+
+template <int N>
+__device__ __forceinline__ float dda_trap_rows(const SSDdaAngle& a, __amdgpu_buffer_rsrc_t rs, int row_bytes,
+                                               int pitch_bytes, int smax, SSDdaTrapState& st) {
+  float blk = 0.f;
 #pragma unroll
-    for (int i = 0; i < DDA_RADON_U; ++i) {
-      const unsigned prev = frac;
-      frac += a.step_fix;
-      const bool carry = frac < prev;
-      const bool active = (unsigned)(kk - 1) < (unsigned)mlim;
-      const bool nbv = active && carry && kk < kslow;
-      const float main = dda_struct_load_f32(rs, pix, 0, 0, 0);
-      const float nb = dda_struct_load_f32(rs, pix + a.dslow, 0, 0, 0);
-      blk_a += active ? main : 0.f;
-      const float wd = (active && carry) ? (float)frac : 0.f;
-      blk_d = fmaf(wd, (nbv ? nb : 0.f) - main, blk_d);
-      pix += fast_step + (carry ? a.dslow : 0);
-      ++kk;
-    }
-    tot_a += (double)blk_a;
-    tot_d += (double)blk_d;
+  for (int u = 0; u < N; ++u) {
+    // below 0 wraps to a large unsigned value and clamps to smax: zeros there too
+    const unsigned idx = min((unsigned)st.hi, (unsigned)smax);
+    const dda_f2v q = dda_raw_load_f2(rs, (int)(idx * sizeof(float)), row_bytes + u * pitch_bytes, 0);
+    const float fr = (float)st.lo;
+    const float z0 = __builtin_amdgcn_fmed3f(fmaf(-fabsf(a.tr_c1 - fr), a.tr_kz, a.tr_pz), 0.f, 1.f);
+    const float z1 = __builtin_amdgcn_fmed3f(fmaf(fr, a.tr_kz, a.tr_pz1), 0.f, 1.f);
+    blk = fmaf(q[0], z0, blk);
+    blk = fmaf(q[1], z1, blk);
+    const unsigned lo = st.lo + a.tr_dlo;
+    st.hi += a.tr_dhi + (lo < st.lo ? 1 : 0);
+    st.lo = lo;
   }
-  return acc + (tot_a * a.len_full + tot_d * a.len_step);
+  return blk;
+}
+
+__device__ __forceinline__ double dda_radon_trap(const SSDdaAngle& a, const float* __restrict__ src, double t,
+                                                 bool lane_ok, int nf, int ns) {
+  const double tau0 = fma(t, a.tr_inv, a.tr_off);
+  // fast indices where floor(tau) can be in [1, ns + 1] (a candidate inside the image)
+  const double ka = (1.0 - tau0) / a.tr_d, kb = (ns + 2.0 - tau0) / a.tr_d;
+  const double kl = fmax(fmin(ka, kb), -1.0), kh = fmin(fmax(ka, kb), (double)nf);
+  int klo = nf, khi = -1;
+  if (lane_ok && kl <= kh) {  // false for NaN
+    klo = (int)fmax(0.0, floor(kl) - 1.0);
+    khi = (int)fmin(nf - 1.0, ceil(kh) + 1.0);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    klo = min(klo, __shfl_xor(klo, off));
+    khi = max(khi, __shfl_xor(khi, off));
+  }
+  klo = __builtin_amdgcn_readfirstlane(klo);  // wave-uniform: rows in scalar registers
+  khi = __builtin_amdgcn_readfirstlane(khi);
+  if (klo > khi) return 0.0;
+  const double v = fmin(fmax(fma((double)klo, a.tr_d, tau0), -1073741824.0), 1073741824.0);
+  const double fv = floor(v);
+  SSDdaTrapState st;
+  st.hi = (int)fv;
+  st.lo = (unsigned)((v - fv) * 4294967296.0);
+  const int pitch_bytes = (ns + 4) * (int)sizeof(float);
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)src, (short)0, nf * pitch_bytes, DDA_RSRC_DWORD3);
+  double tot = 0.0;
+  int k = klo;
+  for (; k + DDA_RADON_U - 1 <= khi; k += DDA_RADON_U)
+    tot += (double)dda_trap_rows<DDA_RADON_U>(a, rs, k * pitch_bytes, pitch_bytes, ns + 2, st);
+  for (; k <= khi; ++k) tot += (double)dda_trap_rows<1>(a, rs, k * pitch_bytes, pitch_bytes, ns + 2, st);
+  return lane_ok ? tot * a.tr_lmax : 0.0;
 }
 
 __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle* __restrict__ angles,
@@ -124,7 +128,14 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   const SSDdaAngle a = angles[ai];
   const int nf = a.fast_i ? g.lines : g.cols;
   const int ns = a.fast_i ? g.cols : g.lines;
-  const float* __restrict__ src = (a.fast_i ? img : imgT) + slice * (size_t)g.lines * g.cols;
+  // padded (fast, slow + 4) layouts: two zero columns either side of each line
+  const int pitch = ns + 4;
+  const float* __restrict__ src = (a.fast_i ? img : imgT) + slice * (size_t)nf * pitch;
+  if (a.fixed && (long long)nf * pitch < (1LL << 29)) {
+    const double acc = dda_radon_trap(a, src, dda_ray_t(g, lane_ok ? r : 0), lane_ok, nf, ns);
+    if (lane_ok) sino[(slice * g.nrays + r) * g.nangles + ai] = (float)acc;
+    return;
+  }
   SSDdaRay ray = dda_ray_start(g, a, dda_ray_t(g, lane_ok ? r : 0));
   bool alive = lane_ok && dda_inside(ray.f, ray.s, nf, ns);
   // FP64 sum: the lengths are FP64 already, so a tap costs one convert and one FMA as
@@ -133,14 +144,9 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   int F = dda_wave_start(alive, ray.f, a.dfast);
   // the pixel's offset in the (fast, slow) image, stepped with the ray (an image side is
   // at most 16384, so a slice's offsets fit an int); meaningful while the lane is alive
-  int pix = alive ? ray.f * ns + ray.s : 0;
-  if (a.fixed && (long long)nf * ns < (1LL << 30)) {
-    // Q32 drift (exact-geometry angles): the fraction's carry is the slow move (the closed
-    // form's virtual pixel offsets stay within an int below 2^30 pixels)
-    const unsigned frac = (unsigned)fmin(fmax(ray.e * a.inv_cap, 0.0), 4294967295.0);
-    acc = dda_radon_fixed(a, src, ray, alive, frac, F, nf, ns);
-  } else {
-    const int fast_step = a.dfast * ns;
+  int pix = alive ? ray.f * pitch + ray.s + 2 : 0;
+  {
+    const int fast_step = a.dfast * pitch;
     // lanes start at most nf moves apart and then take at most nf + 1 moves each
     for (int it = 0; it <= 2 * nf + 1; ++it) {
       if (__ballot(alive) == 0) break;
@@ -475,21 +481,36 @@ __global__ __launch_bounds__(256) void k_dda_fold(int lines, int cols, const T* 
   }
 }
 
-// dst[cols][lines] = src[lines][cols]^T
-__global__ __launch_bounds__(256) void k_dda_transpose_image(int lines, int cols, const float* __restrict__ src,
-                                                             float* __restrict__ dst) {
+// The Radon's padded layouts of img[slice][lines][cols]: pad[slice][lines][cols + 4] and
+// padT[slice][cols][lines + 4] (the transpose), each line with two zeros either side.
+// Block (32, 8) copies a 32 x 32 tile, the transposed copy through LDS.
+__global__ __launch_bounds__(256) void k_dda_pad_images(int lines, int cols, const float* __restrict__ src,
+                                                        float* __restrict__ pad, float* __restrict__ padT) {
   __shared__ float tile[DDA_TILE][DDA_TILE + 1];
   const size_t slice = blockIdx.z;
-  const size_t npix = (size_t)lines * cols;
   const int i0 = blockIdx.y * DDA_TILE, j0 = blockIdx.x * DDA_TILE;
+  const float* __restrict__ s = src + slice * (size_t)lines * cols;
+  float* __restrict__ o = pad + slice * (size_t)lines * (cols + 4);
+  float* __restrict__ oT = padT + slice * (size_t)cols * (lines + 4);
+  const int x = threadIdx.x;
   for (int ii = threadIdx.y; ii < DDA_TILE; ii += blockDim.y) {
-    const int i = i0 + ii, j = j0 + threadIdx.x;
-    tile[ii][threadIdx.x] = (i < lines && j < cols) ? src[slice * npix + (size_t)i * cols + j] : 0.f;
+    const int i = i0 + ii, j = j0 + x;
+    const float v = (i < lines && j < cols) ? s[(size_t)i * cols + j] : 0.f;
+    tile[ii][x] = v;
+    if (i < lines) {
+      if (j < cols) o[(size_t)i * (cols + 4) + j + 2] = v;
+      if (blockIdx.x == 0 && x < 2) o[(size_t)i * (cols + 4) + x] = 0.f;
+      if (j0 + DDA_TILE >= cols && x < 2) o[(size_t)i * (cols + 4) + cols + 2 + x] = 0.f;
+    }
   }
   __syncthreads();
   for (int jj = threadIdx.y; jj < DDA_TILE; jj += blockDim.y) {
-    const int j = j0 + jj, i = i0 + threadIdx.x;
-    if (i < lines && j < cols) dst[slice * npix + (size_t)j * lines + i] = tile[threadIdx.x][jj];
+    const int j = j0 + jj, i = i0 + x;
+    if (j < cols) {
+      if (i < lines) oT[(size_t)j * (lines + 4) + i + 2] = tile[x][jj];
+      if (blockIdx.y == 0 && x < 2) oT[(size_t)j * (lines + 4) + x] = 0.f;
+      if (i0 + DDA_TILE >= lines && x < 2) oT[(size_t)j * (lines + 4) + lines + 2 + x] = 0.f;
+    }
   }
 }
 
@@ -499,12 +520,14 @@ static dim3 dda_tiles(int lines, int cols, int nslices) {
   return dim3((cols + DDA_TILE - 1) / DDA_TILE, (lines + DDA_TILE - 1) / DDA_TILE, nslices);
 }
 
-hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* imgT, float* sino,
+hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* pad, float* sino,
                             int nslices, hipStream_t stream) {
-  hipLaunchKernelGGL(k_dda_transpose_image, dda_tiles(g.lines, g.cols, nslices), dim3(DDA_TILE, 8), 0, stream, g.lines,
-                     g.cols, img, imgT);
+  float* img_pad = pad;
+  float* imgT_pad = pad + (size_t)nslices * g.lines * (g.cols + 4);
+  hipLaunchKernelGGL(k_dda_pad_images, dda_tiles(g.lines, g.cols, nslices), dim3(DDA_TILE, 8), 0, stream, g.lines,
+                     g.cols, img, img_pad, imgT_pad);
   hipLaunchKernelGGL(k_dda_radon, dim3((g.nrays + 255) / 256, g.nangles, nslices), dim3(256), 0, stream, g, angles,
-                     img, (const float*)imgT, sino);
+                     (const float*)img_pad, (const float*)imgT_pad, sino);
   return hipGetLastError();
 }
 

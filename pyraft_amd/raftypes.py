@@ -150,7 +150,6 @@ _decl("raft_dda_radon_transpose", [_vp, _vp, _vp, _i, _vp], _i)
 _decl("raft_dda_backprojection", [_vp, _vp, _vp, _i, _vp], _i)
 _decl("raft_dda_weights", [_vp, _vp, _vp], _i)
 _decl("raft_dda_host_ray", [_i, _i, _i, _i, _d4, _d4, _i, _i, ctypes.POINTER(_i), _d4, _i], _i)
-_decl("raft_dda_host_ray_closed", [_i, _i, _i, _i, _d4, _d4, _i, _i, ctypes.POINTER(_i), _d4, _i], _i)
 _decl("raft_ss_plan_create_mesh", [ctypes.POINTER(_vp), _i, _d4, _i, _d4, _i, _d4, _vp], _i)
 _decl("raft_ss_plan_host_bounds_mesh", [_i, _d4, _i, _d4, _i, _d4, ctypes.POINTER(_i),
                                         ctypes.POINTER(ctypes.c_longlong)], _i)
@@ -366,7 +365,6 @@ EXPORTED_SYMBOLS = (
     "raft_radon_bresenham", "raft_radon_transpose_bresenham", "raft_backprojection_bresenham",
     "raft_dda_plan_create", "raft_dda_plan_destroy", "raft_dda_plan_info", "raft_dda_radon",
     "raft_dda_radon_transpose", "raft_dda_backprojection", "raft_dda_weights", "raft_dda_host_ray",
-    "raft_dda_host_ray_closed",
     "raft_filter1d_ramp", "raft_fbp", "raft_ss_filter_kernel", "raft_ss_filter1d_ramp_taps",
     "raft_ss_filter_rows_toeplitz", "raft_tr_em_gpu", "raft_ss_tr_em", "raft_matrix_read", "raft_matrix_write",
     "raft_matrix_destroy", "raft_image_read", "raft_image_write", "raft_image_destroy",
