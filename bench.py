@@ -96,7 +96,7 @@ def roofline_of(kernel, updates, seconds, traffic, traffic_src, counters=None):
     })
     if counters:
         cu_clk = seconds * CUS * CLOCK_HZ
-        c = {"source": counters["source"]}
+        c = {"source": counters["source"], "launch": counters.get("launch")}
         for name, key in (("valu", "SQ_INSTS_VALU"), ("salu", "SQ_INSTS_SALU"), ("lds", "SQ_INSTS_LDS")):
             if key in counters:
                 c[name + "_lane_ops_per_update"] = round(counters[key] * 64 / updates, 3)
@@ -107,37 +107,46 @@ def roofline_of(kernel, updates, seconds, traffic, traffic_src, counters=None):
     return main
 
 
-def measured_counters(kernel):
-    """SQ counters per launch of `kernel` at this workload from the committed rocprofv3
-    summary (profiles/counters_*2048x1500*.md), or None."""
+def _profile_name_matches(key, kernel):
+    """A profile key ("name/work-items") names `kernel`; a kernel given without its last
+    template arguments ("k_radon<1>") matches every instantiation that extends it
+    ("k_radon<1, 1>": the Radon's block shape is picked per launch)."""
+    name = key.split("/")[0]
+    return name == kernel or (kernel.endswith(">") and name.startswith(kernel[:-1] + ", "))
+
+
+def _profile_json(pattern, field):
     import glob
     import re
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "counters_*2048x1500*.md"))):
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", pattern))):
         m = re.search(r"```json\n(.*?)\n```", open(path).read(), flags=re.S)
-        if not m:
-            continue
-        table = json.loads(m.group(1)).get("per_launch", {})
-        for key, val in table.items():
-            if key.split("/")[0] == kernel:
-                out = dict(val)
-                out["source"] = os.path.relpath(path, ROOT)
-                return out
+        if m:
+            yield os.path.relpath(path, ROOT), json.loads(m.group(1)).get(field, {})
+
+
+def measured_counters(kernel):
+    """SQ counters per launch of `kernel` at this workload from the committed rocprofv3
+    summary (profiles/counters_*2048x1500*.md), or None: of the matching launch shapes the
+    one with the most VALU instructions (the full-size launch)."""
+    for src, table in _profile_json("counters_*2048x1500*.md", "per_launch"):
+        hits = [(key, val) for key, val in table.items() if _profile_name_matches(key, kernel)]
+        if hits:
+            key, val = max(hits, key=lambda kv: kv[1].get("SQ_INSTS_VALU", 0))
+            out = dict(val)
+            out["source"] = src
+            out["launch"] = key
+            return out
     return None
 
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` at this workload from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
-    import glob
-    import re
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_bytes_*2048x1500*.md"))):
-        m = re.search(r"```json\n(.*?)\n```", open(path).read(), flags=re.S)
-        if not m:
-            continue
-        table = json.loads(m.group(1)).get("bytes_per_launch", {})
-        for key, val in table.items():
-            if key.split("/")[0] == kernel:
-                return float(val), os.path.relpath(path, ROOT)
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None: of the matching
+    launch shapes the one moving the most bytes."""
+    for src, table in _profile_json("hbm_bytes_*2048x1500*.md", "bytes_per_launch"):
+        hits = [(key, float(val)) for key, val in table.items() if _profile_name_matches(key, kernel)]
+        if hits:
+            return max(v for _, v in hits), src
     return None, None
 
 
