@@ -193,7 +193,8 @@ static void rt_staging(const std::vector<SSAngle>& angles, int v, int* window, i
   int W = (int)ceil(span) + 5;
   W = (W + 3) & ~3;
   *window = W;
-  *per_stage = W <= kSSBpStage[v] ? std::min(64, kSSBpStage[v] / W) : 0;
+  // the stage holds 2 STAGE floats: per angle a window of W + 1 rays, W + 4 floats apart
+  *per_stage = W + 4 <= 2 * kSSBpStage[v] ? std::min(64, 2 * kSSBpStage[v] / (W + 4)) : 0;
 }
 
 // all variants' staging; the default variant, or the first runnable one

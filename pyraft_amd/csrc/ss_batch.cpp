@@ -284,9 +284,10 @@ extern "C" int raft_ss_plan_tune_transpose(raft_ss_plan* p, int nslices, int rep
   }
   const int n = p->g.n, R = p->g.nrays, A = p->g.nangles;
   DevBuf ws;
-  HIP_TRY(ws.alloc(sizeof(float) * (size_t)nslices * ((size_t)R * A + (size_t)n * n), stream));
+  HIP_TRY(ws.alloc(sizeof(float) * (size_t)nslices * (2 * (size_t)R * A + (size_t)n * n), stream));
   float* sinoT = ws.as<float>();
   float* img = sinoT + (size_t)nslices * R * A;
+  float* scratch = img + (size_t)nslices * n * n;  // the pre-scaled sinogram
   HIP_TRY(ss_launch_fill(sinoT, 1.0f, (size_t)nslices * R * A, stream));
   TuneEntry te;
   int best = -1;
@@ -297,7 +298,7 @@ extern "C" int raft_ss_plan_tune_transpose(raft_ss_plan* p, int nslices, int rep
     p->rt_variant = v;
     const SSDevPlan dp = p->dev();
     auto run = [&]() -> int {
-      HIP_TRY(ss_launch_radon_transpose(dp, sinoT, 0, A, nslices, img, nullptr, stream));
+      HIP_TRY(ss_launch_radon_transpose(dp, sinoT, 0, A, nslices, img, scratch, stream));
       return 0;
     };
     rc = time_best(run, reps, stream, &te.times[v]);

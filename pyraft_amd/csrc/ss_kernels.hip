@@ -1645,23 +1645,26 @@ struct RTPAngle {
 };
 
 // SL = 2: the block takes two slices of a launch and shares each pixel's geometry (kappa,
-// floor, fraction, weights) between them; a staged record then holds both slices' pairs.
+// floor, fraction, weights) between them.  win holds, per staged angle and slice, the
+// window of Delta q_k for rays base .. base + W (rt_window W, WS = W + 4 floats apart),
+// copied straight from the pre-scaled sinogram into LDS; a pixel reads its two
+// candidates with one two-word LDS read per slice.
 template <int STAGE, int SL>
 struct SSRtpLds {
-  typedef typename std::conditional<SL == 2, float4, float2>::type Rec;
-  Rec win[SL == 2 ? STAGE / 2 : STAGE];  // per ray: (Delta q_k, Delta q_{k+1}) of each slice
+  float win[2 * STAGE];  // STAGE pairs' worth: rt_angles x WS floats
   RTPAngle par[64];
 };
 
+// sl0 / sl1: the slices' sinograms [A][R], pre-scaled by each angle's Delta
 template <int ROWS, int STAGE, int SL>
 __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __restrict__ sl0,
                                             const float* __restrict__ sl1, int j0, int j1, int c0, int r0,
                                             float* __restrict__ out0, float* __restrict__ out1,
                                             SSRtpLds<STAGE, SL>& lds) {
-  typedef typename SSRtpLds<STAGE, SL>::Rec Rec;
   const int n = p.g.n, R = p.g.nrays;
   const int W = p.rt_window;
-  const int J = SL == 2 ? p.rt_angles / 2 : p.rt_angles;  // the stage holds half as many records
+  const int WS = W + 4;
+  const int J = SL == 2 ? p.rt_angles / 2 : p.rt_angles;  // two windows per angle with two slices
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1685,7 +1688,6 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
   for (int jb = j0; jb < j1; jb += J) {
     const int nj = min(J, j1 - jb);
     RTPAngle q{0.f, 0.f, 0.f, 0.f, 0, 0, 0, 0.f};
-    float delta = 0.f;
     if (lane < nj) {
       const SSAngle& a = p.angles[jb + lane];
       const double kb = a.branch_a ? -a.rt_b : a.rt_ig;  // kappa per column
@@ -1702,7 +1704,6 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
       q.gs = (float)((double)a.gamma * (1.0 / SS_FIX_ONE));
       q.base = (int)base;
       q.branch_a = a.branch_a;
-      delta = a.delta;
       if (kmax < -1.0 || kmin >= (double)R)
         q.flag = 0;
       else if (!a.regular)
@@ -1724,23 +1725,24 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
       lds.par[lane].branch_a = q.branch_a;
       lds.par[lane].gs = q.gs;
     }
-    // staging: wave wv fills the windows of angles wv, wv+4, ...; kappa - base >= 1 for every
-    // pixel, so the truncated offset is the floor and stays below W - 1
+    // staging: wave wv fills the windows of angles wv, wv+4, ... with direct buffer -> LDS
+    // copies, one ray per lane; rays outside [0, R) (and every ray of an angle without
+    // candidates) read 0 through the buffer's range check.  kappa - base >= 1 for every
+    // pixel, so the truncated offset is the floor and stays below W.
     for (int jj = wv; jj < nj; jj += 4) {
       const int fl = __builtin_amdgcn_readlane(q.flag, jj);
       const int bs = __builtin_amdgcn_readlane(q.base, jj);
-      const float dl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), jj));
-      const float* row0 = sl0 + (size_t)(jb + jj) * R;
-      const float* row1 = SL == 2 ? sl1 + (size_t)(jb + jj) * R : nullptr;
-      for (int w = lane; w < W; w += 64) {
-        const int ray = bs + w;
-        const bool in0 = fl != 0 && (unsigned)ray < (unsigned)R, in1 = fl != 0 && (unsigned)(ray + 1) < (unsigned)R;
-        const float a0 = in0 ? row0[ray] * dl : 0.f, a1 = in1 ? row0[ray + 1] * dl : 0.f;
-        if constexpr (SL == 2) {
-          const float b0 = in0 ? row1[ray] * dl : 0.f, b1 = in1 ? row1[ray + 1] * dl : 0.f;
-          reinterpret_cast<float4*>(lds.win)[jj * W + w] = make_float4(a0, a1, b0, b1);
-        } else {
-          reinterpret_cast<float2*>(lds.win)[jj * W + w] = make_float2(a0, a1);
+      for (int z = 0; z < SL; ++z) {
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)((z == 0 ? sl0 : sl1) + (size_t)(jb + jj) * R), (short)0, R * (int)sizeof(float), 0x00020000);
+        float* dst = lds.win + (jj * SL + z) * WS;
+        for (int w = 0; w < W + 1; w += 64) {
+          // the whole offset in the register (none in the instruction's offset field), so
+          // the range check sees a ray below 0 as out of range
+          int off = fl != 0 ? (bs + w + lane) * (int)sizeof(float) : 0x7ffffff0;
+          __asm__ volatile("" : "+v"(off));
+          if (w + lane <= W)  // active lanes write dst[w + lane]: stay in this window
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (ss_lds_ptr)(dst + w), 4, off, 0, 0, 0);
         }
       }
     }
@@ -1750,20 +1752,20 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
       for (int jj = 0; jj < nj; ++jj) {
         const float f0 = lds.par[jj].f0, pb = lds.par[jj].b, pc = lds.par[jj].c, g = lds.par[jj].g;
         const float g1 = 1.f - g;
-        const Rec* wrow = lds.win + jj * W;
+        const float* wrow = lds.win + jj * SL * WS;
         const float t0 = fmaf(ftx, pb, f0);
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
           const float kap = fmaf(fty[i], pc, t0);
-          const Rec t = wrow[(int)kap];
+          const int kr = (int)kap;
           const float f = __builtin_amdgcn_fractf(kap);
           const float w0 = __builtin_amdgcn_fmed3f(fmaf(-g, f, 1.f), 0.f, 1.f);
           const float w1 = __builtin_amdgcn_fmed3f(fmaf(g, f, g1), 0.f, 1.f);
-          acc0[0][i] = fmaf(w0, t.x, acc0[0][i]);
-          acc1[0][i] = fmaf(w1, t.y, acc1[0][i]);
+          acc0[0][i] = fmaf(w0, wrow[kr], acc0[0][i]);
+          acc1[0][i] = fmaf(w1, wrow[kr + 1], acc1[0][i]);
           if constexpr (SL == 2) {
-            acc0[1][i] = fmaf(w0, t.z, acc0[1][i]);
-            acc1[1][i] = fmaf(w1, t.w, acc1[1][i]);
+            acc0[1][i] = fmaf(w0, wrow[WS + kr], acc0[1][i]);
+            acc1[1][i] = fmaf(w1, wrow[WS + kr + 1], acc1[1][i]);
           }
         }
       }
@@ -1773,7 +1775,7 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
         if (flag == 0) continue;
         const float f0 = lds.par[jj].f0, pb = lds.par[jj].b, pc = lds.par[jj].c, g = lds.par[jj].g;
         const float g1 = 1.f - g;
-        const Rec* wrow = lds.win + jj * W;
+        const float* wrow = lds.win + jj * SL * WS;
         const float t0 = fmaf(ftx, pb, f0);
         const int base = lds.par[jj].base;
         const int branch_a = __builtin_amdgcn_readfirstlane(lds.par[jj].branch_a);
@@ -1785,7 +1787,6 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
         for (int i = 0; i < ROWS; ++i) {
           const float kap = fmaf(fty[i], pc, t0);
           const int kr = (int)kap;
-          const Rec t = wrow[kr];
           const float f = __builtin_amdgcn_fractf(kap);
           float w0 = __builtin_amdgcn_fmed3f(fmaf(-g, f, 1.f), 0.f, 1.f);
           float w1 = __builtin_amdgcn_fmed3f(fmaf(g, f, g1), 0.f, 1.f);
@@ -1820,11 +1821,11 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
             w0 = (keep & 1) ? w0 : 0.f;
             w1 = (keep & 2) ? w1 : 0.f;
           }
-          acc0[0][i] = fmaf(w0, t.x, acc0[0][i]);
-          acc1[0][i] = fmaf(w1, t.y, acc1[0][i]);
+          acc0[0][i] = fmaf(w0, wrow[kr], acc0[0][i]);
+          acc1[0][i] = fmaf(w1, wrow[kr + 1], acc1[0][i]);
           if constexpr (SL == 2) {
-            acc0[1][i] = fmaf(w0, t.z, acc0[1][i]);
-            acc1[1][i] = fmaf(w1, t.w, acc1[1][i]);
+            acc0[1][i] = fmaf(w0, wrow[WS + kr], acc0[1][i]);
+            acc1[1][i] = fmaf(w1, wrow[WS + kr + 1], acc1[1][i]);
           }
         }
       }
@@ -1878,14 +1879,34 @@ static int ss_rt_lines(int n, int sl) {
   return 0;
 }
 
+// out[slice][a][r] = in[slice][a][r] * Delta_a for the angles j0 .. j1 - 1 (the
+// pixel-driven transpose stages pre-scaled rays)
+__global__ __launch_bounds__(256) void k_rtp_scale(SSDevPlan p, const float* __restrict__ in, int j0, int j1,
+                                                   float* __restrict__ out) {
+  const int R = p.g.nrays;
+  const size_t slice = blockIdx.z;
+  const int a = j0 + blockIdx.y;
+  if (a >= j1) return;
+  const size_t row = (slice * p.g.nangles + a) * (size_t)R;
+  const float d = p.angles[a].delta;
+  for (int r = blockIdx.x * 256 + threadIdx.x; r < R; r += gridDim.x * 256) out[row + r] = in[row + r] * d;
+}
+
 size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices) {
-  return p.rt_angles > 0 ? 0 : sizeof(float) * (size_t)nslices * p.g.n * p.g.n;
+  return p.rt_angles > 0 ? sizeof(float) * (size_t)nslices * p.g.nangles * p.g.nrays
+                         : sizeof(float) * (size_t)nslices * p.g.n * p.g.n;
 }
 
 hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int j0, int j1, int nslices,
                                      float* out, float* scratch, hipStream_t stream) {
   const int n = p.g.n;
   if (p.rt_angles > 0) {  // the pixel-driven form (every angle's rays at least one cell apart)
+    if (!scratch) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rtp_scale, dim3((p.g.nrays + 255) / 256, j1 - j0, nslices), dim3(256), 0, stream, p, sinoT,
+                       j0, j1, scratch);
+    const hipError_t es = hipGetLastError();
+    if (es != hipSuccess) return es;
+    const float* scaled = scratch;
     const int th = 8 * kSSBpRows[p.rt_variant];
     // slices in pairs (a pair shares each pixel's geometry) when a pair's stage holds at
     // least 8 angles, the last odd slice alone
@@ -1899,10 +1920,10 @@ hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int
 #define SS_RTP_LAUNCH(V, ROWS, STAGE)                                                                            \
   case V:                                                                                                        \
     if (part == 0)                                                                                               \
-      hipLaunchKernelGGL((k_radon_transpose_px<ROWS, STAGE, 2>), grid, dim3(256), 0, stream, p, sinoT, j0, j1, \
+      hipLaunchKernelGGL((k_radon_transpose_px<ROWS, STAGE, 2>), grid, dim3(256), 0, stream, p, scaled, j0, j1, \
                          slice0, out);                                                                           \
     else                                                                                                         \
-      hipLaunchKernelGGL((k_radon_transpose_px<ROWS, STAGE, 1>), grid, dim3(256), 0, stream, p, sinoT, j0, j1, \
+      hipLaunchKernelGGL((k_radon_transpose_px<ROWS, STAGE, 1>), grid, dim3(256), 0, stream, p, scaled, j0, j1, \
                          slice0, out);                                                                           \
     break;
       switch (p.rt_variant) {
