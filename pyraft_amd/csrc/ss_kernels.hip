@@ -1645,21 +1645,22 @@ struct RTPAngle {
 };
 
 // SL = 2: the block takes two slices of a launch and shares each pixel's geometry (kappa,
-// floor, fraction, weights) between them.  win holds, per staged angle and slice, the
-// window of Delta q_k for rays base .. base + W (rt_window W, WS = W + 4 floats apart),
-// copied straight from the pre-scaled sinogram into LDS; a pixel reads its two
-// candidates with one two-word LDS read per slice.
+// floor, fraction, weights) between them.  win holds, per staged angle, the window of
+// Delta q_k for rays base .. base + W (rt_window W; windows WS = W + 4 rays apart),
+// copied straight from the pre-scaled sinogram into LDS: one float per ray, or with two
+// slices the pair (slice 0, slice 1) per ray, as the pre-scaled copy interleaves them.  A
+// pixel reads its two candidates with one two-word (two-pair) LDS read.
 template <int STAGE, int SL>
 struct SSRtpLds {
-  float win[2 * STAGE];  // STAGE pairs' worth: rt_angles x WS floats
+  alignas(16) float win[2 * STAGE];  // STAGE pairs' worth: rt_angles x WS floats
   RTPAngle par[64];
 };
 
-// sl0 / sl1: the slices' sinograms [A][R], pre-scaled by each angle's Delta
+// sl: the sinogram [A][R] pre-scaled by each angle's Delta; with SL = 2 the two slices
+// interleaved, [A][R][2]
 template <int ROWS, int STAGE, int SL>
-__device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __restrict__ sl0,
-                                            const float* __restrict__ sl1, int j0, int j1, int c0, int r0,
-                                            float* __restrict__ out0, float* __restrict__ out1,
+__device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __restrict__ sl, int j0, int j1,
+                                            int c0, int r0, float* __restrict__ out0, float* __restrict__ out1,
                                             SSRtpLds<STAGE, SL>& lds) {
   const int n = p.g.n, R = p.g.nrays;
   const int W = p.rt_window;
@@ -1732,18 +1733,16 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
     for (int jj = wv; jj < nj; jj += 4) {
       const int fl = __builtin_amdgcn_readlane(q.flag, jj);
       const int bs = __builtin_amdgcn_readlane(q.base, jj);
-      for (int z = 0; z < SL; ++z) {
-        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)((z == 0 ? sl0 : sl1) + (size_t)(jb + jj) * R), (short)0, R * (int)sizeof(float), 0x00020000);
-        float* dst = lds.win + (jj * SL + z) * WS;
-        for (int w = 0; w < W + 1; w += 64) {
-          // the whole offset in the register (none in the instruction's offset field), so
-          // the range check sees a ray below 0 as out of range
-          int off = fl != 0 ? (bs + w + lane) * (int)sizeof(float) : 0x7ffffff0;
-          __asm__ volatile("" : "+v"(off));
-          if (w + lane <= W)  // active lanes write dst[w + lane]: stay in this window
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (ss_lds_ptr)(dst + w), 4, off, 0, 0, 0);
-        }
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(sl + (size_t)(jb + jj) * R * SL), (short)0, R * SL * (int)sizeof(float), 0x00020000);
+      float* dst = lds.win + jj * SL * WS;
+      for (int w = 0; w < SL * (W + 1); w += 64) {
+        // the whole offset in the register (none in the instruction's offset field), so
+        // the range check sees a ray below 0 as out of range
+        int off = fl != 0 ? (SL * bs + w + lane) * (int)sizeof(float) : 0x7ffffff0;
+        __asm__ volatile("" : "+v"(off));
+        if (w + lane < SL * (W + 1))  // active lanes write dst[w + lane]: stay in this window
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (ss_lds_ptr)(dst + w), 4, off, 0, 0, 0);
       }
     }
     __syncthreads();
@@ -1761,11 +1760,16 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
           const float f = __builtin_amdgcn_fractf(kap);
           const float w0 = __builtin_amdgcn_fmed3f(fmaf(-g, f, 1.f), 0.f, 1.f);
           const float w1 = __builtin_amdgcn_fmed3f(fmaf(g, f, g1), 0.f, 1.f);
-          acc0[0][i] = fmaf(w0, wrow[kr], acc0[0][i]);
-          acc1[0][i] = fmaf(w1, wrow[kr + 1], acc1[0][i]);
           if constexpr (SL == 2) {
-            acc0[1][i] = fmaf(w0, wrow[WS + kr], acc0[1][i]);
-            acc1[1][i] = fmaf(w1, wrow[WS + kr + 1], acc1[1][i]);
+            const float2 t0 = reinterpret_cast<const float2*>(wrow)[kr];
+            const float2 t1 = reinterpret_cast<const float2*>(wrow)[kr + 1];
+            acc0[0][i] = fmaf(w0, t0.x, acc0[0][i]);
+            acc1[0][i] = fmaf(w1, t1.x, acc1[0][i]);
+            acc0[1][i] = fmaf(w0, t0.y, acc0[1][i]);
+            acc1[1][i] = fmaf(w1, t1.y, acc1[1][i]);
+          } else {
+            acc0[0][i] = fmaf(w0, wrow[kr], acc0[0][i]);
+            acc1[0][i] = fmaf(w1, wrow[kr + 1], acc1[0][i]);
           }
         }
       }
@@ -1821,11 +1825,16 @@ __device__ __forceinline__ void ss_rtp_item(const SSDevPlan& p, const float* __r
             w0 = (keep & 1) ? w0 : 0.f;
             w1 = (keep & 2) ? w1 : 0.f;
           }
-          acc0[0][i] = fmaf(w0, wrow[kr], acc0[0][i]);
-          acc1[0][i] = fmaf(w1, wrow[kr + 1], acc1[0][i]);
           if constexpr (SL == 2) {
-            acc0[1][i] = fmaf(w0, wrow[WS + kr], acc0[1][i]);
-            acc1[1][i] = fmaf(w1, wrow[WS + kr + 1], acc1[1][i]);
+            const float2 t0 = reinterpret_cast<const float2*>(wrow)[kr];
+            const float2 t1 = reinterpret_cast<const float2*>(wrow)[kr + 1];
+            acc0[0][i] = fmaf(w0, t0.x, acc0[0][i]);
+            acc1[0][i] = fmaf(w1, t1.x, acc1[0][i]);
+            acc0[1][i] = fmaf(w0, t0.y, acc0[1][i]);
+            acc1[1][i] = fmaf(w1, t1.y, acc1[1][i]);
+          } else {
+            acc0[0][i] = fmaf(w0, wrow[kr], acc0[0][i]);
+            acc1[0][i] = fmaf(w1, wrow[kr + 1], acc1[0][i]);
           }
         }
       }
@@ -1856,8 +1865,9 @@ __global__ __launch_bounds__(256, RTP_MIN_BLOCKS(ROWS, SL)) void k_radon_transpo
   __shared__ SSRtpLds<STAGE, SL> lds;
   const size_t slice = (size_t)slice0 + (size_t)blockIdx.z * SL;
   const size_t sino = (size_t)p.g.nangles * p.g.nrays, img = (size_t)p.g.n * p.g.n;
-  ss_rtp_item<ROWS, STAGE, SL>(p, qT + slice * sino, qT + (slice + SL - 1) * sino, j0, j1, blockIdx.x * BP_TILE_W,
-                               blockIdx.y * 8 * ROWS, out + slice * img, out + (slice + SL - 1) * img, lds);
+  // the pre-scaled copy holds slice pairs interleaved in the pairs' own space
+  ss_rtp_item<ROWS, STAGE, SL>(p, qT + slice * sino, j0, j1, blockIdx.x * BP_TILE_W, blockIdx.y * 8 * ROWS,
+                               out + slice * img, out + (slice + SL - 1) * img, lds);
 }
 
 // LDS a block may use on the current device (gfx950: 160 KiB, also assumed if the query
@@ -1879,17 +1889,22 @@ static int ss_rt_lines(int n, int sl) {
   return 0;
 }
 
-// out[slice][a][r] = in[slice][a][r] * Delta_a for the angles j0 .. j1 - 1 (the
-// pixel-driven transpose stages pre-scaled rays)
+// The pixel-driven transpose's pre-scaled rays: in[slice][a][r] * Delta_a for the angles
+// j0 .. j1 - 1; slices 2i, 2i + 1 (i < pairs) interleaved in their own space,
+// out[2i][a][r][2], the others as they are
 __global__ __launch_bounds__(256) void k_rtp_scale(SSDevPlan p, const float* __restrict__ in, int j0, int j1,
-                                                   float* __restrict__ out) {
+                                                   int pairs, float* __restrict__ out) {
   const int R = p.g.nrays;
   const size_t slice = blockIdx.z;
   const int a = j0 + blockIdx.y;
   if (a >= j1) return;
-  const size_t row = (slice * p.g.nangles + a) * (size_t)R;
+  const size_t sino = (size_t)p.g.nangles * R;
+  const size_t row = slice * sino + (size_t)a * R;
   const float d = p.angles[a].delta;
-  for (int r = blockIdx.x * 256 + threadIdx.x; r < R; r += gridDim.x * 256) out[row + r] = in[row + r] * d;
+  const bool paired = slice < 2 * (size_t)pairs;
+  float* __restrict__ dst = paired ? out + (slice & ~(size_t)1) * sino + (size_t)a * R * 2 + (slice & 1) : out + row;
+  const int step = paired ? 2 : 1;
+  for (int r = blockIdx.x * 256 + threadIdx.x; r < R; r += gridDim.x * 256) dst[(size_t)r * step] = in[row + r] * d;
 }
 
 size_t ss_radon_transpose_scratch(const SSDevPlan& p, int nslices) {
@@ -1902,15 +1917,15 @@ hipError_t ss_launch_radon_transpose(const SSDevPlan& p, const float* sinoT, int
   const int n = p.g.n;
   if (p.rt_angles > 0) {  // the pixel-driven form (every angle's rays at least one cell apart)
     if (!scratch) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rtp_scale, dim3((p.g.nrays + 255) / 256, j1 - j0, nslices), dim3(256), 0, stream, p, sinoT,
-                       j0, j1, scratch);
-    const hipError_t es = hipGetLastError();
-    if (es != hipSuccess) return es;
-    const float* scaled = scratch;
     const int th = 8 * kSSBpRows[p.rt_variant];
     // slices in pairs (a pair shares each pixel's geometry) when a pair's stage holds at
     // least 8 angles, the last odd slice alone
     const int pairs = (p.rt_angles / 2 >= 8) ? nslices / 2 : 0;
+    hipLaunchKernelGGL(k_rtp_scale, dim3((p.g.nrays + 255) / 256, j1 - j0, nslices), dim3(256), 0, stream, p, sinoT,
+                       j0, j1, pairs, scratch);
+    const hipError_t es = hipGetLastError();
+    if (es != hipSuccess) return es;
+    const float* scaled = scratch;
     const int singles = nslices - 2 * pairs;
     for (int part = 0; part < 2; ++part) {
       const int count = part == 0 ? pairs : singles;
