@@ -550,6 +550,10 @@ int ensure_ramp(raft_ss_plan* p, hipStream_t stream) {
   return 0;
 }
 
+// Sinograms of at most this many values are filtered straight from / to the (R, A)
+// layout (strided rows; for small launches the layout copies cost more than the strides)
+constexpr size_t kFilterInPlaceLayout = (size_t)1 << 20;
+
 // the plan's FBP filter over `nrows` angle rows: the cached FFT form where it fits
 hipError_t plan_filter(const raft_ss_plan* p, const float* in, float* out, int nrows, hipStream_t stream) {
   if (p->d_ramp_fft) return ss_launch_fft_filter(in, out, p->ramp_fft, nrows, stream);
@@ -681,6 +685,10 @@ extern "C" int raft_ss_fbp_filter(raft_ss_plan* p, const float* d_sino, float* d
     HIP_TRY(plan_filter(p, d_sino, d_out, nslices * A, stream));
     return 0;
   }
+  if (p->d_ramp_fft && total <= kFilterInPlaceLayout) {
+    HIP_TRY(ss_launch_fft_filter(d_sino, d_out, p->ramp_fft, nslices * A, stream, A, A));
+    return 0;
+  }
   Scratch ws;
   HIP_TRY(ws.alloc(2 * total * sizeof(float), stream));
   float* t0 = ws.as<float>();
@@ -705,8 +713,12 @@ extern "C" int raft_ss_fbp(raft_ss_plan* p, const float* d_sino, float* d_image,
   HIP_TRY(ws.alloc(2 * total * sizeof(float) + ss_bp_scratch(dp, 0, A, nslices), stream));
   float* t0 = ws.as<float>();
   float* t1 = t0 + total;
-  HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
-  HIP_TRY(plan_filter(p, t0, t1, nslices * A, stream));
+  if (p->d_ramp_fft && total <= kFilterInPlaceLayout) {
+    HIP_TRY(ss_launch_fft_filter(d_sino, t1, p->ramp_fft, nslices * A, stream, A, 0));
+  } else {
+    HIP_TRY(ss_launch_sino_to_t(d_sino, t0, R, A, nslices, stream));
+    HIP_TRY(plan_filter(p, t0, t1, nslices * A, stream));
+  }
   SSEmUpdateArgs none{nullptr, nullptr};
   HIP_TRY(ss_launch_backprojection(dp, t1, 0, A, nslices, SS_BP_PLAIN, d_image, none, t1 + total, stream));
   return 0;

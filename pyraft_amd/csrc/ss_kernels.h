@@ -196,7 +196,9 @@ struct SSFftTaps {
 int ss_fft_size(int R, int* lg);     // M = the power of two >= 2R (0 if above SS_FFT_MAX)
 size_t ss_fft_taps_bytes(int R);     // device bytes of W and G (0 if the FFT form does not fit)
 hipError_t ss_fft_taps_build(const float* g, int R, bool toeplitz, void* mem, SSFftTaps* t, hipStream_t stream);
-hipError_t ss_launch_fft_filter(const float* in, float* out, const SSFftTaps& t, int nrows, hipStream_t stream);
+// in_cols / out_cols: 0 for angle-major rows, A for the (R, A) layout (row = slice * A + a)
+hipError_t ss_launch_fft_filter(const float* in, float* out, const SSFftTaps& t, int nrows, hipStream_t stream,
+                                int in_cols = 0, int out_cols = 0);
 hipError_t ss_launch_ramp_filter(const float* in, float* out, const float* g, int R, int nrows, hipStream_t stream,
                                  bool toeplitz = false);
 

@@ -2163,21 +2163,30 @@ __global__ __launch_bounds__(FF_THREADS) void k_fft_taps(const float* __restrict
   for (int m = threadIdx.x; m < M; m += FF_THREADS) G[m] = f[m];
 }
 
+// Row r of the filter's input / output: angle-major (cols = 0: row r at r * R, unit
+// stride) or the (R, A) layout (cols = A: row r = slice * A + a at slice * R * A + a,
+// stride A), which spares a small sinogram the layout copies around the filter.
+__device__ __forceinline__ size_t ss_ff_row(int r, int R, int cols) {
+  return cols > 0 ? (size_t)(r / cols) * R * cols + (r % cols) : (size_t)r * R;
+}
+
 __global__ __launch_bounds__(FF_THREADS) void k_fft_filter(const float* __restrict__ in, float* __restrict__ out,
                                                            const double2* __restrict__ G,
                                                            const double2* __restrict__ W, int R, int M, int lgM,
-                                                           int nrows) {
+                                                           int nrows, int in_cols, int out_cols) {
   extern __shared__ double2 zb[];
   double2* a = zb;
   double2* b = zb + M;
   const int r0 = 2 * blockIdx.x;
   const bool two = r0 + 1 < nrows;
-  const float* x0 = in + (size_t)r0 * R;
+  const size_t is = in_cols > 0 ? (size_t)in_cols : 1;
+  const float* x0 = in + ss_ff_row(r0, R, in_cols);
+  const float* x1 = in + ss_ff_row(two ? r0 + 1 : r0, R, in_cols);
   for (int m = threadIdx.x; m < M; m += FF_THREADS) {
     double2 z = make_double2(0.0, 0.0);
     if (m < R) {
-      z.x = x0[m];
-      if (two) z.y = x0[R + m];
+      z.x = x0[m * is];
+      if (two) z.y = x1[m * is];
     }
     a[m] = z;
   }
@@ -2187,11 +2196,13 @@ __global__ __launch_bounds__(FF_THREADS) void k_fft_filter(const float* __restri
   __syncthreads();
   const double2* c = ss_fft_lds<true>(f, f == a ? b : a, W, M, lgM);
   const double scale = 1.0 / (double)M;
-  float* y0 = out + (size_t)r0 * R;
+  const size_t os = out_cols > 0 ? (size_t)out_cols : 1;
+  float* y0 = out + ss_ff_row(r0, R, out_cols);
+  float* y1 = out + ss_ff_row(two ? r0 + 1 : r0, R, out_cols);
   for (int n = threadIdx.x; n < R; n += FF_THREADS) {
     const double2 v = c[n + R];
-    y0[n] = (float)(v.x * scale);
-    if (two) y0[R + n] = (float)(v.y * scale);
+    y0[n * os] = (float)(v.x * scale);
+    if (two) y1[n * os] = (float)(v.y * scale);
   }
 }
 
@@ -2232,10 +2243,11 @@ hipError_t ss_fft_taps_build(const float* g, int R, bool toeplitz, void* mem, SS
   return hipSuccess;
 }
 
-hipError_t ss_launch_fft_filter(const float* in, float* out, const SSFftTaps& t, int nrows, hipStream_t stream) {
+hipError_t ss_launch_fft_filter(const float* in, float* out, const SSFftTaps& t, int nrows, hipStream_t stream,
+                                int in_cols, int out_cols) {
   if (nrows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fft_filter, dim3((nrows + 1) / 2), dim3(FF_THREADS), 2 * sizeof(double2) * (size_t)t.M,
-                     stream, in, out, t.G, t.W, t.R, t.M, t.lgM, nrows);
+                     stream, in, out, t.G, t.W, t.R, t.M, t.lgM, nrows, in_cols, out_cols);
   return hipGetLastError();
 }
 
