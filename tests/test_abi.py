@@ -187,7 +187,9 @@ def _rt_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC, variant=3):
     W, J = Wv[variant], Jv[variant]
     if J == 0:
         return None
-    assert 1 <= J <= 64 and J * W <= BP_STAGE[variant]
+    # a stage of 2 STAGE floats holds J windows of W + 1 rays, W + 4 floats apart (with
+    # slice pairs J / 2 angles of two interleaved windows: the same floats)
+    assert 1 <= J <= 64 and J * (W + 4) <= 2 * BP_STAGE[variant]
     assert np.all(table[:, 4] >= 1.0)
     tile_h = 8 * BP_ROWS[variant]
     ig, a0, rb, br = table[:, 0], table[:, 1], table[:, 2], table[:, 3] != 0
@@ -216,8 +218,9 @@ def _rt_window_extremes(lib, n, R, A, ic=DEF_IC, sc=DEF_SC, variant=3):
                                          (100, 40, 50, DEF_IC, DEF_SC),
                                          (48, 40, 91, (-0.7, 1.3, 1.2, -0.9), (-math.pi / 2, 1.1, math.pi / 2, -0.8))])
 def test_transpose_window_in_bounds(lib, n, R, A, ic, sc, variant):
-    """The pixel-driven transpose reads the staged pair at floor(kappa - base): inside the
-    window, and kappa - base >= 0 so truncation equals the floor."""
+    """The pixel-driven transpose reads the staged rays floor(kappa - base) and the next
+    one: both inside the window of W + 1 rays, and kappa - base >= 0 so truncation equals
+    the floor."""
     r = _rt_window_extremes(lib, n, R, A, ic, sc, variant=variant)
     if r is None:
         pytest.skip("rays closer than a cell, or the window does not fit this variant")
