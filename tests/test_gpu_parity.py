@@ -293,6 +293,28 @@ def test_transpose_slice_batches(torch_mod, nslices, R):
     assert rel_l2(got[-1], O.radon_transpose(y[-1].cpu().numpy().astype(np.float64), n)) <= REL_L2
 
 
+def test_transpose_angle_ranges(torch_mod):
+    """The transpose of an angle range [j0, j1) (the autograd backward of a range
+    forward) from both layouts and with slice pairs plus a single slice: the oracle's
+    transpose of the sinogram with the other angles zeroed.  The launch pre-scales only
+    the range's rows and interleaves the pair, so a wrong row or slice offset fails."""
+    torch = torch_mod
+    n, R, A, ns = 128, 120, 90, 3
+    p = _plan(n, R, A)
+    s = np.random.default_rng(31).random((ns, R, A)).astype(np.float32)
+    st = torch.from_numpy(s).cuda()
+    for j0, j1 in ((0, A), (17, 61), (89, 90)):
+        print("transpose range", j0, j1, flush=True)
+        got = p.radon_transpose(st, angles=(j0, j1)).cpu().numpy()
+        got_am = p.radon_transpose(st.transpose(-1, -2).contiguous(), angle_major=True, angles=(j0, j1)).cpu().numpy()
+        for i in range(ns):
+            y = np.zeros((R, A))
+            y[:, j0:j1] = s[i, :, j0:j1]
+            ref = O.radon_transpose(y, n)
+            assert rel_l2(got[i], ref) <= REL_L2, (j0, j1, i)
+            assert np.array_equal(got_am[i], got[i]), (j0, j1, i)
+
+
 @pytest.mark.parametrize("n,R,A,ns", [(256, 256, 180, 1), (100, 103, 61, 3), (64, 64, 200, 2)])
 def test_small_image_angle_groups(torch_mod, n, R, A, ns):
     """Small images divide a backprojection's angles into groups (partial images added
