@@ -213,16 +213,18 @@ hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const 
                             int nslices, hipStream_t stream);
 // The transpose's angles split into the pixel-driven set (gather table, ngather) and the
 // ray-traced set (walk list of sinogram columns, nwalk).
-// Two-candidate gather (k_dda_gather2): block of DDA_G2_COLS columns x 4 DDA_G2_LINES
-// lines, angles staged DDA_G2_ANGLES at a time, each in a segment of at most
-// DDA_G2_WSMAX rays
+// Two-candidate gather (k_dda_gather2): block of DDA_G2_COLS columns x 4 G lines (G
+// lines per thread: DDA_G2_LINES, or DDA_G2_LINES_SMALL where the larger tiles would
+// leave the CUs short of blocks), angles staged DDA_G2_ANGLES at a time, each in a
+// segment of at most DDA_G2_WSMAX rays
 #define DDA_G2_COLS 64
-#define DDA_G2_LINES 8
+#define DDA_G2_LINES 16
+#define DDA_G2_LINES_SMALL 8
 #define DDA_G2_ANGLES 16
 #define DDA_G2_WSMAX 160
 
-// segment length of the two-candidate gather for an angle: the block's span of
-// rc - half, (cols - 1)|kx| + (lines - 1)|ky|, plus 8
+// segment length of the two-candidate gather for an angle (either tile height): the
+// block's span of rc - half, (cols - 1)|kx| + (lines - 1)|ky|, plus 8
 static inline int dda_gather2_ws(const SSDdaGatherAngle& ga) {
   const double span = (DDA_G2_COLS - 1) * fabs(ga.kx) + (4 * DDA_G2_LINES - 1) * fabs(ga.ky);
   return span < DDA_G2_WSMAX ? (int)ceil(span) + 8 : DDA_G2_WSMAX + 1;

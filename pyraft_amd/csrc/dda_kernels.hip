@@ -278,14 +278,14 @@ __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGath
 }
 
 // The two-candidate gather of a sinogram (k <= 2 for every angle) in FP32 ray units.
-// A block covers DDA_G2_COLS columns x 4 DDA_G2_LINES lines of pixels and takes the
+// A block covers DDA_G2_COLS columns x 4 G lines of pixels and takes the
 // angles in groups of DDA_G2_ANGLES: each wave stages, for a quarter of the group's
 // angles, the segment of the angle's (lmax-scaled, zero-padded) row that the block's
 // candidate rays span into LDS (rays rb .. rb + ws - 1 with rb = floor(min over the block
 // of rc - half) - 2; the resource spans the padded row, so rays off the row read 0),
 // or marks the angle as missing the block.  Then per (thread, angle) FP64 gives
 // rc - half - rb at the thread's first line and splits it into an integer base and an
-// FP32 fraction g0 in [0, 1); each of the thread's DDA_G2_LINES lines takes
+// FP32 fraction g0 in [0, 1); each of the thread's G lines takes
 // fl = floor(g), fr = fract(g) of g = g0 + rr ky, reads q[r0], q[r0 + 1]
 // (r0 = rb + base + fl + 1) from the segment with one two-word LDS read and adds
 // q0 z0 + q1 z1 with the angle's hat weights.  The segment index base + fl + 1 lies in
@@ -305,10 +305,9 @@ __device__ __forceinline__ int dda_floor_i32(float v) {
 #define DDA_G2_MISS (-0x7fffffff - 1)
 typedef __attribute__((address_space(3))) float dda_lds_float;
 
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(256) void k_dda_gather2(SSDdaGeom g, const SSDdaGatherAngle* __restrict__ ga, int nga,
                                                      const float* __restrict__ rows, int ws, T* __restrict__ acc) {
-  constexpr int G = DDA_G2_LINES;
   constexpr int NA = DDA_G2_ANGLES;
   __shared__ float seg[NA * DDA_G2_WSMAX];
   __shared__ int seg_rb[NA];
@@ -541,10 +540,18 @@ hipError_t dda_launch_transpose(const SSDdaGeom& g, const SSDdaTransposePlan& tp
       hipLaunchKernelGGL(k_dda_sino_rows, dda_tiles(g.nrays, g.nangles, nslices), dim3(DDA_TILE, 8), 0, stream,
                          g.nrays, g.nangles, sino, tp.row_scale, sinoT);
     const dim3 grid((g.cols + DDA_GX - 1) / DDA_GX, (g.lines + DDA_GY - 1) / DDA_GY, nslices);
-    if (tp.kmax <= 2 && !ones && tp.gather_ws > 0)
-      hipLaunchKernelGGL((k_dda_gather2<T>), dim3((g.cols + DDA_G2_COLS - 1) / DDA_G2_COLS,
-                                                  (g.lines + 4 * DDA_G2_LINES - 1) / (4 * DDA_G2_LINES), nslices),
-                         dim3(256), 0, stream, g, tp.gather, tp.ngather, (const float*)sinoT, tp.gather_ws, acc);
+    if (tp.kmax <= 2 && !ones && tp.gather_ws > 0) {
+      // the taller tiles halve the per-angle work per pixel; they need a block per CU
+      const int gx = (g.cols + DDA_G2_COLS - 1) / DDA_G2_COLS;
+      const int gy = (g.lines + 4 * DDA_G2_LINES - 1) / (4 * DDA_G2_LINES);
+      if ((long long)gx * gy * nslices >= 256)
+        hipLaunchKernelGGL((k_dda_gather2<T, DDA_G2_LINES>), dim3(gx, gy, nslices), dim3(256), 0, stream, g,
+                           tp.gather, tp.ngather, (const float*)sinoT, tp.gather_ws, acc);
+      else
+        hipLaunchKernelGGL((k_dda_gather2<T, DDA_G2_LINES_SMALL>), dim3(gx, (g.lines + 4 * DDA_G2_LINES_SMALL - 1) /
+                                                                          (4 * DDA_G2_LINES_SMALL), nslices),
+                           dim3(256), 0, stream, g, tp.gather, tp.ngather, (const float*)sinoT, tp.gather_ws, acc);
+    }
     else if (tp.kmax <= 2)
       hipLaunchKernelGGL((k_dda_gather<T, 2>), grid, dim3(256), 0, stream, g, tp.gather, tp.ngather,
                          (const float*)sinoT, ones ? 1 : 0, acc);

@@ -332,6 +332,26 @@ def test_gpu_random_vs_oracle(torch_mod, shape, R, A, ic, sc):
 
 
 @pytest.mark.gpu
+def test_gpu_transpose_tall_tiles_vs_oracle(torch_mod):
+    """A launch of at least 256 gather blocks takes the 64 x 64-pixel tiles (16 lines
+    per thread): a 1024^2 image against the oracle's transpose and backprojection."""
+    torch = torch_mod
+    from pyraft_amd.device import DdaPlan
+    n, R, A = 1024, 1024, 48
+    geo = _geo(DEF_IC, DEF_SC)
+    y = np.random.default_rng(12).random((R, A)).astype(np.float32)
+    p = DdaPlan((n, n), R, A)
+    got_t = p.radon_transpose(torch.from_numpy(y).cuda()).cpu().numpy()
+    got_b = p.backprojection(torch.from_numpy(y).cuda()).cpu().numpy()
+    y64 = y.astype(np.float64)
+    assert rel_l2(got_t, O.radon_transpose_bresenham(y64, (n, n), **geo)) <= REL_L2
+    ref = O.backprojection_bresenham(y64, (n, n), **geo)
+    assert _nan_match(got_b, ref)
+    ok = ~np.isnan(ref)
+    assert rel_l2(got_b[ok], ref[ok]) <= REL_L2
+
+
+@pytest.mark.gpu
 def test_gpu_straight_angles(torch_mod):
     """Square pixels, 181 angles over [0, pi]: the axis angles 0, pi/2 and pi run straight
     (one column or line per ray, full length), so the transpose sums them per column or
