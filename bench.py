@@ -519,7 +519,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only (got --gpus %d, WORLD_SIZE %d); "
+        print("bench.py: measures one GPU only (got --gpus %d, WORLD_SIZE %d); "
               "run with --gpus 1" % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
 

@@ -36,8 +36,8 @@ def backprojection_plan(sino_shape=(256, 180), shape=(256, 256), padding=2, meth
     if method in (radon_method.SLANTSTACK, radon_method.BRESENHAM):
         return (method, shape, padding, )
     if method in (radon_method.BST, radon_method.ANDERSSON, radon_method.ANDERSSONPB):
-        raise NotImplementedError("backprojection_plan: only the SLANTSTACK and BRESENHAM methods are part "
-                                  "of this build")
+        raise NotImplementedError("backprojection_plan: only the SLANTSTACK and BRESENHAM methods are "
+                                  "implemented in pyraft_amd")
     raise TypeError('Unsupported method for Backprojection!')
 
 

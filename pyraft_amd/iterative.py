@@ -13,7 +13,7 @@ raft/raft_cuda_aux.cu:21) and FP64 log-likelihood partials, the backprojection
 applies the update in its epilogue.  OSEM subset b covers the contiguous angle
 columns [b*A/S, (b+1)*A/S) (pyraft/iterative/em.py:284-312) on the full
 sinogram's angle grid.  Pixels with BP_b(1) = 0 keep their value.  (pyraft's
-``osem`` uses the NFFT slice pair, which is not part of this build.)
+``osem`` uses the NFFT slice pair, which pyraft_amd does not implement.)
 """
 import ctypes
 import math
@@ -49,7 +49,7 @@ def _device(a):
 
 
 def _composed_em(sino, shape, projection_method, backprojection_method, weight, **kwargs):
-    """pyraft/iterative/em.py:109-142 with this build's radon / radon_transpose."""
+    """pyraft/iterative/em.py:109-142 with pyraft_amd's radon / radon_transpose."""
     from .backprojection import radon_transpose
     from .sinogram import radon
     if weight is None:

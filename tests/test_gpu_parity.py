@@ -1,7 +1,7 @@
 """GPU parity of the HIP slant-stack path against the FP64 oracle.
 
 Inputs are FP32 arrays; the oracle receives the same values cast to FP64.  Bars:
-relative L2 <= 1e-5 for Radon, backprojection, FBP and EM (north-star tolerance),
+relative L2 <= 1e-5 for Radon, backprojection, FBP and EM,
 adjoint mismatch <= 1e-6 for the exact transpose, bit-exact plan ranges.
 """
 import math
