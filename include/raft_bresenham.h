@@ -81,6 +81,8 @@ int raft_dda_weights(raft_dda_plan *plan, float *out, void *stream);
  * by move (FP64 drift) the visited pixels; on exact-geometry angles the trapezoid form's
  * nonzero candidates per fast index (Q32 stepping as on the device, weights in FP64).
  * ij[2k], ij[2k+1] = line, column and len[k] = intersection length of the k-th pixel.
+ * Where the device stages a wave's image segments in LDS, it also checks that both
+ * candidates lie inside the segment the ray's wave copies (RAFT_SS_EINTERNAL if not).
  * Returns the number of pixels (at most cap are written) or a RAFT_SS_E* code. */
 int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, const double *img_corners,
                       const double *sino_corners, int angle, int ray, int *ij, double *len, int cap);

@@ -112,6 +112,8 @@ SSDdaAngle make_angle(const SSDdaGeom& g, double theta) {
   a.axis = a.s == 0.0 ? 1 : (a.c == 0.0 ? 2 : 0);
   a.k_len = a.e_step > 0.0 ? a.len_full / a.e_step : 0.0;
   a.fixed = 0;
+  a.tr_wstage = a.tr_lowlane = a.tr_wgap = 0;
+  a.tr_qchunk = 0;
   a.tr_inv = a.tr_off = a.tr_d = a.tr_lmax = 0.0;
   a.tr_dhi = 0;
   a.tr_dlo = 0u;
@@ -222,7 +224,7 @@ void set_trapezoid(const SSDdaGeom& g, const SSDdaGatherAngle& ga, SSDdaAngle& a
   if (!isfinite(tt.d) || !(fabs(tt.d) <= 2.0) || !(tt.half_s <= 1.0 + 1e-9) || !isfinite(tt.kz)) return;
   const long long dfix = llround(tt.d * 4294967296.0);
   a.tr_inv = 1.0 / tt.step;
-  a.tr_off = -(g.ox * a.c + g.oy * a.s) / tt.step - tt.half_s + 3.0;
+  a.tr_off = -(g.ox * a.c + g.oy * a.s) / tt.step - tt.half_s + (DDA_PADW + 1.0);
   a.tr_d = tt.d;
   a.tr_lmax = ga.lmax;
   a.tr_dhi = (int)(dfix >> 32);
@@ -231,6 +233,11 @@ void set_trapezoid(const SSDdaGeom& g, const SSDdaGatherAngle& ga, SSDdaAngle& a
   a.tr_kz = (float)(tt.kz / 4294967296.0);
   a.tr_pz = (float)tt.pz;
   a.tr_pz1 = (float)(tt.pz - tt.kz * (2.0 - tt.half_s));
+  const double spread = 63.0 * fabs(g.dt * a.tr_inv);  // tau spread over a wave's rays
+  a.tr_wstage = spread + (DDA_WS_ROWS - 1) * fabs(a.tr_d) + 9.0 <= DDA_WS_SEG - 2.0 ? 1 : 0;
+  a.tr_lowlane = g.dt * a.tr_inv >= 0.0 ? 0 : 63;
+  a.tr_wgap = a.tr_d < 0.0 ? (int)floor((DDA_WS_ROWS - 1) * fabs(a.tr_d)) + 1 : 0;
+  a.tr_qchunk = dfix * DDA_WS_ROWS;
   a.fixed = 1;
 }
 
@@ -461,25 +468,50 @@ extern "C" int raft_dda_host_ray(int lines, int cols, int nrays, int nangles, co
   };
   if (a.fixed) {
     const TrapTerms tt = trap_terms(g, ga, a);
-    const double tau0 = fma(t, a.tr_inv, a.tr_off);
-    const double ka = (1.0 - tau0) / a.tr_d, kb = (ns + 2.0 - tau0) / a.tr_d;
-    const double kl = fmax(fmin(ka, kb), -1.0), kh = fmin(fmax(ka, kb), (double)nf);
-    if (!(kl <= kh)) return 0;
-    const int klo = (int)fmax(0.0, floor(kl) - 1.0), khi = (int)fmin(nf - 1.0, ceil(kh) + 1.0);
-    const double v = fmin(fmax(fma((double)klo, a.tr_d, tau0), -1073741824.0), 1073741824.0);
-    const double fv = floor(v);
-    int hi = (int)fv;
-    unsigned lo = (unsigned)((v - fv) * 4294967296.0);
+    const int pitch = dda_pad_pitch(ns);
+    auto tau_of = [&](int rr) { return fma(g.t0 + (rr * g.dt), a.tr_inv, a.tr_off); };
+    // the wave's fast range: the union over its 64 rays (those below nrays)
+    const int w0 = ray & ~63;
+    int klo = nf, khi = -1;
+    for (int l = 0; l < 64 && w0 + l < nrays; ++l) {
+      const double tl = tau_of(w0 + l);
+      const double ka = (DDA_PADW - 1.0 - tl) / a.tr_d, kb = (ns + DDA_PADW - tl) / a.tr_d;
+      const double kl = fmax(fmin(ka, kb), -1.0), kh = fmin(fmax(ka, kb), (double)nf);
+      if (!(kl <= kh)) continue;
+      klo = std::min(klo, (int)fmax(0.0, floor(kl) - 1.0));
+      khi = std::max(khi, (int)fmin(nf - 1.0, ceil(kh) + 1.0));
+    }
+    if (klo > khi) return 0;
+    auto start = [&](double tau) {
+      const double v = fmin(fmax(fma((double)klo, a.tr_d, tau), -1073741824.0), 1073741824.0);
+      const double fv = floor(v);
+      return std::make_pair((int)fv, (unsigned)((v - fv) * 4294967296.0));
+    };
+    auto step = [&](std::pair<int, unsigned>& st) {
+      const unsigned nlo = st.second + a.tr_dlo;
+      st.first += a.tr_dhi + (nlo < st.second ? 1 : 0);
+      st.second = nlo;
+    };
+    std::pair<int, unsigned> st = start(tau_of(ray)), low = start(tau_of(w0 + a.tr_lowlane));
+    int sb = 0;
     for (int k = klo; k <= khi; ++k) {
-      const int idx = (int)std::min((unsigned)hi, (unsigned)(ns + 2));  // padded slow index
-      const double fr = lo / 4294967296.0;
+      int idx = st.first;  // padded slow index of the first candidate
+      if (a.tr_wstage) {
+        // the chunk's staged segment holds both candidates
+        if ((k - klo) % DDA_WS_ROWS == 0) sb = (low.first - 1 - a.tr_wgap) & ~3;
+        if (idx - sb < 0 || idx + 1 - sb > DDA_WS_SEG - 1 || sb < 0 || sb + DDA_WS_SEG > pitch)
+          return ss_set_error(RAFT_SS_EINTERNAL, "ray %d angle %d: candidate %d outside the staged segment [%d, %d)",
+                              ray, angle, idx, sb, sb + DDA_WS_SEG);
+      } else {
+        idx = (int)std::min((unsigned)idx, (unsigned)(pitch - 2));  // clamped into the padding
+      }
+      const double fr = st.second / 4294967296.0;
       const double u0 = fabs(1.0 - tt.half_s - fr), u1 = 2.0 - tt.half_s - fr;
       const double z0 = fmin(1.0, fmax(0.0, tt.pz - tt.kz * u0)), z1 = fmin(1.0, fmax(0.0, tt.pz - tt.kz * u1));
-      if ((unsigned)(idx - 2) < (unsigned)ns && z0 > 0.0) emit(k, idx - 2, a.tr_lmax * z0);
-      if ((unsigned)(idx - 1) < (unsigned)ns && z1 > 0.0) emit(k, idx - 1, a.tr_lmax * z1);
-      const unsigned nlo = lo + a.tr_dlo;
-      hi += a.tr_dhi + (nlo < lo ? 1 : 0);
-      lo = nlo;
+      if ((unsigned)(idx - DDA_PADW) < (unsigned)ns && z0 > 0.0) emit(k, idx - DDA_PADW, a.tr_lmax * z0);
+      if ((unsigned)(idx + 1 - DDA_PADW) < (unsigned)ns && z1 > 0.0) emit(k, idx + 1 - DDA_PADW, a.tr_lmax * z1);
+      step(st);
+      step(low);
     }
     return count;
   }

@@ -57,16 +57,26 @@ struct SSDdaAngle {
   int axis;                 // 1: sin == 0, 2: cos == 0, 0: oblique
   int fixed;                // exact-geometry angle: the Radon takes the trapezoid form
   // Trapezoid form of the Radon on these angles: along fast index k the ray crosses
-  // the slow axis at sigma = t tr_inv + (tr_off - 3) + half_s + k tr_d (slow-pixel units),
+  // the slow axis at sigma = t tr_inv + (tr_off - DDA_PADW - 1) + half_s + k tr_d (slow-pixel units),
   // and its length in the pixel at slow index j is lmax z(|sigma - j|) with
   // z(u) = clamp((p - u |step|) / (p - q)) (p, q, lmax of SSDdaGatherAngle; |step| the
-  // slow pitch times the trig factor).  With tau = sigma - half_s + 3 the candidates
-  // are the padded slow indices floor(tau), floor(tau) + 1 (two zero columns either side),
-  // at u0 = 1 - half_s - fr and u1 = 2 - half_s - fr > 0 for fr the fraction of tau.
+  // slow pitch times the trig factor).  With tau = sigma - half_s + DDA_PADW + 1 the
+  // candidates are the padded slow indices floor(tau), floor(tau) + 1, at
+  // u0 = 1 - half_s - fr and u1 = 2 - half_s - fr > 0 for fr the fraction of tau.
   double tr_inv, tr_off, tr_d, tr_lmax;
   int tr_dhi;               // tr_d in Q32: tr_dhi + tr_dlo 2^-32
   unsigned tr_dlo;
   float tr_c1, tr_kz, tr_pz, tr_pz1;  // (1 - half_s) 2^32, kz 2^-32, pz, pz - kz (2 - half_s)
+  // Wave-staged form: the 64 rays of a wave span tau differences up to 63 |dt tr_inv|,
+  // and over the DDA_WS_ROWS fast indices of a chunk tau moves by at most
+  // (DDA_WS_ROWS - 1) |tr_d|, so one DDA_WS_SEG-pixel segment per fast index, starting at
+  // the lowest ray's floor(tau) at the chunk's first index, less 1 and tr_wgap, aligned
+  // down to 4, holds every lane's candidates on every row of the chunk when
+  // spread + (DDA_WS_ROWS - 1) |tr_d| + 9 fits.
+  int tr_wstage;            // 1: the wave-staged form applies
+  int tr_lowlane;           // the wave's lane with the lowest tau (0, or 63 when tau falls with the ray)
+  int tr_wgap;              // floor((DDA_WS_ROWS - 1) |tr_d|) + 1 when tr_d < 0, else 0
+  long long tr_qchunk;      // DDA_WS_ROWS tr_d in Q32 (2^32 units): tau's step over a chunk
 };
 
 // Pixel-driven form of the transpose for one angle.  Where the traversal is exact
@@ -205,9 +215,16 @@ static inline DDA_HD SSDdaRay dda_ray_start(const SSDdaGeom& g, const SSDdaAngle
 
 // Radon: sino[slice][R][A] (overwritten) of img[slice][lines][cols]; pad
 // (dda_radon_scratch_floats) is scratch that the launcher fills with the image's padded
-// (line, column + 4) and (column, line + 4) layouts.
+// (line, column) and (column, line) layouts: DDA_PADW zeros either side of every line,
+// the line length rounded up to 4 floats (dda_pad_pitch).  The padding covers a whole
+// staged segment (DDA_WS_SEG) of a wave whose lowest ray lies left of the image while
+// another of its rays reaches in.
+#define DDA_PADW 132
+#define DDA_WS_ROWS 8   // fast indices per wave-staged chunk (even)
+#define DDA_WS_SEG 128  // padded pixels staged per fast index and wave
+static inline DDA_HD int dda_pad_pitch(int ns) { return (ns + 2 * DDA_PADW + 3) & ~3; }
 static inline size_t dda_radon_scratch_floats(const SSDdaGeom& g, int nslices) {
-  return (size_t)nslices * ((size_t)g.lines * (g.cols + 4) + (size_t)g.cols * (g.lines + 4));
+  return (size_t)nslices * ((size_t)g.lines * dda_pad_pitch(g.cols) + (size_t)g.cols * dda_pad_pitch(g.lines));
 }
 hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* pad, float* sino,
                             int nslices, hipStream_t stream);

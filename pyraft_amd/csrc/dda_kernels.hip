@@ -46,15 +46,36 @@ __device__ __forceinline__ int dda_wave_start(bool alive, int f, int dfast) {
 
 // Exact-geometry angles in trapezoid form (SSDdaAngle::tr_*): the ray's length in the
 // pixel at fast index k, slow index j is lmax z(|sigma_k - j|), and at most two slow
-// indices per k have z > 0.  Each lane carries tau = sigma - half_s + 3 in Q32 (integer
-// part, 32-bit fraction), stepped by tr_d per fast index with an add and a carry; per k
-// it reads the two candidates (padded slow indices floor(tau), floor(tau) + 1, clamped
-// to the padded line, whose two zero columns either side make the clamp exact) with one
-// 8-byte load and adds q0 z0 + q1 z1, z from the FP32 fraction.  The wave runs the
-// union of its lanes' fast ranges where a candidate can be inside the image; rows are
-// summed in FP32 in blocks of DDA_RADON_U, the blocks in FP64.  src is the slice's
-// padded (fast, slow + 4) image.
+// indices per k have z > 0.  Each lane carries tau = sigma - half_s + DDA_PADW + 1 in Q32
+// (integer part, 32-bit fraction), stepped by tr_d per fast index with an add and a
+// carry; per k it adds q0 z0 + q1 z1 of its candidates (padded slow indices floor(tau),
+// floor(tau) + 1), z from the FP32 fraction.  The wave runs the union of its lanes' fast
+// ranges where a candidate can be inside the image; rows are summed in FP32 per block of
+// rows, the blocks in FP64.  src is the slice's padded image (dda_pad_pitch lines).
+//
+// Wave-staged form (tr_wstage): per chunk of DDA_WS_ROWS fast indices the wave copies,
+// for each, the DDA_WS_SEG padded pixels from one chunk-wide start sb into its own LDS
+// rows, two fast indices per 16-byte buffer-to-LDS load (lanes 0-31 one, 32-63 the
+// next), and then reads each lane's candidates from there with LDS reads: half the
+// loads of the direct form, whose one 8-byte load per ray and fast index keeps the
+// texture addresser busy.  sb is the lowest ray's floor(tau) at the chunk's first index,
+// less 1 and the chunk's possible fall (tr_wgap), aligned down to 4: one scalar per
+// chunk, stepped in Q32 by tr_qchunk, so the copies take a fixed lane offset and a
+// scalar row offset.  Every lane's candidates on the chunk's rows then lie in
+// [sb, sb + spread + (DDA_WS_ROWS - 1) |tr_d| + 9), inside the segment, and the DDA_PADW
+// padding keeps the segment inside its line for every fast index of the wave's range.
+// Two chunk buffers per wave: the next chunk's copies are in flight while the wave sums
+// the current one.  Otherwise the direct form: one 8-byte buffer load per ray and fast
+// index, the index clamped to the padded line (its padding makes the clamp exact).
+//
+// s_waitcnt immediates (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
+// 11:8; a counter at its maximum is not waited on)
+#define DDA_WAIT_VM0 0x0F70
+#define DDA_WAIT_LGKM0 0xC07F
+#define DDA_WAIT_VM_CHUNK (0x0F70 | (DDA_WS_ROWS / 2))  // vmcnt(DDA_WS_ROWS / 2), below 16
 typedef float dda_f2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) void* dda_lds_ptr;
+typedef __attribute__((address_space(3))) const float* dda_lds_fptr;
 __device__ dda_f2v dda_raw_load_f2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
                                    int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
 
@@ -63,32 +84,32 @@ struct SSDdaTrapState {
   unsigned lo;
 };
 
-template <int N>
-__device__ __forceinline__ float dda_trap_rows(const SSDdaAngle& a, __amdgpu_buffer_rsrc_t rs, int row_bytes,
-                                               int pitch_bytes, int smax, SSDdaTrapState& st) {
-  float blk = 0.f;
-#pragma unroll
-  for (int u = 0; u < N; ++u) {
-    // below 0 wraps to a large unsigned value and clamps to smax: zeros there too
-    const unsigned idx = min((unsigned)st.hi, (unsigned)smax);
-    const dda_f2v q = dda_raw_load_f2(rs, (int)(idx * sizeof(float)), row_bytes + u * pitch_bytes, 0);
-    const float fr = (float)st.lo;
-    const float z0 = __builtin_amdgcn_fmed3f(fmaf(-fabsf(a.tr_c1 - fr), a.tr_kz, a.tr_pz), 0.f, 1.f);
-    const float z1 = __builtin_amdgcn_fmed3f(fmaf(fr, a.tr_kz, a.tr_pz1), 0.f, 1.f);
-    blk = fmaf(q[0], z0, blk);
-    blk = fmaf(q[1], z1, blk);
-    const unsigned lo = st.lo + a.tr_dlo;
-    st.hi += a.tr_dhi + (lo < st.lo ? 1 : 0);
-    st.lo = lo;
-  }
-  return blk;
+__device__ __forceinline__ void dda_trap_step(const SSDdaAngle& a, SSDdaTrapState& st) {
+  const unsigned lo = st.lo + a.tr_dlo;
+  st.hi += a.tr_dhi + (lo < st.lo ? 1 : 0);
+  st.lo = lo;
 }
 
-__device__ __forceinline__ double dda_radon_trap(const SSDdaAngle& a, const float* __restrict__ src, double t,
-                                                 bool lane_ok, int nf, int ns) {
-  const double tau0 = fma(t, a.tr_inv, a.tr_off);
-  // fast indices where floor(tau) can be in [1, ns + 1] (a candidate inside the image)
-  const double ka = (1.0 - tau0) / a.tr_d, kb = (ns + 2.0 - tau0) / a.tr_d;
+__device__ __forceinline__ float dda_trap_term(const SSDdaAngle& a, float x0, float x1, unsigned lo, float acc) {
+  const float fr = (float)lo;
+  const float z0 = __builtin_amdgcn_fmed3f(fmaf(-fabsf(a.tr_c1 - fr), a.tr_kz, a.tr_pz), 0.f, 1.f);
+  const float z1 = __builtin_amdgcn_fmed3f(fmaf(fr, a.tr_kz, a.tr_pz1), 0.f, 1.f);
+  return fmaf(x1, z1, fmaf(x0, z0, acc));
+}
+
+__device__ __forceinline__ SSDdaTrapState dda_trap_start(const SSDdaAngle& a, double tau0, int k) {
+  const double v = fmin(fmax(fma((double)k, a.tr_d, tau0), -1073741824.0), 1073741824.0);
+  const double fv = floor(v);
+  SSDdaTrapState st;
+  st.hi = (int)fv;
+  st.lo = (unsigned)((v - fv) * 4294967296.0);
+  return st;
+}
+
+// the wave's fast range: indices where floor(tau) can be in [DDA_PADW - 1,
+// ns + DDA_PADW - 1] (a candidate inside the image) for some lane, one index of margin
+__device__ __forceinline__ int2 dda_trap_range(const SSDdaAngle& a, double tau0, bool lane_ok, int nf, int ns) {
+  const double ka = (DDA_PADW - 1.0 - tau0) / a.tr_d, kb = (ns + DDA_PADW - tau0) / a.tr_d;
   const double kl = fmax(fmin(ka, kb), -1.0), kh = fmin(fmax(ka, kb), (double)nf);
   int klo = nf, khi = -1;
   if (lane_ok && kl <= kh) {  // false for NaN
@@ -99,22 +120,145 @@ __device__ __forceinline__ double dda_radon_trap(const SSDdaAngle& a, const floa
     klo = min(klo, __shfl_xor(klo, off));
     khi = max(khi, __shfl_xor(khi, off));
   }
-  klo = __builtin_amdgcn_readfirstlane(klo);  // wave-uniform: rows in scalar registers
-  khi = __builtin_amdgcn_readfirstlane(khi);
-  if (klo > khi) return 0.0;
-  const double v = fmin(fmax(fma((double)klo, a.tr_d, tau0), -1073741824.0), 1073741824.0);
-  const double fv = floor(v);
-  SSDdaTrapState st;
-  st.hi = (int)fv;
-  st.lo = (unsigned)((v - fv) * 4294967296.0);
-  const int pitch_bytes = (ns + 4) * (int)sizeof(float);
+  // wave-uniform: rows in scalar registers
+  return make_int2(__builtin_amdgcn_readfirstlane(klo), __builtin_amdgcn_readfirstlane(khi));
+}
+
+template <int N>
+__device__ __forceinline__ float dda_trap_rows(const SSDdaAngle& a, __amdgpu_buffer_rsrc_t rs, int row_bytes,
+                                               int pitch_bytes, int smax, SSDdaTrapState& st) {
+  float blk = 0.f;
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    // below 0 wraps to a large unsigned value and clamps to smax: zeros there too
+    const unsigned idx = min((unsigned)st.hi, (unsigned)smax);
+    const dda_f2v q = dda_raw_load_f2(rs, (int)(idx * sizeof(float)), row_bytes + u * pitch_bytes, 0);
+    blk = dda_trap_term(a, q[0], q[1], st.lo, blk);
+    dda_trap_step(a, st);
+  }
+  return blk;
+}
+
+__device__ __forceinline__ double dda_radon_trap(const SSDdaAngle& a, const float* __restrict__ src, double t,
+                                                 bool lane_ok, int nf, int ns) {
+  const double tau0 = fma(t, a.tr_inv, a.tr_off);
+  const int2 kr = dda_trap_range(a, tau0, lane_ok, nf, ns);
+  if (kr.x > kr.y) return 0.0;
+  SSDdaTrapState st = dda_trap_start(a, tau0, kr.x);
+  const int pitch = dda_pad_pitch(ns);
+  const int pitch_bytes = pitch * (int)sizeof(float);
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc((void*)src, (short)0, nf * pitch_bytes, DDA_RSRC_DWORD3);
   double tot = 0.0;
-  int k = klo;
-  for (; k + DDA_RADON_U - 1 <= khi; k += DDA_RADON_U)
-    tot += (double)dda_trap_rows<DDA_RADON_U>(a, rs, k * pitch_bytes, pitch_bytes, ns + 2, st);
-  for (; k <= khi; ++k) tot += (double)dda_trap_rows<1>(a, rs, k * pitch_bytes, pitch_bytes, ns + 2, st);
+  int k = kr.x;
+  for (; k + DDA_RADON_U - 1 <= kr.y; k += DDA_RADON_U)
+    tot += (double)dda_trap_rows<DDA_RADON_U>(a, rs, k * pitch_bytes, pitch_bytes, pitch - 2, st);
+  for (; k <= kr.y; ++k) tot += (double)dda_trap_rows<1>(a, rs, k * pitch_bytes, pitch_bytes, pitch - 2, st);
+  return lane_ok ? tot * a.tr_lmax : 0.0;
+}
+
+// copies chunk rows k0 .. k0 + nrow - 1 of the segments from sb into buf (DDA_WS_ROWS
+// rows of DDA_WS_SEG floats), DDA_WS_ROWS / 2 loads
+__device__ __forceinline__ void dda_ws_copy(__amdgpu_buffer_rsrc_t rs, float* buf, int k0, int sb, int nrow,
+                                            int pitch_bytes, int l16, int upper) {
+  const int chunk_bytes = k0 * pitch_bytes + sb * (int)sizeof(float);
+  if (nrow == DDA_WS_ROWS) {
+    // a fixed lane offset, the row pair as a scalar offset
+    const int lane_off = l16 + upper * pitch_bytes;
+#pragma unroll
+    for (int i = 0; i < DDA_WS_ROWS / 2; ++i) {
+      int off = lane_off;
+      __asm__ volatile("" : "+v"(off));
+      // the scalar offset through readfirstlane: a uniform value that instruction
+      // selection placed in a vector register would otherwise become a waterfall loop
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (dda_lds_ptr)(buf + 2 * i * DDA_WS_SEG), 16, off,
+                                               __builtin_amdgcn_readfirstlane(chunk_bytes + 2 * i * pitch_bytes), 0,
+                                               0);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < DDA_WS_ROWS / 2; ++i) {
+      // rows past the range repeat the last one (inside the image, never read); the whole
+      // offset per lane, so that no scalar offset is built from lane values
+      int off = chunk_bytes + l16 + min(2 * i + upper, nrow - 1) * pitch_bytes;
+      __asm__ volatile("" : "+v"(off));
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (dda_lds_ptr)(buf + 2 * i * DDA_WS_SEG), 16, off, 0, 0, 0);
+    }
+  }
+}
+
+// the chunk's segment start from the lowest ray's Q32 state ql at its first index
+__device__ __forceinline__ int dda_ws_start(const SSDdaAngle& a, long long ql) {
+  return ((int)(ql >> 32) - 1 - a.tr_wgap) & ~3;
+}
+
+// wlds: this wave's 2 DDA_WS_ROWS x DDA_WS_SEG floats of LDS (two chunk buffers: the
+// next chunk's copies are in flight while the wave sums the current one)
+__device__ __forceinline__ double dda_radon_trap_ws(const SSDdaAngle& a, const float* __restrict__ src, double t,
+                                                    bool lane_ok, int nf, int ns, float* wlds) {
+  const double tau0 = fma(t, a.tr_inv, a.tr_off);
+  const int2 kr = dda_trap_range(a, tau0, lane_ok, nf, ns);
+  if (kr.x > kr.y) return 0.0;
+  SSDdaTrapState st = dda_trap_start(a, tau0, kr.x);
+  const int lane = threadIdx.x & 63;
+  const int pitch_bytes = dda_pad_pitch(ns) * (int)sizeof(float);
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)src, (short)0, nf * pitch_bytes, DDA_RSRC_DWORD3);
+  const int l16 = (lane & 31) * 16;  // this lane's 16 bytes of a staged segment
+  const int upper = lane >> 5;       // 1: the second fast index of a load
+  // the lowest ray's Q32 state (scalar) at the first index of the next chunk to copy
+  long long ql = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(st.hi, a.tr_lowlane) << 32) |
+                             (unsigned)__builtin_amdgcn_readlane((int)st.lo, a.tr_lowlane));
+  const int nchunk = (kr.y - kr.x) / DDA_WS_ROWS + 1;
+  int sb = dda_ws_start(a, ql);
+  dda_ws_copy(rs, wlds, kr.x, sb, min(DDA_WS_ROWS, kr.y - kr.x + 1), pitch_bytes, l16, upper);
+  double tot = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    const int k0 = kr.x + c * DDA_WS_ROWS;
+    const int nrow = min(DDA_WS_ROWS, kr.y - k0 + 1);
+    const float* __restrict__ cur = wlds + (c & 1) * (DDA_WS_ROWS * DDA_WS_SEG);
+    int sb_next = 0;
+    if (c + 1 < nchunk) {
+      ql += a.tr_qchunk;
+      sb_next = dda_ws_start(a, ql);
+      // the other buffer's LDS reads (chunk c - 1) have returned before it is overwritten
+      __builtin_amdgcn_s_waitcnt(DDA_WAIT_LGKM0);
+      dda_ws_copy(rs, wlds + ((c + 1) & 1) * (DDA_WS_ROWS * DDA_WS_SEG), k0 + DDA_WS_ROWS, sb_next,
+                  min(DDA_WS_ROWS, kr.y - k0 - DDA_WS_ROWS + 1), pitch_bytes, l16, upper);
+      // buffer-to-LDS copies count on vmcnt, in order, and no barrier orders them here:
+      // this chunk's have landed once at most the next chunk's are outstanding
+      __builtin_amdgcn_s_waitcnt(DDA_WAIT_VM_CHUNK);
+    } else {
+      __builtin_amdgcn_s_waitcnt(DDA_WAIT_VM0);
+    }
+    // LDS byte address of a lane's first candidate on row u: a scalar row base plus 4 hi
+    const int base = __builtin_amdgcn_readfirstlane((int)(size_t)(dda_lds_fptr)cur) - 4 * sb;
+    float blk = 0.f;
+    if (nrow == DDA_WS_ROWS) {
+      // wave-uniform: a full chunk steps its rows, issues its reads together, then sums
+      unsigned lo[DDA_WS_ROWS];
+      float x0[DDA_WS_ROWS], x1[DDA_WS_ROWS];
+#pragma unroll
+      for (int u = 0; u < DDA_WS_ROWS; ++u) {
+        const dda_lds_fptr q = (dda_lds_fptr)(size_t)(unsigned)(__builtin_amdgcn_readfirstlane(base + u * DDA_WS_SEG * 4) +
+                                                                 4 * st.hi);
+        x0[u] = q[0];
+        x1[u] = q[1];
+        lo[u] = st.lo;
+        dda_trap_step(a, st);
+      }
+#pragma unroll
+      for (int u = 0; u < DDA_WS_ROWS; ++u) blk = dda_trap_term(a, x0[u], x1[u], lo[u], blk);
+    } else {
+      for (int u = 0; u < nrow; ++u) {
+        const dda_lds_fptr q = (dda_lds_fptr)(size_t)(unsigned)(base + u * DDA_WS_SEG * 4 + 4 * st.hi);
+        blk = dda_trap_term(a, q[0], q[1], st.lo, blk);
+        dda_trap_step(a, st);
+      }
+    }
+    tot += (double)blk;
+    sb = sb_next;
+  }
   return lane_ok ? tot * a.tr_lmax : 0.0;
 }
 
@@ -128,11 +272,16 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   const SSDdaAngle a = angles[ai];
   const int nf = a.fast_i ? g.lines : g.cols;
   const int ns = a.fast_i ? g.cols : g.lines;
-  // padded (fast, slow + 4) layouts: two zero columns either side of each line
-  const int pitch = ns + 4;
+  // padded layouts: DDA_PADW zero columns either side of each line (dda_pad_pitch)
+  const int pitch = dda_pad_pitch(ns);
   const float* __restrict__ src = (a.fast_i ? img : imgT) + slice * (size_t)nf * pitch;
   if (a.fixed && (long long)nf * pitch < (1LL << 29)) {
-    const double acc = dda_radon_trap(a, src, dda_ray_t(g, lane_ok ? r : 0), lane_ok, nf, ns);
+    __shared__ float wlds[4][2 * DDA_WS_ROWS * DDA_WS_SEG];  // two chunk buffers per wave
+    // every lane's own ray (past R too): the staged form takes the wave's lowest ray from
+    // its lanes
+    const double t = dda_ray_t(g, r);
+    const double acc = a.tr_wstage ? dda_radon_trap_ws(a, src, t, lane_ok, nf, ns, wlds[threadIdx.x >> 6])
+                                   : dda_radon_trap(a, src, t, lane_ok, nf, ns);
     if (lane_ok) sino[(slice * g.nrays + r) * g.nangles + ai] = (float)acc;
     return;
   }
@@ -144,7 +293,7 @@ __global__ __launch_bounds__(256) void k_dda_radon(SSDdaGeom g, const SSDdaAngle
   int F = dda_wave_start(alive, ray.f, a.dfast);
   // the pixel's offset in the (fast, slow) image, stepped with the ray (an image side is
   // at most 16384, so a slice's offsets fit an int); meaningful while the lane is alive
-  int pix = alive ? ray.f * pitch + ray.s + 2 : 0;
+  int pix = alive ? ray.f * pitch + ray.s + DDA_PADW : 0;
   {
     const int fast_step = a.dfast * pitch;
     // lanes start at most nf moves apart and then take at most nf + 1 moves each
@@ -292,6 +441,11 @@ __global__ __launch_bounds__(256) void k_dda_gather(SSDdaGeom g, const SSDdaGath
 // [1, 6 + span] for span = (cols - 1)|kx| + (lines - 1)|ky| of the block (the FP32
 // fraction is within 1e-5 ray units of the FP64 value, so floor is off by at most one),
 // and ws >= span + 8.  Terms are summed in FP32 over a group, those sums in FP64.
+// s_waitcnt immediates (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
+// 11:8; a counter at its maximum is not waited on)
+#define DDA_WAIT_VM0 0x0F70
+#define DDA_WAIT_LGKM0 0xC07F
+#define DDA_WAIT_VM_CHUNK (0x0F70 | (DDA_WS_ROWS / 2))  // vmcnt(DDA_WS_ROWS / 2), below 16
 typedef float dda_f2v __attribute__((ext_vector_type(2)));
 __device__ float dda_raw_load_f32(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f32");
@@ -480,35 +634,44 @@ __global__ __launch_bounds__(256) void k_dda_fold(int lines, int cols, const T* 
   }
 }
 
-// The Radon's padded layouts of img[slice][lines][cols]: pad[slice][lines][cols + 4] and
-// padT[slice][cols][lines + 4] (the transpose), each line with two zeros either side.
-// Block (32, 8) copies a 32 x 32 tile, the transposed copy through LDS.
+// The Radon's padded layouts of img[slice][lines][cols]: pad[slice][lines][pc] and
+// padT[slice][cols][pl] (the transpose), pc = dda_pad_pitch(cols), pl =
+// dda_pad_pitch(lines): each line DDA_PADW zeros, its pixels, then zeros to the pitch.
+// Block (32, 8) copies a 32 x 32 tile, the transposed copy through LDS; the first and
+// last tile of a line also write its padding.
 __global__ __launch_bounds__(256) void k_dda_pad_images(int lines, int cols, const float* __restrict__ src,
                                                         float* __restrict__ pad, float* __restrict__ padT) {
   __shared__ float tile[DDA_TILE][DDA_TILE + 1];
   const size_t slice = blockIdx.z;
   const int i0 = blockIdx.y * DDA_TILE, j0 = blockIdx.x * DDA_TILE;
+  const int pc = dda_pad_pitch(cols), pl = dda_pad_pitch(lines);
   const float* __restrict__ s = src + slice * (size_t)lines * cols;
-  float* __restrict__ o = pad + slice * (size_t)lines * (cols + 4);
-  float* __restrict__ oT = padT + slice * (size_t)cols * (lines + 4);
+  float* __restrict__ o = pad + slice * (size_t)lines * pc;
+  float* __restrict__ oT = padT + slice * (size_t)cols * pl;
   const int x = threadIdx.x;
   for (int ii = threadIdx.y; ii < DDA_TILE; ii += blockDim.y) {
     const int i = i0 + ii, j = j0 + x;
     const float v = (i < lines && j < cols) ? s[(size_t)i * cols + j] : 0.f;
     tile[ii][x] = v;
     if (i < lines) {
-      if (j < cols) o[(size_t)i * (cols + 4) + j + 2] = v;
-      if (blockIdx.x == 0 && x < 2) o[(size_t)i * (cols + 4) + x] = 0.f;
-      if (j0 + DDA_TILE >= cols && x < 2) o[(size_t)i * (cols + 4) + cols + 2 + x] = 0.f;
+      float* __restrict__ line = o + (size_t)i * pc;
+      if (j < cols) line[j + DDA_PADW] = v;
+      if (blockIdx.x == 0)
+        for (int c = x; c < DDA_PADW; c += DDA_TILE) line[c] = 0.f;
+      if (j0 + DDA_TILE >= cols)
+        for (int c = cols + DDA_PADW + x; c < pc; c += DDA_TILE) line[c] = 0.f;
     }
   }
   __syncthreads();
   for (int jj = threadIdx.y; jj < DDA_TILE; jj += blockDim.y) {
     const int j = j0 + jj, i = i0 + x;
     if (j < cols) {
-      if (i < lines) oT[(size_t)j * (lines + 4) + i + 2] = tile[x][jj];
-      if (blockIdx.y == 0 && x < 2) oT[(size_t)j * (lines + 4) + x] = 0.f;
-      if (i0 + DDA_TILE >= lines && x < 2) oT[(size_t)j * (lines + 4) + lines + 2 + x] = 0.f;
+      float* __restrict__ line = oT + (size_t)j * pl;
+      if (i < lines) line[i + DDA_PADW] = tile[x][jj];
+      if (blockIdx.y == 0)
+        for (int c = x; c < DDA_PADW; c += DDA_TILE) line[c] = 0.f;
+      if (i0 + DDA_TILE >= lines)
+        for (int c = lines + DDA_PADW + x; c < pl; c += DDA_TILE) line[c] = 0.f;
     }
   }
 }
@@ -522,7 +685,7 @@ static dim3 dda_tiles(int lines, int cols, int nslices) {
 hipError_t dda_launch_radon(const SSDdaGeom& g, const SSDdaAngle* angles, const float* img, float* pad, float* sino,
                             int nslices, hipStream_t stream) {
   float* img_pad = pad;
-  float* imgT_pad = pad + (size_t)nslices * g.lines * (g.cols + 4);
+  float* imgT_pad = pad + (size_t)nslices * g.lines * dda_pad_pitch(g.cols);
   hipLaunchKernelGGL(k_dda_pad_images, dda_tiles(g.lines, g.cols, nslices), dim3(DDA_TILE, 8), 0, stream, g.lines,
                      g.cols, img, img_pad, imgT_pad);
   hipLaunchKernelGGL(k_dda_radon, dim3((g.nrays + 255) / 256, g.nangles, nslices), dim3(256), 0, stream, g, angles,
