@@ -307,6 +307,9 @@ def test_gpu_golden(torch_mod, golden):
     # the second case pixels more than 64 rays beyond them
     ((96, 96), 60, 47, DEF_IC, (0.0, 0.5, math.pi, -0.5)),
     ((160, 160), 41, 37, DEF_IC, (0.0, 0.2, math.pi, -0.2)),
+    # rays about 6 pixels apart: a wave's rays span more than one staged segment, so the
+    # Radon reads the image directly
+    ((256, 256), 40, 33, DEF_IC, DEF_SC),
 ])
 def test_gpu_random_vs_oracle(torch_mod, shape, R, A, ic, sc):
     torch = torch_mod
