@@ -807,7 +807,10 @@ int ss_em_launch(const raft_ss_plan* p, const SSEmLaunch& L, const float* d_sino
   for (int it = 0; it < niter; ++it) {
     for (int b = 0; b < S; ++b) {
       const int j0 = (int)(((long long)b * A) / S), j1 = (int)(((long long)(b + 1) * A) / S);
-      HIP_TRY(ss_launch_gather_layouts(L.dp, d_x, gA, gB, nslices, stream));
+      // only the layouts the subset's angles read (its Radon reads no other)
+      int need = 0;
+      for (int j = j0; j < j1; ++j) need |= p->h_angles[j].branch_a ? SS_GATHER_A : SS_GATHER_B;
+      HIP_TRY(ss_launch_gather_layouts(L.dp, d_x, gA, gB, nslices, stream, need));
       SSEmRatioArgs ea{measT, partials + L.poff[b]};
       HIP_TRY(ss_launch_radon(L.dps[b], gA, gB, j0, j1, nslices, SS_OUT_EM_RATIO, rhoT, ea, slabs, stream));
       SSEmUpdateArgs ua{d_x, weights + (size_t)b * img_sz};

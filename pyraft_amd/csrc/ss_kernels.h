@@ -89,8 +89,13 @@ hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4*
 // records q = 0..n+1 of each row
 #define SS_GATHER_TILE 64
 static inline __host__ __device__ int ss_gather_tiles(int n) { return (n + 2 + SS_GATHER_TILE - 1) / SS_GATHER_TILE; }
+// which layouts to build: the Radon reads angle j's samples from gatherA when
+// SSAngle::branch_a, else from gatherB, so a range of angles on one branch needs only one
+#define SS_GATHER_A 1
+#define SS_GATHER_B 2
+#define SS_GATHER_BOTH 3
 hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
-                                    int nslices, hipStream_t stream);
+                                    int nslices, hipStream_t stream, int which = SS_GATHER_BOTH);
 hipError_t ss_launch_sino_to_t(const float* sino, float* sinoT, int R, int A, int nslices, hipStream_t stream);
 hipError_t ss_launch_t_to_sino(const float* sinoT, float* sino, int R, int A, int nslices, hipStream_t stream);
 

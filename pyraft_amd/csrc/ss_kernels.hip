@@ -122,7 +122,7 @@ hipError_t ss_launch_plan_wave_bounds(const SSGeom& g, const int2* bounds, int4*
 template <int GT, int NW>
 __device__ __forceinline__ void ss_gather_tile(const float* __restrict__ src, float2* __restrict__ dA,
                                                float2* __restrict__ dB, int n, int pitch, int bx, int by,
-                                               float (*T)[GT + 1]) {
+                                               float (*T)[GT + 1], int which = SS_GATHER_BOTH) {
   constexpr int H = GT + 1;                       // halo side
   constexpr int NT = NW * SS_WAVE;                // threads
   constexpr int LOADS = (H * H + NT - 1) / NT;    // halo values per thread
@@ -147,7 +147,7 @@ __device__ __forceinline__ void ss_gather_tile(const float* __restrict__ src, fl
   const int x = lane % GT, sub = lane / GT;
   // B: record (r, q) = (F(r, q-1), F(r, q) - F(r, q-1)) = T[r - by + 1][q - bx (+1)]
   const int qb = bx + x;
-  for (int i = wv * SUB + sub; i < GT; i += NW * SUB) {
+  for (int i = wv * SUB + sub; (which & SS_GATHER_B) && i < GT; i += NW * SUB) {
     const int r = by + i;
     if (r < n && qb <= n + 1) {
       const float lo = T[i + 1][x];
@@ -156,7 +156,7 @@ __device__ __forceinline__ void ss_gather_tile(const float* __restrict__ src, fl
   }
   // A: record (c, q) = (F(q-1, c), F(q, c) - F(q-1, c)) = T[q - by (+1)][c - bx + 1]
   const int qa = by + x;
-  for (int j = wv * SUB + sub; j < GT; j += NW * SUB) {
+  for (int j = wv * SUB + sub; (which & SS_GATHER_A) && j < GT; j += NW * SUB) {
     const int cc = bx + j;
     if (cc < n && qa <= n + 1) {
       const float lo = T[x][j + 1];
@@ -167,24 +167,25 @@ __device__ __forceinline__ void ss_gather_tile(const float* __restrict__ src, fl
 
 template <int GT, int NW>
 __global__ __launch_bounds__(NW * SS_WAVE) void k_gather_pairs(const float* __restrict__ img, float2* __restrict__ gA,
-                                                               float2* __restrict__ gB, int n, int pitch) {
+                                                               float2* __restrict__ gB, int n, int pitch, int which) {
   __shared__ float T[GT + 1][GT + 1];  // F(by-1+a, bx-1+b)
   const size_t slice = blockIdx.z;
   ss_gather_tile<GT, NW>(img + slice * n * n, gA + slice * n * pitch, gB + slice * n * pitch, n, pitch,
-                         blockIdx.x * GT, blockIdx.y * GT, T);
+                         blockIdx.x * GT, blockIdx.y * GT, T, which);
 }
 
 hipError_t ss_launch_gather_layouts(const SSDevPlan& p, const float* image, float2* gatherA, float2* gatherB,
-                                    int nslices, hipStream_t stream) {
+                                    int nslices, hipStream_t stream, int which) {
   const int n = p.g.n;
   const int tiles = ss_gather_tiles(n);
+  if (!(which & SS_GATHER_BOTH)) return hipSuccess;
   if ((long long)tiles * tiles * nslices >= 1024) {  // at least 4 large tiles per CU
     hipLaunchKernelGGL((k_gather_pairs<SS_GATHER_TILE, 8>), dim3(tiles, tiles, nslices), dim3(8 * SS_WAVE), 0, stream,
-                       image, gatherA, gatherB, n, p.g.pitch);
+                       image, gatherA, gatherB, n, p.g.pitch, which);
   } else {
     const int t32 = (n + 2 + 31) / 32;
     hipLaunchKernelGGL((k_gather_pairs<32, 4>), dim3(t32, t32, nslices), dim3(4 * SS_WAVE), 0, stream, image,
-                       gatherA, gatherB, n, p.g.pitch);
+                       gatherA, gatherB, n, p.g.pitch, which);
   }
   return hipGetLastError();
 }
