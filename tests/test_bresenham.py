@@ -355,6 +355,26 @@ def test_gpu_transpose_tall_tiles_vs_oracle(torch_mod):
 
 
 @pytest.mark.gpu
+def test_gpu_full_size_vs_oracle(torch_mod):
+    """2048^2 image, 2048 rays (the benchmark's ray spacing, so the same staged segments
+    and 8-row chunks over the full fast range), 181 angles: Radon and transpose against
+    the oracle."""
+    torch = torch_mod
+    from pyraft_amd.device import DdaPlan
+    shape, R, A = (2048, 2048), 2048, 181
+    print("dda full size", shape, R, A, flush=True)
+    geo = _geo(DEF_IC, DEF_SC)
+    rng = np.random.default_rng(11)
+    img = rng.random((1,) + shape).astype(np.float32)
+    sino = rng.random((1, R, A)).astype(np.float32)
+    p = DdaPlan(shape, R, A, DEF_IC, DEF_SC)
+    got_r = p.radon(torch.from_numpy(img).cuda()).cpu().numpy()
+    got_t = p.radon_transpose(torch.from_numpy(sino).cuda()).cpu().numpy()
+    assert rel_l2(got_r[0], O.radon_bresenham(img[0].astype(np.float64), R, A, **geo)) <= REL_L2
+    assert rel_l2(got_t[0], O.radon_transpose_bresenham(sino[0].astype(np.float64), shape, **geo)) <= REL_L2
+
+
+@pytest.mark.gpu
 def test_gpu_straight_angles(torch_mod):
     """Square pixels, 181 angles over [0, pi]: the axis angles 0, pi/2 and pi run straight
     (one column or line per ray, full length), so the transpose sums them per column or
