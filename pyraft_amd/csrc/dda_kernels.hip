@@ -192,8 +192,10 @@ __device__ __forceinline__ int dda_ws_start(const SSDdaAngle& a, long long ql) {
   return ((int)(ql >> 32) - 1 - a.tr_wgap) & ~3;
 }
 
-// wlds: this wave's 2 DDA_WS_ROWS x DDA_WS_SEG floats of LDS (two chunk buffers: the
-// next chunk's copies are in flight while the wave sums the current one)
+// wlds: this wave's two chunk buffers in LDS, DDA_WS_ROWS x DDA_WS_SEG floats each (the
+// next chunk's copies are in flight while the wave sums the current one).  Every lane,
+// also one past the last ray, reads inside its buffer: the lanes' tau is linear in the
+// ray index, so each lies between the lowest lane's and that plus the wave's spread.
 __device__ __forceinline__ double dda_radon_trap_ws(const SSDdaAngle& a, const float* __restrict__ src, double t,
                                                     bool lane_ok, int nf, int ns, float* wlds) {
   const double tau0 = fma(t, a.tr_inv, a.tr_off);
