@@ -609,6 +609,11 @@ int build_launch(raft_ss_batch* b, BatchLaunch& L, int kind, int S, int bsub, in
     k.e.nslab = 1;
     k.e.slab_rows = n;
     k.e.part_off = 0;
+    k.e.which = SS_GATHER_BOTH;
+    if (kind == SS_KIND_GATHER && S > 0) {  // a subset's gather: the layouts its angles read
+      k.e.which = 0;
+      for (int j = j0; j < j1; ++j) k.e.which |= p.h_angles[j].branch_a ? SS_GATHER_A : SS_GATHER_B;
+    }
     if (L.slabs && j1 > j0) {
       k.e.nslab = ss_radon_slab_plan(n, blocks, &k.e.slab_rows);
       k.e.part_off = (int64_t)L.part_floats;
@@ -931,15 +936,15 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
   const std::vector<int> vs = variants_used(b);
 
   // ---- everything the captured part needs, before any capture
-  BatchLaunch *Lg, *Lt;
-  if ((rc = sync_slices(b)) || (rc = get_launch(b, &Lg, SS_KIND_GATHER, 0, 0, 0, 0)) ||
-      (rc = get_launch(b, &Lt, SS_KIND_TRANSPOSE, 0, 0, 0, 0)))
-    return rc;
-  std::vector<BatchLaunch*> Lr(S), Lz(S), Lm(S), Lf(S), Lbw((size_t)S * vs.size()), Lbu((size_t)S * vs.size());
+  BatchLaunch* Lt;
+  if ((rc = sync_slices(b)) || (rc = get_launch(b, &Lt, SS_KIND_TRANSPOSE, 0, 0, 0, 0))) return rc;
+  std::vector<BatchLaunch*> Lg(S), Lr(S), Lz(S), Lm(S), Lf(S), Lbw((size_t)S * vs.size()),
+      Lbu((size_t)S * vs.size());
   std::vector<long long> poff(S + 1, 0);
   size_t part_floats = 1;
   for (int s = 0; s < S; ++s) {
-    if ((rc = get_launch(b, &Lr[s], SS_KIND_RADON, S, s, SS_OUT_EM_RATIO, 0)) ||
+    if ((rc = get_launch(b, &Lg[s], SS_KIND_GATHER, S, s, 0, 0)) ||
+        (rc = get_launch(b, &Lr[s], SS_KIND_RADON, S, s, SS_OUT_EM_RATIO, 0)) ||
         (rc = get_launch(b, &Lz[s], SS_KIND_RADON_FINISH, S, s, SS_OUT_EM_RATIO, 0)) ||
         (rc = get_launch(b, &Lf[s], SS_KIND_FIXUP, S, s, SS_OUT_EM_RATIO, 0)))
       return rc;
@@ -1009,7 +1014,7 @@ extern "C" int raft_ss_batch_em(raft_ss_batch* b, const float* d_sinos, float* d
     HIP_TRY(ss_launch_batch_transpose(Lt->dev(ds), d_sinos, measT, 1, st));
     for (int it = 0; it < niter; ++it) {
       for (int s = 0; s < S; ++s) {
-        HIP_TRY(ss_launch_batch_gather(Lg->dev(ds), b->gather_tile, d_x, gA, gB, st));
+        HIP_TRY(ss_launch_batch_gather(Lg[s]->dev(ds), b->gather_tile, d_x, gA, gB, st));
         float* sp = Lr[s]->slabs ? slab : nullptr;
         HIP_TRY(ss_launch_batch_radon(Lr[s]->dev(ds), gA, gB, sp, SS_OUT_EM_RATIO, rhoT, measT, part + poff[s], st));
         if (sp)

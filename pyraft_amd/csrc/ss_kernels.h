@@ -69,6 +69,7 @@ struct SSBatchEntry {
   // nslab slabs of slab_rows, its partial ray sums at part[part_off ...] ([z][j - j0][k])
   int nslab, slab_rows;
   int64_t part_off;
+  int which;  // gather launches: the layouts to build (SS_GATHER_*), those angles j0..j1-1 read
 };
 struct SSBatchLaunch {
   const SSBatchSlice* slices;  // device

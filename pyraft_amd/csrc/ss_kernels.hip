@@ -2368,6 +2368,7 @@ __device__ __forceinline__ SSBatchEntry ss_entry_of(const SSBatchLaunch& L, int 
   e.nslab = __builtin_amdgcn_readfirstlane(e.nslab);
   e.slab_rows = __builtin_amdgcn_readfirstlane(e.slab_rows);
   e.part_off = ss_rfl64(e.part_off);
+  e.which = __builtin_amdgcn_readfirstlane(e.which);
   return e;
 }
 
@@ -2383,7 +2384,8 @@ __global__ __launch_bounds__(NW * SS_WAVE) void k_batch_gather(SSBatchLaunch L, 
     const int tiles = (n + 2 + GT - 1) / GT;
     const int local = item - e.item0;
     const int ty = local / tiles, tx = local - ty * tiles;
-    ss_gather_tile<GT, NW>(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch, tx * GT, ty * GT, T);
+    ss_gather_tile<GT, NW>(images + s.img_off, gA + s.lay_off, gB + s.lay_off, n, s.p.g.pitch, tx * GT, ty * GT, T,
+                           e.which);
     __syncthreads();
   }
 }
