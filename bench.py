@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-em", action="store_true", help="skip the EM leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-slices leg")
     ap.add_argument("--no-dda", action="store_true", help="skip the line-traversal (Bresenham) leg")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the mesh Radon, transmission EM and binary file legs")
     ap.add_argument("--batch-slices", type=int, default=256, help="slices in the batched leg")
     ap.add_argument("--batch-iters", type=int, default=20, help="OSEM iterations in the batched leg")
     ap.add_argument("--calibration-copy", action="store_true",
@@ -266,6 +268,9 @@ def cpu_baseline(n, angles):
     * one OSEM iteration at 1024^2 / 720 angles / 16 subsets composed from the C operators
       with the raft_em_gpu recurrence (raft/raft_em_gpu.cu:41-69): per subset a Radon of
       its 45 angles, the ratio, a backprojection and the update (weights precomputed);
+    * the mesh Radon (the reference's withmesh) on 24 of the mesh leg's angles, and one
+      transmission-EM iteration at 1024^2 / 720 angles composed from the C operators with
+      the raft_tr_em_gpu recurrence (raft/raft_transmission_em.cu:41-59);
     * the 256-slice uneven batch from a stratified sample: per image side the slices with
       the fewest and the most angles, each projected and backprojected on 24 of its
       angles; the per-update rate of each side extrapolates to the whole pass and to
@@ -335,6 +340,37 @@ def cpu_baseline(n, angles):
     out["osem"] = {"workload": "OSEM 1024x1024, 720 angles, 16 subsets: one iteration (weights precomputed)",
                    "s_per_iteration": round(t_it, 3), "iters_per_s": round(1.0 / t_it, 4),
                    "gups": round(2.0 * ne * ne * Ae / t_it / 1e9, 4)}
+
+    # -- the mesh Radon (non-uniform t and theta tables), on 24 of the mesh leg's angles
+    t_m, th_m = _mesh_tables(n, angles)
+    th_s = th_m[::max(1, angles // 24)][:24].copy()
+    if ref:  # the reference's withmesh computes every ray when nthreads divides R x A
+        mesh_radon = lambda: O.ref_radon_withmesh(img, t_m, th_s, nthreads=nthreads)  # noqa: E731
+    else:
+        mesh_radon = lambda: O.radon_withmesh(img, t_m, th_s, nthreads=nthreads)  # noqa: E731
+    t0 = time.perf_counter()
+    mesh_radon()
+    t_mesh = time.perf_counter() - t0
+    out["mesh"] = {"workload": "%dx%d image, %d detector positions x 24 of the mesh leg's angles" % (n, n, n),
+                   "radon_s": round(t_mesh, 4), "radon_gups": round(float(n) * n * 24 / t_mesh / 1e9, 4)}
+
+    # -- one transmission-EM iteration, 1024^2, 720 angles (raft/raft_transmission_em.cu:41-59
+    # composed from the C operators: W = BP(raw) beforehand)
+    nt, At = 1024, 720
+    xt = O.phantom_fromdesc(random_ellipses(32, 6), nt)
+    pt = radon(xt, nt, At)
+    xt = xt * (2.0 / pt.max())
+    flat_t = np.full((nt, At), 1e4)
+    raw_t = flat_t * np.exp(-radon(xt, nt, At))
+    w_t = bp(raw_t, nt, At)
+    x = np.ones((nt, nt))
+    t0 = time.perf_counter()
+    r_t = np.where(w_t != 0, bp(flat_t * np.exp(-radon(x, nt, At)), nt, At) / np.where(w_t != 0, w_t, 1.0), 1.0)
+    x = (x * r_t) * r_t
+    t_tr = time.perf_counter() - t0
+    out["transmission_em"] = {"workload": "transmission EM 1024x1024, 720 angles: one iteration (W precomputed)",
+                              "s_per_iteration": round(t_tr, 3), "iters_per_s": round(1.0 / t_tr, 4),
+                              "gups": round(2.0 * nt * nt * At / t_tr / 1e9, 4)}
 
     # -- the 256-slice uneven batch, stratified sample, extrapolated
     rng = np.random.default_rng(4)
@@ -521,6 +557,100 @@ def dda_leg(dev, img, n, R, A, reps=3):
     return out
 
 
+def _mesh_tables(R, A, seed=5):
+    """Detector positions and angles of the mesh leg: each sorted uniform random, t in
+    [-1, 1] and theta in [0, pi)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.uniform(-1.0, 1.0, R)), np.sort(rng.uniform(0.0, math.pi, A))
+
+
+def mesh_leg(dev, img, n, R, A, reps=5):
+    """Slant-stack Radon on non-uniform (t, theta) tables (raft_radon_slantstack_withmesh,
+    pyraft's radon_withmesh) on the headline image: event-timed, plan built beforehand."""
+    print("bench: mesh Radon", file=sys.stderr, flush=True)
+    import torch
+    from pyraft_amd.device import Plan
+    t, theta = _mesh_tables(R, A)
+    plan = Plan.mesh(n, t, theta)
+    plan.radon(img)
+    torch.cuda.synchronize()
+    ms = _event_ms(lambda: plan.radon(img), reps, torch.cuda.current_stream())
+    upd = float(n) * R * A
+    return {"workload": "%d^2 image, %d detector positions x %d angles, each sorted uniform random "
+                        "(t in [-1, 1], theta in [0, pi))" % (n, R, A),
+            "radon_ms": round(ms, 4), "radon_gups": round(upd / ms / 1e6, 1)}
+
+
+def _tr_em_case(plan, n, dev):
+    """Transmission data for the transmission-EM leg: raw = flat exp(-R x*) for a
+    32-ellipse attenuation phantom x* scaled so that its longest line integral is 2, flat
+    field 1e4."""
+    import torch
+    from pyraft_amd.device import phantom
+    from pyraft_amd.phantom import random_ellipses
+    xs = phantom(random_ellipses(32, 6), n)
+    xs = xs * (2.0 / float(plan.radon(xs).max()))
+    flat = torch.full((plan.nrays, plan.nangles), 1e4, device=dev)
+    return flat * torch.exp(-plan.radon(xs)), flat
+
+
+def tr_em_leg(dev, n=1024, A=720, niter=20):
+    """Transmission EM (raft_tr_em_gpu's recurrence, raft/raft_transmission_em.cu:16-66,
+    through raft_ss_tr_em on device buffers): iterations per second from x = 1, the fixed
+    part of a call (W = BP(raw)) subtracted as in the OSEM leg."""
+    print("bench: transmission EM %d^2, %d angles" % (n, A), file=sys.stderr, flush=True)
+    import ctypes
+    import torch
+    from pyraft_amd.device import Plan
+    from pyraft_amd.raftypes import check, libraft
+    plan = Plan(n, n, A)
+    raw, flat = _tr_em_case(plan, n, dev)
+    x = torch.ones((n, n), device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def run(k):
+        x.fill_(1.0)
+        check(libraft.raft_ss_tr_em(plan.handle, raw.data_ptr(), flat.data_ptr(), x.data_ptr(), k, 1, stream),
+              "raft_ss_tr_em")
+
+    run(1)  # workspace, kernels
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    run(0)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    run(niter)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    it_s = niter / max(1e-9, (tc - tb) - (tb - ta))
+    return {"workload": "transmission EM %dx%d, %d angles, %d bins, %d iterations from x = 1; raw = 1e4 "
+                        "exp(-R x*), line integrals up to 2" % (n, n, A, n, niter),
+            "iters_per_s": round(it_s, 2), "gups": round(2.0 * n * n * A * it_s / 1e9, 2)}
+
+
+def io_leg(n=2048):
+    """Binary raft image files (raft_image_write / raft_image_read, raft/raft_image.c:5-38,
+    135-151): an n x n FP64 image written to a temporary file and read back (host only)."""
+    print("bench: binary image files", file=sys.stderr, flush=True)
+    import tempfile
+    import numpy as np
+    import pyraft_amd as pr
+    img = pr.image(np.random.default_rng(7).random((n, n)))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "image.raft")
+        pr.image_write(img, path)  # file created, pages cached
+        t0 = time.perf_counter()
+        pr.image_write(img, path)
+        t1 = time.perf_counter()
+        back = pr.image_read(path)
+        t2 = time.perf_counter()
+    mb = n * n * 8 / 1e6
+    return {"workload": "%dx%d FP64 image to a file and back" % (n, n),
+            "write_mb_s": round(mb / (t1 - t0), 1), "read_mb_s": round(mb / (t2 - t1), 1),
+            "round_trip_exact": bool(np.array_equal(np.asarray(back), np.asarray(img)))}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -691,6 +821,11 @@ def main():
                         "rel_l2_batched_vs_plain": em_diff,
                         "ll_first_last": [round(ll[0], 3), round(ll[-1], 3)],
                         "ll_first_last_batched": [round(llb[0][0], 3), round(llb[-1][0], 3)]}
+
+    if not args.no_rows:
+        result["mesh"] = mesh_leg(dev, img, n, R, A)
+        result["transmission_em"] = tr_em_leg(dev)
+        result["binary_io"] = io_leg()
 
     # the drop-in host path (pyraft.radon / backprojection: FP64 numpy in and out,
     # PCIe transfers and FP64<->FP32 conversion included), same workload
