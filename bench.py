@@ -268,6 +268,8 @@ def cpu_baseline(n, angles):
     * one OSEM iteration at 1024^2 / 720 angles / 16 subsets composed from the C operators
       with the raft_em_gpu recurrence (raft/raft_em_gpu.cu:41-69): per subset a Radon of
       its 45 angles, the ratio, a backprojection and the update (weights precomputed);
+    * the line-traversal Radon and transpose (the reference's iterator) on 24 angles, and
+      the reference's binary image write and read of the 2048^2 image;
     * the mesh Radon (the reference's withmesh) on 24 of the mesh leg's angles, and one
       transmission-EM iteration at 1024^2 / 720 angles composed from the C operators with
       the raft_tr_em_gpu recurrence (raft/raft_transmission_em.cu:41-59);
@@ -353,6 +355,40 @@ def cpu_baseline(n, angles):
     t_mesh = time.perf_counter() - t0
     out["mesh"] = {"workload": "%dx%d image, %d detector positions x 24 of the mesh leg's angles" % (n, n, n),
                    "radon_s": round(t_mesh, 4), "radon_gups": round(float(n) * n * 24 / t_mesh / 1e9, 4)}
+
+    # -- the line-traversal pair: the reference's own iterator (raft/raft_ddaiterator.c,
+    # driven by oracle/ref_dda_harness.c), 24 angles over [0, pi]
+    if ref:
+        dda_r = lambda: O.ref_radon_bresenham(img, n, 24)  # noqa: E731
+        dda_t = lambda s_: O.ref_radon_transpose_bresenham(s_, (n, n))  # noqa: E731
+    else:
+        dda_r = lambda: O.radon_bresenham(img, n, 24, nthreads=nthreads)  # noqa: E731
+        dda_t = lambda s_: O.radon_transpose_bresenham(s_, (n, n), nthreads=nthreads)  # noqa: E731
+    t0 = time.perf_counter()
+    s24 = dda_r()
+    t1 = time.perf_counter()
+    dda_t(s24)
+    t2 = time.perf_counter()
+    upd24 = float(n) * n * 24
+    out["bresenham"] = {"workload": "%dx%d image, %d rays x 24 angles over [0, pi]" % (n, n, n),
+                        "radon_s": round(t1 - t0, 4), "transpose_s": round(t2 - t1, 4),
+                        "radon_gups": round(upd24 / (t1 - t0) / 1e9, 4),
+                        "transpose_gups": round(upd24 / (t2 - t1) / 1e9, 4)}
+
+    # -- binary image files: the reference's raft_image_write / raft_image_read
+    if ref:
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "image.raft")
+            O.ref_image_write(path, img, (-1.0, 1.0), (1.0, -1.0))  # file created, pages cached
+            t0 = time.perf_counter()
+            O.ref_image_write(path, img, (-1.0, 1.0), (1.0, -1.0))
+            t1 = time.perf_counter()
+            O.ref_image_read(path, cap=img.size)
+            t2 = time.perf_counter()
+        mb = img.size * 8 / 1e6
+        out["binary_io"] = {"workload": "%dx%d FP64 image to a file and back" % (n, n),
+                            "write_mb_s": round(mb / (t1 - t0), 1), "read_mb_s": round(mb / (t2 - t1), 1)}
 
     # -- one transmission-EM iteration, 1024^2, 720 angles (raft/raft_transmission_em.cu:41-59
     # composed from the C operators: W = BP(raw) beforehand)
