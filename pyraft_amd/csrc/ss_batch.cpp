@@ -367,7 +367,7 @@ int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipSt
     out->radon_slab_rows = p->radon_rows_fixed;
     return 0;
   }
-  if (j1 <= j0 || p->mesh) return 0;  // mesh plans keep the rule (their key would not name the tables)
+  if (j1 <= j0) return 0;
   SSDevPlan dp = p->dev();
   int r0 = n;
   const bool divides = ss_radon_slabs(dp, j0, j1, nslices, &r0) > 1;
@@ -394,9 +394,13 @@ int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipSt
     cand = {{0, 0}, {0, 1}};
   }
   const std::string key = plan_key(p) + "|radon|range" + std::to_string(j1 - j0) + "|s" + std::to_string(nslices);
+  // mesh plans are timed per plan but kept out of the shared cache: its key does not
+  // name their t and theta tables
+  const bool shared = !p->mesh;
   TuneEntry cached;
   int chosen = -1;
-  if (tune_find(key, &cached) && cached.variant >= 0 && cached.variant < (int)cand.size()) chosen = cached.variant;
+  if (shared && tune_find(key, &cached) && cached.variant >= 0 && cached.variant < (int)cand.size())
+    chosen = cached.variant;
   if (chosen < 0) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIP_TRY(hipStreamIsCapturing(stream, &cs));
@@ -432,7 +436,7 @@ int ss_plan_radon_rows(const raft_ss_plan* p, int j0, int j1, int nslices, hipSt
       if (te.times[c] < te.times[best]) best = c;
     }
     te.variant = best;
-    tune_store(key, te);
+    if (shared) tune_store(key, te);
     chosen = best;
   }
   std::lock_guard<std::mutex> lock(p->radon_mutex);

@@ -59,7 +59,8 @@ int ss_em_launch(const raft_ss_plan* p, const SSEmLaunch& L, const float* d_sino
 // set in *dp.  Candidates: both shapes without row slabs and, where the default rule
 // divides the launch, both shapes at the rule's slab rows and shape 0 at twice and half of
 // them.  The fastest is timed once per process (gather layouts built first, not timed)
-// and cached with the backprojection choices; no timing while `stream` is captured, and
+// and cached with the backprojection choices (mesh plans: per plan only, since the cache
+// key does not name their tables); no timing while `stream` is captured, and
 // with `may_time` false only the plan's earlier choice or the rule with shape 0 (no host
 // allocation).  Rows pinned by raft_ss_plan_set_radon_slab_rows apply to every launch,
 // with shape 0.
